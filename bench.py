@@ -1,0 +1,194 @@
+"""Benchmark of the HIP chunk executor on one MI355X: the fused store exp(a)*0.5 + log(|b|+1)*c over
+N = 2^28 float32 elements (a, b, c seeded uniform[-4, 4], 1% of b NaN), as GB/s of algorithmic
+traffic (3 arrays read, 1 written: 16 bytes per element) and as a fraction of the 8 TB/s HBM peak.
+
+A step is one pass of the chunk over the whole arrays, inputs resident in HBM. Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK = 8.0e12  # bytes / s, MI355X
+
+
+class Events:
+    """HIP events on the library's own stream (torch's events would only see torch's streams)."""
+
+    def __init__(self, stream: int):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.stream = ctypes.c_void_p(stream)
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, event):
+        assert self.hip.hipEventRecord(event, self.stream) == 0
+
+    def elapsed_ms(self, start, stop) -> float:
+        assert self.hip.hipEventSynchronize(stop) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), start, stop) == 0
+        return ms.value
+
+
+def time_launches(device, events, launch, steps: int, warmup: int):
+    """Runs `launch` warmup + steps times; returns (wall ms per step, event ms per launch)."""
+    for _ in range(warmup):
+        launch()
+    device.synchronize()
+    start, stop = events.create(), events.create()
+    t0 = time.perf_counter()
+    events.record(start)
+    for _ in range(steps):
+        launch()
+    events.record(stop)
+    device.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, events.elapsed_ms(start, stop) / steps
+
+
+def cpu_baseline(sample: int, threads: int) -> dict:
+    """The oracle (a restatement of the reference's CPU path) on a bounded sample of the workload."""
+    import numpy as np
+
+    import oracle
+    from libsci_amd.chunk import Code, Precision, record_postfix
+
+    C2_STORE = [("in", 0), ("op", Code.exponential), ("const", 0.5), ("op", Code.multiply),
+                ("in", 1), ("op", Code.absolute), ("const", 1.0), ("op", Code.add), ("op", Code.logarithm),
+                ("in", 2), ("op", Code.multiply), ("op", Code.add)]
+
+    a = oracle.uniform(sample, 1, -4.0, 4.0)
+    b = oracle.uniform(sample, 2, -4.0, 4.0, nan_seed=3, nan_fraction=0.01)
+    c = oracle.uniform(sample, 4, -4.0, 4.0)
+    chunk = record_postfix(C2_STORE, Precision.single, "output", [a.ctypes.data, b.ctypes.data, c.ctypes.data])
+    out = np.empty(sample, dtype=np.float32)
+    lib = oracle.lib()
+    lib.oracle_execute(ctypes.addressof(chunk.struct), out.ctypes.data, sample, threads)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        lib.oracle_execute(ctypes.addressof(chunk.struct), out.ctypes.data, sample, threads)
+        times.append(time.perf_counter() - t0)
+    seconds = statistics.median(times)
+    return {"value": round(16 * sample / seconds / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"the same store over N=2^{sample.bit_length() - 1} float32 elements, median of 3 passes "
+                      f"after 1 warm-up, oracle/oracle.cpp (block-wise restatement of dispatch.cpp) on "
+                      f"{threads} host threads"}
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=20)
+    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--log2n", type=int, default=28)
+    parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    parser.add_argument("--extra", action="store_true", help="also time reductions and the interpreter")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU design is described in DESIGN.md "
+                                   "and not built", "n_gpus": args.gpus}))
+        return 2
+
+    import libsci_amd as sci
+    from libsci_amd import _abi
+    from libsci_amd.chunk import Fold
+
+    n = 1 << args.log2n
+    device = sci.Device(0)
+    stream = ctypes.c_void_p()
+    _abi.check(_abi.lib().scihip_stream(device.handle, ctypes.byref(stream)))
+    events = Events(stream.value)
+
+    a = sci.uniform(n, device, 1, -4.0, 4.0)
+    b = sci.uniform(n, device, 2, -4.0, 4.0, nan_seed=3, nan_fraction=0.01)
+    c = sci.uniform(n, device, 4, -4.0, 4.0)
+    out = sci.Array(device, (n,), "float32")
+    expression = sci.exponential(a) * 0.5 + sci.logarithm(sci.absolute(b) + 1.0) * c
+    chunk = sci.record(expression)
+    device.prepare(expression)  # steady state: the chunk's specialised kernel is compiled and ready
+    device.set_policy(2)
+
+    def store():
+        _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), out.pointer, n))
+
+    bytes_per_step = 16 * n
+    wall_ms, kernel_ms = time_launches(device, events, store, args.steps, args.warmup)
+    achieved = bytes_per_step / (kernel_ms * 1e-3)
+    value = bytes_per_step / (wall_ms * 1e-3) / 1e9
+    traffic = None
+    traffic_file = os.path.join(HERE, "profiles", "traffic_c2_store.txt")
+    if os.path.exists(traffic_file):
+        for line in open(traffic_file):
+            if line.startswith("traffic_bytes:"):
+                traffic = int(line.split(":")[1])
+
+    extra = {}
+    if args.extra:
+        device.set_policy(1)
+        w, k = time_launches(device, events, store, max(args.steps // 2, 3), 2)
+        extra["interpreter_store_GBps"] = round(bytes_per_step / (k * 1e-3) / 1e9, 1)
+        device.set_policy(0)
+        reduce_expr = sci.sin(a) * sci.cos(c) + sci.tanh(b)
+        device.prepare(reduce_expr, reduce=True)
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            sci.reduce_raw(reduce_expr, Fold.sum)
+        extra["c2_reduce_sum_GBps"] = round(12 * n / ((time.perf_counter() - t0) / reps) / 1e9, 1)
+        dot_expr = a * c
+        device.prepare(dot_expr, reduce=True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sci.reduce_raw(dot_expr, Fold.sum)
+        extra["c2_dot_GBps"] = round(8 * n / ((time.perf_counter() - t0) / reps) / 1e9, 1)
+
+    result = {
+        "metric": "fused-chunk HBM GB/s (% peak); LU solve time at FP64 accuracy, mixed vs plain",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded uniform[-4,4] generated on the device, 1% NaN in b",
+        "config": {"workload": f"fused chunk store exp(a)*0.5 + log(|b|+1)*c, N=2^{args.log2n} float32, "
+                               f"specialised kernel", "elements": n, "bytes_per_step": bytes_per_step},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic},
+    }
+    if extra:
+        result["extra"] = extra
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        result["cpu_baseline"] = cpu_baseline(min(n, 1 << 26), threads)
+    print(json.dumps(result), flush=True)
+    device.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

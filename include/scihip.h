@@ -1,0 +1,178 @@
+/*
+ * scihip.h - the C ABI of the MI355X (gfx950) HIP backend for libsci.
+ *
+ * This is the drop-in boundary: every entry point replaces one member of the reference's
+ * device boundary, `sci::core::drivers::Device` (include/core/drivers/device.hpp:27-89), or one
+ * host-only routine of include/linear/ that the backend moves onto the GPU. Plain pointers and
+ * sizes only: no C++ or torch types cross it.
+ *
+ * Conventions.
+ *  - Every entry point returns an int status: SCIHIP_OK (0) on success, SCIHIP_DECLINED (1)
+ *    when the GPU cannot run the request as given (an operand that is not device memory of this
+ *    device, an unsupported precision or size); the caller then runs its own CPU path, as the
+ *    reference's Device does after a wait (src/core/drivers/device.cpp:260-262, 313-315,
+ *    349-351). Negative values are errors (SCIHIP_E*). scihip_error() gives the last message of
+ *    the calling thread.
+ *  - Arrays are row-major and contiguous. Device pointers come from scihip_allocate; interior
+ *    pointers are accepted wherever a device pointer is (as the reference's Allocator::locate,
+ *    src/core/drivers/vulkan/allocator.cpp:81-89).
+ *  - scihip_execute, scihip_product and the copies are asynchronous and ordered on the device's
+ *    one stream, like the reference's Vulkan stream (src/core/drivers/vulkan/stream.cpp:71-83).
+ *    scihip_reduce and every call that returns a value to the host are synchronous.
+ *  - Each device serialises its callers with a mutex, as the reference's `guard` does
+ *    (src/core/drivers/device.cpp:127).
+ */
+#ifndef SCIHIP_H
+#define SCIHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCIHIP_OK 0
+#define SCIHIP_DECLINED 1
+#define SCIHIP_EINVAL (-1)
+#define SCIHIP_ENOMEM (-2)
+#define SCIHIP_EDEVICE (-3)
+#define SCIHIP_ECOMPILE (-4)
+
+/* Chunk capacity: instructions, inputs and constants (include/core/bytecode/chunk.hpp:31). */
+#define SCIHIP_CAPACITY 64
+
+/* Value ids: a 2-bit tag and a 6-bit index (include/core/bytecode/instruction.hpp:11-25). */
+#define SCIHIP_SLOT_INPUT 0x00u
+#define SCIHIP_SLOT_LITERAL 0x40u
+#define SCIHIP_SLOT_TEMPORARY 0x80u
+#define SCIHIP_SLOT_OUTPUT 0xC0u
+#define SCIHIP_SLOT_NONE 0xFFu
+#define SCIHIP_SLOT_TYPES 0xC0u
+
+/* Operation codes, in the order of core::Code (include/core/bytecode/code.hpp:18-61). */
+enum scihip_code {
+    SCIHIP_COPY, SCIHIP_NEGATE, SCIHIP_ABSOLUTE,
+    SCIHIP_ROOT, SCIHIP_EXPONENTIAL, SCIHIP_LOGARITHM,
+    SCIHIP_SIN, SCIHIP_COS, SCIHIP_TAN, SCIHIP_ASIN, SCIHIP_ACOS, SCIHIP_ATAN,
+    SCIHIP_CSC, SCIHIP_SEC, SCIHIP_COT, SCIHIP_ACSC, SCIHIP_ASEC, SCIHIP_ACOT,
+    SCIHIP_SINH, SCIHIP_COSH, SCIHIP_TANH, SCIHIP_ASINH, SCIHIP_ACOSH, SCIHIP_ATANH,
+    SCIHIP_CSCH, SCIHIP_SECH, SCIHIP_COTH, SCIHIP_ACSCH, SCIHIP_ASECH, SCIHIP_ACOTH,
+    SCIHIP_ADD, SCIHIP_SUBTRACT, SCIHIP_MULTIPLY, SCIHIP_DIVIDE, SCIHIP_MINIMUM, SCIHIP_MAXIMUM,
+    SCIHIP_FUSE,
+    SCIHIP_CODES
+};
+
+/* Folds (include/core/bytecode/fold.hpp:11). */
+enum scihip_fold { SCIHIP_FOLD_SUM = 0, SCIHIP_FOLD_MINIMUM = 1, SCIHIP_FOLD_MAXIMUM = 2 };
+
+/* Element precision (include/core/bytecode/precision.hpp:12). */
+enum scihip_precision { SCIHIP_SINGLE = 0, SCIHIP_WIDE = 1 };
+
+/* One instruction: target = code(sources...) (include/core/bytecode/instruction.hpp:28-35). */
+typedef struct scihip_instruction {
+    uint8_t code;
+    uint8_t target;
+    uint8_t sources[3];
+} scihip_instruction;
+
+/*
+ * A recorded chunk: the public fields of core::Chunk (include/core/bytecode/chunk.hpp:178-196),
+ * in the same order and with the same meaning. Only the first `length` instructions, `arguments`
+ * inputs and `literals` constants are read. Inputs are device pointers (interior pointers
+ * allowed). The chunk is borrowed for the call only.
+ */
+typedef struct scihip_chunk {
+    uint8_t precision;
+    uint8_t length;
+    uint8_t arguments;
+    uint8_t literals;
+    uint8_t temporaries;
+    uint8_t result;
+    scihip_instruction instructions[SCIHIP_CAPACITY];
+    const void *inputs[SCIHIP_CAPACITY];
+    double constants[SCIHIP_CAPACITY];
+} scihip_chunk;
+
+typedef struct scihip_device scihip_device;
+
+/* The last error message of the calling thread, or "" (never null). */
+const char *scihip_error(void);
+
+/* Opens GPU `ordinal`; replaces Device::Device(Target::graphics, index)
+ * (include/core/drivers/device.hpp:32, src/core/drivers/device.cpp:186-199). Returns
+ * SCIHIP_DECLINED when no such GPU exists, so the caller can become a CPU device. */
+int scihip_open(uint32_t ordinal, scihip_device **device);
+
+/* Waits for queued work and releases the device (src/core/drivers/device.cpp:201). */
+int scihip_close(scihip_device *device);
+
+/* The device's name (include/core/drivers/device.hpp:46). Valid until scihip_close. */
+const char *scihip_name(const scihip_device *device);
+
+/* Blocks until all queued work has finished (include/core/drivers/device.hpp:49). */
+int scihip_synchronize(scihip_device *device);
+
+/* Device memory of `bytes` bytes, 256-byte aligned (include/core/drivers/device.hpp:55,
+ * src/core/drivers/device.cpp:212-224). Reuses a cached block when one fits. */
+int scihip_allocate(scihip_device *device, size_t bytes, void **pointer);
+
+/* Returns memory scihip_allocate gave; the block is reused or freed only after the work queued
+ * so far has finished (include/core/drivers/device.hpp:58, src/core/drivers/device.cpp:226-233). */
+int scihip_dispose(scihip_device *device, void *pointer);
+
+/* 1 when `pointer` lies inside a live allocation of this device, else 0
+ * (src/core/drivers/vulkan/allocator.cpp:81-89). */
+int scihip_locate(scihip_device *device, const void *pointer);
+
+/* Stream-ordered copies between host and device memory, or within the device. The reference
+ * copies through the mapped pointer (include/array/array.hpp:223-237); MI355X HBM is not
+ * host-mapped, so Array::to / host() go through these. Upload and download wait for the copy. */
+int scihip_upload(scihip_device *device, void *destination, const void *source, size_t bytes);
+int scihip_download(scihip_device *device, void *destination, const void *source, size_t bytes);
+int scihip_copy(scihip_device *device, void *destination, const void *source, size_t bytes);
+
+/* Evaluates a chunk over `count` elements into `output` (include/core/drivers/device.hpp:63,
+ * src/core/drivers/device.cpp:235-263). Asynchronous. A chunk whose specialised kernel is ready
+ * runs that kernel; otherwise the bytecode interpreter runs it and a specialised kernel is
+ * compiled on a background thread (src/core/drivers/vulkan/compiler.cpp:169-222). */
+int scihip_execute(scihip_device *device, const scihip_chunk *chunk, void *output, size_t count);
+
+/* Sum, minimum or maximum of the chunk's result over `count` elements, in double
+ * (include/core/drivers/device.hpp:69, src/core/drivers/device.cpp:265-316). Synchronous.
+ * NaN anywhere gives NaN; the minimum and maximum of nothing are NaN and the sum of nothing 0.
+ * A sum whose chunk ends in a multiply into its result is contracted into a dot product
+ * (include/core/bytecode/chunk.hpp:151-154). */
+int scihip_reduce(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t count, double *result);
+
+/* Dispatch policy: 0 = automatic (interpreter until the specialised kernel is ready),
+ * 1 = interpreter only, 2 = specialised kernel only (compiles and waits on first use).
+ * Mirrors SCI_JIT=0 (src/core/drivers/vulkan/compiler.cpp:153-158); the SCIHIP_JIT environment
+ * variable sets the initial policy ("0" = 1, "force" = 2). */
+int scihip_set_policy(scihip_device *device, int policy);
+
+/* Compiles the specialised kernel of a chunk now and waits for it (the background compile that
+ * scihip_execute starts, done in the foreground). SCIHIP_ECOMPILE when compilation fails. */
+int scihip_prepare(scihip_device *device, const scihip_chunk *chunk);
+
+/* Counts of launches since open: interpreter and specialised kernels, and completed compiles. */
+int scihip_statistics(scihip_device *device, uint64_t *interpreted, uint64_t *specialised, uint64_t *compiled);
+
+/* The device's HIP stream (a hipStream_t), on which every call above queues its work. For callers
+ * that order or time their own work against it (HIP events, graphs). No reference counterpart: the
+ * reference's Vulkan stream is internal (src/core/drivers/vulkan/stream.cpp:25-156). */
+int scihip_stream(scihip_device *device, void **stream);
+
+/* Seeded synthetic data, generated in place on the device and identical bit for bit to the
+ * oracle's CPU generator: element i = lower + (upper - lower) * u(seed, i), u = top 24 bits of a
+ * SplitMix64 hash of (seed, i) times 2^-24 (single) or top 53 bits times 2^-53 (wide); then every
+ * element whose hash under `nan_seed` falls below nan_fraction * 2^32 becomes NaN (nan_fraction 0
+ * disables it). No reference counterpart: it feeds benchmarks and tests without host traffic. */
+int scihip_fill_uniform(scihip_device *device, void *output, uint32_t precision, size_t count, uint64_t seed,
+                        double lower, double upper, uint64_t nan_seed, double nan_fraction);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

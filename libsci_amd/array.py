@@ -1,0 +1,496 @@
+"""Arrays on a HIP device and lazy elementwise expressions over them.
+
+The Python mirror of the reference's array layer: ``Device`` is ``sci::Device``
+(include/core/drivers/device.hpp:27-89), ``Array`` is ``sci::Array`` (include/array/array.hpp:35-285)
+for arrays that live on the device, and the expression nodes record a Chunk the way
+``Node`` / ``Reference`` / ``Literal`` do (include/array/node.hpp:75-88, reference.hpp:45,
+literal.hpp:34-36). Writing ``a * b + 2.0`` computes nothing; assigning it (``Array(expr)``,
+``target.assign(expr)``) or reducing it (``sum(expr)``) records one Chunk and runs it in one pass
+through the C ABI. Host data enters with ``to(device)`` and leaves with ``host()``.
+
+There is no CPU engine here: every computation runs on the device, and an expression whose arrays
+are not on one device raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _abi
+from .chunk import Chunk, Code, Fold, Precision
+
+_DTYPES = {np.dtype(np.float32): Precision.single, np.dtype(np.float64): Precision.wide}
+
+
+class Device:
+    """A HIP device (include/core/drivers/device.hpp:27-89). Raises when no GPU is usable."""
+
+    def __init__(self, index: int = 0):
+        handle = ctypes.c_void_p()
+        _abi.check(_abi.lib().scihip_open(index, ctypes.byref(handle)), "scihip_open")
+        self.handle = handle
+        self.index = index
+
+    @property
+    def name(self) -> str:
+        return _abi.lib().scihip_name(self.handle).decode()
+
+    def synchronize(self) -> None:
+        _abi.check(_abi.lib().scihip_synchronize(self.handle), "scihip_synchronize")
+
+    def allocate(self, nbytes: int) -> int:
+        pointer = ctypes.c_void_p()
+        _abi.check(_abi.lib().scihip_allocate(self.handle, max(int(nbytes), 1), ctypes.byref(pointer)),
+                   "scihip_allocate")
+        return pointer.value
+
+    def dispose(self, pointer: int) -> None:
+        if self.handle:
+            _abi.check(_abi.lib().scihip_dispose(self.handle, pointer), "scihip_dispose")
+
+    def locate(self, pointer: int) -> bool:
+        return bool(_abi.lib().scihip_locate(self.handle, pointer))
+
+    def set_policy(self, policy: int) -> None:
+        """0 automatic, 1 interpreter only, 2 specialised kernels only."""
+        _abi.check(_abi.lib().scihip_set_policy(self.handle, int(policy)), "scihip_set_policy")
+
+    def prepare(self, expression, precision=Precision.single, reduce: bool = False) -> None:
+        """Compiles the specialised kernel of an expression's chunk now."""
+        chunk = record(expression, reduce=reduce)
+        _abi.check(_abi.lib().scihip_prepare(self.handle, ctypes.byref(chunk.struct)), "scihip_prepare")
+
+    def statistics(self) -> dict:
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _abi.check(_abi.lib().scihip_statistics(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"interpreted": a.value, "specialised": b.value, "compiled": c.value}
+
+    def close(self) -> None:
+        if self.handle:
+            _abi.lib().scihip_close(self.handle)
+            self.handle = None
+
+    def __repr__(self) -> str:
+        return f"Device({self.index}: {self.name})"
+
+
+# ------------------------------------------------------------------------------------------ arrays
+
+class _Storage:
+    """A device allocation, returned to the device when the last array using it goes."""
+
+    def __init__(self, device: Device, nbytes: int):
+        self.device = device
+        self.pointer = device.allocate(nbytes)
+        self.nbytes = nbytes
+
+    def __del__(self):
+        try:
+            if self.device.handle and self.pointer:
+                self.device.dispose(self.pointer)
+        except Exception:
+            pass
+
+
+class Array:
+    """A contiguous row-major float32 / float64 array on a Device (include/array/array.hpp:35-285)."""
+
+    def __init__(self, device: Device, shape, dtype=np.float32, storage: _Storage | None = None, offset: int = 0):
+        self.device = device
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in _DTYPES:
+            raise TypeError("arrays hold float32 or float64")
+        self.size = int(np.prod(self.shape, dtype=np.int64)) if self.shape else 1
+        self.storage = storage if storage is not None else _Storage(device, self.size * self.dtype.itemsize)
+        self.offset = offset
+
+    # -- identity
+    @property
+    def pointer(self) -> int:
+        return self.storage.pointer + self.offset
+
+    @property
+    def ndim(self) -> int:
+        return len(self.shape)
+
+    @property
+    def precision(self) -> Precision:
+        return _DTYPES[self.dtype]
+
+    def __len__(self) -> int:
+        return self.shape[0]
+
+    # -- residency (array.hpp:223-237)
+    @classmethod
+    def from_numpy(cls, values, device: Device, dtype=None) -> "Array":
+        data = np.ascontiguousarray(values, dtype=dtype if dtype is not None else None)
+        if data.dtype not in _DTYPES:
+            data = data.astype(np.float32)
+        out = cls(device, data.shape, data.dtype)
+        if data.size:
+            _abi.check(_abi.lib().scihip_upload(device.handle, out.pointer, data.ctypes.data, data.nbytes),
+                       "scihip_upload")
+        return out
+
+    def host(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        if out.size:
+            _abi.check(_abi.lib().scihip_download(self.device.handle, out.ctypes.data, self.pointer, out.nbytes),
+                       "scihip_download")
+        return out
+
+    numpy = host
+
+    def to(self, device: Device) -> "Array":
+        if device is self.device:
+            out = Array(device, self.shape, self.dtype)
+            _abi.check(_abi.lib().scihip_copy(device.handle, out.pointer, self.pointer, self.size * self.dtype.itemsize),
+                       "scihip_copy")
+            return out
+        return Array.from_numpy(self.host(), device)
+
+    def __getitem__(self, index):
+        return self.host()[index]
+
+    def reshape(self, *shape) -> "Array":
+        shape = shape[0] if len(shape) == 1 and isinstance(shape[0], (tuple, list)) else shape
+        if int(np.prod(shape)) != self.size:
+            raise ValueError("reshape keeps the element count")
+        return Array(self.device, shape, self.dtype, self.storage, self.offset)
+
+    def view(self, start: int, count: int) -> "Array":
+        """Elements [start, start + count) of a flat array, sharing its memory (an interior pointer)."""
+        if start < 0 or start + count > self.size:
+            raise IndexError("view out of range")
+        return Array(self.device, (count,), self.dtype, self.storage, self.offset + start * self.dtype.itemsize)
+
+    # -- expression protocol
+    def record(self, chunk: Chunk) -> int:
+        return chunk.input(self.pointer)
+
+    def devices(self):
+        return [self.device]
+
+    def leaf_dtype(self):
+        return self.dtype
+
+    def element_shape(self):
+        return self.shape
+
+    # -- evaluation
+    def assign(self, expression) -> "Array":
+        """Writes an expression into this array (include/array/expression.hpp:158-185)."""
+        assign(self, expression)
+        return self
+
+    def sum(self):
+        return sum(self)
+
+    def mean(self):
+        return mean(self)
+
+    def minimum(self):
+        return minimum(self)
+
+    def maximum(self):
+        return maximum(self)
+
+    def __repr__(self) -> str:
+        return f"Array(shape={self.shape}, dtype={self.dtype.name}, device={self.device.index})"
+
+    # operators are shared with Node below
+    __add__ = lambda self, other: _binary(Code.add, self, other)
+    __radd__ = lambda self, other: _binary(Code.add, other, self)
+    __sub__ = lambda self, other: _binary(Code.subtract, self, other)
+    __rsub__ = lambda self, other: _binary(Code.subtract, other, self)
+    __mul__ = lambda self, other: _binary(Code.multiply, self, other)
+    __rmul__ = lambda self, other: _binary(Code.multiply, other, self)
+    __truediv__ = lambda self, other: _binary(Code.divide, self, other)
+    __rtruediv__ = lambda self, other: _binary(Code.divide, other, self)
+    __neg__ = lambda self: Node(Code.negate, (self,))
+
+
+class Literal:
+    """A scalar leaf, already narrowed to the element type of the array side (expression.hpp:88-104)."""
+
+    def __init__(self, value, dtype):
+        self.value = float(np.dtype(dtype).type(value))
+        self.dtype = np.dtype(dtype)
+
+    def record(self, chunk: Chunk) -> int:
+        return chunk.constant(self.value)
+
+    def devices(self):
+        return []
+
+    def leaf_dtype(self):
+        return self.dtype
+
+    def element_shape(self):
+        return None
+
+
+class Node:
+    """An operation over operands (include/array/node.hpp:250-331)."""
+
+    def __init__(self, code: Code, children):
+        self.code = Code(code)
+        self.children = tuple(children)
+        dtypes = {c.leaf_dtype() for c in self.children}
+        if len(dtypes) != 1:
+            raise TypeError("operands must share an element type")
+        shapes = [c.element_shape() for c in self.children if c.element_shape() is not None]
+        if not shapes:
+            raise TypeError("an expression needs at least one array operand")
+        if any(s != shapes[0] for s in shapes):
+            raise ValueError("operands must share a shape (no broadcasting apart from scalars)")
+        self.shape = shapes[0]
+        self.dtype = dtypes.pop()
+
+    def record(self, chunk: Chunk) -> int:
+        # Operands first, left to right, then this operation (node.hpp:299-312).
+        slots = [c.record(chunk) for c in self.children]
+        return chunk.emit(self.code, *slots)
+
+    def devices(self):
+        found = []
+        for c in self.children:
+            for d in c.devices():
+                if d not in found:
+                    found.append(d)
+        return found
+
+    def leaf_dtype(self):
+        return self.dtype
+
+    def element_shape(self):
+        return self.shape
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.shape)) if self.shape else 1
+
+    def copy(self) -> Array:
+        return evaluate(self)
+
+    def sum(self):
+        return sum(self)
+
+    def mean(self):
+        return mean(self)
+
+    def minimum(self):
+        return minimum(self)
+
+    def maximum(self):
+        return maximum(self)
+
+    __add__ = Array.__add__
+    __radd__ = Array.__radd__
+    __sub__ = Array.__sub__
+    __rsub__ = Array.__rsub__
+    __mul__ = Array.__mul__
+    __rmul__ = Array.__rmul__
+    __truediv__ = Array.__truediv__
+    __rtruediv__ = Array.__rtruediv__
+    __neg__ = Array.__neg__
+
+
+def _operand(x):
+    return isinstance(x, (Array, Node))
+
+
+def _binary(code: Code, left, right):
+    if not _operand(left) and not _operand(right):
+        return NotImplemented
+    if not _operand(left):
+        left = Literal(left, right.leaf_dtype())
+    elif not _operand(right):
+        right = Literal(right, left.leaf_dtype())
+    return Node(code, (left, right))
+
+
+# ------------------------------------------------------------------------------------------ running
+
+def _device_of(expression) -> Device:
+    devices = expression.devices()
+    if not devices:
+        raise TypeError("an expression needs an array operand")
+    if len(devices) > 1:
+        raise ValueError("every array of an expression must live on the same device")
+    return devices[0]
+
+
+def record(expression, reduce: bool = False) -> Chunk:
+    precision = _DTYPES[expression.leaf_dtype()]
+    chunk = Chunk(precision)
+    value = expression.record(chunk)
+    if reduce:
+        chunk.reduce(value)
+    else:
+        chunk.output(value)
+    if not chunk.valid():
+        raise ValueError("the expression outgrew a chunk (64 instructions, inputs or constants)")
+    return chunk
+
+
+def assign(target: Array, expression) -> None:
+    """Runs an expression into `target` on the target's device (expression.hpp:158-185)."""
+    if tuple(target.shape) != tuple(expression.element_shape()):
+        raise ValueError("the target must have the expression's shape")
+    if expression.leaf_dtype() != target.dtype:
+        raise TypeError("the target must have the expression's element type")
+    device = _device_of(expression)
+    if device is not target.device:
+        raise ValueError("the target and the expression's arrays must live on the same device")
+    chunk = record(expression)
+    _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), target.pointer, target.size),
+               "scihip_execute")
+
+
+def evaluate(expression) -> Array:
+    """A new array holding the expression (Array(const source&), array.hpp:107-114)."""
+    if isinstance(expression, Array):
+        expression = Node(Code.copy, (expression,))
+    device = _device_of(expression)
+    out = Array(device, expression.element_shape(), expression.leaf_dtype())
+    assign(out, expression)
+    return out
+
+
+def reduce_raw(expression, fold: Fold) -> float:
+    """The fold of an expression in double, before the cast to the element type."""
+    device = _device_of(expression)
+    chunk = record(expression, reduce=True)
+    count = expression.size
+    result = ctypes.c_double()
+    _abi.check(_abi.lib().scihip_reduce(device.handle, ctypes.byref(chunk.struct), int(fold), count,
+                                        ctypes.byref(result)), "scihip_reduce")
+    return result.value
+
+
+def reduce(expression, fold: Fold):
+    """Sum, minimum or maximum, cast to the element type (expression.hpp:188-202)."""
+    return expression.leaf_dtype().type(reduce_raw(expression, fold))
+
+
+def sum(expression):
+    return reduce(expression, Fold.sum)
+
+
+def mean(expression):
+    """float(sum) / float(count) in the element type (include/operations/reduce.hpp:24-28)."""
+    dtype = expression.leaf_dtype().type
+    count = expression.size
+    return dtype(0) if count == 0 else dtype(sum(expression) / dtype(count))
+
+
+def minimum(left, right=None):
+    """minimum(a) reduces (reduce.hpp:36-38); minimum(a, b) is elementwise (binary.hpp:60-66)."""
+    if right is None:
+        return reduce(left, Fold.minimum)
+    return _binary(Code.minimum, left, right)
+
+
+def maximum(left, right=None):
+    if right is None:
+        return reduce(left, Fold.maximum)
+    return _binary(Code.maximum, left, right)
+
+
+def dot(left, right):
+    """sum(a * b): contracted into a dot product (include/linear/product.hpp:72-76)."""
+    return sum(_binary(Code.multiply, left, right))
+
+
+def _unary(code: Code):
+    def op(x):
+        return Node(code, (x,))
+    op.__name__ = code.name
+    op.__doc__ = f"Elementwise {code.name} (include/operations/unary.hpp)."
+    return op
+
+
+def _named_binary(code: Code):
+    def op(a, b):
+        return _binary(code, a, b)
+    op.__name__ = code.name
+    return op
+
+
+copy = _unary(Code.copy)
+negate = _unary(Code.negate)
+absolute = _unary(Code.absolute)
+root = _unary(Code.root)
+exponential = _unary(Code.exponential)
+logarithm = _unary(Code.logarithm)
+sin = _unary(Code.sin)
+cos = _unary(Code.cos)
+tan = _unary(Code.tan)
+asin = _unary(Code.asin)
+acos = _unary(Code.acos)
+atan = _unary(Code.atan)
+csc = _unary(Code.csc)
+sec = _unary(Code.sec)
+cot = _unary(Code.cot)
+acsc = _unary(Code.acsc)
+asec = _unary(Code.asec)
+acot = _unary(Code.acot)
+sinh = _unary(Code.sinh)
+cosh = _unary(Code.cosh)
+tanh = _unary(Code.tanh)
+asinh = _unary(Code.asinh)
+acosh = _unary(Code.acosh)
+atanh = _unary(Code.atanh)
+csch = _unary(Code.csch)
+sech = _unary(Code.sech)
+coth = _unary(Code.coth)
+acsch = _unary(Code.acsch)
+asech = _unary(Code.asech)
+acoth = _unary(Code.acoth)
+add = _named_binary(Code.add)
+subtract = _named_binary(Code.subtract)
+multiply = _named_binary(Code.multiply)
+divide = _named_binary(Code.divide)
+
+
+# ------------------------------------------------------------------------------------------ creation
+
+def linspace(start: float, stop: float, count: int, device: Device, dtype=np.float32) -> Array:
+    """x_i = T(start + i * step), step = double(stop - start) / (count - 1) (creation.hpp:65-84)."""
+    t = np.dtype(dtype).type
+    values = np.empty(count, dtype=dtype)
+    if count == 1:
+        values[0] = t(start)
+    elif count > 1:
+        step = float(t(t(stop) - t(start))) / (float(count) - 1.0)
+        values[:] = (float(t(start)) + np.arange(count, dtype=np.float64) * step).astype(dtype)
+    return Array.from_numpy(values, device)
+
+
+def full(shape, value, device: Device, dtype=np.float32) -> Array:
+    return Array.from_numpy(np.full(shape, value, dtype=dtype), device)
+
+
+def ones(shape, device: Device, dtype=np.float32) -> Array:
+    return full(shape, 1, device, dtype)
+
+
+def zeros(shape, device: Device, dtype=np.float32) -> Array:
+    return full(shape, 0, device, dtype)
+
+
+def uniform(count: int, device: Device, seed: int, lower: float, upper: float, dtype=np.float32,
+            nan_seed: int = 0, nan_fraction: float = 0.0) -> Array:
+    """Seeded uniform data generated on the device (scihip_fill_uniform)."""
+    out = Array(device, (count,), dtype)
+    _abi.check(_abi.lib().scihip_fill_uniform(device.handle, out.pointer, int(_DTYPES[np.dtype(dtype)]), count,
+                                              seed, lower, upper, nan_seed, nan_fraction), "scihip_fill_uniform")
+    return out
+
+
+__all__ = [name for name in dir() if not name.startswith("_") and name not in ("ctypes", "math", "np")]

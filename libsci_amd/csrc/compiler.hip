@@ -1,0 +1,180 @@
+// Specialised chunk kernels compiled with hipRTC on a background thread (see compiler.hpp).
+#include "compiler.hpp"
+#include "runtime.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+#include <vector>
+
+namespace scihip {
+
+    // device_math.hpp and interpreter.hpp, embedded at build time (build.py writes embedded.inc).
+#include "embedded.inc"
+
+    namespace {
+
+        std::string name_of(uint8_t id, const std::string *temporaries) {
+            const uint8_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3F;
+            if (id == SCIHIP_SLOT_NONE) return "i0";
+            if (tag == SCIHIP_SLOT_INPUT) return "i" + std::to_string(index);
+            if (tag == SCIHIP_SLOT_LITERAL) return "k" + std::to_string(index);
+            if (tag == SCIHIP_SLOT_TEMPORARY) return temporaries[index];
+            return temporaries[64];
+        }
+
+    } // namespace
+
+    std::string Compiler::generate(const scihip_chunk &chunk, int mode, bool contract) {
+        const bool single = chunk.precision == SCIHIP_SINGLE;
+        const char *type = single ? "float" : "double";
+        const int width = single ? 4 : 2;
+        std::ostringstream o;
+        o << "extern \"C\" __global__ __launch_bounds__(256) void scihip_specialised(const scihip::Launch<" << type
+          << "> a) {\n"
+          << "    using namespace scihip;\n"
+          << "    typedef " << type << " T;\n"
+          << "    constexpr int W = " << width << ";\n"
+          << "    typedef Lanes<T, W> L;\n"
+          << "    const Program<T> &p = a.p;\n";
+        for (int i = 0; i < chunk.literals; ++i)
+            o << "    const L k" << i << " = splat<T, W>(p.constants[" << i << "]);\n";
+        if (mode == 1) o << "    double total = identity(p.fold);\n    uint32_t poisoned = 0;\n";
+        o << "    const uint64_t steps = (p.count + W - 1) / W;\n"
+          << "    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;\n"
+          << "    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < steps; v += stride) {\n"
+          << "        const uint64_t at = v * W;\n"
+          << "        const bool full = at + W <= p.count;\n";
+        for (int i = 0; i < chunk.arguments; ++i)
+            o << "        const L i" << i << " = fetch<T, W>(p, " << i << ", at, full);\n";
+        std::string temporaries[65];
+        for (auto &t : temporaries) t = "i0";
+        const int steps = contract ? chunk.length - 1 : chunk.length;
+        for (int s = 0; s < steps; ++s) {
+            const scihip_instruction &in = chunk.instructions[s];
+            const std::string a = name_of(in.sources[0], temporaries);
+            const std::string b = in.sources[1] == SCIHIP_SLOT_NONE ? a : name_of(in.sources[1], temporaries);
+            const std::string c = in.sources[2] == SCIHIP_SLOT_NONE ? a : name_of(in.sources[2], temporaries);
+            const std::string target = "s" + std::to_string(s);
+            o << "        const L " << target << " = lanes<" << unsigned(in.code) << "u, T, W>(" << a << ", " << b
+              << ", " << c << ");\n";
+            if (in.target == SCIHIP_SLOT_OUTPUT) temporaries[64] = target;
+            else temporaries[in.target & 0x3F] = target;
+        }
+        if (mode == 0) {
+            o << "        store<T, W>(p, at, full, " << name_of(SCIHIP_SLOT_OUTPUT, temporaries) << ");\n    }\n}\n";
+        } else if (contract) {
+            const scihip_instruction &last = chunk.instructions[chunk.length - 1];
+            o << "        contract<T, W>(total, " << name_of(last.sources[0], temporaries) << ", "
+              << name_of(last.sources[1], temporaries) << ", at, full, p.count);\n    }\n"
+              << "    finish<T>(a, total, poisoned);\n}\n";
+        } else {
+            o << "        consume<T, W>(p.fold, total, poisoned, " << name_of(chunk.result, temporaries)
+              << ", at, full, p.count);\n    }\n"
+              << "    finish<T>(a, total, poisoned);\n}\n";
+        }
+        return o.str();
+    }
+
+    Compiler::Compiler(int ordinal) : ordinal(ordinal) {
+        if (const char *text = std::getenv("SCIHIP_JIT")) trace = std::strcmp(text, "trace") == 0;
+        worker = std::thread([this] { loop(); });
+    }
+
+    Compiler::~Compiler() {
+        {
+            std::lock_guard<std::mutex> lock(guard);
+            stopping = true;
+        }
+        signal.notify_all();
+        if (worker.joinable()) worker.join();
+        for (auto &entry : kernels)
+            if (entry.second->module) (void)hipModuleUnload(entry.second->module);
+    }
+
+    const Kernel *Compiler::find(const std::string &key, const std::string &source, bool wait) {
+        std::unique_lock<std::mutex> lock(guard);
+        auto found = kernels.find(key);
+        if (found == kernels.end()) {
+            if (kernels.size() >= limit) return nullptr;
+            kernels.emplace(key, std::make_unique<Kernel>());
+            ready[key] = false;
+            if (wait) {
+                Kernel *kernel = kernels[key].get();
+                lock.unlock();
+                build(key, source, *kernel);
+                lock.lock();
+                ready[key] = true;
+                signal.notify_all();
+                return kernel->failed ? nullptr : kernel;
+            }
+            queue.emplace_back(key, source);
+            signal.notify_all();
+            return nullptr;
+        }
+        if (!ready[key]) {
+            if (!wait) return nullptr;
+            signal.wait(lock, [&] { return ready[key]; });
+        }
+        const Kernel *kernel = found->second.get();
+        return kernel->failed ? nullptr : kernel;
+    }
+
+    void Compiler::loop() {
+        (void)hipSetDevice(ordinal);
+        std::unique_lock<std::mutex> lock(guard);
+        for (;;) {
+            signal.wait(lock, [&] { return stopping || !queue.empty(); });
+            if (stopping) return;
+            auto job = std::move(queue.front());
+            queue.pop_front();
+            Kernel *kernel = kernels[job.first].get();
+            lock.unlock();
+            build(job.first, job.second, *kernel);
+            lock.lock();
+            ready[job.first] = true;
+            signal.notify_all();
+        }
+    }
+
+    void Compiler::build(const std::string &key, const std::string &source, Kernel &kernel) {
+        (void)key;
+        (void)hipSetDevice(ordinal);
+        const std::string text = std::string(kEmbeddedSource) + "\n" + source;
+        hiprtcProgram program;
+        if (hiprtcCreateProgram(&program, text.c_str(), "chunk.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+            kernel.failed = true;
+            return;
+        }
+        const char *options[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-DSCIHIP_RTC",
+                                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+        const hiprtcResult compiled = hiprtcCompileProgram(program, sizeof(options) / sizeof(options[0]), options);
+        size_t size = 0;
+        hiprtcGetProgramLogSize(program, &size);
+        if (size > 1) {
+            kernel.log.resize(size);
+            hiprtcGetProgramLog(program, kernel.log.data());
+        }
+        if (compiled != HIPRTC_SUCCESS) {
+            if (trace) std::fprintf(stderr, "[scihip] specialised compile failed:\n%s\n", kernel.log.c_str());
+            hiprtcDestroyProgram(&program);
+            kernel.failed = true;
+            return;
+        }
+        hiprtcGetCodeSize(program, &size);
+        std::vector<char> code(size);
+        hiprtcGetCode(program, code.data());
+        hiprtcDestroyProgram(&program);
+        if (hipModuleLoadData(&kernel.module, code.data()) != hipSuccess ||
+            hipModuleGetFunction(&kernel.function, kernel.module, "scihip_specialised") != hipSuccess) {
+            kernel.failed = true;
+            return;
+        }
+        ++done;
+        if (trace) std::fprintf(stderr, "[scihip] compiled a specialised kernel (%zu bytes)\n", code.size());
+    }
+
+} // namespace scihip

@@ -1,0 +1,63 @@
+// Specialised chunk kernels: each distinct chunk signature is turned into straight-line HIP source,
+// compiled with hipRTC on a background thread and loaded as a module. Replaces the reference's
+// background Slang JIT (src/core/drivers/vulkan/compiler.cpp:35-222): the first sightings of a chunk
+// run on the interpreter, later ones on its own kernel once that is ready.
+#pragma once
+
+#include "scihip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+
+namespace scihip {
+
+    struct Kernel {
+        hipModule_t module = nullptr;
+        hipFunction_t function = nullptr;
+        bool failed = false;
+        std::string log;
+    };
+
+    class Compiler {
+    public:
+        explicit Compiler(int ordinal);
+        ~Compiler();
+
+        // The kernel for `key`, or null while it is being compiled. The first call for a key queues
+        // a compile of `source`; with `wait` the call compiles in the foreground instead.
+        const Kernel *find(const std::string &key, const std::string &source, bool wait);
+
+        uint64_t compiled() const { return done; }
+
+        // Straight-line source for a chunk. `mode` 0 evaluates, 1 reduces; `contract` marks a sum
+        // whose last multiply is a dot product.
+        static std::string generate(const scihip_chunk &chunk, int mode, bool contract);
+
+        static constexpr size_t limit = 1024; // distinct chunks compiled at most (vulkan/compiler.cpp:27)
+
+    private:
+        void loop();
+        void build(const std::string &key, const std::string &source, Kernel &kernel);
+
+        int ordinal;
+        std::mutex guard;
+        std::condition_variable signal;
+        std::unordered_map<std::string, std::unique_ptr<Kernel>> kernels;
+        std::unordered_map<std::string, bool> ready;
+        std::deque<std::pair<std::string, std::string>> queue;
+        std::thread worker;
+        bool stopping = false;
+        std::atomic<uint64_t> done{0};
+        bool trace = false;
+    };
+
+} // namespace scihip

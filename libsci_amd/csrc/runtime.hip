@@ -1,0 +1,502 @@
+// The C ABI (include/scihip.h): device, allocator, copies, chunk execute / reduce.
+#include "runtime.hpp"
+#include "compiler.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+
+namespace scihip {
+
+    thread_local std::string last_error;
+
+    void fail(const std::string &message) { last_error = message; }
+
+    int status(hipError_t error, const char *what) {
+        if (error == hipSuccess) return SCIHIP_OK;
+        last_error = std::string(what) + ": " + hipGetErrorString(error);
+        return error == hipErrorOutOfMemory ? SCIHIP_ENOMEM : SCIHIP_EDEVICE;
+    }
+
+    // ------------------------------------------------------------------------------------ Allocator
+
+    Allocator::~Allocator() {
+        (void)hipStreamSynchronize(stream);
+        reclaim(true);
+        trim(0);
+        for (auto &block : live) (void)hipFree(reinterpret_cast<void *>(block.first));
+    }
+
+    void Allocator::reclaim(bool all) {
+        size_t kept = 0;
+        for (Retired &r : retired) {
+            if (all || hipEventQuery(r.event) == hipSuccess) {
+                (void)hipEventDestroy(r.event);
+                cache.emplace(r.bytes, r.pointer);
+                cached += r.bytes;
+            } else {
+                retired[kept++] = r;
+            }
+        }
+        retired.resize(kept);
+        if (cached > limit) trim(limit / 2);
+    }
+
+    void Allocator::trim(size_t keep) {
+        while (cached > keep && !cache.empty()) {
+            auto largest = std::prev(cache.end());
+            (void)hipFree(largest->second);
+            cached -= largest->first;
+            cache.erase(largest);
+        }
+    }
+
+    int Allocator::allocate(size_t bytes, void **pointer) {
+        reclaim(false);
+        const size_t rounded = std::max<size_t>((bytes + 4095) / 4096 * 4096, 4096);
+        // A cached block at most 25% larger than asked for (allocator.cpp:44-54).
+        auto found = cache.lower_bound(rounded);
+        if (found != cache.end() && found->first <= rounded + rounded / 4) {
+            *pointer = found->second;
+            live[reinterpret_cast<uintptr_t>(*pointer)] = found->first;
+            cached -= found->first;
+            cache.erase(found);
+            return SCIHIP_OK;
+        }
+        hipError_t error = hipMalloc(pointer, rounded);
+        if (error == hipErrorOutOfMemory) {
+            // Out of memory: drop the cache and retry once (allocator.cpp:57-65).
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(stream);
+            reclaim(true);
+            trim(0);
+            error = hipMalloc(pointer, rounded);
+        }
+        if (error != hipSuccess) {
+            *pointer = nullptr;
+            return status(error, "hipMalloc");
+        }
+        live[reinterpret_cast<uintptr_t>(*pointer)] = rounded;
+        return SCIHIP_OK;
+    }
+
+    int Allocator::dispose(void *pointer) {
+        auto found = live.find(reinterpret_cast<uintptr_t>(pointer));
+        if (found == live.end()) {
+            fail("scihip_dispose: not an allocation of this device");
+            return SCIHIP_EINVAL;
+        }
+        Retired r{pointer, found->second, nullptr};
+        live.erase(found);
+        hipError_t error = hipEventCreateWithFlags(&r.event, hipEventDisableTiming);
+        if (error == hipSuccess) error = hipEventRecord(r.event, stream);
+        if (error != hipSuccess) {
+            (void)hipStreamSynchronize(stream);
+            if (r.event) (void)hipEventDestroy(r.event);
+            cache.emplace(r.bytes, r.pointer);
+            cached += r.bytes;
+            return SCIHIP_OK;
+        }
+        retired.push_back(r);
+        return SCIHIP_OK;
+    }
+
+    bool Allocator::locate(const void *pointer, size_t bytes) const {
+        const uintptr_t at = reinterpret_cast<uintptr_t>(pointer);
+        auto found = live.upper_bound(at);
+        if (found == live.begin()) return false;
+        --found;
+        return at >= found->first && at + bytes <= found->first + found->second;
+    }
+
+    // ------------------------------------------------------------------------------------ encoding
+
+    // The register number of a value id: inputs, then constants, then temporaries, then the output.
+    struct Numbering {
+        uint32_t arguments, literals, temporaries;
+        uint32_t operator()(uint8_t id) const {
+            const uint32_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3Fu;
+            if (id == SCIHIP_SLOT_NONE) return 0;
+            if (tag == SCIHIP_SLOT_INPUT) return index;
+            if (tag == SCIHIP_SLOT_LITERAL) return arguments + index;
+            if (tag == SCIHIP_SLOT_TEMPORARY) return arguments + literals + index;
+            return arguments + literals + temporaries;
+        }
+        uint32_t registers() const { return arguments + literals + temporaries + 1; }
+    };
+
+    bool contracted(const scihip_chunk &chunk, uint32_t fold) { // chunk.hpp:151-154
+        return fold == SCIHIP_FOLD_SUM && chunk.length > 0 &&
+               chunk.instructions[chunk.length - 1].code == SCIHIP_MULTIPLY &&
+               chunk.instructions[chunk.length - 1].target == chunk.result;
+    }
+
+    // Checks a chunk's structure: every id it uses names a slot it has, codes are known.
+    bool wellformed(const scihip_chunk &chunk) {
+        if (chunk.length > SCIHIP_CAPACITY || chunk.arguments > SCIHIP_CAPACITY || chunk.literals > SCIHIP_CAPACITY ||
+            chunk.temporaries > SCIHIP_CAPACITY)
+            return false;
+        auto valid = [&](uint8_t id, bool output_ok) {
+            if (id == SCIHIP_SLOT_NONE) return true;
+            const uint8_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3F;
+            if (tag == SCIHIP_SLOT_INPUT) return index < chunk.arguments;
+            if (tag == SCIHIP_SLOT_LITERAL) return index < chunk.literals;
+            if (tag == SCIHIP_SLOT_TEMPORARY) return index < chunk.temporaries;
+            return output_ok && id == SCIHIP_SLOT_OUTPUT;
+        };
+        for (int s = 0; s < chunk.length; ++s) {
+            const scihip_instruction &in = chunk.instructions[s];
+            if (in.code >= SCIHIP_CODES) return false;
+            if (!valid(in.target, true) || in.target == SCIHIP_SLOT_NONE) return false;
+            if ((in.target & SCIHIP_SLOT_TYPES) == SCIHIP_SLOT_INPUT || (in.target & SCIHIP_SLOT_TYPES) == SCIHIP_SLOT_LITERAL)
+                return false;
+            const int arity = in.code == SCIHIP_FUSE ? 3 : in.code >= SCIHIP_ADD ? 2 : 1;
+            for (int k = 0; k < 3; ++k) {
+                if (!valid(in.sources[k], false)) return false;
+                if (k < arity && in.sources[k] == SCIHIP_SLOT_NONE) return false;
+            }
+        }
+        return valid(chunk.result, true) && chunk.result != SCIHIP_SLOT_NONE;
+    }
+
+    template <typename T>
+    int encode(scihip_device *device, const scihip_chunk &chunk, T *output, size_t count, uint32_t fold, bool reduce,
+               Launch<T> &launch, int &registers) {
+        if (!wellformed(chunk)) {
+            fail("malformed chunk");
+            return SCIHIP_EINVAL;
+        }
+        Program<T> &p = launch.p;
+        std::memset(&p, 0, sizeof(p));
+        const Numbering number{chunk.arguments, chunk.literals, chunk.temporaries};
+        const bool contract = reduce && contracted(chunk, fold);
+        p.length = chunk.length;
+        p.arguments = chunk.arguments;
+        p.literals = chunk.literals;
+        p.result = number(chunk.result);
+        p.fold = fold;
+        p.contract = contract ? 1u : 0u;
+        if (contract) {
+            p.left = number(chunk.instructions[chunk.length - 1].sources[0]);
+            p.right = number(chunk.instructions[chunk.length - 1].sources[1]);
+        }
+        p.count = count;
+        bool aligned = (reinterpret_cast<uintptr_t>(output) & 15u) == 0;
+        for (int s = 0; s < chunk.length; ++s) {
+            const scihip_instruction &in = chunk.instructions[s];
+            p.word[s] = uint32_t(in.code) | number(in.target) << 8 | number(in.sources[0]) << 16 |
+                        number(in.sources[1]) << 24;
+            p.third[s] = static_cast<uint8_t>(number(in.sources[2]));
+        }
+        // Constants are narrowed to the element type, as the CPU does (dispatch.cpp:52-56).
+        for (int i = 0; i < chunk.literals; ++i) p.constants[i] = static_cast<T>(chunk.constants[i]);
+        for (int i = 0; i < chunk.arguments; ++i) {
+            if (!device->allocator->locate(chunk.inputs[i], count * sizeof(T))) {
+                fail("an input of the chunk is not device memory of this device");
+                return SCIHIP_DECLINED;
+            }
+            p.inputs[i] = static_cast<const T *>(chunk.inputs[i]);
+            aligned = aligned && (reinterpret_cast<uintptr_t>(chunk.inputs[i]) & 15u) == 0;
+        }
+        p.aligned = aligned ? 1u : 0u;
+        p.output = output;
+        launch.partials = static_cast<Partial *>(device->partials);
+        launch.done = device->done;
+        launch.result = device->result;
+        const uint32_t needed = number.registers();
+        if (needed > uint32_t(kLarge)) {
+            fail("chunk needs more registers than the interpreter has");
+            return SCIHIP_EINVAL;
+        }
+        registers = needed <= kSmall ? kSmall : needed <= kMedium ? kMedium : kLarge;
+        return SCIHIP_OK;
+    }
+
+    constexpr unsigned kMaxBlocks = 4096; // partials buffer capacity
+
+    unsigned blocks_for(scihip_device *device, size_t count, int width) {
+        const size_t steps = (count + width - 1) / width;
+        const size_t wanted = (steps + 255) / 256;
+        // Enough workgroups to fill every CU several times over; each sweeps the array grid-stride.
+        const size_t fill = size_t(device->units) * 8;
+        return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>({wanted, fill, kMaxBlocks})));
+    }
+
+    std::string key_of(const scihip_chunk &chunk, int mode, bool contract) {
+        std::string key;
+        key.reserve(8 + 5 * chunk.length);
+        key += char(mode);
+        key += char(contract);
+        key += char(chunk.precision);
+        key += char(chunk.arguments);
+        key += char(chunk.literals);
+        key += char(chunk.temporaries);
+        key += char(chunk.result);
+        for (int s = 0; s < chunk.length; ++s) {
+            const scihip_instruction &in = chunk.instructions[s];
+            key += char(in.code);
+            key += char(in.target);
+            key += char(in.sources[0]);
+            key += char(in.sources[1]);
+            key += char(in.sources[2]);
+        }
+        return key;
+    }
+
+    // Runs a chunk: its specialised kernel when ready (or forced), else the interpreter.
+    template <typename T>
+    int launch(scihip_device *device, const scihip_chunk &chunk, Launch<T> &arguments, int registers, int mode,
+               bool contract, unsigned blocks) {
+        const Kernel *kernel = nullptr;
+        if (device->policy != 1) {
+            const std::string key = key_of(chunk, mode, contract);
+            kernel = device->compiler->find(key, Compiler::generate(chunk, mode, contract), device->policy == 2);
+            if (!kernel && device->policy == 2) {
+                fail("specialised kernel failed to compile");
+                return SCIHIP_ECOMPILE;
+            }
+        }
+        if (kernel) {
+            size_t size = sizeof(arguments);
+            void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &arguments, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
+                              HIP_LAUNCH_PARAM_END};
+            ++device->specialised;
+            return status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream, nullptr,
+                                                config),
+                          "specialised launch");
+        }
+        ++device->interpreted;
+        return status(interpret(arguments, registers, mode, blocks, device->stream), "interpreter launch");
+    }
+
+    template <typename T>
+    int execute(scihip_device *device, const scihip_chunk &chunk, void *output, size_t count) {
+        if (!device->allocator->locate(output, count * sizeof(T))) {
+            fail("the output is not device memory of this device");
+            return SCIHIP_DECLINED;
+        }
+        Launch<T> arguments;
+        int registers = 0;
+        if (const int rc = encode<T>(device, chunk, static_cast<T *>(output), count, 0, false, arguments, registers))
+            return rc;
+        if (chunk.result != SCIHIP_SLOT_OUTPUT) {
+            fail("a stored chunk ends in its output");
+            return SCIHIP_EINVAL;
+        }
+        const int width = registers == kLarge ? 1 : Traits<T>::width;
+        return launch<T>(device, chunk, arguments, registers, 0, false, blocks_for(device, count, width));
+    }
+
+    template <typename T>
+    int reduce(scihip_device *device, const scihip_chunk &chunk, uint32_t fold, size_t count, double *result) {
+        Launch<T> arguments;
+        int registers = 0;
+        if (const int rc = encode<T>(device, chunk, nullptr, count, fold, true, arguments, registers)) return rc;
+        const int width = registers == kLarge ? 1 : Traits<T>::width;
+        const bool contract = contracted(chunk, fold);
+        if (const int rc = launch<T>(device, chunk, arguments, registers, 1, contract, blocks_for(device, count, width)))
+            return rc;
+        if (const int rc = status(hipMemcpyAsync(device->staging, device->result, sizeof(double),
+                                                 hipMemcpyDeviceToHost, device->stream),
+                                  "result copy"))
+            return rc;
+        if (const int rc = status(hipStreamSynchronize(device->stream), "reduce")) return rc;
+        *result = *device->staging;
+        return SCIHIP_OK;
+    }
+
+} // namespace scihip
+
+using namespace scihip;
+
+extern "C" {
+
+const char *scihip_error(void) { return last_error.c_str(); }
+
+int scihip_open(uint32_t ordinal, scihip_device **out) {
+    if (!out) return SCIHIP_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || int(ordinal) >= count) {
+        (void)hipGetLastError();
+        fail("no usable HIP device at this index");
+        return SCIHIP_DECLINED;
+    }
+    auto device = std::make_unique<scihip_device>();
+    device->ordinal = int(ordinal);
+    if (int rc = status(hipSetDevice(device->ordinal), "hipSetDevice")) return rc;
+    hipDeviceProp_t properties;
+    if (int rc = status(hipGetDeviceProperties(&properties, device->ordinal), "hipGetDeviceProperties")) return rc;
+    device->name = properties.name;
+    device->units = properties.multiProcessorCount;
+    if (int rc = status(hipStreamCreateWithFlags(&device->stream, hipStreamNonBlocking), "hipStreamCreate")) return rc;
+    device->allocator = std::make_unique<Allocator>(device->stream);
+    if (int rc = status(hipMalloc(&device->partials, kMaxBlocks * sizeof(Partial)), "hipMalloc")) return rc;
+    if (int rc = status(hipMalloc(reinterpret_cast<void **>(&device->done), 256), "hipMalloc")) return rc;
+    if (int rc = status(hipMemset(device->done, 0, 256), "hipMemset")) return rc;
+    device->result = reinterpret_cast<double *>(reinterpret_cast<char *>(device->done) + 128);
+    if (int rc = status(hipHostMalloc(reinterpret_cast<void **>(&device->staging), sizeof(double)), "hipHostMalloc"))
+        return rc;
+    if (const char *text = std::getenv("SCIHIP_JIT")) {
+        if (std::strcmp(text, "0") == 0) device->policy = 1;
+        if (std::strcmp(text, "force") == 0) device->policy = 2;
+    }
+    device->compiler = std::make_unique<Compiler>(device->ordinal);
+    *out = device.release();
+    return SCIHIP_OK;
+}
+
+int scihip_close(scihip_device *device) {
+    if (!device) return SCIHIP_OK;
+    {
+        std::lock_guard<std::mutex> lock(device->guard);
+        (void)hipStreamSynchronize(device->stream);
+        device->compiler.reset();
+        device->allocator.reset();
+        (void)hipFree(device->partials);
+        (void)hipFree(device->done);
+        (void)hipHostFree(device->staging);
+        (void)hipStreamDestroy(device->stream);
+    }
+    delete device;
+    return SCIHIP_OK;
+}
+
+const char *scihip_name(const scihip_device *device) { return device ? device->name.c_str() : "cpu"; }
+
+int scihip_synchronize(scihip_device *device) {
+    if (!device) return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    const int rc = status(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
+    device->allocator->reclaim(true);
+    return rc;
+}
+
+int scihip_allocate(scihip_device *device, size_t bytes, void **pointer) {
+    if (!device || !pointer) return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    return device->allocator->allocate(bytes, pointer);
+}
+
+int scihip_dispose(scihip_device *device, void *pointer) {
+    if (!device) return SCIHIP_EINVAL;
+    if (!pointer) return SCIHIP_OK;
+    std::lock_guard<std::mutex> lock(device->guard);
+    return device->allocator->dispose(pointer);
+}
+
+int scihip_locate(scihip_device *device, const void *pointer) {
+    if (!device) return 0;
+    std::lock_guard<std::mutex> lock(device->guard);
+    return device->allocator->locate(pointer) ? 1 : 0;
+}
+
+static int copy(scihip_device *device, void *destination, const void *source, size_t bytes, hipMemcpyKind kind,
+                bool wait) {
+    if (!device) return SCIHIP_EINVAL;
+    if (bytes == 0) return SCIHIP_OK;
+    std::lock_guard<std::mutex> lock(device->guard);
+    const bool in = kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToDevice;
+    const bool out = kind == hipMemcpyDeviceToHost || kind == hipMemcpyDeviceToDevice;
+    if ((in && !device->allocator->locate(destination, bytes)) || (out && !device->allocator->locate(source, bytes))) {
+        fail("a copy names memory that is not a device allocation of this size");
+        return SCIHIP_EINVAL;
+    }
+    if (int rc = status(hipMemcpyAsync(destination, source, bytes, kind, device->stream), "hipMemcpyAsync")) return rc;
+    return wait ? status(hipStreamSynchronize(device->stream), "copy") : SCIHIP_OK;
+}
+
+int scihip_upload(scihip_device *device, void *destination, const void *source, size_t bytes) {
+    return copy(device, destination, source, bytes, hipMemcpyHostToDevice, true);
+}
+
+int scihip_download(scihip_device *device, void *destination, const void *source, size_t bytes) {
+    return copy(device, destination, source, bytes, hipMemcpyDeviceToHost, true);
+}
+
+int scihip_copy(scihip_device *device, void *destination, const void *source, size_t bytes) {
+    return copy(device, destination, source, bytes, hipMemcpyDeviceToDevice, false);
+}
+
+int scihip_execute(scihip_device *device, const scihip_chunk *chunk, void *output, size_t count) {
+    if (!device || !chunk) return SCIHIP_EINVAL;
+    if (count == 0) return SCIHIP_OK;
+    std::lock_guard<std::mutex> lock(device->guard);
+    return chunk->precision == SCIHIP_SINGLE ? execute<float>(device, *chunk, output, count)
+                                             : execute<double>(device, *chunk, output, count);
+}
+
+int scihip_reduce(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t count, double *result) {
+    if (!device || !chunk || !result || fold > SCIHIP_FOLD_MAXIMUM) return SCIHIP_EINVAL;
+    if (count == 0) { // device.cpp:266-268
+        *result = fold == SCIHIP_FOLD_SUM ? 0.0 : std::numeric_limits<double>::quiet_NaN();
+        return SCIHIP_OK;
+    }
+    std::lock_guard<std::mutex> lock(device->guard);
+    return chunk->precision == SCIHIP_SINGLE ? reduce<float>(device, *chunk, fold, count, result)
+                                             : reduce<double>(device, *chunk, fold, count, result);
+}
+
+int scihip_set_policy(scihip_device *device, int policy) {
+    if (!device || policy < 0 || policy > 2) return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    device->policy = policy;
+    return SCIHIP_OK;
+}
+
+int scihip_prepare(scihip_device *device, const scihip_chunk *chunk) {
+    if (!device || !chunk) return SCIHIP_EINVAL;
+    if (!wellformed(*chunk)) {
+        fail("malformed chunk");
+        return SCIHIP_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(device->guard);
+    const bool stored = chunk->result == SCIHIP_SLOT_OUTPUT;
+    const int mode = stored ? 0 : 1;
+    const bool contract = !stored && contracted(*chunk, SCIHIP_FOLD_SUM);
+    if (!device->compiler->find(key_of(*chunk, mode, false), Compiler::generate(*chunk, mode, false), true)) {
+        fail("specialised kernel failed to compile");
+        return SCIHIP_ECOMPILE;
+    }
+    if (contract &&
+        !device->compiler->find(key_of(*chunk, mode, true), Compiler::generate(*chunk, mode, true), true)) {
+        fail("specialised kernel failed to compile");
+        return SCIHIP_ECOMPILE;
+    }
+    return SCIHIP_OK;
+}
+
+int scihip_statistics(scihip_device *device, uint64_t *interpreted, uint64_t *specialised, uint64_t *compiled) {
+    if (!device) return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    if (interpreted) *interpreted = device->interpreted;
+    if (specialised) *specialised = device->specialised;
+    if (compiled) *compiled = device->compiler->compiled();
+    return SCIHIP_OK;
+}
+
+int scihip_stream(scihip_device *device, void **stream) {
+    if (!device || !stream) return SCIHIP_EINVAL;
+    *stream = device->stream;
+    return SCIHIP_OK;
+}
+
+int scihip_fill_uniform(scihip_device *device, void *output, uint32_t precision, size_t count, uint64_t seed,
+                        double lower, double upper, uint64_t nan_seed, double nan_fraction) {
+    if (!device || precision > SCIHIP_WIDE) return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    const size_t bytes = count * (precision == SCIHIP_SINGLE ? 4 : 8);
+    if (count && !device->allocator->locate(output, bytes)) {
+        fail("the output is not device memory of this device");
+        return SCIHIP_DECLINED;
+    }
+    return status(fill_uniform(output, int(precision), count, seed, lower, upper, nan_seed, nan_fraction,
+                               device->stream),
+                  "fill_uniform");
+}
+
+} // extern "C"

@@ -1,0 +1,72 @@
+// Host side of the HIP backend: the device object behind the C ABI (include/scihip.h).
+#pragma once
+
+#include "scihip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace scihip {
+
+    void fail(const std::string &message);
+    int status(hipError_t error, const char *what);
+
+    // Device memory with the reference allocator's contract (src/core/drivers/vulkan/allocator.cpp:16-136):
+    // blocks are cached and reused when one at most 25% larger fits; `locate` finds the block holding
+    // any interior pointer; a disposed block returns to the cache only once the work queued before
+    // its disposal has finished (src/core/drivers/vulkan/stream.cpp:117-130).
+    class Allocator {
+    public:
+        explicit Allocator(hipStream_t stream) : stream(stream) {}
+        ~Allocator();
+        int allocate(size_t bytes, void **pointer);
+        int dispose(void *pointer);
+        bool locate(const void *pointer, size_t bytes = 1) const;
+        void reclaim(bool all);
+
+    private:
+        struct Retired {
+            void *pointer;
+            size_t bytes;
+            hipEvent_t event;
+        };
+        void trim(size_t keep);
+        hipStream_t stream;
+        std::map<uintptr_t, size_t> live;        // start -> bytes
+        std::multimap<size_t, void *> cache;     // bytes -> block
+        std::vector<Retired> retired;
+        size_t cached = 0;
+        static constexpr size_t limit = size_t{16} << 30; // cache at most 16 GiB of 288
+    };
+
+    class Compiler;
+
+    struct Kernel; // a specialised, loaded kernel
+
+} // namespace scihip
+
+struct scihip_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    std::mutex guard;
+    std::unique_ptr<scihip::Allocator> allocator;
+    std::unique_ptr<scihip::Compiler> compiler;
+    int units = 0;              // compute units
+    void *partials = nullptr;   // reduction partials, one per workgroup
+    uint32_t *done = nullptr;   // reduction completion counter (zero between launches)
+    double *result = nullptr;   // reduction result on the device
+    double *staging = nullptr;  // pinned host copy of it
+    int policy = 0;             // 0 automatic, 1 interpreter only, 2 specialised only
+    uint64_t interpreted = 0, specialised = 0;
+};

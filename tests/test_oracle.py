@@ -1,0 +1,306 @@
+"""Pins the CPU oracle (oracle/) and the chunk recorder before either is trusted as a checker.
+
+Each check is independent of the code it checks: the reference's own compiled code through the
+golden fixtures (tests/golden/, written by make_golden.py from /root/reference), closed forms
+(IEEE-754 correct rounding, exact rationals), float64 libm bounds, LAPACK's pivots, and the
+reference tests' own known answers.
+"""
+
+from __future__ import annotations
+
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle
+from libsci_amd.chunk import ARITY, OUTPUT, Chunk, Code, Fold, Precision, record_postfix
+from tests.helpers import fma32, golden_json, golden_kernels, program_tuples, same_bits, ulp_distance
+
+KERNEL_SEEDS = (11, 12, 13)
+SPECIALS32 = [0.0, -0.0, np.inf, -np.inf, np.nan, 1e-40, -1e-40, 1e-45, 1e30, -1e30, 3e6, -3e6, 1048576.0,
+              1048577.0, 88.7, 89.1, -103.9, -104.5, 0.5, -0.5, 1.0, -1.0, 2.0, 1e-7, 0.70710677, 1.4142135,
+              3.1415927, 1.5707964, 12582912.0, 1e38]
+SPECIALS64 = [0.0, -0.0, np.inf, -np.inf, np.nan, 1e-310, -1e-310, 5e-324, 1e300, -1e300, 3e6, 1048577.0, 709.0,
+              710.5, -745.0, -746.5, 0.5, -1.0, 2.0, 1e-17, 0.7071067811865476, 1.4142135623730951,
+              3.141592653589793]
+
+
+def kernel_inputs(dtype, count=1024):
+    specials = SPECIALS32 if dtype == np.float32 else SPECIALS64
+    inputs = []
+    for seed in KERNEL_SEEDS:
+        x = oracle.uniform(count, seed, -8.0, 8.0, dtype=dtype)
+        x[: len(specials)] = np.array(specials, dtype=dtype)[np.random.default_rng(seed).permutation(len(specials))]
+        inputs.append(x)
+    return inputs
+
+
+def single_instruction_chunk(code: Code, arrays, precision: Precision) -> Chunk:
+    program = [("in", i) for i in range(ARITY[code])] + [("op", code)]
+    return record_postfix(program, precision, "output", [a.ctypes.data for a in arrays])
+
+
+# ------------------------------------------------------------------------------------------ recorder
+
+HAND_DERIVED = {
+    # Derived by hand from chunk.hpp:69-118, 203-211 and node.hpp:75-88.
+    "c1_store": [(Code.fuse, OUTPUT, (0x00, 0x01, 0x40))],
+    "c1_reduce": [(Code.sin, 0x80, (0x00, 0xFF, 0xFF))],
+    "c2_store": [(Code.exponential, 0x80, (0x00, 0xFF, 0xFF)), (Code.multiply, 0x80, (0x80, 0x40, 0xFF)),
+                 (Code.absolute, 0x81, (0x01, 0xFF, 0xFF)), (Code.add, 0x81, (0x81, 0x41, 0xFF)),
+                 (Code.logarithm, 0x81, (0x81, 0xFF, 0xFF)), (Code.fuse, OUTPUT, (0x81, 0x02, 0x80))],
+    "c2_reduce": [(Code.sin, 0x80, (0x00, 0xFF, 0xFF)), (Code.cos, 0x81, (0x01, 0xFF, 0xFF)),
+                  (Code.multiply, 0x80, (0x80, 0x81, 0xFF)), (Code.tanh, 0x81, (0x02, 0xFF, 0xFF)),
+                  (Code.add, 0x80, (0x80, 0x81, 0xFF))],
+    "c2_dot": [(Code.multiply, 0x80, (0x00, 0x01, 0xFF))],
+    "c5_loss": [(Code.fuse, 0x80, (0x00, 0x01, 0x02)), (Code.sinh, 0x80, (0x80, 0xFF, 0xFF)),
+                (Code.multiply, 0x81, (0x03, 0x03, 0xFF)), (Code.negate, 0x81, (0x81, 0xFF, 0xFF)),
+                (Code.exponential, 0x81, (0x81, 0xFF, 0xFF)), (Code.multiply, 0x80, (0x80, 0x81, 0xFF))],
+}
+
+
+@pytest.mark.parametrize("name", sorted(HAND_DERIVED))
+def test_hand_derived_bytecode(name):
+    fixtures = golden_json("chunks.json")
+    finish = "output" if name.endswith("store") else "reduce"
+    entry = fixtures[f"{name}/{finish}"]
+    chunk = record_postfix(program_tuples(entry["program"]), Precision.single, finish)
+    got = chunk.instructions()
+    assert [(Code(c), t, s) for c, t, s in got] == HAND_DERIVED[name]
+    # and the reference's own Chunk recorded the same
+    assert [(Code(c), t, tuple(s)) for c, t, s in entry["instructions"]] == HAND_DERIVED[name]
+
+
+def test_contraction_rules():
+    # c2 dot and c5 loss are contracted sums; c2 reduce is not (tanh sits between multiply and add).
+    for name, contracted in (("c2_dot", True), ("c5_loss", True), ("c2_reduce", False), ("c1_reduce", False)):
+        entry = golden_json("chunks.json")[f"{name}/reduce"]
+        chunk = record_postfix(program_tuples(entry["program"]), Precision.single, "reduce")
+        assert chunk.contracted(Fold.sum) is contracted
+        assert not chunk.contracted(Fold.minimum)
+
+
+def test_recorder_matches_reference_chunk():
+    fixtures = golden_json("chunks.json")
+    assert len(fixtures) > 200
+    for key, entry in fixtures.items():
+        addresses = [0x1000 + 64 * n for n in range(64)]
+        chunk = record_postfix(program_tuples(entry["program"]), Precision.single, entry["finish"], addresses)
+        assert chunk.valid() == entry["valid"], key
+        if not entry["valid"]:
+            continue
+        assert chunk.length == entry["length"], key
+        assert chunk.arguments == entry["arguments"], key
+        assert chunk.literals == entry["literals"], key
+        assert chunk.temporaries == entry["temporaries"], key
+        assert chunk.result == entry["result"], key
+        assert chunk.cost() == entry["cost"], key
+        assert [(c, t, tuple(s)) for c, t, s in chunk.instructions()] == \
+               [(c, t, tuple(s)) for c, t, s in entry["instructions"]], key
+        assert [(a - 0x1000) // 64 for a in chunk.inputs()] == entry["inputs"], key
+        assert chunk.constants() == entry["constants"], key
+
+
+def test_signature_ignores_addresses_and_constants():
+    program = [("in", 0), ("const", 2.0), ("op", Code.multiply)]
+    a = record_postfix(program, addresses=[0x100])
+    b = record_postfix([("in", 0), ("const", 3.0), ("op", Code.multiply)], addresses=[0x900])
+    assert a.signature() == b.signature()
+    c = record_postfix([("in", 0), ("const", 3.0), ("op", Code.add)], addresses=[0x900])
+    assert a.signature() != c.signature()
+
+
+# ------------------------------------------------------------------------------------------ kernels
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_oracle_kernels_match_reference(dtype):
+    """Every code, bit for bit with the reference's AVX-512 table (NaN payloads aside)."""
+    golden = golden_kernels()
+    tag = "f32" if dtype == np.float32 else "f64"
+    inputs = kernel_inputs(dtype)
+    precision = Precision.single if dtype == np.float32 else Precision.wide
+    for code in Code:
+        args = inputs[: ARITY[code]]
+        chunk = single_instruction_chunk(code, args, precision)
+        got = oracle.execute(chunk, args[0].size, threads=2, dtype=dtype)
+        want = golden[f"{tag}/{code.name}"]
+        same = same_bits(got, want)
+        if code in (Code.minimum, Code.maximum):
+            same |= got == want  # the sign of a zero tie is not part of the contract
+        assert same.all(), (code.name, np.flatnonzero(~same)[:5])
+
+
+def test_oracle_closed_forms():
+    """+, -, *, /, sqrt and fma are IEEE-754 correctly rounded float32 operations."""
+    a, b, c = kernel_inputs(np.float32, 4096)
+    finite = np.isfinite(a) & np.isfinite(b) & np.isfinite(c)
+    with np.errstate(all="ignore"):
+        for code, want in ((Code.add, a + b), (Code.subtract, a - b), (Code.multiply, a * b),
+                           (Code.divide, a / b), (Code.root, np.sqrt(a))):
+            got = oracle.map_f32(code, a, b)
+            assert same_bits(got, want).all(), code.name
+    got = oracle.map_f32(Code.fuse, a, b, c)
+    for i in np.flatnonzero(finite)[:1500]:
+        assert got[i] == fma32(a[i], b[i], c[i]) or (np.isnan(got[i])), i
+    # a = b = 1 + 2^-12, c = 2^-60: one rounding gives 0x1.002002p+0, two give 0x1.002p+0.
+    x = np.float32(1 + 2 ** -12)
+    assert oracle.map_f32(Code.fuse, np.array([x]), np.array([x]), np.array([np.float32(2 ** -60)]))[0] == \
+           np.float32(float.fromhex("0x1.002002p+0"))
+
+
+@pytest.mark.parametrize("code,bound,lo,hi", [
+    (Code.exponential, 1, -87.0, 88.0), (Code.logarithm, 1, 1e-30, 1e30),
+    (Code.sin, 2, -1000.0, 1000.0), (Code.cos, 2, -1000.0, 1000.0),
+])
+def test_oracle_transcendental_ulp(code, bound, lo, hi):
+    """exp / log within 1 ulp and sin / cos within 2 ulp of float64 libm (README.md:107)."""
+    if code == Code.logarithm:
+        x = np.exp(np.random.default_rng(5).uniform(np.log(lo), np.log(hi), 200000)).astype(np.float32)
+    else:
+        x = np.random.default_rng(5).uniform(lo, hi, 200000).astype(np.float32)
+    got = oracle.map_f32(code, x)
+    ref = {Code.exponential: np.exp, Code.logarithm: np.log, Code.sin: np.sin, Code.cos: np.cos}[code](
+        x.astype(np.float64))
+    # ulps measured against the float64 value, at the float32 spacing of that value
+    spacing = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    err = np.abs(got.astype(np.float64) - ref) / spacing
+    assert err.max() <= bound, err.max()
+
+
+# ------------------------------------------------------------------------------------------ reductions
+
+def test_oracle_reductions_match_reference():
+    """Sum / minimum / maximum, bit for bit with the reference's Plan, folds and Accumulator."""
+    for key, entry in golden_json("reductions.json").items():
+        arrays = [oracle.uniform(entry["count"], s, -2.0, 2.0) for s in entry["seeds"]]
+        chunk = record_postfix(program_tuples(entry["program"]), Precision.single, "reduce",
+                               [a.ctypes.data for a in arrays])
+        got = oracle.reduce(chunk, entry["fold"], entry["count"], threads=entry["threads"], isa=entry["isa"])
+        want = entry["value"]
+        assert got == want or (math.isnan(got) and math.isnan(want)), (key, got, want)
+
+
+def test_oracle_sum_within_reordering_bound():
+    x = oracle.uniform(1 << 20, 7, -4.0, 4.0)
+    chunk = record_postfix([("in", 0)], Precision.single, "reduce", [x.ctypes.data])
+    exact = math.fsum(x.astype(np.float64))
+    for threads in (1, 3, 8):
+        got = oracle.reduce(chunk, Fold.sum, x.size, threads=threads)
+        # float accumulation inside 1,024-element blocks: |error| <= 1024 * eps32 * sum |x|
+        assert abs(got - exact) <= 1024 * 2 ** -24 * float(np.abs(x).sum())
+
+
+def test_oracle_nan_wins():
+    x = oracle.uniform(5000, 9, -1.0, 1.0)
+    x[1234] = np.nan
+    chunk = record_postfix([("in", 0)], Precision.single, "reduce", [x.ctypes.data])
+    for fold in Fold:
+        assert math.isnan(oracle.reduce(chunk, fold, x.size))
+    empty = oracle.reduce(chunk, Fold.sum, 0)
+    assert empty == 0.0 and math.isnan(oracle.reduce(chunk, Fold.minimum, 0))
+
+
+# ------------------------------------------------------------------------------------------ linear algebra
+
+def permutation_from_lapack(ipiv):
+    rows = np.arange(len(ipiv))
+    for i, p in enumerate(ipiv):
+        rows[[i, p]] = rows[[p, i]]
+    return rows
+
+
+def test_lu_reference_kat():
+    """tests/linear/test_decompose.cpp:21-36: a zero first pivot; rows by hand = [1, 2, 0]."""
+    a = np.array([[0, 2, 1], [4, 1, 3], [2, 5, 7]], dtype=np.float64)
+    lu, rows = oracle.factor(a)
+    assert list(rows) == [1, 2, 0]
+    lower = np.tril(lu, -1) + np.eye(3)
+    upper = np.triu(lu)
+    assert np.abs(a[rows] - lower @ upper).max() < 1e-12
+
+
+def test_solve_reference_kat():
+    """tests/linear/test_solve.cpp:8-25: x = [1, -2, 3] exactly, and a 2-column right side."""
+    a = np.array([[4, -2, 1], [-2, 4, -2], [1, -2, 4]], dtype=np.float64)
+    lu, rows = oracle.factor(a)
+    x = oracle.substitute(lu, rows, np.array([11.0, -16.0, 17.0]))
+    assert np.abs(x - [1, -2, 3]).max() < 1e-12
+    b = np.array([[11, 1], [-16, 0], [17, 2]], dtype=np.float64)
+    xs = oracle.substitute(lu, rows, b)
+    assert np.abs(a @ xs - b).max() < 1e-9
+
+
+@pytest.mark.parametrize("n", [7, 64, 200])
+def test_lu_pivots_match_lapack(n):
+    """The first-maximum pivot rule is LAPACK getrf's: the same row order on random matrices."""
+    a = np.random.default_rng(n).uniform(-1, 1, (n, n))
+    lu, rows = oracle.factor(a, threads=4)
+    ref_lu, piv = scipy.linalg.lu_factor(a)
+    assert (rows == permutation_from_lapack(piv)).all()
+    assert np.abs(lu - ref_lu).max() < 1e-10
+    lower, upper = np.tril(lu, -1) + np.eye(n), np.triu(lu)
+    assert np.abs(a[rows] - lower @ upper).max() < 1e-12 * n
+
+
+def test_diagonally_shifted_matrix_keeps_identity_pivots():
+    n = 300
+    a = np.random.default_rng(3).uniform(-1, 1, (n, n)) + (n / 4) * np.eye(n)
+    _, rows = oracle.factor(a)
+    assert (rows == np.arange(n)).all()
+
+
+def test_product_reference_kat():
+    """tests/linear/test_product.cpp:8-33: 37 x 53 x 29, integer-valued, against a float64 loop."""
+    rows, inner, columns = 37, 53, 29
+    left = np.array([[((r * 7 + c * 3) % 11) - 5.0 for c in range(inner)] for r in range(rows)], np.float32)
+    right = np.array([[((r * 5 + c) % 13) * 0.25 for c in range(columns)] for r in range(inner)], np.float32)
+    got = oracle.product(left, right)
+    assert np.abs(got - left.astype(np.float64) @ right.astype(np.float64)).max() < 1e-3
+
+
+def test_product_against_float64():
+    a = oracle.uniform(96 * 130, 1, -1, 1).reshape(96, 130)
+    b = oracle.uniform(130 * 70, 2, -1, 1).reshape(130, 70)
+    got = oracle.product(a, b, threads=3)
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    assert np.abs(got - ref).max() < 130 * 2 ** -23
+
+
+# ------------------------------------------------------------------------------------------ synthetic data
+
+def splitmix(seed, index):
+    z = (seed * 0x9E3779B97F4A7C15 + index) & (2 ** 64 - 1)
+    z ^= z >> 30
+    z = (z * 0xBF58476D1CE4E5B9) & (2 ** 64 - 1)
+    z ^= z >> 27
+    z = (z * 0x94D049BB133111EB) & (2 ** 64 - 1)
+    z ^= z >> 31
+    return z
+
+
+def test_uniform_closed_form():
+    x = oracle.uniform(2000, 42, -4.0, 4.0, nan_seed=7, nan_fraction=0.01)
+    for i in range(0, 2000, 37):
+        hole = (splitmix(7, i) >> 32) < int(0.01 * 2 ** 32)
+        if hole:
+            assert np.isnan(x[i])
+            continue
+        u = Fraction(splitmix(42, i) >> 40, 2 ** 24)
+        assert Fraction(float(x[i])) == -4 + 8 * u  # exact in float32
+    big = oracle.uniform(1 << 20, 3, -4.0, 4.0, nan_seed=5, nan_fraction=0.01)
+    finite = big[~np.isnan(big)]
+    assert finite.min() >= -4.0 and finite.max() < 4.0
+    assert abs(np.isnan(big).mean() - 0.01) < 0.001
+    d = oracle.uniform(1000, 3, -1.0, 1.0, dtype=np.float64)
+    assert Fraction(float(d[5])) == -1 + 2 * Fraction(splitmix(3, 5) >> 11, 2 ** 53)
+
+
+def test_linspace_closed_form():
+    """x_i = float(start + i * step) with a double step (include/operations/creation.hpp:79-81)."""
+    x = oracle.linspace(0.0, 1.0, 1_000_000)
+    i = np.arange(1_000_000, dtype=np.float64)
+    assert (x == (0.0 + i * (1.0 / 999_999.0)).astype(np.float32)).all()
+    assert x[0] == 0.0 and x[-1] == 1.0
