@@ -26,24 +26,17 @@ def program_tuples(program):
 
 
 def ulp_distance(a: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """Units in the last place between two float arrays of the same dtype; NaN against NaN is 0,
-    NaN against a number is a huge distance."""
+    """Units in the last place between two float arrays of the same dtype, in exact integer
+    arithmetic; NaN against NaN is 0, NaN against a number is a huge distance."""
     a = np.asarray(a)
     b = np.asarray(b)
-    if a.dtype == np.float32:
-        ia = a.view(np.int32).astype(np.int64)
-        ib = b.view(np.int32).astype(np.int64)
-        ia = np.where(ia < 0, -(ia & 0x7FFFFFFF), ia)
-        ib = np.where(ib < 0, -(ib & 0x7FFFFFFF), ib)
-        d = np.abs(ia - ib)
-    else:
-        ia = a.view(np.int64)
-        ib = b.view(np.int64)
-        fa = np.where(ia < 0, -(ia & 0x7FFFFFFFFFFFFFFF), ia).astype(np.float64)
-        fb = np.where(ib < 0, -(ib & 0x7FFFFFFFFFFFFFFF), ib).astype(np.float64)
-        d = np.abs(fa - fb)
+    kind, mask = (np.int32, 0x7FFFFFFF) if a.dtype == np.float32 else (np.int64, 0x7FFFFFFFFFFFFFFF)
+    ia = a.view(kind).astype(object)
+    ib = b.view(kind).astype(object)
+    order = np.vectorize(lambda i: -(i & mask) if i < 0 else i, otypes=[object])
+    d = np.abs(order(ia) - order(ib)).astype(np.float64)
     nan_a, nan_b = np.isnan(a), np.isnan(b)
-    return np.where(nan_a & nan_b, 0, np.where(nan_a | nan_b, 1 << 40, d))
+    return np.where(nan_a & nan_b, 0.0, np.where(nan_a | nan_b, float(1 << 40), d))
 
 
 def same_bits(a: np.ndarray, b: np.ndarray) -> np.ndarray:

@@ -2,8 +2,11 @@
 
 Bar: minimum and maximum equal the oracle's exactly; sums within a stated reordering bound of the
 exact (math.fsum) sum of the same float terms: the GPU accumulates in double, so its error is at
-most n * 2^-53 * sum |x_i|, and the CPU's float blocks at most 1024 * 2^-24 * sum |x_i|; NaN anywhere
-gives NaN; the minimum and maximum of nothing are NaN and the sum of nothing 0
+most n * 2^-53 * sum |x_i|, and the CPU's float blocks at most 1024 * 2^-24 * sum |x_i|. Where a
+chunk uses a code the CPU sends to glibc (tanh, sinh, ...), each GPU term may differ from the CPU
+term by that code's ulp bound, so the sum gets LIBRARY_ULPS * 2^-24 * sum (|x_i| + 1) more. A
+contracted sum (a dot product) folds the exact products of its float operands. NaN anywhere gives
+NaN; the minimum and maximum of nothing are NaN and the sum of nothing 0
 (src/core/drivers/device.cpp:266-268).
 """
 
@@ -21,6 +24,8 @@ from libsci_amd import _abi
 from libsci_amd.chunk import Code, Fold, Precision, record_postfix
 
 pytestmark = pytest.mark.gpu
+
+LIBRARY_ULPS = 8
 
 C1_REDUCE = [("in", 0), ("op", Code.sin)]
 C2_REDUCE = [("in", 0), ("op", Code.sin), ("in", 2), ("op", Code.cos), ("op", Code.multiply),
@@ -53,17 +58,23 @@ def test_reductions_match_oracle(device, count, policy):
     print(f"[test] reductions count={count} policy={policy}", flush=True)
     arrays = [oracle.uniform(count, 100 + i, -4.0, 4.0) for i in range(3)]
     dev = [sci.Array.from_numpy(a, device) for a in arrays]
-    for program in (C1_REDUCE, C2_REDUCE):
+    for program, library in ((C1_REDUCE, False), (C2_REDUCE, True)):
         x = terms(program, arrays).astype(np.float64)
         magnitude = float(np.abs(x).sum())
         exact = math.fsum(x)
         got = gpu_reduce(device, program, dev, Fold.sum, policy)
-        assert abs(got - exact) <= count * 2 ** -53 * magnitude + 1e-300
+        slack = LIBRARY_ULPS * 2 ** -24 * (magnitude + count) if library else 0.0
+        assert abs(got - exact) <= count * 2 ** -53 * magnitude + slack
         chunk = record_postfix(program, Precision.single, "reduce", [a.ctypes.data for a in arrays])
         cpu = oracle.reduce(chunk, Fold.sum, count, threads=8)
         assert abs(cpu - exact) <= 1024 * 2 ** -24 * magnitude
         for fold in (Fold.minimum, Fold.maximum):
-            assert gpu_reduce(device, program, dev, fold, policy) == oracle.reduce(chunk, fold, count, threads=8)
+            got = gpu_reduce(device, program, dev, fold, policy)
+            want = oracle.reduce(chunk, fold, count, threads=8)
+            if library:
+                assert abs(got - want) <= LIBRARY_ULPS * 2 ** -24 * (abs(want) + 1)
+            else:
+                assert got == want
 
 
 @pytest.mark.parametrize("policy", [1, 2], ids=["interpreter", "specialised"])
@@ -84,7 +95,9 @@ def test_contracted_sums(device, policy):
                                           [arrays[3].ctypes.data]), count, threads=8).astype(np.float64)
     exact = math.fsum(left * right)
     got = gpu_reduce(device, C5_LOSS, dev, Fold.sum, policy)
-    assert abs(got - exact) <= count * 2 ** -53 * float(np.abs(left * right).sum())
+    # sinh goes to glibc on the CPU and the device library on the GPU: a few ulps per term
+    magnitude = float(np.abs(left * right).sum())
+    assert abs(got - exact) <= count * 2 ** -53 * magnitude + LIBRARY_ULPS * 2 ** -24 * magnitude
     cpu = oracle.reduce(record_postfix(C5_LOSS, Precision.single, "reduce", [a.ctypes.data for a in arrays]),
                         Fold.sum, count, threads=8)
     assert abs(cpu - exact) <= 1024 * 2 ** -24 * float(np.abs(left * right).sum())
@@ -147,8 +160,10 @@ def test_c2_reduce_full_size(device):
     ch = oracle.uniform(n, 3, -4.0, 4.0)
     chunk = record_postfix(C2_REDUCE, Precision.single, "reduce", [ah.ctypes.data, bh.ctypes.data, ch.ctypes.data])
     for fold in (Fold.minimum, Fold.maximum):
-        assert got[fold] == oracle.reduce(chunk, fold, n, threads=16)
+        want = oracle.reduce(chunk, fold, n, threads=16)
+        assert abs(got[fold] - want) <= LIBRARY_ULPS * 2 ** -24 * (abs(want) + 1)  # tanh: glibc vs device
     cpu = oracle.reduce(chunk, Fold.sum, n, threads=16)
-    # sum |terms| <= 2 * n, so the CPU's float blocks are within 1024 * 2^-24 * 2n of the exact sum
-    assert abs(got[Fold.sum] - cpu) <= 1024 * 2 ** -24 * 2 * n
+    # sum |terms| <= 2n: the CPU's float blocks are within 1024 * 2^-24 * 2n of the exact sum, and the
+    # GPU's terms within LIBRARY_ULPS * 2^-24 * (|term| + 1) of the CPU's
+    assert abs(got[Fold.sum] - cpu) <= (1024 + 2 * LIBRARY_ULPS) * 2 ** -24 * 2 * n
     assert sci.mean(expr) == np.float32(np.float32(got[Fold.sum]) / np.float32(n))
