@@ -145,6 +145,21 @@ int scihip_execute(scihip_device *device, const scihip_chunk *chunk, void *outpu
  * (include/core/bytecode/chunk.hpp:151-154). */
 int scihip_reduce(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t count, double *result);
 
+/* Product modes. */
+enum scihip_product_mode {
+    SCIHIP_PRODUCT_FLOAT32 = 0,  /* float operands and result, fp32 matrix cores */
+    SCIHIP_PRODUCT_BFLOAT16 = 1, /* float operands rounded to bf16, fp32 accumulation */
+    SCIHIP_PRODUCT_SPLIT = 2,    /* float operands as bf16 hi + lo parts, three bf16 products (bf16x3) */
+    SCIHIP_PRODUCT_FLOAT64 = 3   /* double operands and result, fp64 matrix cores */
+};
+
+/* Row-major output = left * right with left rows x depth and right depth x columns
+ * (include/core/drivers/device.hpp:79-80, src/core/drivers/device.cpp:318-352). The reference's
+ * Precision::single is SCIHIP_PRODUCT_FLOAT32 and Precision::wide SCIHIP_PRODUCT_FLOAT64.
+ * Asynchronous. depth = 0 writes zeros (src/core/drivers/processor/dispatch.cpp:270-273). */
+int scihip_product(scihip_device *device, uint32_t mode, const void *left, const void *right, void *output,
+                   size_t rows, size_t columns, size_t depth);
+
 /* Dispatch policy: 0 = automatic (interpreter until the specialised kernel is ready),
  * 1 = interpreter only, 2 = specialised kernel only (compiles and waits on first use).
  * Mirrors SCI_JIT=0 (src/core/drivers/vulkan/compiler.cpp:153-158); the SCIHIP_JIT environment

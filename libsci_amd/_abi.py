@@ -36,6 +36,7 @@ SIGNATURES = {
     "scihip_set_policy": (_i, [_vp, _i]),
     "scihip_prepare": (_i, [_vp, _vp]),
     "scihip_statistics": (_i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
+    "scihip_product": (_i, [_vp, _u32, _vp, _vp, _vp, _sz, _sz, _sz]),
     "scihip_stream": (_i, [_vp, _pp]),
     "scihip_fill_uniform": (_i, [_vp, _vp, _u32, _sz, _u64, _d, _d, _u64, _d]),
 }

@@ -1,0 +1,318 @@
+// Matrix products on the gfx950 matrix cores (see product.hpp).
+//
+// Tiling: a 256-thread workgroup (four wave64s, 2 x 2) computes a 128 x 128 tile of C; each wave a
+// 64 x 64 sub-tile as 4 x 4 MFMA 16 x 16 blocks held in accumulator registers. A and B (packed, the
+// shared dimension contiguous) pass through LDS in k-tiles of 128 bytes per row (64 for kSplit),
+// double-buffered: the next k-tile's global loads are in flight while the current one is
+// multiplied. LDS rows are padded by 16 bytes so that the 16 lanes reading 16 consecutive rows hit
+// distinct banks. Workgroups are numbered so that consecutive tiles run on the same XCD (blocks are
+// dealt round-robin over the 8 XCDs) and neighbouring tiles share rows of A, for L2 reuse.
+//
+// MFMA 16 x 16 register layout (lane l, group g = l / 16, r = l % 16): the A operand is row r of the
+// block at k-slot g, the B operand column r at k-slot g, and accumulator element i is C[4g + i][r].
+// Each lane reads 16 bytes of its row per step, so one step covers 64 bytes of k per row; for fp32
+// (fp64) those 16 bytes feed four (two) MFMAs, each over the k values at the same position of every
+// lane group's 16 bytes: a fixed permutation of k, the same for A and B, so the dot products are
+// unchanged.
+#include "product.hpp"
+
+namespace scihip {
+
+    namespace {
+
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        typedef double v4d __attribute__((ext_vector_type(4)));
+        typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+        typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+        constexpr int kTile = 128;      // rows of C (and columns) per workgroup
+        constexpr int kThreads = 256;
+        constexpr int kPad = 16;        // bytes of padding per LDS row
+        constexpr int kXcds = 8;
+        constexpr int kGroup = 8;       // tile rows per L2 group
+
+        template <int M> struct Shape {
+            static constexpr int row = M == kSplit ? 64 : 128;       // bytes of a k-tile row
+            static constexpr int stride = row + kPad;                // LDS bytes per row
+            static constexpr int operands = M == kSplit ? 4 : 2;     // A_hi, B_hi (, A_lo, B_lo)
+            static constexpr int chunks = kTile * row / 16 / kThreads; // 16-byte loads per thread per operand
+            static constexpr int steps = row / 64;                   // 64-byte k steps per k-tile
+            static constexpr int k = row / element_bytes(M);         // k values per k-tile
+        };
+
+        template <int M> struct Accumulator {
+            typedef v4f type;
+        };
+        template <> struct Accumulator<kFloat64> {
+            typedef v4d type;
+        };
+
+        union Fragment {
+            v4u raw;
+            v4f f;
+            v8bf b;
+            double d[2];
+        };
+
+        template <int M>
+        __device__ __forceinline__ void multiply(typename Accumulator<M>::type &acc, const Fragment &a,
+                                                 const Fragment &b, const Fragment &a_lo, const Fragment &b_lo) {
+            if constexpr (M == kFloat32) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.f[0], b.f[0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.f[1], b.f[1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.f[2], b.f[2], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.f[3], b.f[3], acc, 0, 0, 0);
+            } else if constexpr (M == kBfloat16) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.b, b.b, acc, 0, 0, 0);
+            } else if constexpr (M == kSplit) {
+                // the small cross terms first, then the leading term
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo.b, b.b, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.b, b_lo.b, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.b, b.b, acc, 0, 0, 0);
+            } else {
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.d[0], b.d[0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.d[1], b.d[1], acc, 0, 0, 0);
+            }
+        }
+
+        // Which 128 x 128 tile this workgroup computes: XCD-contiguous numbering, then groups of
+        // kGroup tile rows walked column by column.
+        __device__ __forceinline__ void tile_of(int tiles_m, int tiles_n, int &tm, int &tn) {
+            const int total = tiles_m * tiles_n;
+            int id = blockIdx.x;
+            if (total % kXcds == 0) id = (id % kXcds) * (total / kXcds) + id / kXcds;
+            const int width = kGroup * tiles_n;
+            const int group = id / width;
+            const int first = group * kGroup;
+            const int rows = min(tiles_m - first, kGroup);
+            const int within = id % width;
+            tm = first + within % rows;
+            tn = within / rows;
+        }
+
+        template <int M> __global__ __launch_bounds__(kThreads) void gemm(const ProductArgs p) {
+            typedef Shape<M> S;
+            typedef typename Accumulator<M>::type Acc;
+            extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+            constexpr int eb = element_bytes(M);
+            constexpr int stage_bytes = S::operands * kTile * S::stride;
+
+            const int tiles_m = (int)((p.m + kTile - 1) / kTile);
+            const int tiles_n = (int)((p.n + kTile - 1) / kTile);
+            int tm, tn;
+            tile_of(tiles_m, tiles_n, tm, tn);
+            const int64_t row0 = (int64_t)tm * kTile, col0 = (int64_t)tn * kTile;
+
+            const int t = threadIdx.x;
+            const int lane = t & 63, wave = t >> 6;
+            const int g = lane >> 4, r = lane & 15;
+            const int wm = wave >> 1, wn = wave & 1;
+
+            // Global sources of this thread's chunks: operand o (0 A_hi, 1 B_hi, 2 A_lo, 3 B_lo).
+            const unsigned char *source[S::operands];
+            int64_t limit[S::operands], ld[S::operands], first[S::operands];
+            source[0] = static_cast<const unsigned char *>(p.a_hi);
+            source[1] = static_cast<const unsigned char *>(p.b_hi);
+            if constexpr (S::operands == 4) {
+                source[2] = static_cast<const unsigned char *>(p.a_lo);
+                source[3] = static_cast<const unsigned char *>(p.b_lo);
+            }
+#pragma unroll
+            for (int o = 0; o < S::operands; ++o) {
+                const bool is_a = (o & 1) == 0;
+                limit[o] = is_a ? p.m : p.n;
+                ld[o] = (is_a ? p.lda : p.ldb) * eb;
+                first[o] = is_a ? row0 : col0;
+            }
+
+            v4u staged[S::operands][S::chunks];
+            auto fetch = [&](int64_t k_byte) {
+#pragma unroll
+                for (int o = 0; o < S::operands; ++o)
+#pragma unroll
+                    for (int j = 0; j < S::chunks; ++j) {
+                        const int c = t + j * kThreads;
+                        const int row = c / (S::row / 16), piece = c % (S::row / 16);
+                        const int64_t global_row = first[o] + row;
+                        v4u value = {0u, 0u, 0u, 0u};
+                        if (global_row < limit[o])
+                            value = *reinterpret_cast<const v4u *>(source[o] + global_row * ld[o] + k_byte + piece * 16);
+                        staged[o][j] = value;
+                    }
+            };
+            auto deposit = [&](int stage) {
+#pragma unroll
+                for (int o = 0; o < S::operands; ++o)
+#pragma unroll
+                    for (int j = 0; j < S::chunks; ++j) {
+                        const int c = t + j * kThreads;
+                        const int row = c / (S::row / 16), piece = c % (S::row / 16);
+                        *reinterpret_cast<v4u *>(lds + stage * stage_bytes + (o * kTile + row) * S::stride + piece * 16) =
+                                staged[o][j];
+                    }
+            };
+
+            Acc acc[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+
+            const int64_t tiles_k = p.kp / S::k;
+            if (tiles_k > 0) {
+                fetch(0);
+                deposit(0);
+                __syncthreads();
+            }
+            for (int64_t kt = 0; kt < tiles_k; ++kt) {
+                const int stage = (int)(kt & 1);
+                if (kt + 1 < tiles_k) fetch((kt + 1) * S::row);
+                const unsigned char *base = lds + stage * stage_bytes;
+#pragma unroll
+                for (int s = 0; s < S::steps; ++s) {
+                    Fragment a[4], b[4], a_lo[4], b_lo[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int arow = wm * 64 + i * 16 + r, brow = wn * 64 + i * 16 + r;
+                        const int offset = s * 64 + g * 16;
+                        a[i].raw = *reinterpret_cast<const v4u *>(base + arow * S::stride + offset);
+                        b[i].raw = *reinterpret_cast<const v4u *>(base + (kTile + brow) * S::stride + offset);
+                        if constexpr (M == kSplit) {
+                            a_lo[i].raw = *reinterpret_cast<const v4u *>(base + (2 * kTile + arow) * S::stride + offset);
+                            b_lo[i].raw = *reinterpret_cast<const v4u *>(base + (3 * kTile + brow) * S::stride + offset);
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) multiply<M>(acc[i][j], a[i], b[j], a_lo[i], b_lo[j]);
+                }
+                if (kt + 1 < tiles_k) deposit(stage ^ 1);
+                __syncthreads();
+            }
+
+            // Epilogue: accumulator element e of block (i, j) is C[row0 + 64 wm + 16 i + 4 g + e][col0 + 64 wn + 16 j + r].
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int64_t row = row0 + wm * 64 + i * 16 + g * 4 + e;
+                        const int64_t col = col0 + wn * 64 + j * 16 + r;
+                        if (row >= p.m || col >= p.n) continue;
+                        if constexpr (M == kFloat64) {
+                            double *target = static_cast<double *>(p.c) + row * p.ldc + col;
+                            const double v = acc[i][j][e];
+                            *target = p.beta == 0.0 ? p.alpha * v : p.alpha * v + p.beta * *target;
+                        } else {
+                            float *target = static_cast<float *>(p.c) + row * p.ldc + col;
+                            const float v = acc[i][j][e];
+                            const float alpha = (float)p.alpha, beta = (float)p.beta;
+                            *target = beta == 0.0f ? (alpha == 1.0f ? v : alpha * v) : alpha * v + beta * *target;
+                        }
+                    }
+        }
+
+        // 64 x 64 tiles through LDS, so that both the plain and the transposed copy read and write
+        // global memory along rows.
+        template <typename S, typename D, bool split>
+        __global__ __launch_bounds__(256) void pack_tile(const S *source, int64_t rows, int64_t depth, int64_t ld,
+                                                         bool transpose, D *hi, D *lo, int64_t kp) {
+            __shared__ S tile[64][65];
+            const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+            const int64_t k0 = (int64_t)blockIdx.x * 64, i0 = (int64_t)blockIdx.y * 64;
+            for (int y = ty; y < 64; y += 4) {
+                if (transpose) { // source is depth x rows: element (i, k) = source[k * ld + i]
+                    const int64_t k = k0 + y, i = i0 + tx;
+                    tile[tx][y] = (k < depth && i < rows) ? source[k * ld + i] : S(0);
+                } else {
+                    const int64_t i = i0 + y, k = k0 + tx;
+                    tile[y][tx] = (k < depth && i < rows) ? source[i * ld + k] : S(0);
+                }
+            }
+            __syncthreads();
+            for (int y = ty; y < 64; y += 4) {
+                const int64_t i = i0 + y, k = k0 + tx;
+                if (i >= rows || k >= kp) continue;
+                const S x = tile[y][tx];
+                if constexpr (sizeof(D) == 2) {
+                    const float f = (float)x;
+                    const __bf16 h = (__bf16)f;
+                    hi[i * kp + k] = h;
+                    if constexpr (split) {
+                        const float rest = __builtin_isfinite(f) ? f - (float)h : 0.0f;
+                        lo[i * kp + k] = (__bf16)rest;
+                    }
+                } else {
+                    hi[i * kp + k] = (D)x;
+                }
+            }
+        }
+
+        template <int M> constexpr int lds_bytes() { return 2 * Shape<M>::operands * kTile * Shape<M>::stride; }
+
+    } // namespace
+
+    hipError_t product(int mode, const ProductArgs &args, hipStream_t stream) {
+        const int64_t tiles = ((args.m + kTile - 1) / kTile) * ((args.n + kTile - 1) / kTile);
+        if (tiles == 0) return hipSuccess;
+        const dim3 grid((unsigned)tiles), block(kThreads);
+        switch (mode) {
+        case kFloat32:
+            hipLaunchKernelGGL(gemm<kFloat32>, grid, block, lds_bytes<kFloat32>(), stream, args);
+            break;
+        case kBfloat16:
+            hipLaunchKernelGGL(gemm<kBfloat16>, grid, block, lds_bytes<kBfloat16>(), stream, args);
+            break;
+        case kSplit:
+            hipLaunchKernelGGL(gemm<kSplit>, grid, block, lds_bytes<kSplit>(), stream, args);
+            break;
+        default:
+            hipLaunchKernelGGL(gemm<kFloat64>, grid, block, lds_bytes<kFloat64>(), stream, args);
+            break;
+        }
+        return hipGetLastError();
+    }
+
+    hipError_t pack(int mode, const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,
+                    void *hi, void *lo, int64_t kp, hipStream_t stream) {
+        if (rows == 0 || kp == 0) return hipSuccess;
+        const dim3 grid((unsigned)((kp + 63) / 64), (unsigned)((rows + 63) / 64)), block(256);
+        if (mode == kFloat64) {
+            if (!wide) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((pack_tile<double, double, false>), grid, block, 0, stream,
+                               static_cast<const double *>(source), rows, depth, ld, transpose,
+                               static_cast<double *>(hi), nullptr, kp);
+        } else if (mode == kFloat32) {
+            if (wide)
+                hipLaunchKernelGGL((pack_tile<double, float, false>), grid, block, 0, stream,
+                                   static_cast<const double *>(source), rows, depth, ld, transpose,
+                                   static_cast<float *>(hi), nullptr, kp);
+            else
+                hipLaunchKernelGGL((pack_tile<float, float, false>), grid, block, 0, stream,
+                                   static_cast<const float *>(source), rows, depth, ld, transpose,
+                                   static_cast<float *>(hi), nullptr, kp);
+        } else if (mode == kBfloat16) {
+            if (wide)
+                hipLaunchKernelGGL((pack_tile<double, __bf16, false>), grid, block, 0, stream,
+                                   static_cast<const double *>(source), rows, depth, ld, transpose,
+                                   static_cast<__bf16 *>(hi), nullptr, kp);
+            else
+                hipLaunchKernelGGL((pack_tile<float, __bf16, false>), grid, block, 0, stream,
+                                   static_cast<const float *>(source), rows, depth, ld, transpose,
+                                   static_cast<__bf16 *>(hi), nullptr, kp);
+        } else {
+            if (wide)
+                hipLaunchKernelGGL((pack_tile<double, __bf16, true>), grid, block, 0, stream,
+                                   static_cast<const double *>(source), rows, depth, ld, transpose,
+                                   static_cast<__bf16 *>(hi), static_cast<__bf16 *>(lo), kp);
+            else
+                hipLaunchKernelGGL((pack_tile<float, __bf16, true>), grid, block, 0, stream,
+                                   static_cast<const float *>(source), rows, depth, ld, transpose,
+                                   static_cast<__bf16 *>(hi), static_cast<__bf16 *>(lo), kp);
+        }
+        return hipGetLastError();
+    }
+
+} // namespace scihip

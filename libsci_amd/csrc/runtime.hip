@@ -2,6 +2,7 @@
 #include "runtime.hpp"
 #include "compiler.hpp"
 #include "kernels.hpp"
+#include "product.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -477,6 +478,46 @@ int scihip_statistics(scihip_device *device, uint64_t *interpreted, uint64_t *sp
     if (specialised) *specialised = device->specialised;
     if (compiled) *compiled = device->compiler->compiled();
     return SCIHIP_OK;
+}
+
+int scihip_product(scihip_device *device, uint32_t mode, const void *left, const void *right, void *output,
+                   size_t rows, size_t columns, size_t depth) {
+    if (!device || mode > SCIHIP_PRODUCT_FLOAT64) return SCIHIP_EINVAL;
+    if (rows == 0 || columns == 0) return SCIHIP_OK; // device.cpp:320
+    std::lock_guard<std::mutex> lock(device->guard);
+    const bool wide = mode == SCIHIP_PRODUCT_FLOAT64;
+    const size_t element = wide ? 8 : 4;
+    if (!device->allocator->locate(left, rows * depth * element) ||
+        !device->allocator->locate(right, depth * columns * element) ||
+        !device->allocator->locate(output, rows * columns * element)) {
+        fail("an operand of the product is not device memory of this device");
+        return SCIHIP_DECLINED;
+    }
+    const int m = int(mode);
+    const int64_t tile = k_tile(m);
+    const int64_t kp = (int64_t(depth) + tile - 1) / tile * tile;
+    const size_t eb = element_bytes(m);
+    const bool split = m == kSplit;
+    // Packed operands: A as rows x kp, B transposed as columns x kp (the split form twice each).
+    void *a_hi = nullptr, *a_lo = nullptr, *b_hi = nullptr, *b_lo = nullptr;
+    int rc = SCIHIP_OK;
+    const size_t a_bytes = std::max<size_t>(rows * kp * eb, 16), b_bytes = std::max<size_t>(columns * kp * eb, 16);
+    if (!rc) rc = device->allocator->allocate(a_bytes, &a_hi);
+    if (!rc) rc = device->allocator->allocate(b_bytes, &b_hi);
+    if (!rc && split) rc = device->allocator->allocate(a_bytes, &a_lo);
+    if (!rc && split) rc = device->allocator->allocate(b_bytes, &b_lo);
+    if (!rc) rc = status(pack(m, left, wide, int64_t(rows), int64_t(depth), int64_t(depth), false, a_hi, a_lo, kp,
+                              device->stream), "pack");
+    if (!rc) rc = status(pack(m, right, wide, int64_t(columns), int64_t(depth), int64_t(columns), true, b_hi, b_lo,
+                              kp, device->stream), "pack");
+    if (!rc) {
+        ProductArgs args{a_hi, a_lo, b_hi, b_lo, output, int64_t(rows), int64_t(columns), kp,
+                         kp, kp, int64_t(columns), 1.0, 0.0};
+        rc = status(scihip::product(m, args, device->stream), "product");
+    }
+    for (void *block : {a_hi, a_lo, b_hi, b_lo})
+        if (block) device->allocator->dispose(block);
+    return rc;
 }
 
 int scihip_stream(scihip_device *device, void **stream) {

@@ -1,0 +1,76 @@
+"""Matrix products on the matrix cores against a float64 product (the oracle for products).
+
+Bars, per mode, on entries in [-1, 1]: max |C - C64| <= K * u * max-row-col magnitude bound with
+u = 2^-23 (fp32 accumulation of fp32 products), 2^-8 (bf16 inputs), 2^-16 (bf16x3), 2^-52 (fp64),
+and the reference test's 37 x 53 x 29 integer-valued KAT within 1e-3
+(tests/linear/test_product.cpp:8-33).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import libsci_amd as sci
+import oracle
+from libsci_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"fp32": 0, "bf16": 1, "bf16x3": 2, "fp64": 3}
+UNIT = {"fp32": 2.0 ** -23, "bf16": 2.0 ** -8, "bf16x3": 2.0 ** -16, "fp64": 2.0 ** -52}
+
+
+def gpu_product(device, left, right, mode):
+    dtype = np.float64 if mode == "fp64" else np.float32
+    a = sci.Array.from_numpy(left.astype(dtype), device)
+    b = sci.Array.from_numpy(right.astype(dtype), device)
+    c = sci.Array(device, (left.shape[0], right.shape[1]), dtype)
+    _abi.check(_abi.lib().scihip_product(device.handle, MODES[mode], a.pointer, b.pointer, c.pointer,
+                                         left.shape[0], right.shape[1], left.shape[1]), "scihip_product")
+    return c.host()
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_reference_kat(device, mode):
+    rows, inner, columns = 37, 53, 29
+    left = np.array([[((r * 7 + c * 3) % 11) - 5.0 for c in range(inner)] for r in range(rows)])
+    right = np.array([[((r * 5 + c) % 13) * 0.25 for c in range(columns)] for r in range(inner)])
+    got = gpu_product(device, left, right, mode)
+    want = left @ right
+    assert got.shape == (rows, columns)
+    assert np.abs(got - want).max() < 1e-3  # the entries are exact in bf16, so every mode is exact here
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (5, 3, 7), (129, 257, 130), (300, 64, 200), (512, 512, 512)])
+@pytest.mark.parametrize("mode", list(MODES))
+def test_against_float64(device, mode, shape):
+    m, k, n = shape
+    print(f"[test] product {mode} {shape}", flush=True)
+    left = oracle.uniform(m * k, 1, -1.0, 1.0, dtype=np.float64).reshape(m, k)
+    right = oracle.uniform(k * n, 2, -1.0, 1.0, dtype=np.float64).reshape(k, n)
+    if mode != "fp64":
+        left, right = left.astype(np.float32), right.astype(np.float32)
+    got = gpu_product(device, left, right, mode)
+    want = left.astype(np.float64) @ right.astype(np.float64)
+    bound = 2 * k * UNIT[mode] * (np.abs(left).astype(np.float64) @ np.abs(right).astype(np.float64))
+    assert (np.abs(got - want) <= bound + 1e-300).all(), np.abs(got - want).max()
+
+
+def test_zero_depth_writes_zeros(device):
+    c = sci.full((4, 5), 7.0, device)
+    a = sci.Array(device, (4, 1), np.float32)
+    assert _abi.lib().scihip_product(device.handle, 0, a.pointer, a.pointer, c.pointer, 4, 5, 0) == 0
+    assert (c.host() == 0).all()
+
+
+def test_large_fp32_and_split(device):
+    n = 2048
+    left = oracle.uniform(n * n, 3, -1.0, 1.0).reshape(n, n)
+    right = oracle.uniform(n * n, 4, -1.0, 1.0).reshape(n, n)
+    want = left.astype(np.float64) @ right.astype(np.float64)
+    for mode in ("fp32", "bf16x3", "bf16"):
+        got = gpu_product(device, left, right, mode)
+        relative = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"[test] 2048^3 {mode}: relative Frobenius error {relative:.3e}", flush=True)
+        assert relative < {"fp32": 1e-6, "bf16x3": 1e-4, "bf16": 1e-2}[mode]
