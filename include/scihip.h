@@ -160,6 +160,45 @@ enum scihip_product_mode {
 int scihip_product(scihip_device *device, uint32_t mode, const void *left, const void *right, void *output,
                    size_t rows, size_t columns, size_t depth);
 
+/* Factorisation modes: the element type of the matrix and the precision of its trailing updates. */
+enum scihip_factor_mode {
+    SCIHIP_FACTOR_FLOAT64 = 0,  /* double matrix, fp64 matrix-core updates */
+    SCIHIP_FACTOR_FLOAT32 = 1,  /* float matrix, fp32 matrix-core updates */
+    SCIHIP_FACTOR_SPLIT = 2,    /* float matrix, bf16x3 matrix-core updates */
+    SCIHIP_FACTOR_BFLOAT16 = 3  /* float matrix, bf16 matrix-core updates */
+};
+
+/* LU with partial pivoting of the order x order row-major matrix, in place: unit lower L below the
+ * diagonal, U on and above it; rows[i] receives the original index of the row now at position i.
+ * Replaces linear::factor (include/linear/decompose.hpp:29-72): the same pivot rule (the first row
+ * with the largest |a|), row swaps and zero-pivot skip; blocked, with the trailing updates on the
+ * matrix cores. `rows` is host memory of `order` entries. Synchronous. */
+int scihip_factor(scihip_device *device, uint32_t mode, void *matrix, size_t order, int64_t *rows);
+
+/* Forward substitution through unit L in the order `rows`, then back substitution through U:
+ * solution = U^-1 L^-1 (right permuted by rows), `width` right-hand sides, all row-major
+ * (include/linear/solve.hpp:25-52). `factors` and `rows` as scihip_factor gives them (rows in host
+ * memory); precision SCIHIP_SINGLE for float, SCIHIP_WIDE for double. Asynchronous. */
+int scihip_substitute(scihip_device *device, uint32_t precision, const void *factors, const int64_t *rows,
+                      const void *right, void *solution, size_t order, size_t width);
+
+typedef struct scihip_solve_stats {
+    int32_t iterations;     /* refinement steps taken */
+    int32_t converged;      /* 1 when the backward error reached the tolerance */
+    double backward_error;  /* max over columns of |b - Ax|_inf / (|A|_inf |x|_inf + |b|_inf) */
+    double factor_ms;       /* wall time of the factorisation */
+    double solve_ms;        /* wall time of the whole call */
+} scihip_solve_stats;
+
+/* Solves A X = B for double A (order x order) and B (order x width), all device memory: a copy of A
+ * is factored in `mode` precision, and X is refined in double (residual r = B - A X on the fp64
+ * matrix cores, correction solved with the factors) until the backward error is at most
+ * `tolerance` or `max_iterations` refinements were made. Replaces linear::solve
+ * (include/linear/solve.hpp:60-76), which factors in the matrix's own precision on the host.
+ * Synchronous; `stats` may be null. */
+int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, const double *right, double *solution,
+                 size_t order, size_t width, double tolerance, int32_t max_iterations, scihip_solve_stats *stats);
+
 /* Dispatch policy: 0 = automatic (interpreter until the specialised kernel is ready),
  * 1 = interpreter only, 2 = specialised kernel only (compiles and waits on first use).
  * Mirrors SCI_JIT=0 (src/core/drivers/vulkan/compiler.cpp:153-158); the SCIHIP_JIT environment

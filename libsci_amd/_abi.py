@@ -37,9 +37,21 @@ SIGNATURES = {
     "scihip_prepare": (_i, [_vp, _vp]),
     "scihip_statistics": (_i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     "scihip_product": (_i, [_vp, _u32, _vp, _vp, _vp, _sz, _sz, _sz]),
+    "scihip_factor": (_i, [_vp, _u32, _vp, _sz, _vp]),
+    "scihip_substitute": (_i, [_vp, _u32, _vp, _vp, _vp, _vp, _sz, _sz]),
+    "scihip_solve": (_i, [_vp, _u32, _vp, _vp, _vp, _sz, _sz, _d, ctypes.c_int32, _vp]),
     "scihip_stream": (_i, [_vp, _pp]),
     "scihip_fill_uniform": (_i, [_vp, _vp, _u32, _sz, _u64, _d, _d, _u64, _d]),
 }
+
+
+
+class SolveStats(ctypes.Structure):
+    """scihip_solve_stats (include/scihip.h)."""
+
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("backward_error", ctypes.c_double),
+                ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+
 
 _lib = None
 

@@ -1,0 +1,111 @@
+// Small elementwise and reduction kernels of the solvers: precision changes, x += d, norms.
+#include "linear.hpp"
+
+#include <algorithm>
+
+namespace scihip {
+
+    namespace {
+
+        template <typename D, typename S>
+        __global__ __launch_bounds__(256) void convert_kernel(const S *source, D *target, int64_t count) {
+            for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+                 i += (int64_t)gridDim.x * blockDim.x)
+                target[i] = (D)source[i];
+        }
+
+        template <typename S>
+        __global__ __launch_bounds__(256) void accumulate_kernel(double *target, const S *delta, int64_t count) {
+            for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+                 i += (int64_t)gridDim.x * blockDim.x)
+                target[i] = target[i] + (double)delta[i];
+        }
+
+        // One workgroup per row: sum of |a_ij| over the row.
+        __global__ __launch_bounds__(256) void row_sums_kernel(const double *a, int64_t n, int64_t columns,
+                                                               double *sums) {
+            const int64_t row = blockIdx.x;
+            double s = 0.0;
+            for (int64_t j = threadIdx.x; j < columns; j += blockDim.x) s += __builtin_fabs(a[row * columns + j]);
+#pragma unroll
+            for (int offset = 32; offset > 0; offset >>= 1) s += __shfl_xor(s, offset, 64);
+            __shared__ double waves[4];
+            if ((threadIdx.x & 63) == 0) waves[threadIdx.x >> 6] = s;
+            __syncthreads();
+            if (threadIdx.x == 0) sums[row] = (waves[0] + waves[1]) + (waves[2] + waves[3]);
+        }
+
+        // Partial column maxima of |x| over row chunks: partial[chunk * width + c].
+        __global__ __launch_bounds__(256) void column_max_kernel(const double *x, int64_t rows, int64_t width,
+                                                                 int64_t chunk, double *partial) {
+            const int64_t c = threadIdx.x % 64 + (int64_t)blockIdx.y * 64;
+            const int64_t lane_rows = threadIdx.x / 64;
+            const int64_t first = (int64_t)blockIdx.x * chunk;
+            const int64_t last = std::min(rows, first + chunk);
+            double m = 0.0;
+            bool nan = false;
+            if (c < width)
+                for (int64_t r = first + lane_rows; r < last; r += 4) {
+                    const double v = __builtin_fabs(x[r * width + c]);
+                    nan = nan || __builtin_isnan(v);
+                    m = v > m ? v : m;
+                }
+            __shared__ double best[4][64];
+            best[lane_rows][threadIdx.x % 64] = nan ? __builtin_nan("") : m;
+            __syncthreads();
+            if (lane_rows == 0 && c < width) {
+                double v = best[0][threadIdx.x];
+                for (int k = 1; k < 4; ++k) {
+                    const double o = best[k][threadIdx.x];
+                    v = (__builtin_isnan(v) || __builtin_isnan(o)) ? v + o : (o > v ? o : v);
+                }
+                partial[(int64_t)blockIdx.x * width + c] = v;
+            }
+        }
+
+        unsigned grid_for(int64_t count) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((count + 255) / 256, 8192)); }
+
+    } // namespace
+
+    hipError_t convert(const double *source, float *target, int64_t count, hipStream_t stream) {
+        if (count == 0) return hipSuccess;
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(count)), dim3(256), 0, stream, source, target, count);
+        return hipGetLastError();
+    }
+
+    hipError_t convert(const float *source, double *target, int64_t count, hipStream_t stream) {
+        if (count == 0) return hipSuccess;
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(count)), dim3(256), 0, stream, source, target, count);
+        return hipGetLastError();
+    }
+
+    hipError_t accumulate(double *target, const float *delta, int64_t count, hipStream_t stream) {
+        if (count == 0) return hipSuccess;
+        hipLaunchKernelGGL(accumulate_kernel<float>, dim3(grid_for(count)), dim3(256), 0, stream, target, delta, count);
+        return hipGetLastError();
+    }
+
+    hipError_t accumulate(double *target, const double *delta, int64_t count, hipStream_t stream) {
+        if (count == 0) return hipSuccess;
+        hipLaunchKernelGGL(accumulate_kernel<double>, dim3(grid_for(count)), dim3(256), 0, stream, target, delta, count);
+        return hipGetLastError();
+    }
+
+    hipError_t row_sums(const double *a, int64_t rows, int64_t columns, double *sums, hipStream_t stream) {
+        if (rows == 0) return hipSuccess;
+        hipLaunchKernelGGL(row_sums_kernel, dim3((unsigned)rows), dim3(256), 0, stream, a, rows, columns, sums);
+        return hipGetLastError();
+    }
+
+    int64_t column_max_chunks(int64_t rows) { return std::max<int64_t>(1, std::min<int64_t>(256, (rows + 1023) / 1024)); }
+
+    hipError_t column_max(const double *x, int64_t rows, int64_t width, double *partial, hipStream_t stream) {
+        if (rows == 0 || width == 0) return hipSuccess;
+        const int64_t chunks = column_max_chunks(rows);
+        const int64_t chunk = (rows + chunks - 1) / chunks;
+        hipLaunchKernelGGL(column_max_kernel, dim3((unsigned)chunks, (unsigned)((width + 63) / 64)), dim3(256), 0,
+                           stream, x, rows, width, chunk, partial);
+        return hipGetLastError();
+    }
+
+} // namespace scihip

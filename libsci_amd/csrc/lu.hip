@@ -1,0 +1,391 @@
+// LU with partial pivoting, triangular solves (see lu.hpp).
+#include "lu.hpp"
+#include "product.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace scihip {
+
+    namespace {
+
+        constexpr unsigned kSpinLimit = 1u << 21; // a bounded wait: about two seconds
+
+        __device__ __forceinline__ unsigned acquire(unsigned *p) {
+            return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+
+        // Workgroup waits until *counter >= target (thread 0 spins). Gives up, and raises *error, past
+        // kSpinLimit polls or as soon as another workgroup has given up.
+        __device__ __forceinline__ void wait_until(unsigned *counter, unsigned target, unsigned *error) {
+            if (threadIdx.x == 0) {
+                unsigned spins = 0;
+                while (acquire(counter) < target) {
+                    if (++spins > kSpinLimit) {
+                        __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    if ((spins & 63u) == 0 && acquire(error)) break;
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __syncthreads();
+        }
+
+        // Every workgroup of the grid arrives, then waits for all of them (`target` = barriers so far
+        // times the grid size, on a counter zeroed before the launch).
+        __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target, unsigned *error) {
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            wait_until(counter, target, error);
+        }
+
+        // The better of two pivot candidates: the larger |a|, and on a tie the lower row (the first
+        // maximum, include/linear/decompose.hpp:35-38).
+        template <typename T> __device__ __forceinline__ void better(T &v, int64_t &r, T ov, int64_t orow) {
+            if (ov > v || (ov == v && orow < r)) {
+                v = ov;
+                r = orow;
+            }
+        }
+
+        template <typename T> __device__ __forceinline__ void block_best(T &v, int64_t &r, T *sv, int64_t *sr) {
+#pragma unroll
+            for (int offset = 32; offset > 0; offset >>= 1) {
+                const T ov = __shfl_xor(v, offset, 64);
+                const int64_t orow = __shfl_xor(r, offset, 64);
+                better(v, r, ov, orow);
+            }
+            const int wave = threadIdx.x >> 6;
+            __syncthreads();
+            if ((threadIdx.x & 63) == 0) {
+                sv[wave] = v;
+                sr[wave] = r;
+            }
+            __syncthreads();
+            v = sv[0];
+            r = sr[0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) better(v, r, sv[w], sr[w]);
+        }
+
+        // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b owns rows
+        // [j0 + b * kPanelRows, ...). Publications, two parities: value[2][G], row[2][G],
+        // rows[2][G][kPanel], diagonal[2][kPanel].
+        template <typename T>
+        __global__ __launch_bounds__(256) void panel(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
+                                                     T *pub_value, int64_t *pub_row, T *pub_rows, T *diagonal,
+                                                     unsigned *counter, unsigned *error) {
+            const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+            const int64_t first = j0 + (int64_t)b * kPanelRows;
+            const int64_t last = min(n, first + (int64_t)kPanelRows);
+            __shared__ T multipliers[kPanelRows];
+            __shared__ T pivot_row[kPanel];
+            __shared__ T diagonal_row[kPanel];
+            __shared__ T sv[4];
+            __shared__ int64_t sr[4];
+
+            for (int c = 0; c < w; ++c) {
+                const int64_t d = j0 + c;
+                const int parity = c & 1;
+                // A: this workgroup's candidate among its rows >= d, and its panel row.
+                T best = T(-1);
+                int64_t row = LLONG_MAX;
+                if (t < kPanelRows) {
+                    const int64_t r = first + t;
+                    if (r < last && r >= d) {
+                        best = __builtin_fabs(a[r * n + d]);
+                        row = r;
+                    }
+                }
+                block_best(best, row, sv, sr);
+                if (t == 0) {
+                    pub_value[parity * G + b] = best;
+                    pub_row[parity * G + b] = row;
+                }
+                if (row != LLONG_MAX)
+                    for (int k = t; k < w; k += blockDim.x)
+                        pub_rows[((int64_t)parity * G + b) * kPanel + k] = a[row * n + j0 + k];
+                if (d >= first && d < last)
+                    for (int k = t; k < w; k += blockDim.x) diagonal[parity * kPanel + k] = a[d * n + j0 + k];
+                grid_barrier(counter, (unsigned)(c + 1) * (unsigned)G, error);
+
+                // B: agree on the pivot, swap, eliminate.
+                T v = T(-1);
+                int64_t p = LLONG_MAX;
+                for (int g = t; g < G; g += blockDim.x) better(v, p, pub_value[parity * G + g], pub_row[parity * G + g]);
+                block_best(v, p, sv, sr);
+                const int owner = (int)((p - j0) / kPanelRows); // the workgroup that published row p
+                for (int k = t; k < w; k += blockDim.x) {
+                    pivot_row[k] = pub_rows[((int64_t)parity * G + owner) * kPanel + k];
+                    diagonal_row[k] = diagonal[parity * kPanel + k];
+                }
+                __syncthreads();
+                if (p != d) {
+                    if (p >= first && p < last)
+                        for (int k = t; k < w; k += blockDim.x) a[p * n + j0 + k] = diagonal_row[k];
+                    if (d >= first && d < last)
+                        for (int k = t; k < w; k += blockDim.x) a[d * n + j0 + k] = pivot_row[k];
+                }
+                if (b == 0 && t == 0) pivots[d] = p;
+                __syncthreads();
+                const T pivot = pivot_row[c];
+                if (pivot != T(0)) { // a zero pivot skips the column (decompose.hpp:44-45)
+                    if (t < kPanelRows) {
+                        const int64_t r = first + t;
+                        T l = T(0);
+                        if (r < last && r > d) {
+                            l = a[r * n + d] / pivot;
+                            a[r * n + d] = l;
+                        }
+                        multipliers[t] = l;
+                    }
+                    __syncthreads();
+                    const int remaining = w - c - 1;
+                    const int64_t lo = max(first, d + 1);
+                    if (remaining > 0 && lo < last) {
+                        const int count = (int)(last - lo) * remaining;
+                        for (int idx = t; idx < count; idx += blockDim.x) {
+                            const int64_t r = lo + idx / remaining;
+                            const int k = c + 1 + idx % remaining;
+                            const T scale = multipliers[r - first];
+                            T &target = a[r * n + j0 + k];
+                            target = target - scale * pivot_row[k];
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // Applies the panel's swaps to every column outside it, in step order.
+        template <typename T>
+        __global__ __launch_bounds__(256) void swap_rows(T *a, int64_t n, int64_t j0, int w, const int64_t *pivots) {
+            const int64_t raw = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            if (raw >= n - w) return;
+            const int64_t col = raw < j0 ? raw : raw + w;
+            for (int k = 0; k < w; ++k) {
+                const int64_t d = j0 + k, p = pivots[d];
+                if (p != d) {
+                    const T x = a[d * n + col];
+                    a[d * n + col] = a[p * n + col];
+                    a[p * n + col] = x;
+                }
+            }
+        }
+
+        // U12 = L11^-1 A12: each thread carries one column of A12 down the w rows of the panel, in the
+        // reference's update order (target -= scale * top, step by step: decompose.hpp:50-55).
+        template <typename T>
+        __global__ __launch_bounds__(64) void lower_columns(T *a, int64_t n, int64_t j0, int w) {
+            const int64_t col = j0 + w + (int64_t)blockIdx.x * 64 + threadIdx.x;
+            if (col >= n) return;
+            for (int r = 1; r < w; ++r) {
+                const T *lrow = a + (j0 + r) * n + j0;
+                T s = a[(j0 + r) * n + col];
+                for (int k = 0; k < r; ++k) s = s - lrow[k] * a[(j0 + k) * n + col];
+                a[(j0 + r) * n + col] = s;
+            }
+        }
+
+        // Sync-free blocked substitution over columns [col0, col0 + kSolveColumns) of x (order x width):
+        // workgroup q solves block q (lower, unit diagonal) or block nb - 1 - q (upper), after folding in
+        // every block it depends on as soon as that block's ready flag reaches `epoch`.
+        template <typename T, bool upper>
+        __global__ __launch_bounds__(256) void solve_blocks(const T *f, int64_t n, T *x, int64_t width, int64_t col0,
+                                                            unsigned *flags, unsigned epoch, unsigned *error) {
+            const int nb = gridDim.x, q = blockIdx.x, t = threadIdx.x;
+            const int blk = upper ? nb - 1 - q : q;
+            const int64_t r0 = (int64_t)blk * kSolveRows;
+            const int rows = (int)min((int64_t)kSolveRows, n - r0);
+            const int cols = (int)min((int64_t)kSolveColumns, width - col0);
+            __shared__ T rhs[kSolveRows][kSolveColumns + 1];
+            __shared__ T tile[kSolveRows][kSolveRows + 1];
+            __shared__ T known[kSolveRows][kSolveColumns + 1];
+
+            for (int idx = t; idx < kSolveRows * kSolveColumns; idx += blockDim.x) {
+                const int i = idx / kSolveColumns, c = idx % kSolveColumns;
+                rhs[i][c] = (i < rows && c < cols) ? x[(r0 + i) * width + col0 + c] : T(0);
+            }
+            for (int qq = 0; qq < q; ++qq) {
+                const int other = upper ? nb - 1 - qq : qq;
+                wait_until(&flags[other], epoch, error);
+                const int64_t o0 = (int64_t)other * kSolveRows;
+                const int orows = (int)min((int64_t)kSolveRows, n - o0);
+                for (int idx = t; idx < kSolveRows * kSolveRows; idx += blockDim.x) {
+                    const int i = idx / kSolveRows, k = idx % kSolveRows;
+                    tile[i][k] = (i < rows && k < orows) ? f[(r0 + i) * n + o0 + k] : T(0);
+                }
+                for (int idx = t; idx < kSolveRows * kSolveColumns; idx += blockDim.x) {
+                    const int k = idx / kSolveColumns, c = idx % kSolveColumns;
+                    known[k][c] = (k < orows && c < cols) ? x[(o0 + k) * width + col0 + c] : T(0);
+                }
+                __syncthreads();
+                const int c = t % kSolveColumns;
+                for (int i = t / kSolveColumns; i < kSolveRows; i += blockDim.x / kSolveColumns) {
+                    T s = rhs[i][c];
+                    for (int k = 0; k < kSolveRows; ++k) s = s - tile[i][k] * known[k][c];
+                    rhs[i][c] = s;
+                }
+                __syncthreads();
+            }
+            // The diagonal block: kSolveColumns columns, eight threads per column splitting the sum.
+            for (int idx = t; idx < kSolveRows * kSolveRows; idx += blockDim.x) {
+                const int i = idx / kSolveRows, k = idx % kSolveRows;
+                tile[i][k] = (i < rows && k < rows) ? f[(r0 + i) * n + r0 + k] : T(0);
+            }
+            __syncthreads();
+            const int c = t >> 3, s = t & 7;
+            for (int step = 0; step < rows; ++step) {
+                const int i = upper ? rows - 1 - step : step;
+                T partial = T(0);
+                if (upper) {
+                    for (int k = i + 1 + s; k < rows; k += 8) partial = partial + tile[i][k] * rhs[k][c];
+                } else {
+                    for (int k = s; k < i; k += 8) partial = partial + tile[i][k] * rhs[k][c];
+                }
+                partial = partial + __shfl_xor(partial, 1, 64);
+                partial = partial + __shfl_xor(partial, 2, 64);
+                partial = partial + __shfl_xor(partial, 4, 64);
+                if (s == 0) {
+                    const T y = rhs[i][c] - partial;
+                    rhs[i][c] = upper ? y / tile[i][i] : y;
+                }
+                __syncthreads();
+            }
+            for (int idx = t; idx < rows * cols; idx += blockDim.x) {
+                const int i = idx / cols, cc = idx % cols;
+                x[(r0 + i) * width + col0 + cc] = rhs[i][cc];
+            }
+            __syncthreads();
+            if (t == 0) {
+                __threadfence();
+                __hip_atomic_store(&flags[blk], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+
+        template <typename T, typename S>
+        __global__ __launch_bounds__(256) void permute_rows(const S *source, T *target, const int64_t *rows,
+                                                            int64_t order, int64_t width) {
+            const int64_t total = order * width;
+            for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+                 idx += (int64_t)gridDim.x * blockDim.x) {
+                const int64_t k = idx / width, c = idx % width;
+                target[idx] = (T)source[rows[k] * width + c];
+            }
+        }
+
+        const char *kSpinFailure = "a bounded wait in the LU kernels gave up";
+
+        template <typename T> struct PackMode;
+        template <> struct PackMode<double> {
+            static bool wide() { return true; }
+        };
+        template <> struct PackMode<float> {
+            static bool wide() { return false; }
+        };
+
+    } // namespace
+
+    size_t factor_scratch(int64_t order, int mode) {
+        const size_t eb = element_bytes(mode) * (mode == kSplit ? 2 : 1);
+        const size_t kp = (size_t)kPanel;
+        return 2 * (size_t)order * kp * eb + 256;
+    }
+
+    template <typename T>
+    int factor(T *a, int64_t n, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
+               hipStream_t stream, const char **error) {
+        *error = nullptr;
+        if (scratch_bytes < factor_scratch(n, mode)) {
+            *error = "LU scratch too small";
+            return -1;
+        }
+        const size_t eb = element_bytes(mode);
+        const size_t half = (size_t)n * kPanel * eb * (mode == kSplit ? 2 : 1);
+        unsigned char *base = static_cast<unsigned char *>(scratch);
+        void *la_hi = base, *la_lo = base + (size_t)n * kPanel * eb;
+        void *ub_hi = base + half, *ub_lo = base + half + (size_t)n * kPanel * eb;
+        T *pub_value = static_cast<T *>(ws.publish);
+        const int max_groups = (int)((n + kPanelRows - 1) / kPanelRows);
+        int64_t *pub_row = reinterpret_cast<int64_t *>(pub_value + 2 * max_groups);
+        T *pub_rows = reinterpret_cast<T *>(pub_row + 2 * max_groups);
+        T *diagonal = pub_rows + (size_t)2 * max_groups * kPanel;
+
+        for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
+            const int w = (int)std::min<int64_t>(kPanel, n - j0);
+            const int groups = (int)((n - j0 + kPanelRows - 1) / kPanelRows);
+            hipError_t e = hipMemsetAsync(ws.counter, 0, sizeof(unsigned), stream);
+            if (e != hipSuccess) return (int)e;
+            void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &pub_value, &pub_row, &pub_rows, &diagonal,
+                            &ws.counter, &ws.error};
+            e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups), dim3(256), args, 0,
+                                           stream);
+            if (e != hipSuccess) return (int)e;
+            hipLaunchKernelGGL(swap_rows<T>, dim3((unsigned)((n - w + 255) / 256)), dim3(256), 0, stream, a, n, j0, w,
+                               (const int64_t *)pivots);
+            const int64_t rest = n - j0 - w;
+            if (rest > 0) {
+                hipLaunchKernelGGL(lower_columns<T>, dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, stream, a, n, j0, w);
+                // A22 -= L21 U12 on the matrix cores.
+                const int64_t kp = kPanel;
+                e = pack(mode, a + (j0 + w) * n + j0, PackMode<T>::wide(), rest, w, n, false, la_hi, la_lo, kp, stream);
+                if (e == hipSuccess)
+                    e = pack(mode, a + j0 * n + j0 + w, PackMode<T>::wide(), rest, w, n, true, ub_hi, ub_lo, kp, stream);
+                if (e != hipSuccess) return (int)e;
+                ProductArgs args2{la_hi, la_lo, ub_hi, ub_lo, a + (j0 + w) * n + (j0 + w), rest, rest, kp, kp, kp, n,
+                                  -1.0, 1.0};
+                e = product(mode, args2, stream);
+                if (e != hipSuccess) return (int)e;
+            }
+        }
+        return hipGetLastError();
+    }
+
+    template <typename T>
+    int substitute(const T *f, int64_t n, T *x, int64_t width, Workspace &ws, unsigned &epoch, hipStream_t stream,
+                   const char **error) {
+        *error = nullptr;
+        const int blocks = (int)((n + kSolveRows - 1) / kSolveRows);
+        if ((size_t)blocks > ws.flag_count) {
+            *error = "solve flags too few";
+            return -1;
+        }
+        for (int64_t col0 = 0; col0 < width; col0 += kSolveColumns) {
+            for (int pass = 0; pass < 2; ++pass) {
+                const unsigned e = ++epoch; // every launch waits on values no earlier launch has written
+                void *args[] = {(void *)&f, &n, &x, &width, &col0, &ws.flags, (void *)&e, &ws.error};
+                const void *kernel = pass == 0 ? reinterpret_cast<const void *>(&solve_blocks<T, false>)
+                                               : reinterpret_cast<const void *>(&solve_blocks<T, true>);
+                const hipError_t err = hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(256), args, 0, stream);
+                if (err != hipSuccess) return (int)err;
+            }
+        }
+        return hipGetLastError();
+    }
+
+    template <typename T, typename S>
+    hipError_t permute(const S *source, T *target, const int64_t *rows, int64_t order, int64_t width,
+                       hipStream_t stream) {
+        const int64_t total = order * width;
+        const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
+        if (total == 0) return hipSuccess;
+        hipLaunchKernelGGL((permute_rows<T, S>), dim3(blocks), dim3(256), 0, stream, source, target, rows, order, width);
+        return hipGetLastError();
+    }
+
+    template int factor<double>(double *, int64_t, int64_t *, int, Workspace &, void *, size_t, hipStream_t,
+                                const char **);
+    template int factor<float>(float *, int64_t, int64_t *, int, Workspace &, void *, size_t, hipStream_t,
+                               const char **);
+    template int substitute<double>(const double *, int64_t, double *, int64_t, Workspace &, unsigned &, hipStream_t,
+                                    const char **);
+    template int substitute<float>(const float *, int64_t, float *, int64_t, Workspace &, unsigned &, hipStream_t,
+                                   const char **);
+    template hipError_t permute<double, double>(const double *, double *, const int64_t *, int64_t, int64_t,
+                                                hipStream_t);
+    template hipError_t permute<float, double>(const double *, float *, const int64_t *, int64_t, int64_t, hipStream_t);
+    template hipError_t permute<float, float>(const float *, float *, const int64_t *, int64_t, int64_t, hipStream_t);
+
+    const char *spin_failure() { return kSpinFailure; }
+
+} // namespace scihip

@@ -1,0 +1,60 @@
+// LU with partial pivoting on the GPU, triangular solves and mixed-precision iterative refinement.
+//
+// The reference factors and solves on the host only (include/linear/decompose.hpp:29-72,
+// include/linear/solve.hpp:25-76). The same row-major, in-place algorithm runs here blocked:
+// for each panel of kPanel columns,
+//   1. the panel is factored by one cooperative kernel: one grid barrier per column, at which every
+//      workgroup publishes its best pivot candidate (first maximum of |a|, the reference's rule) and
+//      the candidate's panel row; all workgroups then agree on the pivot, swap and eliminate;
+//   2. the panel's row swaps are applied to the columns outside it;
+//   3. U12 = L11^-1 A12 by column-parallel forward substitution;
+//   4. A22 -= L21 U12 on the matrix cores (fp64, or fp32 / bf16x3 / bf16 for a float matrix).
+// Solves run a sync-free blocked substitution: one workgroup per kSolveRows-row block, each waiting on the
+// blocks before it with a bounded spin. Every spin in this file is bounded: past kSpinLimit the
+// kernel records an error and exits, and the host reports it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace scihip {
+
+    constexpr int kPanel = 128;       // panel width (columns factored per cooperative launch)
+    constexpr int kPanelRows = 64;    // rows per workgroup in the panel kernel
+    constexpr int kSolveRows = 48;    // rows per workgroup in the triangular solves
+    constexpr int kSolveColumns = 32; // right-hand sides per pass of the solve kernel
+
+    // Device-side state the LU kernels share: barrier counters, error flag, publications.
+    struct Workspace {
+        unsigned *counter = nullptr; // grid barrier arrivals (zeroed before each launch)
+        unsigned *error = nullptr;   // set when a bounded spin gave up
+        unsigned *flags = nullptr;   // solve: per-block ready epochs
+        void *publish = nullptr;     // panel: candidate values, rows and panel rows, two parities
+        size_t publish_bytes = 0;
+        size_t flag_count = 0;
+    };
+
+    // Factors the order x order row-major matrix in place (T = float or double); pivots[k] is the
+    // row swapped with row k at step k (absolute). `mode` is the product mode of the trailing update
+    // (kFloat64 for double; kFloat32, kSplit or kBfloat16 for float). Workspace buffers are passed in.
+    template <typename T>
+    int factor(T *matrix, int64_t order, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
+               hipStream_t stream, const char **error);
+
+    // Bytes of scratch factor() needs.
+    size_t factor_scratch(int64_t order, int mode);
+
+    // Solves L U x = P b for `width` right-hand sides (row-major order x width), in place in `x`,
+    // which holds P b on entry. `epoch` is the device's running launch count for the ready flags;
+    // each launch takes the next value, so a flag never satisfies a wait of a later launch early.
+    template <typename T>
+    int substitute(const T *factors, int64_t order, T *x, int64_t width, Workspace &ws, unsigned &epoch,
+                   hipStream_t stream, const char **error);
+
+    // Applies the row order of `pivots` to `source` (order x width) into `target`: target row k is
+    // source row rows[k], where rows is the identity permuted by the swaps pivots[0..order).
+    template <typename T, typename S>
+    hipError_t permute(const S *source, T *target, const int64_t *rows, int64_t order, int64_t width,
+                       hipStream_t stream);
+
+} // namespace scihip

@@ -3,8 +3,11 @@
 #include "compiler.hpp"
 #include "kernels.hpp"
 #include "product.hpp"
+#include "linear.hpp"
+#include "lu.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -310,6 +313,127 @@ namespace scihip {
         return SCIHIP_OK;
     }
 
+
+    // ------------------------------------------------------------------------------------ LU support
+
+    const char *spin_failure();
+
+    // Sizes the LU workspace for `order` (element bytes `element`): the barrier counter and error
+    // flag, the solve's ready flags (zeroed when first made), the panel's publications.
+    int ensure_lu(scihip_device *device, int64_t order, size_t element) {
+        Workspace &ws = device->lu;
+        if (!ws.counter) {
+            void *block = nullptr;
+            if (int rc = status(hipMalloc(&block, 256), "hipMalloc")) return rc;
+            if (int rc = status(hipMemset(block, 0, 256), "hipMemset")) return rc;
+            ws.counter = static_cast<unsigned *>(block);
+            ws.error = ws.counter + 32;
+        }
+        const size_t blocks = size_t((order + kSolveRows - 1) / kSolveRows);
+        if (blocks > ws.flag_count) {
+            if (ws.flags) {
+                (void)hipStreamSynchronize(device->stream);
+                (void)hipFree(ws.flags);
+                ws.flags = nullptr;
+            }
+            const size_t count = std::max<size_t>(blocks, 1024);
+            void *block = nullptr;
+            if (int rc = status(hipMalloc(&block, count * sizeof(unsigned)), "hipMalloc")) return rc;
+            if (int rc = status(hipMemset(block, 0, count * sizeof(unsigned)), "hipMemset")) return rc;
+            ws.flags = static_cast<unsigned *>(block);
+            ws.flag_count = count;
+        }
+        const size_t groups = size_t((order + kPanelRows - 1) / kPanelRows);
+        const size_t bytes = 2 * groups * (element + 8 + kPanel * element) + 2 * kPanel * element + 256;
+        if (bytes > ws.publish_bytes) {
+            if (ws.publish) {
+                (void)hipStreamSynchronize(device->stream);
+                (void)hipFree(ws.publish);
+                ws.publish = nullptr;
+            }
+            if (int rc = status(hipMalloc(&ws.publish, bytes), "hipMalloc")) return rc;
+            ws.publish_bytes = bytes;
+        }
+        return SCIHIP_OK;
+    }
+
+    void release_lu(scihip_device *device) {
+        Workspace &ws = device->lu;
+        if (ws.counter) (void)hipFree(ws.counter);
+        if (ws.flags) (void)hipFree(ws.flags);
+        if (ws.publish) (void)hipFree(ws.publish);
+        ws = Workspace{};
+    }
+
+    // Checks the error flag of the bounded waits after the queued work has finished.
+    int check_spins(scihip_device *device) {
+        if (int rc = status(hipStreamSynchronize(device->stream), "LU")) return rc;
+        unsigned flag = 0;
+        if (int rc = status(hipMemcpy(&flag, device->lu.error, sizeof(flag), hipMemcpyDeviceToHost), "hipMemcpy"))
+            return rc;
+        if (flag) {
+            (void)hipMemset(device->lu.error, 0, sizeof(unsigned));
+            fail(spin_failure());
+            return SCIHIP_EDEVICE;
+        }
+        return SCIHIP_OK;
+    }
+
+    // A device block from the allocator, returned when the holder goes out of scope (deferred).
+    struct Scratch {
+        scihip_device *device;
+        void *pointer = nullptr;
+        Scratch(scihip_device *d, size_t bytes, int &rc) : device(d) {
+            if (!rc) rc = d->allocator->allocate(std::max<size_t>(bytes, 16), &pointer);
+        }
+        ~Scratch() {
+            if (pointer) device->allocator->dispose(pointer);
+        }
+        template <typename T> T *as() const { return static_cast<T *>(pointer); }
+    };
+
+    // rows[i] = the original index of the row at position i after the swaps (decompose.hpp:31, 41).
+    std::vector<int64_t> rows_of(const std::vector<int64_t> &pivots) {
+        std::vector<int64_t> rows(pivots.size());
+        for (size_t i = 0; i < rows.size(); ++i) rows[i] = int64_t(i);
+        for (size_t k = 0; k < pivots.size(); ++k) std::swap(rows[k], rows[size_t(pivots[k])]);
+        return rows;
+    }
+
+    template <typename T>
+    int factor_on_device(scihip_device *device, T *matrix, int64_t order, int mode, std::vector<int64_t> &pivots) {
+        int rc = ensure_lu(device, order, sizeof(T));
+        Scratch pivot_block(device, size_t(order) * sizeof(int64_t), rc);
+        Scratch scratch(device, factor_scratch(order, mode), rc);
+        if (rc) return rc;
+        const char *message = nullptr;
+        const int e = factor<T>(matrix, order, pivot_block.as<int64_t>(), mode, device->lu, scratch.pointer,
+                                factor_scratch(order, mode), device->stream, &message);
+        if (e) {
+            fail(message ? message : hipGetErrorString(hipError_t(e)));
+            return SCIHIP_EDEVICE;
+        }
+        if ((rc = check_spins(device))) return rc;
+        pivots.resize(size_t(order));
+        return status(hipMemcpy(pivots.data(), pivot_block.pointer, size_t(order) * sizeof(int64_t),
+                                hipMemcpyDeviceToHost),
+                      "hipMemcpy");
+    }
+
+    // x = U^-1 L^-1 P source (source in S, solved in T), rows on the device.
+    template <typename T, typename S>
+    int solve_on_device(scihip_device *device, const T *factors, const int64_t *rows, const S *source, T *x,
+                        int64_t order, int64_t width) {
+        if (int rc = status(permute<T, S>(source, x, rows, order, width, device->stream), "permute")) return rc;
+        const char *message = nullptr;
+        const int e = substitute<T>(factors, order, x, width, device->lu, device->epoch, device->stream, &message);
+        if (e) {
+            fail(message ? message : hipGetErrorString(hipError_t(e)));
+            return SCIHIP_EDEVICE;
+        }
+        return SCIHIP_OK;
+    }
+
 } // namespace scihip
 
 using namespace scihip;
@@ -357,6 +481,7 @@ int scihip_close(scihip_device *device) {
         std::lock_guard<std::mutex> lock(device->guard);
         (void)hipStreamSynchronize(device->stream);
         device->compiler.reset();
+        release_lu(device);
         device->allocator.reset();
         (void)hipFree(device->partials);
         (void)hipFree(device->done);
@@ -518,6 +643,194 @@ int scihip_product(scihip_device *device, uint32_t mode, const void *left, const
     for (void *block : {a_hi, a_lo, b_hi, b_lo})
         if (block) device->allocator->dispose(block);
     return rc;
+}
+
+
+int scihip_factor(scihip_device *device, uint32_t mode, void *matrix, size_t order, int64_t *rows) {
+    if (!device || !rows || mode > SCIHIP_FACTOR_BFLOAT16) return SCIHIP_EINVAL;
+    if (order == 0) return SCIHIP_OK;
+    std::lock_guard<std::mutex> lock(device->guard);
+    const bool wide = mode == SCIHIP_FACTOR_FLOAT64;
+    if (!device->allocator->locate(matrix, order * order * (wide ? 8 : 4))) {
+        fail("the matrix is not device memory of this device");
+        return SCIHIP_DECLINED;
+    }
+    std::vector<int64_t> pivots;
+    const int product_mode = wide ? kFloat64 : mode == SCIHIP_FACTOR_FLOAT32 ? kFloat32
+                                              : mode == SCIHIP_FACTOR_SPLIT   ? kSplit
+                                                                              : kBfloat16;
+    const int rc = wide ? factor_on_device<double>(device, static_cast<double *>(matrix), int64_t(order), product_mode, pivots)
+                        : factor_on_device<float>(device, static_cast<float *>(matrix), int64_t(order), product_mode, pivots);
+    if (rc) return rc;
+    const std::vector<int64_t> order_of_rows = rows_of(pivots);
+    std::copy(order_of_rows.begin(), order_of_rows.end(), rows);
+    return SCIHIP_OK;
+}
+
+int scihip_substitute(scihip_device *device, uint32_t precision, const void *factors, const int64_t *rows,
+                      const void *right, void *solution, size_t order, size_t width) {
+    if (!device || !rows || precision > SCIHIP_WIDE) return SCIHIP_EINVAL;
+    if (order == 0 || width == 0) return SCIHIP_OK;
+    std::lock_guard<std::mutex> lock(device->guard);
+    const size_t eb = precision == SCIHIP_WIDE ? 8 : 4;
+    if (!device->allocator->locate(factors, order * order * eb) || !device->allocator->locate(right, order * width * eb) ||
+        !device->allocator->locate(solution, order * width * eb)) {
+        fail("an operand of the substitution is not device memory of this device");
+        return SCIHIP_DECLINED;
+    }
+    if (right == solution) {
+        fail("the solution must not alias the right-hand side");
+        return SCIHIP_EINVAL;
+    }
+    int rc = ensure_lu(device, int64_t(order), eb);
+    Scratch device_rows(device, order * sizeof(int64_t), rc);
+    if (rc) return rc;
+    if ((rc = status(hipMemcpyAsync(device_rows.pointer, rows, order * sizeof(int64_t), hipMemcpyHostToDevice,
+                                    device->stream),
+                     "hipMemcpyAsync")))
+        return rc;
+    if (precision == SCIHIP_WIDE)
+        rc = solve_on_device<double, double>(device, static_cast<const double *>(factors), device_rows.as<int64_t>(),
+                                             static_cast<const double *>(right), static_cast<double *>(solution),
+                                             int64_t(order), int64_t(width));
+    else
+        rc = solve_on_device<float, float>(device, static_cast<const float *>(factors), device_rows.as<int64_t>(),
+                                           static_cast<const float *>(right), static_cast<float *>(solution),
+                                           int64_t(order), int64_t(width));
+    if (rc) return rc;
+    // The host rows buffer was copied asynchronously from pageable memory: finish before returning it.
+    return check_spins(device);
+}
+
+int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, const double *right, double *solution,
+                 size_t order, size_t width, double tolerance, int32_t max_iterations, scihip_solve_stats *stats) {
+    if (!device || mode > SCIHIP_FACTOR_BFLOAT16) return SCIHIP_EINVAL;
+    scihip_solve_stats local{0, 0, 0.0, 0.0, 0.0};
+    if (!stats) stats = &local;
+    *stats = local;
+    if (order == 0 || width == 0) {
+        stats->converged = 1;
+        return SCIHIP_OK;
+    }
+    std::lock_guard<std::mutex> lock(device->guard);
+    const int64_t n = int64_t(order), w = int64_t(width);
+    if (!device->allocator->locate(matrix, order * order * 8) || !device->allocator->locate(right, order * width * 8) ||
+        !device->allocator->locate(solution, order * width * 8)) {
+        fail("an operand of the solve is not device memory of this device");
+        return SCIHIP_DECLINED;
+    }
+    const auto start = std::chrono::steady_clock::now();
+    const bool wide = mode == SCIHIP_FACTOR_FLOAT64;
+    const size_t eb = wide ? 8 : 4;
+    int rc = SCIHIP_OK;
+    Scratch factors(device, order * order * eb, rc);
+    Scratch device_rows(device, order * sizeof(int64_t), rc);
+    Scratch correction(device, order * width * eb, rc);
+    Scratch residual(device, order * width * 8, rc);
+    const int64_t kp = (n + 15) / 16 * 16;
+    Scratch packed_x(device, size_t(w) * size_t(kp) * 8, rc);
+    Scratch sums(device, order * 8, rc);
+    const int64_t chunks = column_max_chunks(n);
+    Scratch partial(device, size_t(chunks * w) * 8 * 3, rc);
+    const bool aligned = n % 16 == 0;
+    Scratch packed_a(device, aligned ? 16 : size_t(n) * size_t(kp) * 8, rc);
+    if (rc) return rc;
+    if ((rc = ensure_lu(device, n, eb))) return rc;
+
+    // Factor a copy of A in the chosen precision.
+    std::vector<int64_t> pivots;
+    if (wide) {
+        if ((rc = status(hipMemcpyAsync(factors.pointer, matrix, order * order * 8, hipMemcpyDeviceToDevice,
+                                        device->stream), "copy")))
+            return rc;
+        rc = factor_on_device<double>(device, factors.as<double>(), n, kFloat64, pivots);
+    } else {
+        if ((rc = status(convert(matrix, factors.as<float>(), n * n, device->stream), "convert"))) return rc;
+        const int product_mode = mode == SCIHIP_FACTOR_FLOAT32 ? kFloat32 : mode == SCIHIP_FACTOR_SPLIT ? kSplit : kBfloat16;
+        rc = factor_on_device<float>(device, factors.as<float>(), n, product_mode, pivots);
+    }
+    if (rc) return rc;
+    stats->factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+    const std::vector<int64_t> rows = rows_of(pivots);
+    if ((rc = status(hipMemcpy(device_rows.pointer, rows.data(), order * sizeof(int64_t), hipMemcpyHostToDevice),
+                     "hipMemcpy")))
+        return rc;
+
+    // |A|_inf, once.
+    std::vector<double> host(size_t(std::max<int64_t>(n, chunks * w * 3)));
+    if ((rc = status(row_sums(matrix, n, n, sums.as<double>(), device->stream), "row sums"))) return rc;
+    if ((rc = status(hipMemcpyAsync(host.data(), sums.pointer, order * 8, hipMemcpyDeviceToHost, device->stream), "copy")))
+        return rc;
+    if ((rc = status(hipStreamSynchronize(device->stream), "sync"))) return rc;
+    double norm = 0.0;
+    for (int64_t i = 0; i < n; ++i) norm = std::max(norm, host[size_t(i)]);
+
+    // x = solve(b)
+    auto solve_into = [&](const double *source, bool add) -> int {
+        int e;
+        if (wide) e = solve_on_device<double, double>(device, factors.as<double>(), device_rows.as<int64_t>(), source,
+                                                      correction.as<double>(), n, w);
+        else e = solve_on_device<float, double>(device, factors.as<float>(), device_rows.as<int64_t>(), source,
+                                                correction.as<float>(), n, w);
+        if (e) return e;
+        if (!add) {
+            if (wide) return status(hipMemcpyAsync(solution, correction.pointer, order * width * 8,
+                                                   hipMemcpyDeviceToDevice, device->stream), "copy");
+            return status(convert(correction.as<float>(), solution, n * w, device->stream), "convert");
+        }
+        return wide ? status(accumulate(solution, correction.as<double>(), n * w, device->stream), "accumulate")
+                    : status(accumulate(solution, correction.as<float>(), n * w, device->stream), "accumulate");
+    };
+    if ((rc = solve_into(right, false))) return rc;
+
+    const double *a_operand = matrix;
+    if (!aligned) {
+        if ((rc = status(pack(kFloat64, matrix, true, n, n, n, false, packed_a.pointer, nullptr, kp, device->stream),
+                         "pack")))
+            return rc;
+        a_operand = packed_a.as<double>();
+    }
+    for (int iteration = 0;; ++iteration) {
+        // r = b - A x on the fp64 matrix cores
+        if ((rc = status(hipMemcpyAsync(residual.pointer, right, order * width * 8, hipMemcpyDeviceToDevice,
+                                        device->stream), "copy")))
+            return rc;
+        if ((rc = status(pack(kFloat64, solution, true, w, n, w, true, packed_x.pointer, nullptr, kp, device->stream),
+                         "pack")))
+            return rc;
+        ProductArgs args{a_operand, nullptr, packed_x.pointer, nullptr, residual.pointer, n, w, kp, kp, kp, w, -1.0, 1.0};
+        if ((rc = status(product(kFloat64, args, device->stream), "residual"))) return rc;
+        double *partials = partial.as<double>();
+        if ((rc = status(column_max(residual.as<double>(), n, w, partials, device->stream), "norm"))) return rc;
+        if ((rc = status(column_max(solution, n, w, partials + chunks * w, device->stream), "norm"))) return rc;
+        if ((rc = status(column_max(right, n, w, partials + 2 * chunks * w, device->stream), "norm"))) return rc;
+        if ((rc = status(hipMemcpyAsync(host.data(), partials, size_t(chunks * w * 3) * 8, hipMemcpyDeviceToHost,
+                                        device->stream), "copy")))
+            return rc;
+        if ((rc = check_spins(device))) return rc;
+        double worst = 0.0;
+        for (int64_t c = 0; c < w; ++c) {
+            double r = 0.0, x = 0.0, b = 0.0;
+            for (int64_t k = 0; k < chunks; ++k) {
+                r = std::max(r, host[size_t(k * w + c)]);
+                x = std::max(x, host[size_t((chunks + k) * w + c)]);
+                b = std::max(b, host[size_t((2 * chunks + k) * w + c)]);
+            }
+            const double denominator = norm * x + b;
+            const double e = denominator > 0.0 ? r / denominator : r;
+            worst = std::isnan(e) || std::isnan(worst) ? std::numeric_limits<double>::quiet_NaN() : std::max(worst, e);
+        }
+        stats->backward_error = worst;
+        stats->iterations = iteration;
+        if (worst <= tolerance) {
+            stats->converged = 1;
+            break;
+        }
+        if (iteration >= max_iterations || std::isnan(worst)) break;
+        if ((rc = solve_into(residual.as<double>(), true))) return rc;
+    }
+    stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+    return SCIHIP_OK;
 }
 
 int scihip_stream(scihip_device *device, void **stream) {

@@ -1,6 +1,7 @@
 // Host side of the HIP backend: the device object behind the C ABI (include/scihip.h).
 #pragma once
 
+#include "lu.hpp"
 #include "scihip.h"
 
 #include <hip/hip_runtime.h>
@@ -50,6 +51,7 @@ namespace scihip {
     };
 
     class Compiler;
+    struct Workspace;
 
     struct Kernel; // a specialised, loaded kernel
 
@@ -68,5 +70,7 @@ struct scihip_device {
     double *result = nullptr;   // reduction result on the device
     double *staging = nullptr;  // pinned host copy of it
     int policy = 0;             // 0 automatic, 1 interpreter only, 2 specialised only
+    scihip::Workspace lu;       // LU barrier counter, error flag, solve flags, panel publications
+    unsigned epoch = 0;         // launches of the solve kernels so far (their ready-flag values)
     uint64_t interpreted = 0, specialised = 0;
 };
