@@ -155,6 +155,18 @@ def main() -> int:
         for _ in range(reps):
             sci.reduce_raw(reduce_expr, Fold.sum)
         extra["c2_reduce_sum_GBps"] = round(12 * n / ((time.perf_counter() - t0) / reps) / 1e9, 1)
+        # the same traffic with one fma per element: the streaming ceiling of this kernel shape
+        fma_expr = a * c + b
+        fma_chunk = sci.record(fma_expr)
+        device.prepare(fma_expr)
+        device.set_policy(2)
+
+        def fma_store():
+            _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(fma_chunk.struct), out.pointer, n))
+
+        _, k = time_launches(device, events, fma_store, max(args.steps // 2, 3), 2)
+        extra["fma_store_GBps"] = round(bytes_per_step / (k * 1e-3) / 1e9, 1)
+        device.set_policy(0)
         dot_expr = a * c
         device.prepare(dot_expr, reduce=True)
         t0 = time.perf_counter()

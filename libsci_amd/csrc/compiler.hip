@@ -15,18 +15,6 @@ namespace scihip {
     // device_math.hpp and interpreter.hpp, embedded at build time (build.py writes embedded.inc).
 #include "embedded.inc"
 
-    namespace {
-
-        std::string name_of(uint8_t id, const std::string *temporaries) {
-            const uint8_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3F;
-            if (id == SCIHIP_SLOT_NONE) return "i0";
-            if (tag == SCIHIP_SLOT_INPUT) return "i" + std::to_string(index);
-            if (tag == SCIHIP_SLOT_LITERAL) return "k" + std::to_string(index);
-            if (tag == SCIHIP_SLOT_TEMPORARY) return temporaries[index];
-            return temporaries[64];
-        }
-
-    } // namespace
 
     std::string Compiler::generate(const scihip_chunk &chunk, int mode, bool contract) {
         const bool single = chunk.precision == SCIHIP_SINGLE;
@@ -38,44 +26,64 @@ namespace scihip {
           << "    using namespace scihip;\n"
           << "    typedef " << type << " T;\n"
           << "    constexpr int W = " << width << ";\n"
+          << "    constexpr int U = " << kUnroll << ";\n"
           << "    typedef Lanes<T, W> L;\n"
           << "    const Program<T> &p = a.p;\n";
         for (int i = 0; i < chunk.literals; ++i)
             o << "    const L k" << i << " = splat<T, W>(p.constants[" << i << "]);\n";
         if (mode == 1) o << "    double total = identity(p.fold);\n    uint32_t poisoned = 0;\n";
+        // Each thread takes U vectors 256 apart per sweep; all of their loads are issued before any
+        // of them is computed, so U * inputs 16-byte loads are in flight per thread.
         o << "    const uint64_t steps = (p.count + W - 1) / W;\n"
-          << "    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;\n"
-          << "    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < steps; v += stride) {\n"
-          << "        const uint64_t at = v * W;\n"
-          << "        const bool full = at + W <= p.count;\n";
-        for (int i = 0; i < chunk.arguments; ++i)
-            o << "        const L i" << i << " = fetch<T, W>(p, " << i << ", at, full);\n";
-        std::string temporaries[65];
-        for (auto &t : temporaries) t = "i0";
+          << "    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * U;\n"
+          << "    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * U + threadIdx.x; base < steps; base += stride) {\n";
+        for (int u = 0; u < kUnroll; ++u)
+            o << "        const uint64_t v" << u << " = base + " << 256 * u << ";\n"
+              << "        const bool live" << u << " = v" << u << " < steps;\n"
+              << "        const uint64_t at" << u << " = v" << u << " * W;\n"
+              << "        const bool full" << u << " = at" << u << " + W <= p.count;\n";
+        for (int u = 0; u < kUnroll; ++u)
+            for (int i = 0; i < chunk.arguments; ++i)
+                o << "        L i" << i << "_" << u << "{}; if (live" << u << ") i" << i << "_" << u
+                  << " = fetch<T, W>(p, " << i << ", at" << u << ", full" << u << ");\n";
         const int steps = contract ? chunk.length - 1 : chunk.length;
-        for (int s = 0; s < steps; ++s) {
-            const scihip_instruction &in = chunk.instructions[s];
-            const std::string a = name_of(in.sources[0], temporaries);
-            const std::string b = in.sources[1] == SCIHIP_SLOT_NONE ? a : name_of(in.sources[1], temporaries);
-            const std::string c = in.sources[2] == SCIHIP_SLOT_NONE ? a : name_of(in.sources[2], temporaries);
-            const std::string target = "s" + std::to_string(s);
-            o << "        const L " << target << " = lanes<" << unsigned(in.code) << "u, T, W>(" << a << ", " << b
-              << ", " << c << ");\n";
-            if (in.target == SCIHIP_SLOT_OUTPUT) temporaries[64] = target;
-            else temporaries[in.target & 0x3F] = target;
+        for (int u = 0; u < kUnroll; ++u) {
+            std::string temporaries[65];
+            const std::string suffix = "_" + std::to_string(u);
+            auto name = [&](uint8_t id) -> std::string {
+                const uint8_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3F;
+                if (tag == SCIHIP_SLOT_INPUT) return "i" + std::to_string(index) + suffix;
+                if (tag == SCIHIP_SLOT_LITERAL) return "k" + std::to_string(index);
+                if (tag == SCIHIP_SLOT_TEMPORARY) return temporaries[index];
+                return temporaries[64];
+            };
+            o << "        if (live" << u << ") {\n";
+            for (int s = 0; s < steps; ++s) {
+                const scihip_instruction &in = chunk.instructions[s];
+                const std::string first = name(in.sources[0]);
+                const std::string second = in.sources[1] == SCIHIP_SLOT_NONE ? first : name(in.sources[1]);
+                const std::string third = in.sources[2] == SCIHIP_SLOT_NONE ? first : name(in.sources[2]);
+                const std::string target = "s" + std::to_string(s) + suffix;
+                o << "            const L " << target << " = lanes<" << unsigned(in.code) << "u, T, W>(" << first << ", "
+                  << second << ", " << third << ");\n";
+                if (in.target == SCIHIP_SLOT_OUTPUT) temporaries[64] = target;
+                else temporaries[in.target & 0x3F] = target;
+            }
+            if (mode == 0) {
+                o << "            store<T, W>(p, at" << u << ", full" << u << ", " << name(SCIHIP_SLOT_OUTPUT) << ");\n";
+            } else if (contract) {
+                const scihip_instruction &last = chunk.instructions[chunk.length - 1];
+                o << "            contract<T, W>(total, " << name(last.sources[0]) << ", " << name(last.sources[1])
+                  << ", at" << u << ", full" << u << ", p.count);\n";
+            } else {
+                o << "            consume<T, W>(p.fold, total, poisoned, " << name(chunk.result) << ", at" << u
+                  << ", full" << u << ", p.count);\n";
+            }
+            o << "        }\n";
         }
-        if (mode == 0) {
-            o << "        store<T, W>(p, at, full, " << name_of(SCIHIP_SLOT_OUTPUT, temporaries) << ");\n    }\n}\n";
-        } else if (contract) {
-            const scihip_instruction &last = chunk.instructions[chunk.length - 1];
-            o << "        contract<T, W>(total, " << name_of(last.sources[0], temporaries) << ", "
-              << name_of(last.sources[1], temporaries) << ", at, full, p.count);\n    }\n"
-              << "    finish<T>(a, total, poisoned);\n}\n";
-        } else {
-            o << "        consume<T, W>(p.fold, total, poisoned, " << name_of(chunk.result, temporaries)
-              << ", at, full, p.count);\n    }\n"
-              << "    finish<T>(a, total, poisoned);\n}\n";
-        }
+        o << "    }\n";
+        if (mode == 1) o << "    finish<T>(a, total, poisoned);\n";
+        o << "}\n";
         return o.str();
     }
 

@@ -43,6 +43,7 @@ namespace scihip {
         static std::string generate(const scihip_chunk &chunk, int mode, bool contract);
 
         static constexpr size_t limit = 1024; // distinct chunks compiled at most (vulkan/compiler.cpp:27)
+        static constexpr int kUnroll = 4;     // vectors per thread per sweep of a specialised kernel
 
     private:
         void loop();

@@ -321,8 +321,9 @@ namespace scihip {
             e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups), dim3(256), args, 0,
                                            stream);
             if (e != hipSuccess) return (int)e;
-            hipLaunchKernelGGL(swap_rows<T>, dim3((unsigned)((n - w + 255) / 256)), dim3(256), 0, stream, a, n, j0, w,
-                               (const int64_t *)pivots);
+            if (n > w)
+                hipLaunchKernelGGL(swap_rows<T>, dim3((unsigned)((n - w + 255) / 256)), dim3(256), 0, stream, a, n, j0,
+                                   w, (const int64_t *)pivots);
             const int64_t rest = n - j0 - w;
             if (rest > 0) {
                 hipLaunchKernelGGL(lower_columns<T>, dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, stream, a, n, j0, w);

@@ -9,7 +9,8 @@
 // dealt round-robin over the 8 XCDs) and neighbouring tiles share rows of A, for L2 reuse.
 //
 // MFMA 16 x 16 register layout (lane l, group g = l / 16, r = l % 16): the A operand is row r of the
-// block at k-slot g, the B operand column r at k-slot g, and accumulator element i is C[4g + i][r].
+// block at k-slot g, the B operand column r at k-slot g; accumulator element i is C[4g + i][r] for the
+// fp32-accumulating instructions and C[g + 4i][r] for v_mfma_f64_16x16x4_f64 (measured on gfx950).
 // Each lane reads 16 bytes of its row per step, so one step covers 64 bytes of k per row; for fp32
 // (fp64) those 16 bytes feed four (two) MFMAs, each over the k values at the same position of every
 // lane group's 16 bytes: a fixed permutation of k, the same for A and B, so the dot products are
@@ -191,14 +192,15 @@ namespace scihip {
                 __syncthreads();
             }
 
-            // Epilogue: accumulator element e of block (i, j) is C[row0 + 64 wm + 16 i + 4 g + e][col0 + 64 wn + 16 j + r].
+            // Epilogue: accumulator element e of block (i, j) is C[row0 + 64 wm + 16 i + 4 g + e] for fp32
+            // accumulators and C[row0 + 64 wm + 16 i + g + 4 e] for fp64 ones, column col0 + 64 wn + 16 j + r.
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const int64_t row = row0 + wm * 64 + i * 16 + g * 4 + e;
+                        const int64_t row = row0 + wm * 64 + i * 16 + (M == kFloat64 ? g + 4 * e : g * 4 + e);
                         const int64_t col = col0 + wn * 64 + j * 16 + r;
                         if (row >= p.m || col >= p.n) continue;
                         if constexpr (M == kFloat64) {

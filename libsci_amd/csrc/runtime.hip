@@ -220,13 +220,14 @@ namespace scihip {
         return SCIHIP_OK;
     }
 
-    constexpr unsigned kMaxBlocks = 4096; // partials buffer capacity
+    constexpr unsigned kMaxBlocks = 16384; // partials buffer capacity
 
-    unsigned blocks_for(scihip_device *device, size_t count, int width) {
+    // Workgroups for `count` elements at `width` lanes and `unroll` vectors per thread per sweep:
+    // up to 32 per CU, each sweeping the array grid-stride.
+    unsigned blocks_for(scihip_device *device, size_t count, int width, int unroll = 1) {
         const size_t steps = (count + width - 1) / width;
-        const size_t wanted = (steps + 255) / 256;
-        // Enough workgroups to fill every CU several times over; each sweeps the array grid-stride.
-        const size_t fill = size_t(device->units) * 8;
+        const size_t wanted = (steps + 256 * size_t(unroll) - 1) / (256 * size_t(unroll));
+        const size_t fill = size_t(device->units) * 32;
         return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>({wanted, fill, kMaxBlocks})));
     }
 
@@ -254,7 +255,8 @@ namespace scihip {
     // Runs a chunk: its specialised kernel when ready (or forced), else the interpreter.
     template <typename T>
     int launch(scihip_device *device, const scihip_chunk &chunk, Launch<T> &arguments, int registers, int mode,
-               bool contract, unsigned blocks) {
+               bool contract) {
+        const size_t count = arguments.p.count;
         const Kernel *kernel = nullptr;
         if (device->policy != 1) {
             const std::string key = key_of(chunk, mode, contract);
@@ -265,6 +267,7 @@ namespace scihip {
             }
         }
         if (kernel) {
+            const unsigned blocks = blocks_for(device, count, Traits<T>::width, Compiler::kUnroll);
             size_t size = sizeof(arguments);
             void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &arguments, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
                               HIP_LAUNCH_PARAM_END};
@@ -274,6 +277,7 @@ namespace scihip {
                           "specialised launch");
         }
         ++device->interpreted;
+        const unsigned blocks = blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width);
         return status(interpret(arguments, registers, mode, blocks, device->stream), "interpreter launch");
     }
 
@@ -291,8 +295,7 @@ namespace scihip {
             fail("a stored chunk ends in its output");
             return SCIHIP_EINVAL;
         }
-        const int width = registers == kLarge ? 1 : Traits<T>::width;
-        return launch<T>(device, chunk, arguments, registers, 0, false, blocks_for(device, count, width));
+        return launch<T>(device, chunk, arguments, registers, 0, false);
     }
 
     template <typename T>
@@ -300,10 +303,8 @@ namespace scihip {
         Launch<T> arguments;
         int registers = 0;
         if (const int rc = encode<T>(device, chunk, nullptr, count, fold, true, arguments, registers)) return rc;
-        const int width = registers == kLarge ? 1 : Traits<T>::width;
         const bool contract = contracted(chunk, fold);
-        if (const int rc = launch<T>(device, chunk, arguments, registers, 1, contract, blocks_for(device, count, width)))
-            return rc;
+        if (const int rc = launch<T>(device, chunk, arguments, registers, 1, contract)) return rc;
         if (const int rc = status(hipMemcpyAsync(device->staging, device->result, sizeof(double),
                                                  hipMemcpyDeviceToHost, device->stream),
                                   "result copy"))
@@ -456,7 +457,7 @@ int scihip_open(uint32_t ordinal, scihip_device **out) {
     if (int rc = status(hipSetDevice(device->ordinal), "hipSetDevice")) return rc;
     hipDeviceProp_t properties;
     if (int rc = status(hipGetDeviceProperties(&properties, device->ordinal), "hipGetDeviceProperties")) return rc;
-    device->name = properties.name;
+    device->name = properties.name[0] ? properties.name : properties.gcnArchName;
     device->units = properties.multiProcessorCount;
     if (int rc = status(hipStreamCreateWithFlags(&device->stream, hipStreamNonBlocking), "hipStreamCreate")) return rc;
     device->allocator = std::make_unique<Allocator>(device->stream);
