@@ -95,6 +95,68 @@ def cpu_baseline(sample: int, threads: int) -> dict:
                       f"{threads} host threads"}
 
 
+def measure_linear(sci, _abi, device, events) -> dict:
+    """8192^3 products in every mode (TFLOP/s, error against float64 on 64 sampled rows) and the
+    n = 16384, 32-right-hand-side solves: plain fp64 LU against mixed factorisations refined in fp64."""
+    import numpy as np
+
+    import oracle
+
+    out = {}
+    n = 8192
+    left = sci.uniform(n * n, device, 31, -1.0, 1.0).reshape(n, n)
+    right = sci.uniform(n * n, device, 32, -1.0, 1.0).reshape(n, n)
+    product = sci.Array(device, (n, n), "float32")
+    rows = np.arange(0, n, n // 64)
+    a_rows = oracle.uniform(n * n, 31, -1.0, 1.0).reshape(n, n)[rows].astype(np.float64)
+    b_host = oracle.uniform(n * n, 32, -1.0, 1.0).reshape(n, n).astype(np.float64)
+    exact = a_rows @ b_host
+    for name, mode in (("fp32", 0), ("bf16", 1), ("bf16x3", 2)):
+        def run():
+            _abi.check(_abi.lib().scihip_product(device.handle, mode, left.pointer, right.pointer, product.pointer,
+                                                 n, n, n))
+        _, ms = time_launches(device, events, run, 3, 1)
+        got = product.host()[rows].astype(np.float64)
+        out[f"product_8192_{name}"] = {"TFLOPs": round(2 * n ** 3 / (ms * 1e-3) / 1e12, 1), "ms": round(ms, 2),
+                                       "max_abs_error": float(np.abs(got - exact).max()),
+                                       "relative_frobenius": float(np.linalg.norm(got - exact) / np.linalg.norm(exact))}
+    del left, right, product
+    out.update(measure_solves(sci, _abi, device, 16384))
+    return out
+
+
+def measure_solves(sci, _abi, device, order, width=32, graded_cases=(False, True),
+                   modes=(("fp64", 0), ("fp32", 1), ("bf16x3", 2), ("bf16", 3))) -> dict:
+    """A X = B with A = U[-1,1] + (order/4) I (and the same rows scaled by 10^(-6 i / order)), 32
+    right-hand sides, to backward error 1e-14: plain fp64 LU against mixed factorisations + fp64 refinement."""
+    import numpy as np
+
+    import oracle
+
+    out = {}
+    for graded in graded_cases:
+        a_host = oracle.uniform(order * order, 41, -1.0, 1.0, dtype=np.float64).reshape(order, order)
+        a_host += (order / 4) * np.eye(order)
+        if graded:
+            a_host *= (10.0 ** (-6.0 * np.arange(order) / order))[:, None]
+        b_host = oracle.uniform(order * width, 42, -1.0, 1.0, dtype=np.float64).reshape(order, width)
+        a = sci.Array.from_numpy(a_host, device)
+        b = sci.Array.from_numpy(b_host, device)
+        x = sci.Array(device, (order, width), "float64")
+        del a_host
+        label = "graded" if graded else "shifted"
+        for name, mode in modes:
+            stats = _abi.SolveStats()
+            _abi.check(_abi.lib().scihip_solve(device.handle, mode, a.pointer, b.pointer, x.pointer, order, width,
+                                               1e-14, 30, ctypes.byref(stats)))
+            key = f"solve_{order}_{label}_{name}"
+            out[key] = {"ms": round(stats.solve_ms, 1), "factor_ms": round(stats.factor_ms, 1),
+                        "refinements": stats.iterations, "backward_error": stats.backward_error,
+                        "converged": bool(stats.converged)}
+            print(json.dumps({key: out[key]}), file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -103,6 +165,10 @@ def main() -> int:
     parser.add_argument("--log2n", type=int, default=28)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     parser.add_argument("--extra", action="store_true", help="also time reductions and the interpreter")
+    parser.add_argument("--solve-only", type=int, default=0, metavar="ORDER",
+                        help="only run one fp64 and one fp32-refined solve of this order (for profiling)")
+    parser.add_argument("--linear", action="store_true",
+                        help="also time the 8192^3 products and the n = 16384 LU solves (mixed vs plain)")
     args = parser.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU design is described in DESIGN.md "
@@ -112,6 +178,13 @@ def main() -> int:
     import libsci_amd as sci
     from libsci_amd import _abi
     from libsci_amd.chunk import Fold
+
+    if args.solve_only:
+        device = sci.Device(0)
+        print(json.dumps(measure_solves(sci, _abi, device, args.solve_only, graded_cases=(False,),
+                                        modes=(("fp64", 0), ("fp32", 1)))), flush=True)
+        device.close()
+        return 0
 
     n = 1 << args.log2n
     device = sci.Device(0)
@@ -174,6 +247,10 @@ def main() -> int:
             sci.reduce_raw(dot_expr, Fold.sum)
         extra["c2_dot_GBps"] = round(8 * n / ((time.perf_counter() - t0) / reps) / 1e9, 1)
 
+    linear = {}
+    if args.linear:
+        linear = measure_linear(sci, _abi, device, events)
+
     result = {
         "metric": "fused-chunk HBM GB/s (% peak); LU solve time at FP64 accuracy, mixed vs plain",
         "value": round(value, 1),
@@ -194,6 +271,8 @@ def main() -> int:
     }
     if extra:
         result["extra"] = extra
+    if linear:
+        result["linear"] = linear
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(min(n, 1 << 26), threads)

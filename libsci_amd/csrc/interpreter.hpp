@@ -278,13 +278,13 @@ namespace scihip {
         }
         __syncthreads();
         if (!last) return;
-        __threadfence();
+        __threadfence(); // acquire: every other workgroup's partial, released before its arrival, is visible
         double value = identity(fold);
         uint32_t flag = 0;
-        const volatile Partial *all = a.partials;
         for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-            value = combine(fold, value, all[b].value);
-            flag |= all[b].poisoned;
+            const Partial other = a.partials[b];
+            value = combine(fold, value, other.value);
+            flag |= other.poisoned;
         }
         const Partial folded = fold_block(fold, value, flag);
         if (threadIdx.x == 0) {

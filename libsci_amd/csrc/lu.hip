@@ -11,23 +11,26 @@ namespace scihip {
 
         constexpr unsigned kSpinLimit = 1u << 21; // a bounded wait: about two seconds
 
-        __device__ __forceinline__ unsigned acquire(unsigned *p) {
-            return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __device__ __forceinline__ unsigned relaxed(unsigned *p) {
+            return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
 
-        // Workgroup waits until *counter >= target (thread 0 spins). Gives up, and raises *error, past
-        // kSpinLimit polls or as soon as another workgroup has given up.
+        // Workgroup waits until *counter >= target (thread 0 polls). Polls are relaxed atomic loads,
+        // which read the coherent value without invalidating any cache; one acquire fence follows the
+        // wait. Gives up, and raises *error, past kSpinLimit polls or as soon as another workgroup has
+        // given up.
         __device__ __forceinline__ void wait_until(unsigned *counter, unsigned target, unsigned *error) {
             if (threadIdx.x == 0) {
                 unsigned spins = 0;
-                while (acquire(counter) < target) {
+                while (relaxed(counter) < target) {
                     if (++spins > kSpinLimit) {
                         __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
-                    if ((spins & 63u) == 0 && acquire(error)) break;
-                    __builtin_amdgcn_s_sleep(2);
+                    if ((spins & 63u) == 0 && relaxed(error)) break;
+                    __builtin_amdgcn_s_sleep(1);
                 }
+                __atomic_thread_fence(__ATOMIC_ACQUIRE); // (device scope)
             }
             __syncthreads();
         }
@@ -69,20 +72,29 @@ namespace scihip {
         }
 
         // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b owns rows
-        // [j0 + b * kPanelRows, ...). Publications, two parities: value[2][G], row[2][G],
-        // rows[2][G][kPanel], diagonal[2][kPanel].
+        // [j0 + b * kPanelRows, ...) and keeps their panel part in LDS for the whole launch, so the
+        // fences of the barriers never evict it; global memory carries only the publications, two
+        // parities: value[2][G], row[2][G], rows[2][G][kPanel], diagonal[2][kPanel].
         template <typename T>
         __global__ __launch_bounds__(256) void panel(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
                                                      T *pub_value, int64_t *pub_row, T *pub_rows, T *diagonal,
                                                      unsigned *counter, unsigned *error) {
+            constexpr int stride = kPanel + 1;
             const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
             const int64_t first = j0 + (int64_t)b * kPanelRows;
-            const int64_t last = min(n, first + (int64_t)kPanelRows);
+            const int mine = (int)min((int64_t)kPanelRows, n - first); // rows this workgroup owns
+            __shared__ T rows[kPanelRows * stride];
             __shared__ T multipliers[kPanelRows];
             __shared__ T pivot_row[kPanel];
             __shared__ T diagonal_row[kPanel];
             __shared__ T sv[4];
             __shared__ int64_t sr[4];
+
+            for (int idx = t; idx < mine * w; idx += blockDim.x) {
+                const int i = idx / w, k = idx % w;
+                rows[i * stride + k] = a[(first + i) * n + j0 + k];
+            }
+            __syncthreads();
 
             for (int c = 0; c < w; ++c) {
                 const int64_t d = j0 + c;
@@ -90,12 +102,9 @@ namespace scihip {
                 // A: this workgroup's candidate among its rows >= d, and its panel row.
                 T best = T(-1);
                 int64_t row = LLONG_MAX;
-                if (t < kPanelRows) {
-                    const int64_t r = first + t;
-                    if (r < last && r >= d) {
-                        best = __builtin_fabs(a[r * n + d]);
-                        row = r;
-                    }
+                if (t < mine && first + t >= d) {
+                    best = __builtin_fabs(rows[t * stride + c]);
+                    row = first + t;
                 }
                 block_best(best, row, sv, sr);
                 if (t == 0) {
@@ -104,9 +113,9 @@ namespace scihip {
                 }
                 if (row != LLONG_MAX)
                     for (int k = t; k < w; k += blockDim.x)
-                        pub_rows[((int64_t)parity * G + b) * kPanel + k] = a[row * n + j0 + k];
-                if (d >= first && d < last)
-                    for (int k = t; k < w; k += blockDim.x) diagonal[parity * kPanel + k] = a[d * n + j0 + k];
+                        pub_rows[((int64_t)parity * G + b) * kPanel + k] = rows[(row - first) * stride + k];
+                if (d >= first && d < first + mine)
+                    for (int k = t; k < w; k += blockDim.x) diagonal[parity * kPanel + k] = rows[(d - first) * stride + k];
                 grid_barrier(counter, (unsigned)(c + 1) * (unsigned)G, error);
 
                 // B: agree on the pivot, swap, eliminate.
@@ -121,39 +130,42 @@ namespace scihip {
                 }
                 __syncthreads();
                 if (p != d) {
-                    if (p >= first && p < last)
-                        for (int k = t; k < w; k += blockDim.x) a[p * n + j0 + k] = diagonal_row[k];
-                    if (d >= first && d < last)
-                        for (int k = t; k < w; k += blockDim.x) a[d * n + j0 + k] = pivot_row[k];
+                    if (p >= first && p < first + mine)
+                        for (int k = t; k < w; k += blockDim.x) rows[(p - first) * stride + k] = diagonal_row[k];
+                    if (d >= first && d < first + mine)
+                        for (int k = t; k < w; k += blockDim.x) rows[(d - first) * stride + k] = pivot_row[k];
                 }
                 if (b == 0 && t == 0) pivots[d] = p;
                 __syncthreads();
                 const T pivot = pivot_row[c];
                 if (pivot != T(0)) { // a zero pivot skips the column (decompose.hpp:44-45)
                     if (t < kPanelRows) {
-                        const int64_t r = first + t;
                         T l = T(0);
-                        if (r < last && r > d) {
-                            l = a[r * n + d] / pivot;
-                            a[r * n + d] = l;
+                        if (t < mine && first + t > d) {
+                            l = rows[t * stride + c] / pivot;
+                            rows[t * stride + c] = l;
                         }
                         multipliers[t] = l;
                     }
                     __syncthreads();
                     const int remaining = w - c - 1;
-                    const int64_t lo = max(first, d + 1);
-                    if (remaining > 0 && lo < last) {
-                        const int count = (int)(last - lo) * remaining;
+                    const int lo = (int)max((int64_t)0, d + 1 - first);
+                    if (remaining > 0 && lo < mine) {
+                        const int count = (mine - lo) * remaining;
                         for (int idx = t; idx < count; idx += blockDim.x) {
-                            const int64_t r = lo + idx / remaining;
+                            const int i = lo + idx / remaining;
                             const int k = c + 1 + idx % remaining;
-                            const T scale = multipliers[r - first];
-                            T &target = a[r * n + j0 + k];
-                            target = target - scale * pivot_row[k];
+                            T &target = rows[i * stride + k];
+                            target = target - multipliers[i] * pivot_row[k];
                         }
                     }
                 }
                 __syncthreads();
+            }
+
+            for (int idx = t; idx < mine * w; idx += blockDim.x) {
+                const int i = idx / w, k = idx % w;
+                a[(first + i) * n + j0 + k] = rows[i * stride + k];
             }
         }
 

@@ -222,12 +222,13 @@ namespace scihip {
 
     constexpr unsigned kMaxBlocks = 16384; // partials buffer capacity
 
-    // Workgroups for `count` elements at `width` lanes and `unroll` vectors per thread per sweep:
-    // up to 32 per CU, each sweeping the array grid-stride.
-    unsigned blocks_for(scihip_device *device, size_t count, int width, int unroll = 1) {
+    // Workgroups for `count` elements at `width` lanes and `unroll` vectors per thread per sweep,
+    // each sweeping the array grid-stride: up to 32 per CU for stores, 8 per CU for reductions (whose
+    // last workgroup folds one partial per workgroup).
+    unsigned blocks_for(scihip_device *device, size_t count, int width, int unroll, int mode) {
         const size_t steps = (count + width - 1) / width;
         const size_t wanted = (steps + 256 * size_t(unroll) - 1) / (256 * size_t(unroll));
-        const size_t fill = size_t(device->units) * 32;
+        const size_t fill = size_t(device->units) * (mode == 0 ? 32 : 8);
         return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>({wanted, fill, kMaxBlocks})));
     }
 
@@ -267,7 +268,7 @@ namespace scihip {
             }
         }
         if (kernel) {
-            const unsigned blocks = blocks_for(device, count, Traits<T>::width, Compiler::kUnroll);
+            const unsigned blocks = blocks_for(device, count, Traits<T>::width, Compiler::kUnroll, mode);
             size_t size = sizeof(arguments);
             void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &arguments, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
                               HIP_LAUNCH_PARAM_END};
@@ -277,7 +278,7 @@ namespace scihip {
                           "specialised launch");
         }
         ++device->interpreted;
-        const unsigned blocks = blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width);
+        const unsigned blocks = blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width, 1, mode);
         return status(interpret(arguments, registers, mode, blocks, device->stream), "interpreter launch");
     }
 
