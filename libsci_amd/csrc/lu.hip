@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 
 namespace scihip {
 
@@ -15,10 +18,31 @@ namespace scihip {
             return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
 
-        // Workgroup waits until *counter >= target (thread 0 polls). Polls are relaxed atomic loads,
-        // which read the coherent value without invalidating any cache; one acquire fence follows the
-        // wait. Gives up, and raises *error, past kSpinLimit polls or as soon as another workgroup has
-        // given up.
+        // Values other workgroups read inside the same kernel travel through relaxed agent-scope
+        // atomic stores and loads: those go to the device's point of coherence, so no release
+        // (L2 write-back) or acquire (L2 invalidate) fence is needed; a workgroup barrier, which
+        // waits for every wave's stores to complete, orders the data before the flag that
+        // announces it.
+        template <typename T> __device__ __forceinline__ void put(T *p, T v) {
+            if constexpr (sizeof(T) == 8)
+                __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), __builtin_bit_cast(unsigned long long, v),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                __hip_atomic_store(reinterpret_cast<unsigned *>(p), __builtin_bit_cast(unsigned, v), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template <typename T> __device__ __forceinline__ T get(const T *p) {
+            if constexpr (sizeof(T) == 8)
+                return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            else
+                return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const unsigned *>(p), __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT));
+        }
+
+        // Workgroup waits until *counter >= target (thread 0 polls with relaxed atomic loads; what the
+        // flag announces is read with get()). Gives up, and raises *error, past kSpinLimit polls or as
+        // soon as another workgroup has given up.
         __device__ __forceinline__ void wait_until(unsigned *counter, unsigned target, unsigned *error) {
             if (threadIdx.x == 0) {
                 unsigned spins = 0;
@@ -30,17 +54,8 @@ namespace scihip {
                     if ((spins & 63u) == 0 && relaxed(error)) break;
                     __builtin_amdgcn_s_sleep(1);
                 }
-                __atomic_thread_fence(__ATOMIC_ACQUIRE); // (device scope)
             }
             __syncthreads();
-        }
-
-        // Every workgroup of the grid arrives, then waits for all of them (`target` = barriers so far
-        // times the grid size, on a counter zeroed before the launch).
-        __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target, unsigned *error) {
-            __syncthreads();
-            if (threadIdx.x == 0) __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            wait_until(counter, target, error);
         }
 
         // The better of two pivot candidates: the larger |a|, and on a tie the lower row (the first
@@ -71,35 +86,47 @@ namespace scihip {
             for (int w = 1; w < (int)(blockDim.x >> 6); ++w) better(v, r, sv[w], sr[w]);
         }
 
+        constexpr int kLine = 32; // unsigned flags per 128-byte line: one line per workgroup's flag
+
         // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b owns rows
-        // [j0 + b * kPanelRows, ...) and keeps their panel part in LDS for the whole launch, so the
-        // fences of the barriers never evict it; global memory carries only the publications, two
-        // parities: value[2][G], row[2][G], rows[2][G][kPanel], diagonal[2][kPanel].
+        // [j0 + b * kPanelRows, ...) and keeps their panel part in LDS for the whole launch, so fences
+        // never evict it. Per column, every workgroup publishes its candidate (first maximum of |a|
+        // among its rows at or below the diagonal) with that row's panel values and raises its own
+        // arrival flag; every workgroup waits for all flags, folds all candidates to the same pivot,
+        // then swaps and eliminates. Publications have two parities (column c uses c & 1): a
+        // workgroup overwrites parity p only after every flag of the column in between has risen,
+        // that is after every workgroup has finished reading parity p. value[2][G], row[2][G],
+        // rows[2][G][kPanel], diagonal[2][kPanel].
+        // At the end, workgroup 0 (which owns the diagonal block: kPanelRows == kPanel) writes the
+        // inverse of the unit lower L11 to `inverse` (w x w, row stride kPanel).
         template <typename T>
         __global__ __launch_bounds__(256) void panel(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
                                                      T *pub_value, int64_t *pub_row, T *pub_rows, T *diagonal,
-                                                     unsigned *counter, unsigned *error) {
+                                                     unsigned *flags, unsigned *error, T *inverse) {
             constexpr int stride = kPanel + 1;
             const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
             const int64_t first = j0 + (int64_t)b * kPanelRows;
             const int mine = (int)min((int64_t)kPanelRows, n - first); // rows this workgroup owns
+            unsigned *arrive = flags;                                  // flags[g * kLine]
             __shared__ T rows[kPanelRows * stride];
             __shared__ T multipliers[kPanelRows];
             __shared__ T pivot_row[kPanel];
             __shared__ T diagonal_row[kPanel];
             __shared__ T sv[4];
             __shared__ int64_t sr[4];
+            __shared__ int64_t chosen;
 
-            for (int idx = t; idx < mine * w; idx += blockDim.x) {
-                const int i = idx / w, k = idx % w;
-                rows[i * stride + k] = a[(first + i) * n + j0 + k];
+            for (int i = t / kPanel; i < mine; i += 256 / kPanel) {
+                const int k = t % kPanel;
+                if (k < w) rows[i * stride + k] = a[(first + i) * n + j0 + k];
             }
             __syncthreads();
 
             for (int c = 0; c < w; ++c) {
                 const int64_t d = j0 + c;
                 const int parity = c & 1;
-                // A: this workgroup's candidate among its rows >= d, and its panel row.
+                const unsigned epoch = (unsigned)c + 1;
+                // A: this workgroup's candidate and its panel row; then arrive.
                 T best = T(-1);
                 int64_t row = LLONG_MAX;
                 if (t < mine && first + t >= d) {
@@ -108,25 +135,46 @@ namespace scihip {
                 }
                 block_best(best, row, sv, sr);
                 if (t == 0) {
-                    pub_value[parity * G + b] = best;
-                    pub_row[parity * G + b] = row;
+                    put(&pub_value[parity * G + b], best);
+                    put(&pub_row[parity * G + b], row);
                 }
                 if (row != LLONG_MAX)
                     for (int k = t; k < w; k += blockDim.x)
-                        pub_rows[((int64_t)parity * G + b) * kPanel + k] = rows[(row - first) * stride + k];
+                        put(&pub_rows[((int64_t)parity * G + b) * kPanel + k], rows[(row - first) * stride + k]);
                 if (d >= first && d < first + mine)
-                    for (int k = t; k < w; k += blockDim.x) diagonal[parity * kPanel + k] = rows[(d - first) * stride + k];
-                grid_barrier(counter, (unsigned)(c + 1) * (unsigned)G, error);
+                    for (int k = t; k < w; k += blockDim.x) put(&diagonal[parity * kPanel + k], rows[(d - first) * stride + k]);
+                __syncthreads();
+                if (t == 0) __hip_atomic_store(&arrive[b * kLine], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // Every workgroup gathers: thread g polls workgroup g's arrival, then the candidates are
+                // folded in the same order everywhere, so all workgroups pick the same pivot.
+                for (int g = t; g < G; g += blockDim.x) {
+                    unsigned spins = 0;
+                    while (__hip_atomic_load(&arrive[g * kLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+                        if (++spins > kSpinLimit) {
+                            __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        if ((spins & 63u) == 0 && relaxed(error)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                __syncthreads();
+                {
+                    T v = T(-1);
+                    int64_t p = LLONG_MAX;
+                    for (int g = t; g < G; g += blockDim.x)
+                        better(v, p, get(&pub_value[parity * G + g]), get(&pub_row[parity * G + g]));
+                    block_best(v, p, sv, sr);
+                    if (t == 0) chosen = p;
+                    __syncthreads();
+                }
 
-                // B: agree on the pivot, swap, eliminate.
-                T v = T(-1);
-                int64_t p = LLONG_MAX;
-                for (int g = t; g < G; g += blockDim.x) better(v, p, pub_value[parity * G + g], pub_row[parity * G + g]);
-                block_best(v, p, sv, sr);
+                // B: swap and eliminate.
+                const int64_t p = chosen;
                 const int owner = (int)((p - j0) / kPanelRows); // the workgroup that published row p
                 for (int k = t; k < w; k += blockDim.x) {
-                    pivot_row[k] = pub_rows[((int64_t)parity * G + owner) * kPanel + k];
-                    diagonal_row[k] = diagonal[parity * kPanel + k];
+                    pivot_row[k] = get(&pub_rows[((int64_t)parity * G + owner) * kPanel + k]);
+                    diagonal_row[k] = get(&diagonal[parity * kPanel + k]);
                 }
                 __syncthreads();
                 if (p != d) {
@@ -148,24 +196,46 @@ namespace scihip {
                         multipliers[t] = l;
                     }
                     __syncthreads();
-                    const int remaining = w - c - 1;
+                    // Thread t updates column t % kPanel of every other row from its row parity on.
+                    const int k = t % kPanel;
                     const int lo = (int)max((int64_t)0, d + 1 - first);
-                    if (remaining > 0 && lo < mine) {
-                        const int count = (mine - lo) * remaining;
-                        for (int idx = t; idx < count; idx += blockDim.x) {
-                            const int i = lo + idx / remaining;
-                            const int k = c + 1 + idx % remaining;
+                    if (k > c && k < w)
+                        for (int i = lo + t / kPanel; i < mine; i += 256 / kPanel) {
                             T &target = rows[i * stride + k];
                             target = target - multipliers[i] * pivot_row[k];
                         }
-                    }
                 }
                 __syncthreads();
             }
 
-            for (int idx = t; idx < mine * w; idx += blockDim.x) {
-                const int i = idx / w, k = idx % w;
-                a[(first + i) * n + j0 + k] = rows[i * stride + k];
+            for (int i = t / kPanel; i < mine; i += 256 / kPanel) {
+                const int k = t % kPanel;
+                if (k < w) a[(first + i) * n + j0 + k] = rows[i * stride + k];
+            }
+            if (b != 0) return;
+            // inverse(L11): thread c < w carries column c of X = L11^-1 down the rows. The panel is
+            // back in global memory, so X reuses the LDS rows; L11 is read from global memory with
+            // addresses uniform across the wave.
+            __syncthreads();
+            T *x = rows;
+            if (t < w) {
+                const int c = t;
+                for (int r = 0; r < w; ++r) {
+                    T v;
+                    if (r < c) v = T(0);
+                    else if (r == c) v = T(1);
+                    else {
+                        v = T(0);
+                        const T *lrow = a + (j0 + r) * n + j0;
+                        for (int k = c; k < r; ++k) v = v - lrow[k] * x[k * stride + c];
+                    }
+                    x[r * stride + c] = v;
+                }
+            }
+            __syncthreads();
+            for (int r = t / kPanel; r < w; r += 256 / kPanel) {
+                const int c = t % kPanel;
+                if (c < w) inverse[(int64_t)r * kPanel + c] = x[r * stride + c];
             }
         }
 
@@ -182,20 +252,6 @@ namespace scihip {
                     a[d * n + col] = a[p * n + col];
                     a[p * n + col] = x;
                 }
-            }
-        }
-
-        // U12 = L11^-1 A12: each thread carries one column of A12 down the w rows of the panel, in the
-        // reference's update order (target -= scale * top, step by step: decompose.hpp:50-55).
-        template <typename T>
-        __global__ __launch_bounds__(64) void lower_columns(T *a, int64_t n, int64_t j0, int w) {
-            const int64_t col = j0 + w + (int64_t)blockIdx.x * 64 + threadIdx.x;
-            if (col >= n) return;
-            for (int r = 1; r < w; ++r) {
-                const T *lrow = a + (j0 + r) * n + j0;
-                T s = a[(j0 + r) * n + col];
-                for (int k = 0; k < r; ++k) s = s - lrow[k] * a[(j0 + k) * n + col];
-                a[(j0 + r) * n + col] = s;
             }
         }
 
@@ -229,7 +285,7 @@ namespace scihip {
                 }
                 for (int idx = t; idx < kSolveRows * kSolveColumns; idx += blockDim.x) {
                     const int k = idx / kSolveColumns, c = idx % kSolveColumns;
-                    known[k][c] = (k < orows && c < cols) ? x[(o0 + k) * width + col0 + c] : T(0);
+                    known[k][c] = (k < orows && c < cols) ? get(&x[(o0 + k) * width + col0 + c]) : T(0);
                 }
                 __syncthreads();
                 const int c = t % kSolveColumns;
@@ -266,13 +322,10 @@ namespace scihip {
             }
             for (int idx = t; idx < rows * cols; idx += blockDim.x) {
                 const int i = idx / cols, cc = idx % cols;
-                x[(r0 + i) * width + col0 + cc] = rhs[i][cc];
+                put(&x[(r0 + i) * width + col0 + cc], rhs[i][cc]);
             }
             __syncthreads();
-            if (t == 0) {
-                __threadfence();
-                __hip_atomic_store(&flags[blk], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (t == 0) __hip_atomic_store(&flags[blk], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
 
         template <typename T, typename S>
@@ -299,10 +352,49 @@ namespace scihip {
     } // namespace
 
     size_t factor_scratch(int64_t order, int mode) {
-        const size_t eb = element_bytes(mode) * (mode == kSplit ? 2 : 1);
+        (void)mode;
         const size_t kp = (size_t)kPanel;
-        return 2 * (size_t)order * kp * eb + 256;
+        // L21 and U12 packed (up to 8 bytes, split pairs), L11^-1 and its packed copy, the panel's
+        // arrival flags.
+        return 4 * (size_t)order * kp * 8 + 2 * kp * kp * 8 + ((size_t)order / kPanelRows + 2) * 128 + 1024;
     }
+
+    namespace {
+
+        // SCIHIP_TRACE=lu: HIP events around every stage of the factorisation, totals on stderr.
+        struct StageTimer {
+            bool on = false;
+            hipStream_t stream;
+            hipEvent_t marks[2] = {nullptr, nullptr};
+            double totals[6] = {0, 0, 0, 0, 0, 0};
+            explicit StageTimer(hipStream_t s) : stream(s) {
+                const char *text = std::getenv("SCIHIP_TRACE");
+                on = text && std::strstr(text, "lu");
+                if (on) {
+                    (void)hipEventCreate(&marks[0]);
+                    (void)hipEventCreate(&marks[1]);
+                    (void)hipEventRecord(marks[0], stream);
+                }
+            }
+            void stage(int index) {
+                if (!on) return;
+                (void)hipEventRecord(marks[1], stream);
+                (void)hipEventSynchronize(marks[1]);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, marks[0], marks[1]);
+                totals[index] += ms;
+                std::swap(marks[0], marks[1]);
+            }
+            ~StageTimer() {
+                if (!on) return;
+                std::fprintf(stderr, "[scihip] lu stages ms: panel %.1f swap %.1f u12 %.1f pack %.1f update %.1f\n",
+                             totals[0], totals[1], totals[2], totals[3], totals[4]);
+                (void)hipEventDestroy(marks[0]);
+                (void)hipEventDestroy(marks[1]);
+            }
+        };
+
+    } // namespace
 
     template <typename T>
     int factor(T *a, int64_t n, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
@@ -312,43 +404,60 @@ namespace scihip {
             *error = "LU scratch too small";
             return -1;
         }
-        const size_t eb = element_bytes(mode);
-        const size_t half = (size_t)n * kPanel * eb * (mode == kSplit ? 2 : 1);
+        const bool wide = PackMode<T>::wide();
+        const int own = wide ? kFloat64 : kFloat32; // U12 = L11^-1 A12 runs in the matrix's precision
+        const size_t kp = kPanel;
         unsigned char *base = static_cast<unsigned char *>(scratch);
-        void *la_hi = base, *la_lo = base + (size_t)n * kPanel * eb;
-        void *ub_hi = base + half, *ub_lo = base + half + (size_t)n * kPanel * eb;
-        T *pub_value = static_cast<T *>(ws.publish);
+        const size_t operand = (size_t)n * kp * 8; // bytes of one packed n x kPanel operand, at most
+        void *la_hi = base, *la_lo = base + operand;
+        void *ub_hi = base + 2 * operand, *ub_lo = base + 3 * operand;
+        T *inverse = reinterpret_cast<T *>(base + 4 * operand);
+        void *inverse_packed = base + 4 * operand + kp * kp * 8;
+        unsigned *flags = reinterpret_cast<unsigned *>(base + 4 * operand + 2 * kp * kp * 8);
         const int max_groups = (int)((n + kPanelRows - 1) / kPanelRows);
-        int64_t *pub_row = reinterpret_cast<int64_t *>(pub_value + 2 * max_groups);
+        T *pub_value = static_cast<T *>(ws.publish);
+        int64_t *pub_row = reinterpret_cast<int64_t *>(pub_value + 2 * max_groups + 2);
         T *pub_rows = reinterpret_cast<T *>(pub_row + 2 * max_groups);
         T *diagonal = pub_rows + (size_t)2 * max_groups * kPanel;
 
+        StageTimer timer(stream);
         for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
             const int w = (int)std::min<int64_t>(kPanel, n - j0);
             const int groups = (int)((n - j0 + kPanelRows - 1) / kPanelRows);
-            hipError_t e = hipMemsetAsync(ws.counter, 0, sizeof(unsigned), stream);
+            hipError_t e = hipMemsetAsync(flags, 0, (size_t)(groups + 1) * kLine * sizeof(unsigned), stream);
             if (e != hipSuccess) return (int)e;
             void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &pub_value, &pub_row, &pub_rows, &diagonal,
-                            &ws.counter, &ws.error};
+                            &flags, &ws.error, &inverse};
             e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups), dim3(256), args, 0,
                                            stream);
             if (e != hipSuccess) return (int)e;
+            timer.stage(0);
             if (n > w)
                 hipLaunchKernelGGL(swap_rows<T>, dim3((unsigned)((n - w + 255) / 256)), dim3(256), 0, stream, a, n, j0,
                                    w, (const int64_t *)pivots);
+            timer.stage(1);
             const int64_t rest = n - j0 - w;
             if (rest > 0) {
-                hipLaunchKernelGGL(lower_columns<T>, dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, stream, a, n, j0, w);
-                // A22 -= L21 U12 on the matrix cores.
-                const int64_t kp = kPanel;
-                e = pack(mode, a + (j0 + w) * n + j0, PackMode<T>::wide(), rest, w, n, false, la_hi, la_lo, kp, stream);
-                if (e == hipSuccess)
-                    e = pack(mode, a + j0 * n + j0 + w, PackMode<T>::wide(), rest, w, n, true, ub_hi, ub_lo, kp, stream);
+                // U12 = L11^-1 A12: the packed copy of A12 is the product's operand, so the result can
+                // overwrite A12 in place.
+                e = pack(own, a + j0 * n + j0 + w, wide, rest, w, n, true, ub_hi, nullptr, kp, stream);
+                if (e == hipSuccess) e = pack(own, inverse, wide, w, w, kPanel, false, inverse_packed, nullptr, kp, stream);
                 if (e != hipSuccess) return (int)e;
-                ProductArgs args2{la_hi, la_lo, ub_hi, ub_lo, a + (j0 + w) * n + (j0 + w), rest, rest, kp, kp, kp, n,
-                                  -1.0, 1.0};
-                e = product(mode, args2, stream);
+                ProductArgs solve{inverse_packed, nullptr, ub_hi, nullptr, a + j0 * n + j0 + w, w, rest, (int64_t)kp,
+                                  (int64_t)kp, (int64_t)kp, n, 1.0, 0.0};
+                e = product(own, solve, stream);
                 if (e != hipSuccess) return (int)e;
+                timer.stage(2);
+                // A22 -= L21 U12 on the matrix cores, in the requested mode.
+                e = pack(mode, a + (j0 + w) * n + j0, wide, rest, w, n, false, la_hi, la_lo, kp, stream);
+                if (e == hipSuccess) e = pack(mode, a + j0 * n + j0 + w, wide, rest, w, n, true, ub_hi, ub_lo, kp, stream);
+                if (e != hipSuccess) return (int)e;
+                timer.stage(3);
+                ProductArgs update{la_hi, la_lo, ub_hi, ub_lo, a + (j0 + w) * n + (j0 + w), rest, rest, (int64_t)kp,
+                                   (int64_t)kp, (int64_t)kp, n, -1.0, 1.0};
+                e = product(mode, update, stream);
+                if (e != hipSuccess) return (int)e;
+                timer.stage(4);
             }
         }
         return hipGetLastError();

@@ -3,11 +3,12 @@
 // The reference factors and solves on the host only (include/linear/decompose.hpp:29-72,
 // include/linear/solve.hpp:25-76). The same row-major, in-place algorithm runs here blocked:
 // for each panel of kPanel columns,
-//   1. the panel is factored by one cooperative kernel: one grid barrier per column, at which every
-//      workgroup publishes its best pivot candidate (first maximum of |a|, the reference's rule) and
-//      the candidate's panel row; all workgroups then agree on the pivot, swap and eliminate;
+//   1. the panel is factored by one cooperative kernel that keeps each workgroup's rows in LDS: per
+//      column every workgroup publishes its best pivot candidate (first maximum of |a|, the
+//      reference's rule) with the candidate's panel row, workgroup 0 gathers them and announces the
+//      pivot, and all workgroups swap and eliminate; workgroup 0 then inverts the unit lower L11;
 //   2. the panel's row swaps are applied to the columns outside it;
-//   3. U12 = L11^-1 A12 by column-parallel forward substitution;
+//   3. U12 = L11^-1 A12 on the matrix cores, in the matrix's own precision;
 //   4. A22 -= L21 U12 on the matrix cores (fp64, or fp32 / bf16x3 / bf16 for a float matrix).
 // Solves run a sync-free blocked substitution: one workgroup per kSolveRows-row block, each waiting on the
 // blocks before it with a bounded spin. Every spin in this file is bounded: past kSpinLimit the
@@ -20,7 +21,7 @@
 namespace scihip {
 
     constexpr int kPanel = 128;       // panel width (columns factored per cooperative launch)
-    constexpr int kPanelRows = 64;    // rows per workgroup in the panel kernel
+    constexpr int kPanelRows = 128;   // rows per workgroup in the panel kernel (== kPanel)
     constexpr int kSolveRows = 48;    // rows per workgroup in the triangular solves
     constexpr int kSolveColumns = 32; // right-hand sides per pass of the solve kernel
 

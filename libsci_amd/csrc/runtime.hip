@@ -346,7 +346,7 @@ namespace scihip {
             ws.flag_count = count;
         }
         const size_t groups = size_t((order + kPanelRows - 1) / kPanelRows);
-        const size_t bytes = 2 * groups * (element + 8 + kPanel * element) + 2 * kPanel * element + 256;
+        const size_t bytes = (2 * groups + 2) * (element + 8 + kPanel * element) + 2 * kPanel * element + 256;
         if (bytes > ws.publish_bytes) {
             if (ws.publish) {
                 (void)hipStreamSynchronize(device->stream);
