@@ -225,6 +225,51 @@ int scihip_stream(scihip_device *device, void **stream);
 int scihip_fill_uniform(scihip_device *device, void *output, uint32_t precision, size_t count, uint64_t seed,
                         double lower, double upper, uint64_t nan_seed, double nan_fraction);
 
+/*
+ * Reverse-mode adjoint of a chunk, forward and backward fused in one pass over the data: for every
+ * input i with gradients[i] non-null, gradients[i][e] = seed[e] * d result[e] / d input_i[e], where
+ * result is the chunk's result value and seed is an upstream gradient array (null: all ones, which
+ * makes the gradients those of the loss sum(result)). With `loss` non-null, the sum of the result is
+ * also returned, contracted as scihip_reduce does (synchronous then; asynchronous otherwise). Forward
+ * values are recomputed in registers, never stored. Derivatives follow the closed forms of each code;
+ * minimum and maximum pass the adjoint to the operand they return, half to each on a tie, and
+ * |x| passes sign(x) (0 at 0). Gradient arrays may alias inputs. No reference counterpart: the
+ * reference has no differentiation; the chunk is include/core/bytecode/chunk.hpp:28-232's.
+ */
+int scihip_adjoint(scihip_device *device, const scihip_chunk *chunk, const void *seed, void *const *gradients,
+                   size_t count, double *loss);
+
+/* The adjoint program scihip_adjoint runs for a chunk: the chunk's forward instructions and the
+ * backward ones in the interpreter's register numbering (inputs, the seed last when seeded; then
+ * constants; then values), codes 37-39 being a * sign(b), minimum routing and maximum routing. */
+typedef struct scihip_adjoint_program {
+    uint32_t length;        /* instructions */
+    uint32_t arguments;     /* inputs, the seed included */
+    uint32_t literals;      /* constants */
+    uint32_t registers;     /* registers the program uses */
+    uint32_t gradients;     /* the chunk's inputs */
+    uint32_t loss;          /* 1: the loss is folded */
+    uint32_t contract;      /* 1: the loss is the sum of left * right */
+    uint32_t left, right;   /* loss registers */
+    uint32_t word[256];     /* code | target << 8 | first << 16 | second << 24 */
+    uint8_t third[256];
+    uint8_t slot[SCIHIP_CAPACITY]; /* register holding input i's gradient */
+    double constants[SCIHIP_CAPACITY + 8];
+} scihip_adjoint_program;
+
+/* Fills `program` with the adjoint program of `chunk` (seeded: an upstream gradient is given;
+ * loss: the loss is folded). Needs no device. No reference counterpart. */
+int scihip_adjoint_describe(const scihip_chunk *chunk, int seeded, int loss, scihip_adjoint_program *program);
+
+/* Adjoint of output = left * right (rows x depth times depth x columns, row-major, `mode` as for
+ * scihip_product): left_gradient = seed * right^T (rows x depth) and right_gradient = left^T * seed
+ * (depth x columns) for the upstream gradient `seed` (rows x columns), each as one matrix-core
+ * product; a null gradient is skipped. Asynchronous. For the loss sum((left * right - C)^2),
+ * seed = 2 (left * right - C), the adjoint of that chunk. No reference counterpart: the product is
+ * include/linear/product.hpp:22-65's. */
+int scihip_product_adjoint(scihip_device *device, uint32_t mode, const void *left, const void *right, const void *seed,
+                           void *left_gradient, void *right_gradient, size_t rows, size_t columns, size_t depth);
+
 #ifdef __cplusplus
 }
 #endif

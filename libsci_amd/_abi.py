@@ -42,7 +42,20 @@ SIGNATURES = {
     "scihip_solve": (_i, [_vp, _u32, _vp, _vp, _vp, _sz, _sz, _d, ctypes.c_int32, _vp]),
     "scihip_stream": (_i, [_vp, _pp]),
     "scihip_fill_uniform": (_i, [_vp, _vp, _u32, _sz, _u64, _d, _d, _u64, _d]),
+    "scihip_adjoint": (_i, [_vp, _vp, _vp, _pp, _sz, ctypes.POINTER(_d)]),
+    "scihip_adjoint_describe": (_i, [_vp, _i, _i, _vp]),
+    "scihip_product_adjoint": (_i, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _sz]),
 }
+
+
+class AdjointProgram(ctypes.Structure):
+    """scihip_adjoint_program (include/scihip.h)."""
+
+    _fields_ = [("length", ctypes.c_uint32), ("arguments", ctypes.c_uint32), ("literals", ctypes.c_uint32),
+                ("registers", ctypes.c_uint32), ("gradients", ctypes.c_uint32), ("loss", ctypes.c_uint32),
+                ("contract", ctypes.c_uint32), ("left", ctypes.c_uint32), ("right", ctypes.c_uint32),
+                ("word", ctypes.c_uint32 * 256), ("third", ctypes.c_uint8 * 256), ("slot", ctypes.c_uint8 * 64),
+                ("constants", ctypes.c_double * 72)]
 
 
 

@@ -407,6 +407,80 @@ def dot(left, right):
     return sum(_binary(Code.multiply, left, right))
 
 
+def _leaves(expression, found):
+    if isinstance(expression, Array):
+        found.append(expression)
+    elif isinstance(expression, Node):
+        for child in expression.children:
+            _leaves(child, found)
+    return found
+
+
+def gradient(expression, arrays, seed: Array | None = None, loss: bool = True):
+    """Reverse-mode adjoint of an elementwise expression (scihip_adjoint), forward and backward in one
+    pass: returns (loss, gradients), gradients[k] being d/d arrays[k] of sum(expression) or, with
+    `seed`, of sum(seed * expression) (a vector-Jacobian product). The loss is the sum of the
+    expression in double (None when `loss` is false); an array the expression does not use gets zeros.
+    """
+    device = _device_of(expression)
+    chunk = record(expression, reduce=True)
+    count = expression.size
+    inputs = [chunk.struct.inputs[i] for i in range(chunk.struct.arguments)]
+    outputs = {}
+    for array in arrays:
+        if not isinstance(array, Array) or array.device is not device:
+            raise ValueError("gradients are taken with respect to arrays on the expression's device")
+        if array.pointer in inputs and array.pointer not in outputs:
+            outputs[array.pointer] = Array(device, expression.element_shape(), expression.leaf_dtype())
+    pointers = (ctypes.c_void_p * max(1, len(inputs)))()
+    for i, address in enumerate(inputs):
+        pointers[i] = outputs[address].pointer if address in outputs else None
+    if seed is not None and (tuple(seed.shape) != tuple(expression.element_shape()) or
+                             seed.dtype != expression.leaf_dtype() or seed.device is not device):
+        raise ValueError("the seed must have the expression's shape, element type and device")
+    total = ctypes.c_double()
+    _abi.check(_abi.lib().scihip_adjoint(device.handle, ctypes.byref(chunk.struct),
+                                         seed.pointer if seed is not None else None, pointers, count,
+                                         ctypes.byref(total) if loss else None), "scihip_adjoint")
+    gradients = []
+    for array in arrays:
+        if array.pointer in outputs:
+            gradients.append(outputs[array.pointer])
+        else:
+            gradients.append(zeros(expression.element_shape(), device, expression.leaf_dtype()))
+    return (total.value if loss else None), gradients
+
+
+_PRODUCT_MODES = {"fp32": 0, "bf16": 1, "bf16x3": 2, "fp64": 3}
+
+
+def product(left: Array, right: Array, mode: str = "fp32") -> Array:
+    """left @ right on the matrix cores (scihip_product; include/linear/product.hpp:22-65)."""
+    rows, depth = left.shape
+    depth2, columns = right.shape
+    if depth != depth2 or left.device is not right.device or left.dtype != right.dtype:
+        raise ValueError("product needs conforming matrices of one element type on one device")
+    out = Array(left.device, (rows, columns), left.dtype)
+    _abi.check(_abi.lib().scihip_product(left.device.handle, _PRODUCT_MODES[mode], left.pointer, right.pointer,
+                                         out.pointer, rows, columns, depth), "scihip_product")
+    return out
+
+
+def product_adjoint(left: Array, right: Array, seed: Array, mode: str = "fp32"):
+    """(seed @ right^T, left^T @ seed): the gradients of sum(seed * (left @ right)) with respect to
+    left and right, as two matrix-core products (scihip_product_adjoint)."""
+    rows, depth = left.shape
+    _, columns = right.shape
+    if tuple(seed.shape) != (rows, columns):
+        raise ValueError("the seed must have the product's shape")
+    d_left = Array(left.device, (rows, depth), left.dtype)
+    d_right = Array(left.device, (depth, columns), left.dtype)
+    _abi.check(_abi.lib().scihip_product_adjoint(left.device.handle, _PRODUCT_MODES[mode], left.pointer,
+                                                 right.pointer, seed.pointer, d_left.pointer, d_right.pointer,
+                                                 rows, columns, depth), "scihip_product_adjoint")
+    return d_left, d_right
+
+
 def _unary(code: Code):
     def op(x):
         return Node(code, (x,))

@@ -4,7 +4,7 @@ Usage: python -m libsci_amd.build   (or libsci_amd.build.build() from Python)
 
 Every .hip file under csrc/ is compiled in parallel to an object, then linked with hipRTC into one
 shared library exporting the C ABI of include/scihip.h. The device sources used by run-time
-specialisation (device_math.hpp, interpreter.hpp) are embedded into the library as text.
+specialisation (device_math.hpp, interpreter.hpp, adjoint.hpp) are embedded into the library as text.
 """
 
 from __future__ import annotations
@@ -30,7 +30,7 @@ FLAGS = [
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-I" + BUILD,
 ]
 
-EMBEDDED = ("device_math.hpp", "interpreter.hpp")
+EMBEDDED = ("device_math.hpp", "interpreter.hpp", "adjoint.hpp")
 
 
 def _embed() -> None:

@@ -12,7 +12,7 @@
 
 namespace scihip {
 
-    // device_math.hpp and interpreter.hpp, embedded at build time (build.py writes embedded.inc).
+    // device_math.hpp, interpreter.hpp and adjoint.hpp, embedded at build time (build.py writes embedded.inc).
 #include "embedded.inc"
 
 

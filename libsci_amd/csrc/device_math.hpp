@@ -27,7 +27,10 @@ namespace scihip {
         kCopy, kNegate, kAbsolute, kRoot, kExponential, kLogarithm,
         kSin, kCos, kTan, kAsin, kAcos, kAtan, kCsc, kSec, kCot, kAcsc, kAsec, kAcot,
         kSinh, kCosh, kTanh, kAsinh, kAcosh, kAtanh, kCsch, kSech, kCoth, kAcsch, kAsech, kAcoth,
-        kAdd, kSubtract, kMultiply, kDivide, kMinimum, kMaximum, kFuse, kCodes
+        kAdd, kSubtract, kMultiply, kDivide, kMinimum, kMaximum, kFuse, kCodes,
+        // Codes that only adjoint programs use (never in a recorded chunk): a * sign(b); the share
+        // of g that flows to the first operand of minimum(x, y) or maximum(x, y), half of it on a tie.
+        kSignTimes = kCodes, kRouteMinimum, kRouteMaximum, kAdjointCodes
     };
 
     template <typename T> struct Limits;
@@ -297,6 +300,9 @@ namespace scihip {
         case kDivide: return a / b;
         case kMinimum: return lesser(a, b);
         case kMaximum: return greater(a, b);
+        case kSignTimes: return a * (b > T(0) ? one : (b < T(0) ? -one : b));
+        case kRouteMinimum: return a * (b < c ? one : (b == c ? T(0.5) : (b > c ? T(0) : one)));
+        case kRouteMaximum: return a * (b > c ? one : (b == c ? T(0.5) : (b < c ? T(0) : one)));
         default: return fused(a, b, c);
         }
     }

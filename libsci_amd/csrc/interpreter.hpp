@@ -68,6 +68,7 @@ namespace scihip {
             SCIHIP_CASE(18) SCIHIP_CASE(19) SCIHIP_CASE(20) SCIHIP_CASE(21) SCIHIP_CASE(22) SCIHIP_CASE(23)
             SCIHIP_CASE(24) SCIHIP_CASE(25) SCIHIP_CASE(26) SCIHIP_CASE(27) SCIHIP_CASE(28) SCIHIP_CASE(29)
             SCIHIP_CASE(30) SCIHIP_CASE(31) SCIHIP_CASE(32) SCIHIP_CASE(33) SCIHIP_CASE(34) SCIHIP_CASE(35)
+            SCIHIP_CASE(37) SCIHIP_CASE(38) SCIHIP_CASE(39)
         default:
             return lanes<kFuse, T, W>(a, b, c);
         }
@@ -111,9 +112,10 @@ namespace scihip {
     const File<T, W, R> regs{bank0, *reinterpret_cast<T(*)[R]>(bank1), *reinterpret_cast<T(*)[R]>(bank2),             \
                              *reinterpret_cast<T(*)[R]>(bank3)}
 
-    // W elements at `at` of input `i` (the specialised kernels' load).
-    template <typename T, int W>
-    __device__ __forceinline__ Lanes<T, W> fetch(const Program<T> &p, uint32_t i, uint64_t at, bool full) {
+    // W elements at `at` of input `i` of a program (Program or Adjoint: both have inputs, count and
+    // aligned); the specialised kernels' load.
+    template <typename T, int W, typename P>
+    __device__ __forceinline__ Lanes<T, W> fetch(const P &p, uint32_t i, uint64_t at, bool full) {
         const T *source = p.inputs[i] + at;
         Lanes<T, W> l;
         if constexpr (W * sizeof(T) == 16) {
@@ -148,9 +150,9 @@ namespace scihip {
         for (uint32_t i = 0; i < p.arguments; ++i) regs.set(i, fetch<T, W>(p, i, at, full));
     }
 
-    // Runs the first `steps` instructions on the register file.
-    template <typename T, int W, int R>
-    __device__ __forceinline__ void run(const Program<T> &p, const File<T, W, R> &regs, uint32_t steps) {
+    // Runs the first `steps` instructions of a program (Program or Adjoint) on the register file.
+    template <typename T, int W, int R, typename P>
+    __device__ __forceinline__ void run(const P &p, const File<T, W, R> &regs, uint32_t steps) {
         for (uint32_t s = 0; s < steps; ++s) {
             const uint32_t word = p.word[s];
             const uint32_t code = word & 0xFFu;
@@ -162,12 +164,13 @@ namespace scihip {
         }
     }
 
-    // Writes W elements at `at` of the output.
+    // Writes W elements at `at` of an array of `count` elements (`aligned`: 16-byte aligned base).
     template <typename T, int W>
-    __device__ __forceinline__ void store(const Program<T> &p, uint64_t at, bool full, const Lanes<T, W> &out) {
-        T *target = p.output + at;
+    __device__ __forceinline__ void store_to(T *base, uint64_t count, bool aligned, uint64_t at, bool full,
+                                             const Lanes<T, W> &out) {
+        T *target = base + at;
         if constexpr (W * sizeof(T) == 16) {
-            if (full && p.aligned) {
+            if (full && aligned) {
                 typedef typename Vector<T, W>::type V;
                 V x;
                 __builtin_memcpy(&x, &out, sizeof(x));
@@ -177,7 +180,13 @@ namespace scihip {
         }
 #pragma unroll
         for (int k = 0; k < W; ++k)
-            if (full || at + k < p.count) target[k] = out.v[k];
+            if (full || at + k < count) target[k] = out.v[k];
+    }
+
+    // Writes W elements at `at` of the output.
+    template <typename T, int W>
+    __device__ __forceinline__ void store(const Program<T> &p, uint64_t at, bool full, const Lanes<T, W> &out) {
+        store_to<T, W>(p.output, p.count, p.aligned != 0, at, full, out);
     }
 
     // A reduction partial: the sum, or the extreme of the non-NaN values, and whether a NaN was seen.
@@ -266,15 +275,14 @@ namespace scihip {
     // Ends a reduction: each workgroup leaves its Partial; the last one to finish folds all partials
     // (thread t takes partials t, t + 256, ... in order, then the workgroup folds across threads) and
     // writes the result. The grid is fixed for a given size, so the result is deterministic.
-    template <typename T>
-    __device__ __forceinline__ void finish(const Launch<T> &a, double total, uint32_t poisoned) {
-        const uint32_t fold = a.p.fold;
+    __device__ __forceinline__ void finish_fold(uint32_t fold, Partial *partials, uint32_t *done, double *result,
+                                                double total, uint32_t poisoned) {
         const Partial mine = fold_block(fold, total, poisoned);
         __shared__ bool last;
         if (threadIdx.x == 0) {
-            a.partials[blockIdx.x] = mine;
+            partials[blockIdx.x] = mine;
             __threadfence();
-            last = atomicAdd(a.done, 1u) == gridDim.x - 1;
+            last = atomicAdd(done, 1u) == gridDim.x - 1;
         }
         __syncthreads();
         if (!last) return;
@@ -282,15 +290,20 @@ namespace scihip {
         double value = identity(fold);
         uint32_t flag = 0;
         for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-            const Partial other = a.partials[b];
+            const Partial other = partials[b];
             value = combine(fold, value, other.value);
             flag |= other.poisoned;
         }
         const Partial folded = fold_block(fold, value, flag);
         if (threadIdx.x == 0) {
-            *a.result = folded.poisoned ? __builtin_nan("") : folded.value;
-            *a.done = 0;
+            *result = folded.poisoned ? __builtin_nan("") : folded.value;
+            *done = 0;
         }
+    }
+
+    template <typename T>
+    __device__ __forceinline__ void finish(const Launch<T> &a, double total, uint32_t poisoned) {
+        finish_fold(a.p.fold, a.partials, a.done, a.result, total, poisoned);
     }
 
     template <typename T, int W, int R> __global__ __launch_bounds__(256) void evaluate(const Launch<T> a) {

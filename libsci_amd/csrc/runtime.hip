@@ -220,8 +220,6 @@ namespace scihip {
         return SCIHIP_OK;
     }
 
-    constexpr unsigned kMaxBlocks = 16384; // partials buffer capacity
-
     // Workgroups for `count` elements at `width` lanes and `unroll` vectors per thread per sweep,
     // each sweeping the array grid-stride: up to 32 per CU for stores, 8 per CU for reductions (whose
     // last workgroup folds one partial per workgroup).
@@ -315,6 +313,37 @@ namespace scihip {
         return SCIHIP_OK;
     }
 
+
+    int multiply(scihip_device *device, int mode, const void *left, int64_t left_ld, bool left_transposed,
+                 const void *right, int64_t right_ld, bool right_transposed, void *output, int64_t rows,
+                 int64_t columns, int64_t depth) {
+        const int64_t tile = k_tile(mode);
+        const int64_t kp = (depth + tile - 1) / tile * tile;
+        const size_t eb = element_bytes(mode);
+        const bool split = mode == kSplit, wide = mode == kFloat64;
+        void *a_hi = nullptr, *a_lo = nullptr, *b_hi = nullptr, *b_lo = nullptr;
+        int rc = SCIHIP_OK;
+        const size_t a_bytes = std::max<size_t>(size_t(rows * kp) * eb, 16);
+        const size_t b_bytes = std::max<size_t>(size_t(columns * kp) * eb, 16);
+        if (!rc) rc = device->allocator->allocate(a_bytes, &a_hi);
+        if (!rc) rc = device->allocator->allocate(b_bytes, &b_hi);
+        if (!rc && split) rc = device->allocator->allocate(a_bytes, &a_lo);
+        if (!rc && split) rc = device->allocator->allocate(b_bytes, &b_lo);
+        if (!rc)
+            rc = status(pack(mode, left, wide, rows, depth, left_ld, left_transposed, a_hi, a_lo, kp, device->stream),
+                        "pack");
+        if (!rc)
+            rc = status(pack(mode, right, wide, columns, depth, right_ld, right_transposed, b_hi, b_lo, kp,
+                             device->stream),
+                        "pack");
+        if (!rc) {
+            ProductArgs args{a_hi, a_lo, b_hi, b_lo, output, rows, columns, kp, kp, kp, columns, 1.0, 0.0};
+            rc = status(product(mode, args, device->stream), "product");
+        }
+        for (void *block : {a_hi, a_lo, b_hi, b_lo})
+            if (block) device->allocator->dispose(block);
+        return rc;
+    }
 
     // ------------------------------------------------------------------------------------ LU support
 
@@ -620,31 +649,8 @@ int scihip_product(scihip_device *device, uint32_t mode, const void *left, const
         fail("an operand of the product is not device memory of this device");
         return SCIHIP_DECLINED;
     }
-    const int m = int(mode);
-    const int64_t tile = k_tile(m);
-    const int64_t kp = (int64_t(depth) + tile - 1) / tile * tile;
-    const size_t eb = element_bytes(m);
-    const bool split = m == kSplit;
-    // Packed operands: A as rows x kp, B transposed as columns x kp (the split form twice each).
-    void *a_hi = nullptr, *a_lo = nullptr, *b_hi = nullptr, *b_lo = nullptr;
-    int rc = SCIHIP_OK;
-    const size_t a_bytes = std::max<size_t>(rows * kp * eb, 16), b_bytes = std::max<size_t>(columns * kp * eb, 16);
-    if (!rc) rc = device->allocator->allocate(a_bytes, &a_hi);
-    if (!rc) rc = device->allocator->allocate(b_bytes, &b_hi);
-    if (!rc && split) rc = device->allocator->allocate(a_bytes, &a_lo);
-    if (!rc && split) rc = device->allocator->allocate(b_bytes, &b_lo);
-    if (!rc) rc = status(pack(m, left, wide, int64_t(rows), int64_t(depth), int64_t(depth), false, a_hi, a_lo, kp,
-                              device->stream), "pack");
-    if (!rc) rc = status(pack(m, right, wide, int64_t(columns), int64_t(depth), int64_t(columns), true, b_hi, b_lo,
-                              kp, device->stream), "pack");
-    if (!rc) {
-        ProductArgs args{a_hi, a_lo, b_hi, b_lo, output, int64_t(rows), int64_t(columns), kp,
-                         kp, kp, int64_t(columns), 1.0, 0.0};
-        rc = status(scihip::product(m, args, device->stream), "product");
-    }
-    for (void *block : {a_hi, a_lo, b_hi, b_lo})
-        if (block) device->allocator->dispose(block);
-    return rc;
+    return multiply(device, int(mode), left, int64_t(depth), false, right, int64_t(columns), true, output,
+                    int64_t(rows), int64_t(columns), int64_t(depth));
 }
 
 

@@ -22,6 +22,12 @@ namespace scihip {
     void fail(const std::string &message);
     int status(hipError_t error, const char *what);
 
+    // Chunk checks shared by the entry points (runtime.hip).
+    bool wellformed(const scihip_chunk &chunk);
+    bool contracted(const scihip_chunk &chunk, uint32_t fold);
+
+    constexpr unsigned kMaxBlocks = 16384; // reduction partials buffer capacity
+
     // Device memory with the reference allocator's contract (src/core/drivers/vulkan/allocator.cpp:16-136):
     // blocks are cached and reused when one at most 25% larger fits; `locate` finds the block holding
     // any interior pointer; a disposed block returns to the cache only once the work queued before
@@ -52,6 +58,17 @@ namespace scihip {
 
     class Compiler;
     struct Workspace;
+
+    // Workgroups for `count` elements at `width` lanes and `unroll` vectors per thread per sweep
+    // (mode 0 stores, 1 reduces).
+    unsigned blocks_for(scihip_device *device, size_t count, int width, int unroll, int mode);
+
+    // Packs both operands and multiplies on the matrix cores (product.hpp): output (rows x columns)
+    // = L * R with L rows x depth and R depth x columns. Each operand is given as a row-major source
+    // with its row stride: L itself (R^T) when not transposed, L^T (R) when transposed.
+    int multiply(scihip_device *device, int mode, const void *left, int64_t left_ld, bool left_transposed,
+                 const void *right, int64_t right_ld, bool right_transposed, void *output, int64_t rows,
+                 int64_t columns, int64_t depth);
 
     struct Kernel; // a specialised, loaded kernel
 

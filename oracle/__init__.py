@@ -23,6 +23,9 @@ Pinning status of each oracle function (what checks it besides the GPU compariso
 * LU and substitution: the reference tests' KATs (pivot rows [1, 2, 0] and x = [1, -2, 3]),
   P A = L U, and scipy's LAPACK getrf pivots (same first-maximum rule).
 * synthetic data: closed form (every value exact in float, in [lower, upper)).
+* adjoint (no reference counterpart): float64 central differences of the oracle's own forward
+  pass in wide precision for every code and the C5 loss, and hand-derived gradients at chosen
+  points (tests/test_oracle.py).
 """
 
 from __future__ import annotations
@@ -71,6 +74,7 @@ def lib():
         L.oracle_substitute.argtypes = [i, vp, vp, vp, vp, sz, sz]
         L.oracle_linspace_f32.argtypes = [ctypes.c_float, ctypes.c_float, sz, vp]
         L.oracle_fill_uniform.argtypes = [vp, i, sz, ctypes.c_uint64, d, d, ctypes.c_uint64, d]
+        L.oracle_adjoint.argtypes = [vp, sz, vp, ctypes.POINTER(vp), ctypes.POINTER(d)]
         _lib = L
     return _lib
 
@@ -183,6 +187,18 @@ def uniform(count: int, seed: int, lower: float, upper: float, dtype=np.float32,
     precision = 0 if dtype == np.float32 else 1
     lib().oracle_fill_uniform(out.ctypes.data, precision, count, seed, lower, upper, nan_seed, nan_fraction)
     return out
+
+
+def adjoint(chunk, count: int, seed=None, dtype=np.float32, loss: bool = True):
+    """Gradients of sum(result) (or sum(seed * result)) of a recorded chunk whose inputs are host
+    addresses, one array per chunk input, and the loss (None when not asked)."""
+    arguments = chunk.struct.arguments
+    gradients = [np.empty(count, dtype=dtype) for _ in range(arguments)]
+    table = (ctypes.c_void_p * max(1, arguments))(*[g.ctypes.data for g in gradients])
+    total = ctypes.c_double()
+    lib().oracle_adjoint(ctypes.addressof(chunk.struct), count, _ptr(seed), table,
+                         ctypes.byref(total) if loss else None)
+    return (total.value if loss else None), gradients
 
 
 # ---------------------------------------------------------------- the reference's own code (_ref)

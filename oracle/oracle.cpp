@@ -19,11 +19,18 @@
 //  - LU with partial pivoting and the substitutions (include/linear/decompose.hpp:29-72,
 //    include/linear/solve.hpp:25-52);
 //  - linspace (include/operations/creation.hpp:65-84).
+// And one function with no reference counterpart (the reference has no differentiation):
+//  - adjoint: the gradient of sum(result) (or sum(seed * result)) of a chunk, element by element,
+//    by the chain rule over the chunk's instructions in reverse, each local derivative its textbook
+//    closed form evaluated in double at the forward values (which are computed in the chunk's
+//    precision by the element kernels above). It shares no code with the backend's backward
+//    bytecode; central differences in the tests pin it.
 // Built with -ffp-contract=off so that every separate multiply and add rounds as written.
 
 #include "scihip.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <bit>
 #include <cmath>
@@ -602,9 +609,142 @@ namespace {
         return 1;
     }
 
+    // d y / d operand k of y = code(a, b, c), as closed forms in double at the forward values.
+    // minimum / maximum split a tie evenly between the operands; |x| has derivative sign(x), 0 at 0.
+    void partials(int code, double a, double b, double y, double d[3]) {
+        d[0] = d[1] = d[2] = 0.0;
+        switch (code) {
+        case SCIHIP_COPY: d[0] = 1.0; break;
+        case SCIHIP_NEGATE: d[0] = -1.0; break;
+        case SCIHIP_ABSOLUTE: d[0] = a > 0 ? 1.0 : a < 0 ? -1.0 : 0.0; break;
+        case SCIHIP_ROOT: d[0] = 0.5 / std::sqrt(a); break;
+        case SCIHIP_EXPONENTIAL: d[0] = std::exp(a); break;
+        case SCIHIP_LOGARITHM: d[0] = 1.0 / a; break;
+        case SCIHIP_SIN: d[0] = std::cos(a); break;
+        case SCIHIP_COS: d[0] = -std::sin(a); break;
+        case SCIHIP_TAN: d[0] = 1.0 / (std::cos(a) * std::cos(a)); break;
+        case SCIHIP_ASIN: d[0] = 1.0 / std::sqrt(1.0 - a * a); break;
+        case SCIHIP_ACOS: d[0] = -1.0 / std::sqrt(1.0 - a * a); break;
+        case SCIHIP_ATAN: d[0] = 1.0 / (1.0 + a * a); break;
+        case SCIHIP_CSC: d[0] = -std::cos(a) / (std::sin(a) * std::sin(a)); break;
+        case SCIHIP_SEC: d[0] = std::sin(a) / (std::cos(a) * std::cos(a)); break;
+        case SCIHIP_COT: d[0] = -1.0 / (std::sin(a) * std::sin(a)); break;
+        case SCIHIP_ACSC: d[0] = -1.0 / (std::fabs(a) * std::sqrt(a * a - 1.0)); break;
+        case SCIHIP_ASEC: d[0] = 1.0 / (std::fabs(a) * std::sqrt(a * a - 1.0)); break;
+        case SCIHIP_ACOT: d[0] = -1.0 / (1.0 + a * a); break;
+        case SCIHIP_SINH: d[0] = std::cosh(a); break;
+        case SCIHIP_COSH: d[0] = std::sinh(a); break;
+        case SCIHIP_TANH: d[0] = 1.0 / (std::cosh(a) * std::cosh(a)); break;
+        case SCIHIP_ASINH: d[0] = 1.0 / std::sqrt(a * a + 1.0); break;
+        case SCIHIP_ACOSH: d[0] = 1.0 / std::sqrt(a * a - 1.0); break;
+        case SCIHIP_ATANH: d[0] = 1.0 / (1.0 - a * a); break;
+        case SCIHIP_CSCH: d[0] = -std::cosh(a) / (std::sinh(a) * std::sinh(a)); break;
+        case SCIHIP_SECH: d[0] = -std::sinh(a) / (std::cosh(a) * std::cosh(a)); break;
+        case SCIHIP_COTH: d[0] = -1.0 / (std::sinh(a) * std::sinh(a)); break;
+        case SCIHIP_ACSCH: d[0] = -1.0 / (std::fabs(a) * std::sqrt(1.0 + a * a)); break;
+        case SCIHIP_ASECH: d[0] = -1.0 / (a * std::sqrt(1.0 - a * a)); break;
+        case SCIHIP_ACOTH: d[0] = 1.0 / (1.0 - a * a); break;
+        case SCIHIP_ADD: d[0] = 1.0; d[1] = 1.0; break;
+        case SCIHIP_SUBTRACT: d[0] = 1.0; d[1] = -1.0; break;
+        case SCIHIP_MULTIPLY: d[0] = b; d[1] = a; break;
+        case SCIHIP_DIVIDE: d[0] = 1.0 / b; d[1] = -a / (b * b); break;
+        case SCIHIP_MINIMUM:
+        case SCIHIP_MAXIMUM: {
+            const bool first = code == SCIHIP_MINIMUM ? a < b : a > b;
+            const bool second = code == SCIHIP_MINIMUM ? b < a : b > a;
+            d[0] = first ? 1.0 : a == b ? 0.5 : second ? 0.0 : 1.0;
+            d[1] = second ? 1.0 : a == b ? 0.5 : first ? 0.0 : 1.0;
+            break;
+        }
+        default: d[0] = b; d[1] = a; d[2] = 1.0; break; // fuse
+        }
+        (void)y;
+    }
+
+    template <typename T>
+    void adjoint(const scihip_chunk &chunk, size_t count, const T *seed, T *const *gradients, double *loss) {
+        const int length = chunk.length;
+        std::vector<T> value(size_t(length) + 1);       // forward value of each instruction
+        std::vector<double> bar(size_t(length) + 1);    // adjoint of each instruction's value
+        std::vector<double> input_bar(chunk.arguments);
+        // Which instruction (or input / constant) each operand of each instruction reads.
+        enum { kFromInput, kFromLiteral, kFromStep };
+        struct Source { int kind = kFromLiteral, index = 0; };
+        std::vector<std::array<Source, 3>> sources(static_cast<size_t>(length));
+        Source temporaries[SCIHIP_CAPACITY], output;
+        auto resolve = [&](uint8_t id) {
+            const uint8_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3F;
+            if (tag == SCIHIP_SLOT_INPUT) return Source{kFromInput, index};
+            if (tag == SCIHIP_SLOT_LITERAL) return Source{kFromLiteral, index};
+            if (tag == SCIHIP_SLOT_TEMPORARY) return temporaries[index];
+            return output;
+        };
+        for (int s = 0; s < length; ++s) {
+            const scihip_instruction &in = chunk.instructions[s];
+            for (int k = 0; k < 3; ++k)
+                sources[size_t(s)][size_t(k)] = in.sources[k] == SCIHIP_SLOT_NONE ? Source{} : resolve(in.sources[k]);
+            if (in.target == SCIHIP_SLOT_OUTPUT) output = Source{kFromStep, s};
+            else temporaries[in.target & 0x3F] = Source{kFromStep, s};
+        }
+        const Source result = resolve(chunk.result);
+        const bool contract = !seed && contracted(chunk, SCIHIP_FOLD_SUM);
+        double total = 0.0;
+        for (size_t e = 0; e < count; ++e) {
+            auto read = [&](const Source &src) -> T {
+                if (src.kind == kFromInput) return static_cast<const T *>(chunk.inputs[src.index])[e];
+                if (src.kind == kFromLiteral) return static_cast<T>(chunk.constants[src.index]);
+                return value[size_t(src.index)];
+            };
+            for (int s = 0; s < length; ++s) {
+                const auto &src = sources[size_t(s)];
+                value[size_t(s)] = apply<T>(chunk.instructions[s].code, read(src[0]), read(src[1]), read(src[2]));
+            }
+            if (contract) {
+                const auto &last = sources[size_t(length - 1)];
+                total += static_cast<double>(read(last[0])) * static_cast<double>(read(last[1]));
+            } else {
+                total += static_cast<double>(read(result));
+            }
+            std::fill(bar.begin(), bar.end(), 0.0);
+            std::fill(input_bar.begin(), input_bar.end(), 0.0);
+            const double g0 = seed ? static_cast<double>(seed[e]) : 1.0;
+            if (result.kind == kFromStep) bar[size_t(result.index)] = g0;
+            else if (result.kind == kFromInput) input_bar[size_t(result.index)] += g0;
+            for (int s = length - 1; s >= 0; --s) {
+                const double g = bar[size_t(s)];
+                if (g == 0.0) continue;
+                const auto &src = sources[size_t(s)];
+                double d[3];
+                partials(chunk.instructions[s].code, static_cast<double>(read(src[0])),
+                         static_cast<double>(read(src[1])), static_cast<double>(value[size_t(s)]), d);
+                for (int k = 0; k < 3; ++k) {
+                    if (chunk.instructions[s].sources[k] == SCIHIP_SLOT_NONE) continue;
+                    if (src[size_t(k)].kind == kFromStep) bar[size_t(src[size_t(k)].index)] += g * d[k];
+                    else if (src[size_t(k)].kind == kFromInput) input_bar[size_t(src[size_t(k)].index)] += g * d[k];
+                }
+            }
+            for (int i = 0; i < chunk.arguments; ++i)
+                if (gradients[i]) gradients[i][e] = static_cast<T>(input_bar[size_t(i)]);
+        }
+        if (loss) *loss = total;
+    }
+
 } // namespace
 
 extern "C" {
+
+// The gradient of sum(result) (seed null) or sum(seed * result) of a chunk with respect to each input
+// whose gradients[i] is non-null, and the loss sum(result) (contracted as a dot product when the
+// chunk ends in a multiply into its result and no seed is given). No reference counterpart.
+int oracle_adjoint(const scihip_chunk *chunk, size_t count, const void *seed, void *const *gradients, double *loss) {
+    if (chunk->precision == SCIHIP_SINGLE)
+        adjoint<float>(*chunk, count, static_cast<const float *>(seed), reinterpret_cast<float *const *>(gradients),
+                       loss);
+    else
+        adjoint<double>(*chunk, count, static_cast<const double *>(seed),
+                        reinterpret_cast<double *const *>(gradients), loss);
+    return 0;
+}
 
 int oracle_execute(const scihip_chunk *chunk, void *output, size_t count, int threads) {
     const size_t t = threads > 0 ? static_cast<size_t>(threads) : 1;

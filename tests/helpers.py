@@ -72,3 +72,50 @@ def round_to_float32(value: Fraction) -> np.float32:
 def fma32(a, b, c) -> np.float32:
     """fmaf(a, b, c) rounded once, from exact rationals."""
     return round_to_float32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+# ---------------------------------------------------------------- adjoint cases
+
+def _signed(rng, n, lo, hi):
+    return rng.uniform(lo, hi, n) * rng.choice([-1.0, 1.0], n)
+
+
+def adjoint_operands(code, n: int, seed: int, dtype=np.float64):
+    """Seeded operands inside the domain where `code` is differentiable and its derivative is
+    moderate, one array per operand."""
+    from libsci_amd.chunk import ARITY, Code
+
+    rng = np.random.default_rng(seed)
+    code = Code(code)
+    if code in (Code.root, Code.logarithm):
+        first = rng.uniform(0.1, 3.0, n)
+    elif code in (Code.tan, Code.sec):
+        first = rng.uniform(-1.2, 1.2, n)
+    elif code in (Code.asin, Code.acos, Code.atanh):
+        first = rng.uniform(-0.9, 0.9, n)
+    elif code in (Code.csc, Code.cot, Code.csch, Code.coth, Code.acsch):
+        first = _signed(rng, n, 0.2, 1.5)
+    elif code in (Code.acsc, Code.asec, Code.acoth):
+        first = _signed(rng, n, 1.2, 3.0)
+    elif code == Code.acosh:
+        first = rng.uniform(1.2, 3.0, n)
+    elif code == Code.asech:
+        first = rng.uniform(0.1, 0.9, n)
+    else:
+        first = rng.uniform(-2.0, 2.0, n)
+    operands = [first]
+    if ARITY[code] >= 2:
+        operands.append(_signed(rng, n, 0.5, 2.0) if code == Code.divide else rng.uniform(-2.0, 2.0, n))
+    if ARITY[code] == 3:
+        operands.append(rng.uniform(-2.0, 2.0, n))
+    return [np.ascontiguousarray(o.astype(dtype)) for o in operands]
+
+
+def c5_numpy(a, b, c, d):
+    """The differentiation workload's loss terms sinh(a*b + c) * exp(-(d*d)) and their gradients,
+    in float64 closed form (d/da = cosh(ab+c) b e, d/db = cosh(ab+c) a e, d/dc = cosh(ab+c) e,
+    d/dd = -2 d sinh(ab+c) e with e = exp(-d^2))."""
+    a, b, c, d = (np.asarray(x, dtype=np.float64) for x in (a, b, c, d))
+    z = a * b + c
+    e = np.exp(-(d * d))
+    return np.sinh(z) * e, [np.cosh(z) * b * e, np.cosh(z) * a * e, np.cosh(z) * e, -2.0 * d * np.sinh(z) * e]
