@@ -1,10 +1,16 @@
-"""Benchmark of the HIP chunk executor on one MI355X: the fused store exp(a)*0.5 + log(|b|+1)*c over
-N = 2^28 float32 elements (a, b, c seeded uniform[-4, 4], 1% of b NaN), as GB/s of algorithmic
-traffic (3 arrays read, 1 written: 16 bytes per element) and as a fraction of the 8 TB/s HBM peak.
+"""Benchmark of the HIP backend on one MI355X. Prints one JSON line.
 
-A step is one pass of the chunk over the whole arrays, inputs resident in HBM. Prints one JSON line.
+Headline (`value`): the fused chunk store exp(a)*0.5 + log(|b|+1)*c over N = 2^28 float32 elements
+(a, b, c seeded uniform[-4, 4], 1% of b NaN), as GB/s of algorithmic traffic (3 arrays read, 1
+written: 16 bytes per element) and as a fraction of the 8 TB/s HBM peak. A step is one pass of the
+chunk over the whole arrays, inputs resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+Beside it in the same line: the other half of the metric, the n = 16384 LU solve with 32
+right-hand sides at fp64 accuracy (plain fp64 LU against an fp32 factorisation refined in fp64),
+the differentiation workload (fused forward + backward of sum(sinh(a*b + c) * exp(-(d*d))) at
+N = 2^26 against its forward pass, and the product adjoint at 4096^2), and the CPU baseline.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--extra] [--linear]
 """
 
 from __future__ import annotations
@@ -97,10 +103,8 @@ def cpu_baseline(sample: int, threads: int) -> dict:
 
 def measure_linear(sci, _abi, device, events) -> dict:
     """8192^3 products in every mode (TFLOP/s, error against float64 on 64 sampled rows) and the
-    n = 16384, 32-right-hand-side solves: plain fp64 LU against mixed factorisations refined in fp64."""
+    n = 16384 solves in every mode and both matrix variants."""
     import numpy as np
-
-    import oracle
 
     out = {}
     n = 8192
@@ -108,9 +112,7 @@ def measure_linear(sci, _abi, device, events) -> dict:
     right = sci.uniform(n * n, device, 32, -1.0, 1.0).reshape(n, n)
     product = sci.Array(device, (n, n), "float32")
     rows = np.arange(0, n, n // 64)
-    a_rows = oracle.uniform(n * n, 31, -1.0, 1.0).reshape(n, n)[rows].astype(np.float64)
-    b_host = oracle.uniform(n * n, 32, -1.0, 1.0).reshape(n, n).astype(np.float64)
-    exact = a_rows @ b_host
+    exact = left.host()[rows].astype(np.float64) @ right.host().astype(np.float64)
     for name, mode in (("fp32", 0), ("bf16", 1), ("bf16x3", 2)):
         def run():
             _abi.check(_abi.lib().scihip_product(device.handle, mode, left.pointer, right.pointer, product.pointer,
@@ -125,25 +127,27 @@ def measure_linear(sci, _abi, device, events) -> dict:
     return out
 
 
-def measure_solves(sci, _abi, device, order, width=32, graded_cases=(False, True),
-                   modes=(("fp64", 0), ("fp32", 1), ("bf16x3", 2), ("bf16", 3))) -> dict:
-    """A X = B with A = U[-1,1] + (order/4) I (and the same rows scaled by 10^(-6 i / order)), 32
-    right-hand sides, to backward error 1e-14: plain fp64 LU against mixed factorisations + fp64 refinement."""
+def solve_system(sci, device, order, width, graded):
+    """A = U[-1,1] + (order/4) I, rows scaled by 10^(-6 i / order) when graded; B = U[-1,1]. Generated
+    on the device (seeded), the diagonal shift and the row scaling applied on the host."""
     import numpy as np
 
-    import oracle
+    a_host = sci.uniform(order * order, device, 41, -1.0, 1.0, dtype=np.float64).host().reshape(order, order)
+    a_host[np.diag_indices(order)] += order / 4
+    if graded:
+        a_host *= (10.0 ** (-6.0 * np.arange(order) / order))[:, None]
+    a = sci.Array.from_numpy(a_host, device)
+    b = sci.uniform(order * width, device, 42, -1.0, 1.0, dtype=np.float64).reshape(order, width)
+    return a, b
 
+
+def measure_solves(sci, _abi, device, order, width=32, graded_cases=(False, True),
+                   modes=(("fp64", 0), ("fp32", 1), ("bf16x3", 2), ("bf16", 3))) -> dict:
+    """A X = B to backward error 1e-14: plain fp64 LU against mixed factorisations + fp64 refinement."""
     out = {}
     for graded in graded_cases:
-        a_host = oracle.uniform(order * order, 41, -1.0, 1.0, dtype=np.float64).reshape(order, order)
-        a_host += (order / 4) * np.eye(order)
-        if graded:
-            a_host *= (10.0 ** (-6.0 * np.arange(order) / order))[:, None]
-        b_host = oracle.uniform(order * width, 42, -1.0, 1.0, dtype=np.float64).reshape(order, width)
-        a = sci.Array.from_numpy(a_host, device)
-        b = sci.Array.from_numpy(b_host, device)
+        a, b = solve_system(sci, device, order, width, graded)
         x = sci.Array(device, (order, width), "float64")
-        del a_host
         label = "graded" if graded else "shifted"
         for name, mode in modes:
             stats = _abi.SolveStats()
@@ -157,6 +161,85 @@ def measure_solves(sci, _abi, device, order, width=32, graded_cases=(False, True
     return out
 
 
+def measure_lu_headline(sci, _abi, device, order=16384, width=32, repeats=2) -> dict:
+    """The metric's second half: plain fp64 LU solve against the fp32 factorisation refined in fp64,
+    both to backward error 1e-14, on the well-conditioned system; best of `repeats` calls each."""
+    a, b = solve_system(sci, device, order, width, False)
+    x = sci.Array(device, (order, width), "float64")
+    out = {"order": order, "right_hand_sides": width, "tolerance": 1e-14}
+    for name, mode in (("plain_fp64", 0), ("mixed_fp32", 1)):
+        best = None
+        for _ in range(repeats):
+            stats = _abi.SolveStats()
+            _abi.check(_abi.lib().scihip_solve(device.handle, mode, a.pointer, b.pointer, x.pointer, order, width,
+                                               1e-14, 30, ctypes.byref(stats)))
+            if best is None or stats.solve_ms < best.solve_ms:
+                best = stats
+        lu_flops = 2.0 * order ** 3 / 3.0
+        out[name] = {"ms": round(best.solve_ms, 1), "factor_ms": round(best.factor_ms, 1),
+                     "factor_TFLOPs": round(lu_flops / (best.factor_ms * 1e-3) / 1e12, 1),
+                     "refinements": best.iterations, "backward_error": best.backward_error,
+                     "converged": bool(best.converged)}
+    out["mixed_speedup"] = round(out["plain_fp64"]["ms"] / out["mixed_fp32"]["ms"], 3)
+    return out
+
+
+def measure_adjoint(sci, _abi, device, events, log2n=26, steps=10) -> dict:
+    """Fused forward + backward of sum(sinh(a*b + c) * exp(-(d*d))) over N = 2^26 float32 (a..d seeded
+    uniform[-2, 2]) against the forward (the loss as one fused reduction), both specialised kernels;
+    and the product adjoint (two matrix-core products) at 4096^2 fp32."""
+    import numpy as np
+
+    n = 1 << log2n
+    operands = [sci.uniform(n, device, 40 + k, -2.0, 2.0) for k in range(4)]
+    a, b, c, d = operands
+    expression = sci.sinh(a * b + c) * sci.exponential(-(d * d))
+    chunk = sci.record(expression, reduce=True)
+    gradients = [sci.Array(device, (n,), "float32") for _ in range(4)]
+    pointers = (ctypes.c_void_p * 4)(*[g.pointer for g in gradients])
+    loss = ctypes.c_double()
+    device.prepare(expression, reduce=True)
+    device.set_policy(2)
+
+    def forward():
+        _abi.check(_abi.lib().scihip_reduce(device.handle, ctypes.byref(chunk.struct), 0, n, ctypes.byref(loss)))
+
+    def both():
+        _abi.check(_abi.lib().scihip_adjoint(device.handle, ctypes.byref(chunk.struct), None, pointers, n,
+                                             ctypes.byref(loss)))
+
+    both()  # compiles the specialised adjoint kernel (policy 2 waits for it)
+    _, forward_ms = time_launches(device, events, forward, steps, 2)
+    _, both_ms = time_launches(device, events, both, steps, 2)
+    device.set_policy(0)
+    out = {"workload": f"sum(sinh(a*b+c)*exp(-(d*d))), N=2^{log2n} float32",
+           "forward_ms": round(forward_ms, 4), "forward_GBps": round(16 * n / (forward_ms * 1e-3) / 1e9, 1),
+           "forward_backward_ms": round(both_ms, 4),
+           "forward_backward_GBps": round(32 * n / (both_ms * 1e-3) / 1e9, 1),
+           "ratio": round(both_ms / forward_ms, 3)}
+    del operands, a, b, c, d, gradients
+
+    m = 4096
+    left = sci.uniform(m * m, device, 51, -1.0, 1.0).reshape(m, m)
+    right = sci.uniform(m * m, device, 52, -1.0, 1.0).reshape(m, m)
+    seed = sci.uniform(m * m, device, 53, -1.0, 1.0).reshape(m, m)
+    d_left = sci.Array(device, (m, m), "float32")
+    d_right = sci.Array(device, (m, m), "float32")
+
+    def product_adjoint():
+        _abi.check(_abi.lib().scihip_product_adjoint(device.handle, 0, left.pointer, right.pointer, seed.pointer,
+                                                     d_left.pointer, d_right.pointer, m, m, m))
+
+    _, ms = time_launches(device, events, product_adjoint, 5, 1)
+    rows = np.arange(0, m, m // 32)
+    want = seed.host()[rows].astype(np.float64) @ right.host().astype(np.float64).T
+    got = d_left.host()[rows].astype(np.float64)
+    out["product_adjoint_4096_fp32"] = {"ms": round(ms, 3), "TFLOPs": round(2 * 2 * m ** 3 / (ms * 1e-3) / 1e12, 1),
+                                        "relative_frobenius_d_left": float(np.linalg.norm(got - want) /
+                                                                           np.linalg.norm(want))}
+    return out
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -167,8 +250,10 @@ def main() -> int:
     parser.add_argument("--extra", action="store_true", help="also time reductions and the interpreter")
     parser.add_argument("--solve-only", type=int, default=0, metavar="ORDER",
                         help="only run one fp64 and one fp32-refined solve of this order (for profiling)")
+    parser.add_argument("--no-lu", action="store_true", help="skip the n = 16384 LU solves")
+    parser.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurements")
     parser.add_argument("--linear", action="store_true",
-                        help="also time the 8192^3 products and the n = 16384 LU solves (mixed vs plain)")
+                        help="also time the 8192^3 products and the n = 16384 solves in every mode")
     args = parser.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU design is described in DESIGN.md "
@@ -247,6 +332,10 @@ def main() -> int:
             sci.reduce_raw(dot_expr, Fold.sum)
         extra["c2_dot_GBps"] = round(8 * n / ((time.perf_counter() - t0) / reps) / 1e9, 1)
 
+    device.set_policy(0)
+    lu = measure_lu_headline(sci, _abi, device) if not args.no_lu else None
+    adjoint = measure_adjoint(sci, _abi, device, events) if not args.no_adjoint else None
+
     linear = {}
     if args.linear:
         linear = measure_linear(sci, _abi, device, events)
@@ -269,6 +358,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic},
     }
+    if lu:
+        result["lu_solve"] = lu
+    if adjoint:
+        result["adjoint"] = adjoint
     if extra:
         result["extra"] = extra
     if linear:
