@@ -270,6 +270,24 @@ int scihip_adjoint_describe(const scihip_chunk *chunk, int seeded, int loss, sci
 int scihip_product_adjoint(scihip_device *device, uint32_t mode, const void *left, const void *right, const void *seed,
                            void *left_gradient, void *right_gradient, size_t rows, size_t columns, size_t depth);
 
+/*
+ * Graphs of repeated work (HIP graphs in place of the reference's recorded Vulkan command batches,
+ * src/core/drivers/vulkan/stream.cpp:25-156). Between scihip_capture_begin and scihip_capture_end
+ * the device's asynchronous calls (scihip_execute, scihip_product, scihip_adjoint without a loss,
+ * scihip_product_adjoint, scihip_copy, scihip_fill_uniform) are recorded into a graph instead of
+ * run; calls that wait for the device or return a value (synchronize, reduce, host copies, factor,
+ * substitute, solve, replay) fail with SCIHIP_EINVAL meanwhile. scihip_replay queues the whole graph
+ * on the device's stream with one launch; it reads and writes the same arrays every time, so change
+ * their contents, not the arrays. Scratch memory the recorded calls used stays with the graph until
+ * scihip_graph_release, which may be called while replays are still queued. No reference counterpart
+ * as an API.
+ */
+typedef struct scihip_graph scihip_graph;
+int scihip_capture_begin(scihip_device *device);
+int scihip_capture_end(scihip_device *device, scihip_graph **graph);
+int scihip_replay(scihip_device *device, const scihip_graph *graph);
+int scihip_graph_release(scihip_device *device, scihip_graph *graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,16 @@ class Device:
         _abi.check(_abi.lib().scihip_statistics(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return {"interpreted": a.value, "specialised": b.value, "compiled": c.value}
 
+    def begin_capture(self) -> None:
+        """Records the device's asynchronous work (stores, products, adjoints without a loss, device
+        copies) into a graph until end_capture(), instead of running it."""
+        _abi.check(_abi.lib().scihip_capture_begin(self.handle), "scihip_capture_begin")
+
+    def end_capture(self) -> "Graph":
+        graph = ctypes.c_void_p()
+        _abi.check(_abi.lib().scihip_capture_end(self.handle, ctypes.byref(graph)), "scihip_capture_end")
+        return Graph(self, graph)
+
     def close(self) -> None:
         if self.handle:
             _abi.lib().scihip_close(self.handle)
@@ -75,6 +85,28 @@ class Device:
 
     def __repr__(self) -> str:
         return f"Device({self.index}: {self.name})"
+
+
+class Graph:
+    """Captured device work, queued again as a whole by replay() (scihip_replay)."""
+
+    def __init__(self, device: Device, handle: ctypes.c_void_p):
+        self.device = device
+        self.handle = handle
+
+    def replay(self) -> None:
+        _abi.check(_abi.lib().scihip_replay(self.device.handle, self.handle), "scihip_replay")
+
+    def release(self) -> None:
+        if self.handle and self.device.handle:
+            _abi.check(_abi.lib().scihip_graph_release(self.device.handle, self.handle), "scihip_graph_release")
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------------------------------ arrays

@@ -103,13 +103,14 @@ namespace scihip {
             if (entry.second->module) (void)hipModuleUnload(entry.second->module);
     }
 
-    const Kernel *Compiler::find(const std::string &key, const std::string &source, bool wait) {
+    const Kernel *Compiler::find(const std::string &key, const std::function<std::string()> &generate, bool wait) {
         std::unique_lock<std::mutex> lock(guard);
         auto found = kernels.find(key);
         if (found == kernels.end()) {
             if (kernels.size() >= limit) return nullptr;
             kernels.emplace(key, std::make_unique<Kernel>());
             ready[key] = false;
+            const std::string source = generate();
             if (wait) {
                 Kernel *kernel = kernels[key].get();
                 lock.unlock();

@@ -12,6 +12,7 @@
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -33,8 +34,9 @@ namespace scihip {
         ~Compiler();
 
         // The kernel for `key`, or null while it is being compiled. The first call for a key queues
-        // a compile of `source`; with `wait` the call compiles in the foreground instead.
-        const Kernel *find(const std::string &key, const std::string &source, bool wait);
+        // a compile of the source `generate` returns (called only then); with `wait` the call
+        // compiles in the foreground instead.
+        const Kernel *find(const std::string &key, const std::function<std::string()> &generate, bool wait);
 
         uint64_t compiled() const { return done; }
 

@@ -579,7 +579,7 @@ namespace scihip {
 
             const Kernel *kernel = nullptr;
             if (device->policy != 1)
-                kernel = device->compiler->find(adjoint_key(plan), generate_adjoint(plan, kAdjointUnroll),
+                kernel = device->compiler->find(adjoint_key(plan), [&] { return generate_adjoint(plan, kAdjointUnroll); },
                                                 device->policy == 2);
             if (!kernel && device->policy == 2) {
                 fail("specialised adjoint kernel failed to compile");
@@ -588,12 +588,10 @@ namespace scihip {
             int rc;
             if (kernel) {
                 const unsigned blocks = blocks_for(device, count, lanes_of<T>(), kAdjointUnroll, mode);
-                size_t size = sizeof(launch);
-                void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &launch, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
-                                  HIP_LAUNCH_PARAM_END};
+                void *parameters[] = {&launch}; // copied at launch (and into a graph node under capture)
                 ++device->specialised;
-                rc = status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream, nullptr,
-                                                  config),
+                rc = status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream,
+                                                  parameters, nullptr),
                             "specialised adjoint launch");
             } else {
                 const int registers = plan.registers <= uint32_t(kAdjointSmall) ? kAdjointSmall
@@ -616,6 +614,10 @@ namespace scihip {
         }
 
         int zero(scihip_device *device, void *output, size_t bytes) {
+            if (device->capturing) { // a memset node replays wrongly: keep it out of graphs
+                fail("a degenerate product adjoint cannot be recorded into a graph");
+                return SCIHIP_EINVAL;
+            }
             return status(hipMemsetAsync(output, 0, bytes, device->stream), "hipMemsetAsync");
         }
 
@@ -639,6 +641,10 @@ int scihip_adjoint(scihip_device *device, const scihip_chunk *chunk, const void 
         return SCIHIP_OK;
     }
     std::lock_guard<std::mutex> lock(device->guard);
+    if (loss && device->capturing) {
+        fail("scihip_adjoint with a loss cannot be recorded into a graph (it returns a value)");
+        return SCIHIP_EINVAL;
+    }
     return chunk->precision == SCIHIP_SINGLE ? adjoint<float>(device, *chunk, seed, gradients, count, loss)
                                              : adjoint<double>(device, *chunk, seed, gradients, count, loss);
 }

@@ -94,6 +94,10 @@ namespace scihip {
             fail("scihip_dispose: not an allocation of this device");
             return SCIHIP_EINVAL;
         }
+        if (holding) {
+            holding->push_back(pointer);
+            return SCIHIP_OK;
+        }
         Retired r{pointer, found->second, nullptr};
         live.erase(found);
         hipError_t error = hipEventCreateWithFlags(&r.event, hipEventDisableTiming);
@@ -259,7 +263,8 @@ namespace scihip {
         const Kernel *kernel = nullptr;
         if (device->policy != 1) {
             const std::string key = key_of(chunk, mode, contract);
-            kernel = device->compiler->find(key, Compiler::generate(chunk, mode, contract), device->policy == 2);
+            kernel = device->compiler->find(key, [&] { return Compiler::generate(chunk, mode, contract); },
+                                            device->policy == 2);
             if (!kernel && device->policy == 2) {
                 fail("specialised kernel failed to compile");
                 return SCIHIP_ECOMPILE;
@@ -267,12 +272,10 @@ namespace scihip {
         }
         if (kernel) {
             const unsigned blocks = blocks_for(device, count, Traits<T>::width, Compiler::kUnroll, mode);
-            size_t size = sizeof(arguments);
-            void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &arguments, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
-                              HIP_LAUNCH_PARAM_END};
+            void *parameters[] = {&arguments}; // copied at launch (and into a graph node under capture)
             ++device->specialised;
-            return status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream, nullptr,
-                                                config),
+            return status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream,
+                                                parameters, nullptr),
                           "specialised launch");
         }
         ++device->interpreted;
@@ -343,6 +346,14 @@ namespace scihip {
         for (void *block : {a_hi, a_lo, b_hi, b_lo})
             if (block) device->allocator->dispose(block);
         return rc;
+    }
+
+    // Calls that wait for the device, return values to the host or stamp their launches cannot be
+    // recorded into a graph.
+    int refuse_while_capturing(scihip_device *device, const char *what) {
+        if (!device->capturing) return SCIHIP_OK;
+        fail(std::string(what) + " cannot be recorded into a graph (it waits for the device or returns a value)");
+        return SCIHIP_EINVAL;
     }
 
     // ------------------------------------------------------------------------------------ LU support
@@ -510,6 +521,14 @@ int scihip_close(scihip_device *device) {
     if (!device) return SCIHIP_OK;
     {
         std::lock_guard<std::mutex> lock(device->guard);
+        if (device->capturing) { // an unfinished capture is abandoned
+            hipGraph_t abandoned = nullptr;
+            (void)hipStreamEndCapture(device->stream, &abandoned);
+            if (abandoned) (void)hipGraphDestroy(abandoned);
+            device->capturing = false;
+            device->allocator->hold(nullptr);
+            for (void *block : device->held) device->allocator->dispose(block);
+        }
         (void)hipStreamSynchronize(device->stream);
         device->compiler.reset();
         release_lu(device);
@@ -528,6 +547,7 @@ const char *scihip_name(const scihip_device *device) { return device ? device->n
 int scihip_synchronize(scihip_device *device) {
     if (!device) return SCIHIP_EINVAL;
     std::lock_guard<std::mutex> lock(device->guard);
+    if (int rc = refuse_while_capturing(device, "scihip_synchronize")) return rc;
     const int rc = status(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
     device->allocator->reclaim(true);
     return rc;
@@ -559,6 +579,9 @@ static int copy(scihip_device *device, void *destination, const void *source, si
     std::lock_guard<std::mutex> lock(device->guard);
     const bool in = kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToDevice;
     const bool out = kind == hipMemcpyDeviceToHost || kind == hipMemcpyDeviceToDevice;
+    if (wait) {
+        if (int rc = refuse_while_capturing(device, "a copy to or from the host")) return rc;
+    }
     if ((in && !device->allocator->locate(destination, bytes)) || (out && !device->allocator->locate(source, bytes))) {
         fail("a copy names memory that is not a device allocation of this size");
         return SCIHIP_EINVAL;
@@ -594,6 +617,7 @@ int scihip_reduce(scihip_device *device, const scihip_chunk *chunk, uint32_t fol
         return SCIHIP_OK;
     }
     std::lock_guard<std::mutex> lock(device->guard);
+    if (int rc = refuse_while_capturing(device, "scihip_reduce")) return rc;
     return chunk->precision == SCIHIP_SINGLE ? reduce<float>(device, *chunk, fold, count, result)
                                              : reduce<double>(device, *chunk, fold, count, result);
 }
@@ -615,12 +639,14 @@ int scihip_prepare(scihip_device *device, const scihip_chunk *chunk) {
     const bool stored = chunk->result == SCIHIP_SLOT_OUTPUT;
     const int mode = stored ? 0 : 1;
     const bool contract = !stored && contracted(*chunk, SCIHIP_FOLD_SUM);
-    if (!device->compiler->find(key_of(*chunk, mode, false), Compiler::generate(*chunk, mode, false), true)) {
+    if (!device->compiler->find(key_of(*chunk, mode, false), [&] { return Compiler::generate(*chunk, mode, false); },
+                                true)) {
         fail("specialised kernel failed to compile");
         return SCIHIP_ECOMPILE;
     }
     if (contract &&
-        !device->compiler->find(key_of(*chunk, mode, true), Compiler::generate(*chunk, mode, true), true)) {
+        !device->compiler->find(key_of(*chunk, mode, true), [&] { return Compiler::generate(*chunk, mode, true); },
+                                true)) {
         fail("specialised kernel failed to compile");
         return SCIHIP_ECOMPILE;
     }
@@ -658,6 +684,7 @@ int scihip_factor(scihip_device *device, uint32_t mode, void *matrix, size_t ord
     if (!device || !rows || mode > SCIHIP_FACTOR_BFLOAT16) return SCIHIP_EINVAL;
     if (order == 0) return SCIHIP_OK;
     std::lock_guard<std::mutex> lock(device->guard);
+    if (int rc = refuse_while_capturing(device, "scihip_factor")) return rc;
     const bool wide = mode == SCIHIP_FACTOR_FLOAT64;
     if (!device->allocator->locate(matrix, order * order * (wide ? 8 : 4))) {
         fail("the matrix is not device memory of this device");
@@ -680,6 +707,7 @@ int scihip_substitute(scihip_device *device, uint32_t precision, const void *fac
     if (!device || !rows || precision > SCIHIP_WIDE) return SCIHIP_EINVAL;
     if (order == 0 || width == 0) return SCIHIP_OK;
     std::lock_guard<std::mutex> lock(device->guard);
+    if (int rc = refuse_while_capturing(device, "scihip_substitute")) return rc;
     const size_t eb = precision == SCIHIP_WIDE ? 8 : 4;
     if (!device->allocator->locate(factors, order * order * eb) || !device->allocator->locate(right, order * width * eb) ||
         !device->allocator->locate(solution, order * width * eb)) {
@@ -721,6 +749,7 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
         return SCIHIP_OK;
     }
     std::lock_guard<std::mutex> lock(device->guard);
+    if (int rc = refuse_while_capturing(device, "scihip_solve")) return rc;
     const int64_t n = int64_t(order), w = int64_t(width);
     if (!device->allocator->locate(matrix, order * order * 8) || !device->allocator->locate(right, order * width * 8) ||
         !device->allocator->locate(solution, order * width * 8)) {
@@ -859,6 +888,64 @@ int scihip_fill_uniform(scihip_device *device, void *output, uint32_t precision,
     return status(fill_uniform(output, int(precision), count, seed, lower, upper, nan_seed, nan_fraction,
                                device->stream),
                   "fill_uniform");
+}
+
+int scihip_capture_begin(scihip_device *device) {
+    if (!device) return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    if (device->capturing) {
+        fail("a graph is already being captured on this device");
+        return SCIHIP_EINVAL;
+    }
+    if (int rc = status(hipStreamBeginCapture(device->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture"))
+        return rc;
+    device->capturing = true;
+    device->held.clear();
+    device->allocator->hold(&device->held);
+    return SCIHIP_OK;
+}
+
+int scihip_capture_end(scihip_device *device, scihip_graph **graph) {
+    if (!device || !graph) return SCIHIP_EINVAL;
+    *graph = nullptr;
+    std::lock_guard<std::mutex> lock(device->guard);
+    if (!device->capturing) {
+        fail("no graph is being captured on this device");
+        return SCIHIP_EINVAL;
+    }
+    device->capturing = false;
+    device->allocator->hold(nullptr);
+    auto out = std::make_unique<scihip_graph>();
+    out->held.swap(device->held);
+    int rc = status(hipStreamEndCapture(device->stream, &out->graph), "hipStreamEndCapture");
+    if (!rc) rc = status(hipGraphInstantiate(&out->exec, out->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (rc) {
+        if (out->graph) (void)hipGraphDestroy(out->graph);
+        for (void *block : out->held) device->allocator->dispose(block);
+        return rc;
+    }
+    *graph = out.release();
+    return SCIHIP_OK;
+}
+
+int scihip_replay(scihip_device *device, const scihip_graph *graph) {
+    if (!device || !graph) return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    if (int rc = refuse_while_capturing(device, "scihip_replay")) return rc;
+    return status(hipGraphLaunch(graph->exec, device->stream), "hipGraphLaunch");
+}
+
+int scihip_graph_release(scihip_device *device, scihip_graph *graph) {
+    if (!device) return SCIHIP_EINVAL;
+    if (!graph) return SCIHIP_OK;
+    std::lock_guard<std::mutex> lock(device->guard);
+    if (int rc = refuse_while_capturing(device, "scihip_graph_release")) return rc;
+    // Replays already queued keep running; the graph's scratch is returned only after them.
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    for (void *block : graph->held) device->allocator->dispose(block);
+    delete graph;
+    return SCIHIP_OK;
 }
 
 } // extern "C"

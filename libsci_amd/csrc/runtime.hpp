@@ -38,6 +38,9 @@ namespace scihip {
         ~Allocator();
         int allocate(size_t bytes, void **pointer);
         int dispose(void *pointer);
+        // While a graph is being captured, disposed blocks go to `held` instead (the graph's kernels
+        // keep using them on every replay); null ends that.
+        void hold(std::vector<void *> *held) { holding = held; }
         bool locate(const void *pointer, size_t bytes = 1) const;
         void reclaim(bool all);
 
@@ -53,6 +56,7 @@ namespace scihip {
         std::multimap<size_t, void *> cache;     // bytes -> block
         std::vector<Retired> retired;
         size_t cached = 0;
+        std::vector<void *> *holding = nullptr;
         static constexpr size_t limit = size_t{16} << 30; // cache at most 16 GiB of 288
     };
 
@@ -90,4 +94,12 @@ struct scihip_device {
     scihip::Workspace lu;       // LU barrier counter, error flag, solve flags, panel publications
     unsigned epoch = 0;         // launches of the solve kernels so far (their ready-flag values)
     uint64_t interpreted = 0, specialised = 0;
+    bool capturing = false;     // between scihip_capture_begin and scihip_capture_end
+    std::vector<void *> held;   // scratch blocks the graph being captured uses
+};
+
+struct scihip_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> held;   // scratch blocks its kernels use, returned on release
 };
