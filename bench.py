@@ -8,7 +8,9 @@ chunk over the whole arrays, inputs resident in HBM.
 Beside it in the same line: the other half of the metric, the n = 16384 LU solve with 32
 right-hand sides at fp64 accuracy (plain fp64 LU against an fp32 factorisation refined in fp64),
 the differentiation workload (fused forward + backward of sum(sinh(a*b + c) * exp(-(d*d))) at
-N = 2^26 against its forward pass, and the product adjoint at 4096^2), and the CPU baseline.
+N = 2^26 against its forward pass, and the product adjoint at 4096^2), launch-bound chains of small
+chunks issued call by call against one graph replay, the 16384 x 16384 device transpose and axis
+folds, and the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--extra] [--linear]
 """
@@ -184,6 +186,67 @@ def measure_lu_headline(sci, _abi, device, order=16384, width=32, repeats=2) -> 
     return out
 
 
+def measure_two_dimensional(sci, _abi, device, events, n=16384, steps=10) -> dict:
+    """The 2-D variant: X (n x n float32, seeded uniform[-4, 4]) transposed on the device (8 B per
+    element), then summed along each axis (4 B per element), each as GB/s of that traffic."""
+    from libsci_amd.chunk import Fold
+
+    x = sci.uniform(n * n, device, 61, -4.0, 4.0).reshape(n, n)
+    t = sci.Array(device, (n, n), "float32")
+    out = sci.Array(device, (n,), "float32")
+
+    def transpose():
+        _abi.check(_abi.lib().scihip_transpose(device.handle, 0, x.pointer, t.pointer, n, n))
+
+    result = {"workload": f"{n} x {n} float32"}
+    _, ms = time_launches(device, events, transpose, steps, 2)
+    result["transpose_ms"] = round(ms, 4)
+    result["transpose_GBps"] = round(8 * n * n / (ms * 1e-3) / 1e9, 1)
+    chunk = sci.record(sci.copy(x), reduce=True)
+    for axis, name in ((1, "rows"), (0, "columns")):
+        for fold, label in ((Fold.sum, "sum"), (Fold.maximum, "maximum")):
+            def fold_axis():
+                _abi.check(_abi.lib().scihip_reduce_axis(device.handle, ctypes.byref(chunk.struct), int(fold), n, n,
+                                                         axis, out.pointer))
+            _, ms = time_launches(device, events, fold_axis, steps, 2)
+            result[f"{label}_{name}_GBps"] = round(4 * n * n / (ms * 1e-3) / 1e9, 1)
+    return result
+
+
+def measure_graph(sci, _abi, device, events, count=1_000_000, chain=32, steps=20) -> dict:
+    """The launch-bound case: `chain` dependent chunk stores over 1e6 float32 elements (the CPU-runnable
+    workload's size: c = a*b + 2, then alternately sin / cos of the previous result times b), issued
+    call by call against one replay of the same calls captured in a graph."""
+    a = sci.linspace(0.0, 1.0, count, device)
+    b = sci.ones(count, device)
+    buffers = [sci.Array(device, (count,), "float32") for _ in range(2)]
+    stages = [(a * b + 2.0, buffers[0])]
+    for k in range(1, chain):
+        source, target = buffers[(k - 1) % 2], buffers[k % 2]
+        stages.append(((sci.sin(source) if k % 2 else sci.cos(source)) * b, target))
+    chunks = [(sci.record(e), t) for e, t in stages]
+    for e, _ in stages:
+        device.prepare(e)
+    device.set_policy(2)
+
+    def eager():
+        for chunk, target in chunks:
+            _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), target.pointer, count))
+
+    device.begin_capture()
+    eager()
+    graph = device.end_capture()
+    eager_wall, eager_ms = time_launches(device, events, eager, steps, 3)
+    graph_wall, graph_ms = time_launches(device, events, graph.replay, steps, 3)
+    graph.release()
+    device.set_policy(0)
+    return {"workload": f"{chain} dependent chunk stores over {count} float32 elements",
+            "eager_us_per_chunk": round(eager_wall * 1e3 / chain, 2),
+            "graph_us_per_chunk": round(graph_wall * 1e3 / chain, 2),
+            "eager_device_us_per_chunk": round(eager_ms * 1e3 / chain, 2),
+            "graph_device_us_per_chunk": round(graph_ms * 1e3 / chain, 2)}
+
+
 def measure_adjoint(sci, _abi, device, events, log2n=26, steps=10) -> dict:
     """Fused forward + backward of sum(sinh(a*b + c) * exp(-(d*d))) over N = 2^26 float32 (a..d seeded
     uniform[-2, 2]) against the forward (the loss as one fused reduction), both specialised kernels;
@@ -335,6 +398,8 @@ def main() -> int:
     device.set_policy(0)
     lu = measure_lu_headline(sci, _abi, device) if not args.no_lu else None
     adjoint = measure_adjoint(sci, _abi, device, events) if not args.no_adjoint else None
+    graph = measure_graph(sci, _abi, device, events)
+    two_dimensional = measure_two_dimensional(sci, _abi, device, events)
 
     linear = {}
     if args.linear:
@@ -362,6 +427,8 @@ def main() -> int:
         result["lu_solve"] = lu
     if adjoint:
         result["adjoint"] = adjoint
+    result["graph"] = graph
+    result["two_dimensional"] = two_dimensional
     if extra:
         result["extra"] = extra
     if linear:

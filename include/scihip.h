@@ -270,6 +270,22 @@ int scihip_adjoint_describe(const scihip_chunk *chunk, int seeded, int loss, sci
 int scihip_product_adjoint(scihip_device *device, uint32_t mode, const void *left, const void *right, const void *seed,
                            void *left_gradient, void *right_gradient, size_t rows, size_t columns, size_t depth);
 
+/* output (columns x rows) = input^T (rows x columns), both row-major device memory of the element
+ * type `precision`; input and output must not overlap. Replaces the host index remap of transpose
+ * (include/operations/manipulations.hpp:36-62), which returns a host array. Asynchronous. */
+int scihip_transpose(scihip_device *device, uint32_t precision, const void *input, void *output, size_t rows,
+                     size_t columns);
+
+/* Folds the chunk's result along one axis of a rows x columns row-major matrix (every input of the
+ * chunk is such a matrix): axis 1 gives output[r] = fold of row r (rows values), axis 0 gives
+ * output[c] = fold of column c (columns values), in the element type. Each fold means what
+ * scihip_reduce means for that row or column alone, cast to the element type
+ * (include/array/expression.hpp:198): NaN anywhere gives NaN, a fold of nothing is 0 for a sum and
+ * NaN otherwise, a contracted sum folds exact products. No reference counterpart (the reference has
+ * no axis reductions, README.md:129). Asynchronous. */
+int scihip_reduce_axis(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t rows, size_t columns,
+                       uint32_t axis, void *output);
+
 /*
  * Graphs of repeated work (HIP graphs in place of the reference's recorded Vulkan command batches,
  * src/core/drivers/vulkan/stream.cpp:25-156). Between scihip_capture_begin and scihip_capture_end

@@ -9,7 +9,7 @@ reference's ``sci::Device``. This package is its Python host mirror: Chunk recor
 from .chunk import Chunk, Code, Fold, Precision, record_postfix  # noqa: F401
 from .array import (  # noqa: F401
     Array, Device, Graph, Literal, Node, assign, evaluate, record, reduce, reduce_raw,
-    sum, mean, minimum, maximum, dot, gradient, product, product_adjoint,
+    sum, mean, minimum, maximum, dot, gradient, product, product_adjoint, reduce_axis, transpose,
     copy, negate, absolute, root, exponential, logarithm, sin, cos, tan, asin, acos, atan, csc, sec, cot,
     acsc, asec, acot, sinh, cosh, tanh, asinh, acosh, atanh, csch, sech, coth, acsch, asech, acoth,
     add, subtract, multiply, divide,

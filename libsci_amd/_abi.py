@@ -45,6 +45,8 @@ SIGNATURES = {
     "scihip_adjoint": (_i, [_vp, _vp, _vp, _pp, _sz, ctypes.POINTER(_d)]),
     "scihip_adjoint_describe": (_i, [_vp, _i, _i, _vp]),
     "scihip_product_adjoint": (_i, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _sz]),
+    "scihip_transpose": (_i, [_vp, _u32, _vp, _vp, _sz, _sz]),
+    "scihip_reduce_axis": (_i, [_vp, _vp, _u32, _sz, _sz, _u32, _vp]),
     "scihip_capture_begin": (_i, [_vp]),
     "scihip_capture_end": (_i, [_vp, _pp]),
     "scihip_replay": (_i, [_vp, _vp]),

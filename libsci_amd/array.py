@@ -405,33 +405,67 @@ def reduce_raw(expression, fold: Fold) -> float:
     return result.value
 
 
-def reduce(expression, fold: Fold):
-    """Sum, minimum or maximum, cast to the element type (expression.hpp:188-202)."""
+def reduce_axis(expression, fold: Fold, axis: int) -> Array:
+    """Folds a matrix expression along one axis on the device (scihip_reduce_axis): axis 1 folds each
+    row, axis 0 each column; each fold means what reduce means for that row or column alone."""
+    shape = expression.element_shape()
+    if shape is None or len(shape) != 2 or axis not in (0, 1):
+        raise ValueError("axis reductions take a matrix expression and axis 0 or 1")
+    device = _device_of(expression)
+    chunk = record(expression, reduce=True)
+    rows, columns = shape
+    out = Array(device, (rows if axis == 1 else columns,), expression.leaf_dtype())
+    _abi.check(_abi.lib().scihip_reduce_axis(device.handle, ctypes.byref(chunk.struct), int(fold), rows, columns,
+                                             axis, out.pointer), "scihip_reduce_axis")
+    return out
+
+
+def reduce(expression, fold: Fold, axis=None):
+    """Sum, minimum or maximum, cast to the element type (expression.hpp:188-202); with an axis, a
+    device array of the per-row (axis 1) or per-column (axis 0) folds."""
+    if axis is not None:
+        return reduce_axis(expression, fold, axis)
     return expression.leaf_dtype().type(reduce_raw(expression, fold))
 
 
-def sum(expression):
-    return reduce(expression, Fold.sum)
+def sum(expression, axis=None):
+    return reduce(expression, Fold.sum, axis)
 
 
-def mean(expression):
+def mean(expression, axis=None):
     """float(sum) / float(count) in the element type (include/operations/reduce.hpp:24-28)."""
     dtype = expression.leaf_dtype().type
+    if axis is not None:
+        count = expression.element_shape()[1 - axis]
+        sums = sum(expression, axis)
+        return evaluate(sums / dtype(count)) if count else sums
     count = expression.size
     return dtype(0) if count == 0 else dtype(sum(expression) / dtype(count))
 
 
-def minimum(left, right=None):
+def minimum(left, right=None, axis=None):
     """minimum(a) reduces (reduce.hpp:36-38); minimum(a, b) is elementwise (binary.hpp:60-66)."""
     if right is None:
-        return reduce(left, Fold.minimum)
+        return reduce(left, Fold.minimum, axis)
     return _binary(Code.minimum, left, right)
 
 
-def maximum(left, right=None):
+def maximum(left, right=None, axis=None):
     if right is None:
-        return reduce(left, Fold.maximum)
+        return reduce(left, Fold.maximum, axis)
     return _binary(Code.maximum, left, right)
+
+
+def transpose(array: Array) -> Array:
+    """A new device array holding array^T (scihip_transpose; the reference's transpose,
+    include/operations/manipulations.hpp:36-62, remaps on the host)."""
+    if array.ndim != 2:
+        raise ValueError("transpose takes a matrix")
+    rows, columns = array.shape
+    out = Array(array.device, (columns, rows), array.dtype)
+    _abi.check(_abi.lib().scihip_transpose(array.device.handle, int(array.precision), array.pointer, out.pointer,
+                                           rows, columns), "scihip_transpose")
+    return out
 
 
 def dot(left, right):

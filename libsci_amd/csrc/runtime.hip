@@ -1,5 +1,6 @@
 // The C ABI (include/scihip.h): device, allocator, copies, chunk execute / reduce.
 #include "runtime.hpp"
+#include "axis.hpp"
 #include "compiler.hpp"
 #include "kernels.hpp"
 #include "product.hpp"
@@ -433,6 +434,44 @@ namespace scihip {
         }
         template <typename T> T *as() const { return static_cast<T *>(pointer); }
     };
+
+    template <typename T>
+    int reduce_axis(scihip_device *device, const scihip_chunk &chunk, uint32_t fold, size_t rows, size_t columns,
+                    uint32_t axis, T *output) {
+        const size_t outputs = axis == 1 ? rows : columns;
+        if (!device->allocator->locate(output, outputs * sizeof(T))) {
+            fail("the output of the axis reduction is not device memory of this device");
+            return SCIHIP_DECLINED;
+        }
+        if (rows == 0 || columns == 0) { // folds of nothing: 0 for a sum, NaN otherwise (device.cpp:266-268)
+            const T empty = fold == SCIHIP_FOLD_SUM ? T(0) : std::numeric_limits<T>::quiet_NaN();
+            return status(fill_value<T>(output, outputs, empty, device->stream), "fill");
+        }
+        Launch<T> launch;
+        int registers = 0;
+        if (const int rc = encode<T>(device, chunk, nullptr, rows * columns, fold, true, launch, registers)) return rc;
+        AxisLaunch<T> a;
+        a.p = launch.p;
+        a.p.aligned = a.p.aligned && columns % size_t(Traits<T>::width) == 0 ? 1u : 0u;
+        a.output = output;
+        a.rows = rows;
+        a.columns = columns;
+        a.partials = nullptr;
+        uint32_t slices = 1;
+        int rc = SCIHIP_OK;
+        std::unique_ptr<Scratch> partials;
+        if (axis == 0) {
+            const int w = registers == kLarge ? 1 : Traits<T>::width;
+            const size_t tiles = (columns + 256 * size_t(w) - 1) / (256 * size_t(w));
+            const size_t wanted = (size_t(device->units) * 8 + tiles - 1) / tiles;
+            slices = uint32_t(std::max<size_t>(1, std::min<size_t>({wanted, rows, 65535})));
+            partials = std::make_unique<Scratch>(device, size_t(slices) * columns * sizeof(Partial), rc);
+            if (rc) return rc;
+            a.partials = partials->as<Partial>();
+        }
+        ++device->interpreted;
+        return status(fold_axis<T>(a, registers, int(axis), slices, device->stream), "axis reduction");
+    }
 
     // rows[i] = the original index of the row at position i after the swaps (decompose.hpp:31, 41).
     std::vector<int64_t> rows_of(const std::vector<int64_t> &pivots) {
@@ -888,6 +927,34 @@ int scihip_fill_uniform(scihip_device *device, void *output, uint32_t precision,
     return status(fill_uniform(output, int(precision), count, seed, lower, upper, nan_seed, nan_fraction,
                                device->stream),
                   "fill_uniform");
+}
+
+int scihip_transpose(scihip_device *device, uint32_t precision, const void *input, void *output, size_t rows,
+                     size_t columns) {
+    if (!device || precision > SCIHIP_WIDE || (input == output && rows * columns > 0)) return SCIHIP_EINVAL;
+    if (rows == 0 || columns == 0) return SCIHIP_OK;
+    std::lock_guard<std::mutex> lock(device->guard);
+    const size_t bytes = rows * columns * (precision == SCIHIP_WIDE ? 8 : 4);
+    if (!device->allocator->locate(input, bytes) || !device->allocator->locate(output, bytes)) {
+        fail("an operand of the transpose is not device memory of this device");
+        return SCIHIP_DECLINED;
+    }
+    const hipError_t e = precision == SCIHIP_WIDE
+                                 ? transpose<double>(static_cast<const double *>(input), static_cast<double *>(output),
+                                                     int64_t(rows), int64_t(columns), device->stream)
+                                 : transpose<float>(static_cast<const float *>(input), static_cast<float *>(output),
+                                                    int64_t(rows), int64_t(columns), device->stream);
+    return status(e, "transpose");
+}
+
+int scihip_reduce_axis(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t rows, size_t columns,
+                       uint32_t axis, void *output) {
+    if (!device || !chunk || fold > SCIHIP_FOLD_MAXIMUM || axis > 1 || chunk->precision > SCIHIP_WIDE)
+        return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    return chunk->precision == SCIHIP_SINGLE
+                   ? reduce_axis<float>(device, *chunk, fold, rows, columns, axis, static_cast<float *>(output))
+                   : reduce_axis<double>(device, *chunk, fold, rows, columns, axis, static_cast<double *>(output));
 }
 
 int scihip_capture_begin(scihip_device *device) {
