@@ -313,6 +313,9 @@ def main() -> int:
     parser.add_argument("--extra", action="store_true", help="also time reductions and the interpreter")
     parser.add_argument("--solve-only", type=int, default=0, metavar="ORDER",
                         help="only run one fp64 and one fp32-refined solve of this order (for profiling)")
+    parser.add_argument("--store-only", action="store_true",
+                        help="only the headline store, then a plain copy chunk of the same size (8 B per element) "
+                             "that calibrates hardware byte counters; for profiling")
     parser.add_argument("--no-lu", action="store_true", help="skip the n = 16384 LU solves")
     parser.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurements")
     parser.add_argument("--linear", action="store_true",
@@ -394,6 +397,20 @@ def main() -> int:
         for _ in range(reps):
             sci.reduce_raw(dot_expr, Fold.sum)
         extra["c2_dot_GBps"] = round(8 * n / ((time.perf_counter() - t0) / reps) / 1e9, 1)
+
+    if args.store_only:
+        copy_chunk = sci.record(sci.copy(a))
+        device.prepare(sci.copy(a))
+
+        def copy():
+            _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(copy_chunk.struct), out.pointer, n))
+
+        _, copy_ms = time_launches(device, events, copy, args.steps, args.warmup)
+        print(json.dumps({"store_GBps": round(achieved / 1e9, 1), "store_ms": round(kernel_ms, 4),
+                          "copy_GBps": round(8 * n / (copy_ms * 1e-3) / 1e9, 1), "copy_ms": round(copy_ms, 4)}),
+              flush=True)
+        device.close()
+        return 0
 
     device.set_policy(0)
     lu = measure_lu_headline(sci, _abi, device) if not args.no_lu else None
