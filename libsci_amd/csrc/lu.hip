@@ -67,59 +67,75 @@ namespace scihip {
             }
         }
 
-        template <typename T> __device__ __forceinline__ void block_best(T &v, int64_t &r, T *sv, int64_t *sr) {
+        // One DPP move of a 32-bit value within rows of 16 lanes (every lane reads a lane of its row).
+        template <int control> __device__ __forceinline__ unsigned dpp(unsigned x) {
+            return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, control, 0xF, 0xF, false);
+        }
+
+        template <int control, typename T> __device__ __forceinline__ void dpp_step(T &v, int64_t &r) {
+            T ov;
+            if constexpr (sizeof(T) == 8) {
+                const unsigned long long bits = __builtin_bit_cast(unsigned long long, v);
+                const unsigned long long moved = (unsigned long long)dpp<control>((unsigned)(bits >> 32)) << 32 |
+                                                 dpp<control>((unsigned)bits);
+                ov = __builtin_bit_cast(T, moved);
+            } else {
+                ov = __builtin_bit_cast(T, dpp<control>(__builtin_bit_cast(unsigned, v)));
+            }
+            const unsigned long long rb = (unsigned long long)r;
+            const int64_t orow = (int64_t)((unsigned long long)dpp<control>((unsigned)(rb >> 32)) << 32 |
+                                           dpp<control>((unsigned)rb));
+            better(v, r, ov, orow);
+        }
+
+        // The best candidate across the wave; every lane ends with it. Within each row of 16 lanes by
+        // DPP (lane pairs at distance 1 and 2, then mirrored halves and mirrored rows: each step pairs
+        // every lane with one holding a disjoint set of candidates), then across rows by shuffles.
+        template <typename T> __device__ __forceinline__ void wave_best(T &v, int64_t &r) {
+            dpp_step<0xB1>(v, r);  // quad_perm [1, 0, 3, 2]
+            dpp_step<0x4E>(v, r);  // quad_perm [2, 3, 0, 1]
+            dpp_step<0x141>(v, r); // row_half_mirror
+            dpp_step<0x140>(v, r); // row_mirror
 #pragma unroll
-            for (int offset = 32; offset > 0; offset >>= 1) {
+            for (int offset = 16; offset <= 32; offset <<= 1) {
                 const T ov = __shfl_xor(v, offset, 64);
                 const int64_t orow = __shfl_xor(r, offset, 64);
                 better(v, r, ov, orow);
             }
-            const int wave = threadIdx.x >> 6;
-            __syncthreads();
-            if ((threadIdx.x & 63) == 0) {
-                sv[wave] = v;
-                sr[wave] = r;
-            }
-            __syncthreads();
-            v = sv[0];
-            r = sr[0];
-            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) better(v, r, sv[w], sr[w]);
         }
 
-        constexpr int kLine = 32; // unsigned flags per 128-byte line: one line per workgroup's flag
+        constexpr int kLine = 32;          // unsigned flags per 128-byte line: one line per workgroup's flag
+        constexpr int kPanelThreads = 64;  // one wave per panel workgroup: its barriers cost nothing
+        constexpr int kOwned = kPanelRows / kPanelThreads; // panel rows per lane
 
-        // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b owns rows
-        // [j0 + b * kPanelRows, ...) and keeps their panel part in LDS for the whole launch, so fences
-        // never evict it. Per column, every workgroup publishes its candidate (first maximum of |a|
-        // among its rows at or below the diagonal) with that row's panel values and raises its own
-        // arrival flag; every workgroup waits for all flags, folds all candidates to the same pivot,
-        // then swaps and eliminates. Publications have two parities (column c uses c & 1): a
-        // workgroup overwrites parity p only after every flag of the column in between has risen,
-        // that is after every workgroup has finished reading parity p. value[2][G], row[2][G],
-        // rows[2][G][kPanel], diagonal[2][kPanel].
-        // At the end, workgroup 0 (which owns the diagonal block: kPanelRows == kPanel) writes the
-        // inverse of the unit lower L11 to `inverse` (w x w, row stride kPanel).
+        // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b is one wave
+        // and owns rows [j0 + b * kPanelRows, ...); it keeps their panel part in LDS for the whole
+        // launch, lane l searching rows l, l + 64. Per column, every workgroup publishes its candidate
+        // (first maximum of |a| among its rows at or below the diagonal) with that row's panel values
+        // and raises its own arrival flag; every workgroup waits for all flags, folds all candidates to
+        // the same pivot, swaps and eliminates. Publications have two parities (column c uses c & 1): a workgroup overwrites
+        // parity p only after every flag of the column in between has risen, that is after every
+        // workgroup has finished reading parity p. value[2][G], row[2][G], rows[2][G][kPanel],
+        // diagonal[2][kPanel].
+        // At the end, workgroup 0 (which owns the diagonal block: kPanelRows == kPanel) inverts the unit
+        // lower L11 in its LDS tile and writes it to `inverse` (w x w, row stride kPanel).
         template <typename T>
-        __global__ __launch_bounds__(256) void panel(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
-                                                     T *pub_value, int64_t *pub_row, T *pub_rows, T *diagonal,
-                                                     unsigned *flags, unsigned *error, T *inverse) {
+        __global__ __launch_bounds__(kPanelThreads) void panel(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
+                                                                T *pub_value, int64_t *pub_row, T *pub_rows,
+                                                                T *diagonal, unsigned *flags, unsigned *error,
+                                                                T *inverse) {
             constexpr int stride = kPanel + 1;
             const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
             const int64_t first = j0 + (int64_t)b * kPanelRows;
             const int mine = (int)min((int64_t)kPanelRows, n - first); // rows this workgroup owns
             unsigned *arrive = flags;                                  // flags[g * kLine]
             __shared__ T rows[kPanelRows * stride];
-            __shared__ T multipliers[kPanelRows];
             __shared__ T pivot_row[kPanel];
             __shared__ T diagonal_row[kPanel];
-            __shared__ T sv[4];
-            __shared__ int64_t sr[4];
-            __shared__ int64_t chosen;
+            __shared__ T multipliers[kPanelRows];
 
-            for (int i = t / kPanel; i < mine; i += 256 / kPanel) {
-                const int k = t % kPanel;
-                if (k < w) rows[i * stride + k] = a[(first + i) * n + j0 + k];
-            }
+            for (int i = 0; i < mine; ++i)
+                for (int k = t; k < w; k += kPanelThreads) rows[i * stride + k] = a[(first + i) * n + j0 + k];
             __syncthreads();
 
             for (int c = 0; c < w; ++c) {
@@ -129,25 +145,27 @@ namespace scihip {
                 // A: this workgroup's candidate and its panel row; then arrive.
                 T best = T(-1);
                 int64_t row = LLONG_MAX;
-                if (t < mine && first + t >= d) {
-                    best = __builtin_fabs(rows[t * stride + c]);
-                    row = first + t;
+#pragma unroll
+                for (int o = 0; o < kOwned; ++o) {
+                    const int i = t + o * kPanelThreads;
+                    if (i < mine && first + i >= d) better(best, row, T(__builtin_fabs(rows[i * stride + c])), first + i);
                 }
-                block_best(best, row, sv, sr);
+                wave_best(best, row);
                 if (t == 0) {
                     put(&pub_value[parity * G + b], best);
                     put(&pub_row[parity * G + b], row);
                 }
                 if (row != LLONG_MAX)
-                    for (int k = t; k < w; k += blockDim.x)
+                    for (int k = t; k < w; k += kPanelThreads)
                         put(&pub_rows[((int64_t)parity * G + b) * kPanel + k], rows[(row - first) * stride + k]);
                 if (d >= first && d < first + mine)
-                    for (int k = t; k < w; k += blockDim.x) put(&diagonal[parity * kPanel + k], rows[(d - first) * stride + k]);
-                __syncthreads();
+                    for (int k = t; k < w; k += kPanelThreads)
+                        put(&diagonal[parity * kPanel + k], rows[(d - first) * stride + k]);
+                __syncthreads(); // every publication has completed before the flag that announces it
                 if (t == 0) __hip_atomic_store(&arrive[b * kLine], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                // Every workgroup gathers: thread g polls workgroup g's arrival, then the candidates are
+                // Every workgroup gathers: lane g polls workgroup g's arrival, then the candidates are
                 // folded in the same order everywhere, so all workgroups pick the same pivot.
-                for (int g = t; g < G; g += blockDim.x) {
+                for (int g = t; g < G; g += kPanelThreads) {
                     unsigned spins = 0;
                     while (__hip_atomic_load(&arrive[g * kLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
                         if (++spins > kSpinLimit) {
@@ -158,85 +176,96 @@ namespace scihip {
                         __builtin_amdgcn_s_sleep(1);
                     }
                 }
-                __syncthreads();
-                {
-                    T v = T(-1);
-                    int64_t p = LLONG_MAX;
-                    for (int g = t; g < G; g += blockDim.x)
-                        better(v, p, get(&pub_value[parity * G + g]), get(&pub_row[parity * G + g]));
-                    block_best(v, p, sv, sr);
-                    if (t == 0) chosen = p;
-                    __syncthreads();
-                }
+                T v = T(-1);
+                int64_t p = LLONG_MAX;
+                for (int g = t; g < G; g += kPanelThreads)
+                    better(v, p, get(&pub_value[parity * G + g]), get(&pub_row[parity * G + g]));
+                wave_best(v, p);
+                if (p == LLONG_MAX) p = d; // only after a failed wait: keep every index in range
 
                 // B: swap and eliminate.
-                const int64_t p = chosen;
                 const int owner = (int)((p - j0) / kPanelRows); // the workgroup that published row p
-                for (int k = t; k < w; k += blockDim.x) {
+                for (int k = t; k < w; k += kPanelThreads) {
                     pivot_row[k] = get(&pub_rows[((int64_t)parity * G + owner) * kPanel + k]);
                     diagonal_row[k] = get(&diagonal[parity * kPanel + k]);
                 }
                 __syncthreads();
                 if (p != d) {
                     if (p >= first && p < first + mine)
-                        for (int k = t; k < w; k += blockDim.x) rows[(p - first) * stride + k] = diagonal_row[k];
+                        for (int k = t; k < w; k += kPanelThreads) rows[(p - first) * stride + k] = diagonal_row[k];
                     if (d >= first && d < first + mine)
-                        for (int k = t; k < w; k += blockDim.x) rows[(d - first) * stride + k] = pivot_row[k];
+                        for (int k = t; k < w; k += kPanelThreads) rows[(d - first) * stride + k] = pivot_row[k];
                 }
                 if (b == 0 && t == 0) pivots[d] = p;
                 __syncthreads();
                 const T pivot = pivot_row[c];
                 if (pivot != T(0)) { // a zero pivot skips the column (decompose.hpp:44-45)
-                    if (t < kPanelRows) {
+                    // Multipliers by row owner, then the rank-one update by column: lane t updates
+                    // columns c + 1 + t, c + 65 + t of every row below the diagonal, whose iterations
+                    // are independent (unrolled, many LDS accesses in flight).
+#pragma unroll
+                    for (int o = 0; o < kOwned; ++o) {
+                        const int i = t + o * kPanelThreads;
                         T l = T(0);
-                        if (t < mine && first + t > d) {
-                            l = rows[t * stride + c] / pivot;
-                            rows[t * stride + c] = l;
+                        if (i < mine && first + i > d) {
+                            l = rows[i * stride + c] / pivot;
+                            rows[i * stride + c] = l;
                         }
-                        multipliers[t] = l;
+                        multipliers[i] = l;
                     }
                     __syncthreads();
-                    // Thread t updates column t % kPanel of every other row from its row parity on.
-                    const int k = t % kPanel;
                     const int lo = (int)max((int64_t)0, d + 1 - first);
-                    if (k > c && k < w)
-                        for (int i = lo + t / kPanel; i < mine; i += 256 / kPanel) {
-                            T &target = rows[i * stride + k];
-                            target = target - multipliers[i] * pivot_row[k];
+                    for (int k = c + 1 + t; k < w; k += kPanelThreads) {
+                        const T u = pivot_row[k];
+                        T *column = rows + k;
+                        int i = lo;
+                        for (; i + 8 <= mine; i += 8) { // eight loads in flight before any update
+                            T x[8], m[8];
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                x[j] = column[(i + j) * stride];
+                                m[j] = multipliers[i + j];
+                            }
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) column[(i + j) * stride] = x[j] - m[j] * u;
                         }
+                        for (; i < mine; ++i) column[i * stride] = column[i * stride] - multipliers[i] * u;
+                    }
                 }
                 __syncthreads();
             }
 
-            for (int i = t / kPanel; i < mine; i += 256 / kPanel) {
-                const int k = t % kPanel;
-                if (k < w) a[(first + i) * n + j0 + k] = rows[i * stride + k];
-            }
+            for (int i = 0; i < mine; ++i)
+                for (int k = t; k < w; k += kPanelThreads) a[(first + i) * n + j0 + k] = rows[i * stride + k];
             if (b != 0) return;
-            // inverse(L11): thread c < w carries column c of X = L11^-1 down the rows. The panel is
-            // back in global memory, so X reuses the LDS rows; L11 is read from global memory with
-            // addresses uniform across the wave.
-            __syncthreads();
-            T *x = rows;
-            if (t < w) {
-                const int c = t;
-                for (int r = 0; r < w; ++r) {
-                    T v;
-                    if (r < c) v = T(0);
-                    else if (r == c) v = T(1);
-                    else {
-                        v = T(0);
-                        const T *lrow = a + (j0 + r) * n + j0;
-                        for (int k = c; k < r; ++k) v = v - lrow[k] * x[k * stride + c];
+            // inverse(L11) in place in the LDS tile, which holds L11 strictly below its diagonal: with
+            // M = L11^-1, M L11 = I gives, for i > j, M[i][j] = -(L[i][j] + sum_{j<k<i} M[i][k] L[k][j]);
+            // columns are done from the last to the first, so M[i][k] (k > j) is already in place
+            // and column j still holds L. Each lane computes its rows into registers before any lane
+            // overwrites column j.
+            for (int j = w - 2; j >= 0; --j) {
+                T value[kOwned];
+#pragma unroll
+                for (int o = 0; o < kOwned; ++o) {
+                    const int i = t + o * kPanelThreads;
+                    value[o] = T(0);
+                    if (i > j && i < w) {
+                        T s = rows[i * stride + j];
+                        for (int k = j + 1; k < i; ++k) s = s + rows[i * stride + k] * rows[k * stride + j];
+                        value[o] = -s;
                     }
-                    x[r * stride + c] = v;
                 }
+                __syncthreads();
+#pragma unroll
+                for (int o = 0; o < kOwned; ++o) {
+                    const int i = t + o * kPanelThreads;
+                    if (i > j && i < w) rows[i * stride + j] = value[o];
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            for (int r = t / kPanel; r < w; r += 256 / kPanel) {
-                const int c = t % kPanel;
-                if (c < w) inverse[(int64_t)r * kPanel + c] = x[r * stride + c];
-            }
+            for (int r = 0; r < w; ++r)
+                for (int c = t; c < w; c += kPanelThreads)
+                    inverse[(int64_t)r * kPanel + c] = c < r ? rows[r * stride + c] : (c == r ? T(1) : T(0));
         }
 
         // Applies the panel's swaps to every column outside it, in step order.
@@ -326,6 +355,74 @@ namespace scihip {
             }
             __syncthreads();
             if (t == 0) __hip_atomic_store(&flags[blk], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+
+        // Inverts each kBlock x kBlock diagonal block of the unit lower L (upper false) or of U (upper
+        // true) in LDS, one workgroup per block, and stores it transposed: out[b][k][r] = inv(block
+        // b)[r][k]. Lower: M L = I gives, for i > j, M[i][j] = -(L[i][j] + sum_{j<k<i} M[i][k] L[k][j]),
+        // columns from the last; upper: N U = I gives N[j][j] = 1 / U[j][j] and, for i < j,
+        // N[i][j] = -(sum_{i<=k<j} N[i][k] U[k][j]) / U[j][j], columns from the first. Column j is
+        // overwritten only after every thread has read it.
+        template <typename T, bool upper>
+        __global__ __launch_bounds__(256) void invert_diagonal(const T *f, int64_t n, T *out) {
+            constexpr int stride = kBlock + 1;
+            __shared__ T m[kBlock * stride];
+            const int64_t r0 = (int64_t)blockIdx.x * kBlock;
+            const int t = threadIdx.x;
+            for (int idx = t; idx < kBlock * kBlock; idx += blockDim.x) {
+                const int i = idx / kBlock, k = idx % kBlock;
+                m[i * stride + k] = f[(r0 + i) * n + r0 + k];
+            }
+            __syncthreads();
+            const int i = t; // threads below kBlock each carry one row of the inverse
+            for (int step = 0; step < kBlock; ++step) {
+                const int j = upper ? step : kBlock - 1 - step;
+                T v = T(0);
+                const bool active = i < kBlock && (upper ? i <= j : i > j);
+                if (active) {
+                    if (upper) {
+                        if (i == j) {
+                            v = T(1) / m[j * stride + j];
+                        } else {
+                            T sum = T(0);
+                            for (int k = i; k < j; ++k) sum = __builtin_fma(m[i * stride + k], m[k * stride + j], sum);
+                            v = -sum / m[j * stride + j];
+                        }
+                    } else {
+                        T sum = m[i * stride + j];
+                        for (int k = j + 1; k < i; ++k) sum = __builtin_fma(m[i * stride + k], m[k * stride + j], sum);
+                        v = -sum;
+                    }
+                }
+                __syncthreads();
+                if (active) m[i * stride + j] = v;
+                __syncthreads();
+            }
+            T *block = out + (int64_t)blockIdx.x * kBlock * kBlock;
+            for (int idx = t; idx < kBlock * kBlock; idx += blockDim.x) {
+                const int r = idx % kBlock, k = idx / kBlock; // consecutive threads write consecutive r
+                const T value = upper ? (k >= r ? m[r * stride + k] : T(0))
+                                      : (k < r ? m[r * stride + k] : (k == r ? T(1) : T(0)));
+                block[(int64_t)k * kBlock + r] = value;
+            }
+        }
+
+        // x[r0 + r][c] = sum_k inv[r][k] x[r0 + k][c] over one diagonal block (inverse given transposed,
+        // inverse_t[k][r]); one workgroup of kBlock threads per column c. Also writes the packed B
+        // operand of the update that follows, packed[c][r] (width x kBlock).
+        template <typename T>
+        __global__ __launch_bounds__(kBlock) void apply_inverse(const T *inverse_t, T *x, int64_t r0, int64_t width,
+                                                                T *packed) {
+            __shared__ T column[kBlock];
+            const int64_t c = blockIdx.x;
+            const int r = threadIdx.x;
+            column[r] = x[(r0 + r) * width + c];
+            __syncthreads();
+            T sum = T(0);
+#pragma unroll 16
+            for (int k = 0; k < kBlock; ++k) sum = __builtin_fma(inverse_t[k * kBlock + r], column[k], sum);
+            x[(r0 + r) * width + c] = sum;
+            packed[c * kBlock + r] = sum;
         }
 
         template <typename T, typename S>
@@ -428,8 +525,8 @@ namespace scihip {
             if (e != hipSuccess) return (int)e;
             void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &pub_value, &pub_row, &pub_rows, &diagonal,
                             &flags, &ws.error, &inverse};
-            e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups), dim3(256), args, 0,
-                                           stream);
+            e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
+                                           dim3(kPanelThreads), args, 0, stream);
             if (e != hipSuccess) return (int)e;
             timer.stage(0);
             if (n > w)
@@ -463,10 +560,45 @@ namespace scihip {
         return hipGetLastError();
     }
 
+    template <typename T> hipError_t invert_blocks(const T *f, int64_t n, T *inverses, hipStream_t stream) {
+        if (!blocked_substitution(n)) return hipErrorInvalidValue;
+        const unsigned blocks = (unsigned)(n / kBlock);
+        hipLaunchKernelGGL((invert_diagonal<T, false>), dim3(blocks), dim3(256), 0, stream, f, n, inverses);
+        hipLaunchKernelGGL((invert_diagonal<T, true>), dim3(blocks), dim3(256), 0, stream, f, n,
+                           inverses + n * kBlock);
+        return hipGetLastError();
+    }
+
     template <typename T>
     int substitute(const T *f, int64_t n, T *x, int64_t width, Workspace &ws, unsigned &epoch, hipStream_t stream,
-                   const char **error) {
+                   const char **error, const T *inverses, T *packed) {
         *error = nullptr;
+        if (inverses && packed && blocked_substitution(n)) {
+            // Block by block: x_j = inv(L_jj) x_j, then the rows below lose L[below, j] x_j (forward);
+            // x_j = inv(U_jj) x_j, then the rows above lose U[above, j] x_j (backward). The updates
+            // read L and U in place (row stride n) on the matrix cores.
+            const int mode = sizeof(T) == 8 ? kFloat64 : kFloat32;
+            const int64_t nb = n / kBlock;
+            const T *inverse_lower = inverses, *inverse_upper = inverses + n * kBlock;
+            for (int pass = 0; pass < 2; ++pass) {
+                const bool upper = pass == 1;
+                for (int64_t step = 0; step < nb; ++step) {
+                    const int64_t j = upper ? nb - 1 - step : step;
+                    const int64_t r0 = j * kBlock;
+                    hipLaunchKernelGGL(apply_inverse<T>, dim3((unsigned)width), dim3(kBlock), 0, stream,
+                                       (upper ? inverse_upper : inverse_lower) + j * kBlock * kBlock, x, r0, width, packed);
+                    const int64_t rows = upper ? r0 : n - r0 - kBlock;
+                    if (rows == 0) continue;
+                    const T *a_block = upper ? f + r0 : f + (r0 + kBlock) * n + r0;
+                    T *c_block = upper ? x : x + (r0 + kBlock) * width;
+                    ProductArgs update{a_block, nullptr, packed, nullptr, c_block, rows, width, (int64_t)kBlock,
+                                       n, (int64_t)kBlock, width, -1.0, 1.0};
+                    const hipError_t e = product(mode, update, stream);
+                    if (e != hipSuccess) return (int)e;
+                }
+            }
+            return hipGetLastError();
+        }
         const int blocks = (int)((n + kSolveRows - 1) / kSolveRows);
         if ((size_t)blocks > ws.flag_count) {
             *error = "solve flags too few";
@@ -500,9 +632,11 @@ namespace scihip {
     template int factor<float>(float *, int64_t, int64_t *, int, Workspace &, void *, size_t, hipStream_t,
                                const char **);
     template int substitute<double>(const double *, int64_t, double *, int64_t, Workspace &, unsigned &, hipStream_t,
-                                    const char **);
+                                    const char **, const double *, double *);
     template int substitute<float>(const float *, int64_t, float *, int64_t, Workspace &, unsigned &, hipStream_t,
-                                   const char **);
+                                   const char **, const float *, float *);
+    template hipError_t invert_blocks<double>(const double *, int64_t, double *, hipStream_t);
+    template hipError_t invert_blocks<float>(const float *, int64_t, float *, hipStream_t);
     template hipError_t permute<double, double>(const double *, double *, const int64_t *, int64_t, int64_t,
                                                 hipStream_t);
     template hipError_t permute<float, double>(const double *, float *, const int64_t *, int64_t, int64_t, hipStream_t);

@@ -5,14 +5,18 @@
 // for each panel of kPanel columns,
 //   1. the panel is factored by one cooperative kernel that keeps each workgroup's rows in LDS: per
 //      column every workgroup publishes its best pivot candidate (first maximum of |a|, the
-//      reference's rule) with the candidate's panel row, workgroup 0 gathers them and announces the
-//      pivot, and all workgroups swap and eliminate; workgroup 0 then inverts the unit lower L11;
+//      reference's rule) with the candidate's panel row, every workgroup gathers them all and picks
+//      the same pivot, and all workgroups swap and eliminate; workgroup 0 then inverts the unit lower
+//      L11;
 //   2. the panel's row swaps are applied to the columns outside it;
 //   3. U12 = L11^-1 A12 on the matrix cores, in the matrix's own precision;
 //   4. A22 -= L21 U12 on the matrix cores (fp64, or fp32 / bf16x3 / bf16 for a float matrix).
-// Solves run a sync-free blocked substitution: one workgroup per kSolveRows-row block, each waiting on the
-// blocks before it with a bounded spin. Every spin in this file is bounded: past kSpinLimit the
-// kernel records an error and exits, and the host reports it.
+// Solves of an order that is a multiple of kBlock invert the kBlock x kBlock diagonal blocks of L and
+// U once (all blocks in parallel), then walk the blocks: x_j = inv(D_jj) x_j and a matrix-core
+// update of the remaining rows that reads the factors in place. Other orders run a sync-free blocked
+// substitution: one workgroup per kSolveRows-row block, each waiting on the blocks before it with a
+// bounded spin. Every spin in this file is bounded: past kSpinLimit the kernel records an error and
+// exits, and the host reports it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,6 +28,10 @@ namespace scihip {
     constexpr int kPanelRows = 128;   // rows per workgroup in the panel kernel (== kPanel)
     constexpr int kSolveRows = 48;    // rows per workgroup in the triangular solves
     constexpr int kSolveColumns = 32; // right-hand sides per pass of the solve kernel
+    constexpr int kBlock = 128;       // diagonal block of the blocked substitution
+
+    // Whether an order takes the blocked substitution (whole blocks, rows 16-byte aligned).
+    constexpr bool blocked_substitution(int64_t order) { return order >= 2 * kBlock && order % kBlock == 0; }
 
     // Device-side state the LU kernels share: barrier counters, error flag, publications.
     struct Workspace {
@@ -48,9 +56,15 @@ namespace scihip {
     // Solves L U x = P b for `width` right-hand sides (row-major order x width), in place in `x`,
     // which holds P b on entry. `epoch` is the device's running launch count for the ready flags;
     // each launch takes the next value, so a flag never satisfies a wait of a later launch early.
+    // With `inverses` (2 * order * kBlock elements from invert_blocks) and `packed` (width * kBlock
+    // elements of scratch) and an order that takes it, the blocked substitution runs.
     template <typename T>
     int substitute(const T *factors, int64_t order, T *x, int64_t width, Workspace &ws, unsigned &epoch,
-                   hipStream_t stream, const char **error);
+                   hipStream_t stream, const char **error, const T *inverses = nullptr, T *packed = nullptr);
+
+    // The transposed inverses of the diagonal blocks of L (first order * kBlock elements) and of U
+    // (the next order * kBlock) of packed LU factors, for the blocked substitution.
+    template <typename T> hipError_t invert_blocks(const T *factors, int64_t order, T *inverses, hipStream_t stream);
 
     // Applies the row order of `pivots` to `source` (order x width) into `target`: target row k is
     // source row rows[k], where rows is the identity permuted by the swaps pivots[0..order).

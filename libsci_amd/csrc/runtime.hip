@@ -504,10 +504,11 @@ namespace scihip {
     // x = U^-1 L^-1 P source (source in S, solved in T), rows on the device.
     template <typename T, typename S>
     int solve_on_device(scihip_device *device, const T *factors, const int64_t *rows, const S *source, T *x,
-                        int64_t order, int64_t width) {
+                        int64_t order, int64_t width, const T *inverses = nullptr, T *packed = nullptr) {
         if (int rc = status(permute<T, S>(source, x, rows, order, width, device->stream), "permute")) return rc;
         const char *message = nullptr;
-        const int e = substitute<T>(factors, order, x, width, device->lu, device->epoch, device->stream, &message);
+        const int e = substitute<T>(factors, order, x, width, device->lu, device->epoch, device->stream, &message,
+                                    inverses, packed);
         if (e) {
             fail(message ? message : hipGetErrorString(hipError_t(e)));
             return SCIHIP_EDEVICE;
@@ -764,14 +765,30 @@ int scihip_substitute(scihip_device *device, uint32_t precision, const void *fac
                                     device->stream),
                      "hipMemcpyAsync")))
         return rc;
-    if (precision == SCIHIP_WIDE)
-        rc = solve_on_device<double, double>(device, static_cast<const double *>(factors), device_rows.as<int64_t>(),
-                                             static_cast<const double *>(right), static_cast<double *>(solution),
-                                             int64_t(order), int64_t(width));
-    else
-        rc = solve_on_device<float, float>(device, static_cast<const float *>(factors), device_rows.as<int64_t>(),
-                                           static_cast<const float *>(right), static_cast<float *>(solution),
-                                           int64_t(order), int64_t(width));
+    // The blocked substitution needs the inverses of the diagonal blocks of these factors.
+    const bool blocked = blocked_substitution(int64_t(order));
+    Scratch inverses(device, blocked ? 2 * order * kBlock * eb : 16, rc);
+    Scratch packed(device, blocked ? width * kBlock * eb : 16, rc);
+    if (rc) return rc;
+    if (precision == SCIHIP_WIDE) {
+        const double *f = static_cast<const double *>(factors);
+        if (blocked && (rc = status(invert_blocks<double>(f, int64_t(order), inverses.as<double>(), device->stream),
+                                    "invert blocks")))
+            return rc;
+        rc = solve_on_device<double, double>(device, f, device_rows.as<int64_t>(), static_cast<const double *>(right),
+                                             static_cast<double *>(solution), int64_t(order), int64_t(width),
+                                             blocked ? inverses.as<double>() : nullptr,
+                                             blocked ? packed.as<double>() : nullptr);
+    } else {
+        const float *f = static_cast<const float *>(factors);
+        if (blocked && (rc = status(invert_blocks<float>(f, int64_t(order), inverses.as<float>(), device->stream),
+                                    "invert blocks")))
+            return rc;
+        rc = solve_on_device<float, float>(device, f, device_rows.as<int64_t>(), static_cast<const float *>(right),
+                                           static_cast<float *>(solution), int64_t(order), int64_t(width),
+                                           blocked ? inverses.as<float>() : nullptr,
+                                           blocked ? packed.as<float>() : nullptr);
+    }
     if (rc) return rc;
     // The host rows buffer was copied asynchronously from pageable memory: finish before returning it.
     return check_spins(device);
@@ -796,6 +813,21 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
         return SCIHIP_DECLINED;
     }
     const auto start = std::chrono::steady_clock::now();
+    // SCIHIP_TRACE=lu: the wall time of every stage of the solve on stderr (the stream is waited at
+    // each stage boundary, so the stages do not overlap while tracing).
+    const char *trace_text = std::getenv("SCIHIP_TRACE");
+    const bool tracing = trace_text && std::strstr(trace_text, "lu");
+    auto last_mark = start;
+    std::string trace_line;
+    auto mark = [&](const char *stage) {
+        if (!tracing) return;
+        (void)hipStreamSynchronize(device->stream);
+        const auto now = std::chrono::steady_clock::now();
+        char text[64];
+        std::snprintf(text, sizeof(text), " %s %.1f", stage, std::chrono::duration<double, std::milli>(now - last_mark).count());
+        trace_line += text;
+        last_mark = now;
+    };
     const bool wide = mode == SCIHIP_FACTOR_FLOAT64;
     const size_t eb = wide ? 8 : 4;
     int rc = SCIHIP_OK;
@@ -810,6 +842,9 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     Scratch partial(device, size_t(chunks * w) * 8 * 3, rc);
     const bool aligned = n % 16 == 0;
     Scratch packed_a(device, aligned ? 16 : size_t(n) * size_t(kp) * 8, rc);
+    const bool blocked = blocked_substitution(n);
+    Scratch inverses(device, blocked ? 2 * order * kBlock * eb : 16, rc);
+    Scratch packed(device, blocked ? width * kBlock * eb : 16, rc);
     if (rc) return rc;
     if ((rc = ensure_lu(device, n, eb))) return rc;
 
@@ -827,10 +862,16 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     }
     if (rc) return rc;
     stats->factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+    mark("factor");
     const std::vector<int64_t> rows = rows_of(pivots);
     if ((rc = status(hipMemcpy(device_rows.pointer, rows.data(), order * sizeof(int64_t), hipMemcpyHostToDevice),
                      "hipMemcpy")))
         return rc;
+    if (blocked) {
+        const hipError_t e = wide ? invert_blocks<double>(factors.as<double>(), n, inverses.as<double>(), device->stream)
+                                  : invert_blocks<float>(factors.as<float>(), n, inverses.as<float>(), device->stream);
+        if ((rc = status(e, "invert blocks"))) return rc;
+    }
 
     // |A|_inf, once.
     std::vector<double> host(size_t(std::max<int64_t>(n, chunks * w * 3)));
@@ -845,9 +886,12 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     auto solve_into = [&](const double *source, bool add) -> int {
         int e;
         if (wide) e = solve_on_device<double, double>(device, factors.as<double>(), device_rows.as<int64_t>(), source,
-                                                      correction.as<double>(), n, w);
+                                                      correction.as<double>(), n, w,
+                                                      blocked ? inverses.as<double>() : nullptr,
+                                                      blocked ? packed.as<double>() : nullptr);
         else e = solve_on_device<float, double>(device, factors.as<float>(), device_rows.as<int64_t>(), source,
-                                                correction.as<float>(), n, w);
+                                                correction.as<float>(), n, w, blocked ? inverses.as<float>() : nullptr,
+                                                blocked ? packed.as<float>() : nullptr);
         if (e) return e;
         if (!add) {
             if (wide) return status(hipMemcpyAsync(solution, correction.pointer, order * width * 8,
@@ -858,6 +902,7 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
                     : status(accumulate(solution, correction.as<float>(), n * w, device->stream), "accumulate");
     };
     if ((rc = solve_into(right, false))) return rc;
+    mark("solve");
 
     const double *a_operand = matrix;
     if (!aligned) {
@@ -884,6 +929,7 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
                                         device->stream), "copy")))
             return rc;
         if ((rc = check_spins(device))) return rc;
+        mark("residual");
         double worst = 0.0;
         for (int64_t c = 0; c < w; ++c) {
             double r = 0.0, x = 0.0, b = 0.0;
@@ -904,8 +950,10 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
         }
         if (iteration >= max_iterations || std::isnan(worst)) break;
         if ((rc = solve_into(residual.as<double>(), true))) return rc;
+        mark("correct");
     }
     stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+    if (tracing) std::fprintf(stderr, "[scihip] solve stages ms:%s\n", trace_line.c_str());
     return SCIHIP_OK;
 }
 

@@ -102,9 +102,10 @@ def test_float_factor_reconstructs(device, mode):
     assert error < 4 * n * unit * np.abs(a).max(), error
 
 
-def test_substitute_matches_oracle(device):
-    n, width = 300, 5
-    a = oracle.uniform(n * n, 8, -1.0, 1.0, dtype=np.float64).reshape(n, n)
+@pytest.mark.parametrize("n,width", [(300, 5), (512, 5), (512, 33)])
+def test_substitute_matches_oracle(device, n, width):
+    # 300: the sync-free substitution; multiples of 128: the blocked one (inverted diagonal blocks)
+    a = oracle.uniform(n * n, 8, -1.0, 1.0, dtype=np.float64).reshape(n, n) + (n / 4) * np.eye(n)
     b = oracle.uniform(n * width, 9, -1.0, 1.0, dtype=np.float64).reshape(n, width)
     lu, rows = oracle.factor(a)
     want = oracle.substitute(lu, rows, b)
