@@ -17,6 +17,8 @@
 // unchanged.
 #include "product.hpp"
 
+#include <type_traits>
+
 namespace scihip {
 
     namespace {
@@ -194,8 +196,25 @@ namespace scihip {
 
             // Epilogue: accumulator element e of block (i, j) is C[row0 + 64 wm + 16 i + 4 g + e] for fp32
             // accumulators and C[row0 + 64 wm + 16 i + g + 4 e] for fp64 ones, column col0 + 64 wn + 16 j + r.
+            // With beta != 0, the 16 C values of each block row are loaded before any of them is stored,
+            // so their latencies overlap (stores to the same array would otherwise order every load
+            // after the store before it).
+            typedef typename std::conditional<M == kFloat64, double, float>::type CT;
+            CT *const c_base = static_cast<CT *>(p.c);
+            const CT alpha = (CT)p.alpha, beta = (CT)p.beta;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i) {
+                CT old[4][4];
+                if (beta != CT(0)) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int64_t row = row0 + wm * 64 + i * 16 + (M == kFloat64 ? g + 4 * e : g * 4 + e);
+                            const int64_t col = col0 + wn * 64 + j * 16 + r;
+                            old[j][e] = (row < p.m && col < p.n) ? c_base[row * p.ldc + col] : CT(0);
+                        }
+                }
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -203,17 +222,11 @@ namespace scihip {
                         const int64_t row = row0 + wm * 64 + i * 16 + (M == kFloat64 ? g + 4 * e : g * 4 + e);
                         const int64_t col = col0 + wn * 64 + j * 16 + r;
                         if (row >= p.m || col >= p.n) continue;
-                        if constexpr (M == kFloat64) {
-                            double *target = static_cast<double *>(p.c) + row * p.ldc + col;
-                            const double v = acc[i][j][e];
-                            *target = p.beta == 0.0 ? p.alpha * v : p.alpha * v + p.beta * *target;
-                        } else {
-                            float *target = static_cast<float *>(p.c) + row * p.ldc + col;
-                            const float v = acc[i][j][e];
-                            const float alpha = (float)p.alpha, beta = (float)p.beta;
-                            *target = beta == 0.0f ? (alpha == 1.0f ? v : alpha * v) : alpha * v + beta * *target;
-                        }
+                        const CT v = (CT)acc[i][j][e];
+                        c_base[row * p.ldc + col] = beta == CT(0) ? (alpha == CT(1) ? v : alpha * v)
+                                                                  : alpha * v + beta * old[j][e];
                     }
+            }
         }
 
         // 64 x 64 tiles through LDS, so that both the plain and the transposed copy read and write
