@@ -484,7 +484,8 @@ namespace scihip {
             }
             ~StageTimer() {
                 if (!on) return;
-                std::fprintf(stderr, "[scihip] lu stages ms: panel %.1f swap %.1f u12 %.1f pack %.1f update %.1f\n",
+                std::fprintf(stderr,
+                             "[scihip] lu stages ms: panel (not hidden) %.1f swap %.1f u12 %.1f pack %.1f update %.1f\n",
                              totals[0], totals[1], totals[2], totals[3], totals[4]);
                 (void)hipEventDestroy(marks[0]);
                 (void)hipEventDestroy(marks[1]);
@@ -518,16 +519,26 @@ namespace scihip {
         T *diagonal = pub_rows + (size_t)2 * max_groups * kPanel;
 
         StageTimer timer(stream);
-        for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
-            const int w = (int)std::min<int64_t>(kPanel, n - j0);
+        // Factors the panel at column j0 on stream `s`.
+        auto launch_panel = [&](int64_t j0, hipStream_t s) -> hipError_t {
+            int w = (int)std::min<int64_t>(kPanel, n - j0);
             const int groups = (int)((n - j0 + kPanelRows - 1) / kPanelRows);
-            hipError_t e = hipMemsetAsync(flags, 0, (size_t)(groups + 1) * kLine * sizeof(unsigned), stream);
-            if (e != hipSuccess) return (int)e;
+            hipError_t e = hipMemsetAsync(flags, 0, (size_t)(groups + 1) * kLine * sizeof(unsigned), s);
+            if (e != hipSuccess) return e;
             void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &pub_value, &pub_row, &pub_rows, &diagonal,
                             &flags, &ws.error, &inverse};
-            e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
-                                           dim3(kPanelThreads), args, 0, stream);
-            if (e != hipSuccess) return (int)e;
+            return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
+                                              dim3(kPanelThreads), args, 0, s);
+        };
+        const size_t eb = (size_t)element_bytes(mode);
+        hipError_t e = launch_panel(0, stream);
+        if (e != hipSuccess) return (int)e;
+        for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
+            const int w = (int)std::min<int64_t>(kPanel, n - j0);
+            if (j0 > 0) { // this panel ran on the side stream, during the previous update
+                e = hipStreamWaitEvent(stream, ws.panel_done, 0);
+                if (e != hipSuccess) return (int)e;
+            }
             timer.stage(0);
             if (n > w)
                 hipLaunchKernelGGL(swap_rows<T>, dim3((unsigned)((n - w + 255) / 256)), dim3(256), 0, stream, a, n, j0,
@@ -545,15 +556,31 @@ namespace scihip {
                 e = product(own, solve, stream);
                 if (e != hipSuccess) return (int)e;
                 timer.stage(2);
-                // A22 -= L21 U12 on the matrix cores, in the requested mode.
+                // A22 -= L21 U12 on the matrix cores, in the requested mode: the next panel's columns
+                // first, then (while that panel is factored on the side stream) the rest.
                 e = pack(mode, a + (j0 + w) * n + j0, wide, rest, w, n, false, la_hi, la_lo, kp, stream);
                 if (e == hipSuccess) e = pack(mode, a + j0 * n + j0 + w, wide, rest, w, n, true, ub_hi, ub_lo, kp, stream);
                 if (e != hipSuccess) return (int)e;
                 timer.stage(3);
-                ProductArgs update{la_hi, la_lo, ub_hi, ub_lo, a + (j0 + w) * n + (j0 + w), rest, rest, (int64_t)kp,
-                                   (int64_t)kp, (int64_t)kp, n, -1.0, 1.0};
-                e = product(mode, update, stream);
+                const int64_t next = std::min<int64_t>(kPanel, rest);
+                T *c = a + (j0 + w) * n + (j0 + w);
+                ProductArgs narrow{la_hi, la_lo, ub_hi, ub_lo, c, rest, next, (int64_t)kp, (int64_t)kp, (int64_t)kp, n,
+                                   -1.0, 1.0};
+                e = product(mode, narrow, stream);
+                if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
+                if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
+                if (e == hipSuccess) e = launch_panel(j0 + w, ws.side);
+                if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
                 if (e != hipSuccess) return (int)e;
+                if (rest > next) {
+                    const unsigned char *b_hi = static_cast<const unsigned char *>(ub_hi) + (size_t)next * kp * eb;
+                    const unsigned char *b_lo =
+                            ub_lo ? static_cast<const unsigned char *>(ub_lo) + (size_t)next * kp * eb : nullptr;
+                    ProductArgs wide_update{la_hi, la_lo, b_hi, mode == kSplit ? b_lo : nullptr, c + next, rest,
+                                            rest - next, (int64_t)kp, (int64_t)kp, (int64_t)kp, n, -1.0, 1.0};
+                    e = product(mode, wide_update, stream);
+                    if (e != hipSuccess) return (int)e;
+                }
                 timer.stage(4);
             }
         }

@@ -10,7 +10,10 @@
 //      L11;
 //   2. the panel's row swaps are applied to the columns outside it;
 //   3. U12 = L11^-1 A12 on the matrix cores, in the matrix's own precision;
-//   4. A22 -= L21 U12 on the matrix cores (fp64, or fp32 / bf16x3 / bf16 for a float matrix).
+//   4. A22 -= L21 U12 on the matrix cores (fp64, or fp32 / bf16x3 / bf16 for a float matrix), in two
+//      products: first the next panel's columns, then the rest. Once the first is done, the next
+//      panel is factored on a second stream while the rest of the update runs (look-ahead), and the
+//      next step waits for it.
 // Solves of an order that is a multiple of kBlock invert the kBlock x kBlock diagonal blocks of L and
 // U once (all blocks in parallel), then walk the blocks: x_j = inv(D_jj) x_j and a matrix-core
 // update of the remaining rows that reads the factors in place. Other orders run a sync-free blocked
@@ -41,6 +44,9 @@ namespace scihip {
         void *publish = nullptr;     // panel: candidate values, rows and panel rows, two parities
         size_t publish_bytes = 0;
         size_t flag_count = 0;
+        hipStream_t side = nullptr;         // the look-ahead panels run here (default priority)
+        hipEvent_t columns_ready = nullptr; // main stream: the next panel's columns are up to date
+        hipEvent_t panel_done = nullptr;    // side stream: that panel is factored
     };
 
     // Factors the order x order row-major matrix in place (T = float or double); pivots[k] is the

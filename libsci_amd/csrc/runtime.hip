@@ -365,6 +365,13 @@ namespace scihip {
     // flag, the solve's ready flags (zeroed when first made), the panel's publications.
     int ensure_lu(scihip_device *device, int64_t order, size_t element) {
         Workspace &ws = device->lu;
+        if (!ws.side) {
+            if (int rc = status(hipStreamCreateWithFlags(&ws.side, hipStreamNonBlocking), "hipStreamCreate")) return rc;
+            if (int rc = status(hipEventCreateWithFlags(&ws.columns_ready, hipEventDisableTiming), "hipEventCreate"))
+                return rc;
+            if (int rc = status(hipEventCreateWithFlags(&ws.panel_done, hipEventDisableTiming), "hipEventCreate"))
+                return rc;
+        }
         if (!ws.counter) {
             void *block = nullptr;
             if (int rc = status(hipMalloc(&block, 256), "hipMalloc")) return rc;
@@ -402,6 +409,12 @@ namespace scihip {
 
     void release_lu(scihip_device *device) {
         Workspace &ws = device->lu;
+        if (ws.side) {
+            (void)hipStreamSynchronize(ws.side);
+            (void)hipStreamDestroy(ws.side);
+        }
+        if (ws.columns_ready) (void)hipEventDestroy(ws.columns_ready);
+        if (ws.panel_done) (void)hipEventDestroy(ws.panel_done);
         if (ws.counter) (void)hipFree(ws.counter);
         if (ws.flags) (void)hipFree(ws.flags);
         if (ws.publish) (void)hipFree(ws.publish);
