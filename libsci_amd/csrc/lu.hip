@@ -106,24 +106,22 @@ namespace scihip {
 
         constexpr int kLine = 32;          // unsigned flags per 128-byte line: one line per workgroup's flag
         constexpr int kPanelThreads = 64;  // one wave per panel workgroup: its barriers cost nothing
-        constexpr int kOwned = kPanelRows / kPanelThreads; // panel rows per lane
+        constexpr int kOwned = (kPanelRows + kPanelThreads - 1) / kPanelThreads; // panel rows per lane, at most
 
         // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b is one wave
         // and owns rows [j0 + b * kPanelRows, ...); it keeps their panel part in LDS for the whole
-        // launch, lane l searching rows l, l + 64. Per column, every workgroup publishes its candidate
+        // launch, lane l searching rows l, l + 64, ... Per column, every workgroup publishes its candidate
         // (first maximum of |a| among its rows at or below the diagonal) with that row's panel values
         // and raises its own arrival flag; every workgroup waits for all flags, folds all candidates to
-        // the same pivot, swaps and eliminates. Publications have two parities (column c uses c & 1): a workgroup overwrites
-        // parity p only after every flag of the column in between has risen, that is after every
-        // workgroup has finished reading parity p. value[2][G], row[2][G], rows[2][G][kPanel],
+        // the same pivot, swaps and eliminates; the next column's candidate is published before the
+        // bulk of the elimination, which then overlaps the flags' round trip. Publications have two
+        // parities (column c uses c & 1): a workgroup overwrites parity p only after every flag of the
+        // column in between has risen, that is after every workgroup has finished reading parity p. value[2][G], row[2][G], rows[2][G][kPanel],
         // diagonal[2][kPanel].
-        // At the end, workgroup 0 (which owns the diagonal block: kPanelRows == kPanel) inverts the unit
-        // lower L11 in its LDS tile and writes it to `inverse` (w x w, row stride kPanel).
         template <typename T>
         __global__ __launch_bounds__(kPanelThreads) void panel(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
                                                                 T *pub_value, int64_t *pub_row, T *pub_rows,
-                                                                T *diagonal, unsigned *flags, unsigned *error,
-                                                                T *inverse) {
+                                                                T *diagonal, unsigned *flags, unsigned *error) {
             constexpr int stride = kPanel + 1;
             const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
             const int64_t first = j0 + (int64_t)b * kPanelRows;
@@ -138,32 +136,46 @@ namespace scihip {
                 for (int k = t; k < w; k += kPanelThreads) rows[i * stride + k] = a[(first + i) * n + j0 + k];
             __syncthreads();
 
-            for (int c = 0; c < w; ++c) {
-                const int64_t d = j0 + c;
-                const int parity = c & 1;
-                const unsigned epoch = (unsigned)c + 1;
-                // A: this workgroup's candidate and its panel row; then arrive.
+            // Publishes this workgroup's candidate for column `col` (the first maximum of |a| among its
+            // rows at or below the diagonal) with the candidate's panel row, and the diagonal row if it
+            // owns it; then raises its arrival flag for `col`. Column `col` of the rows is up to date;
+            // with `pending`, their columns beyond it still lack the previous column's update, which
+            // is applied to the published values (x - l * u, as the elimination will store it).
+            auto publish = [&](int col, bool pending) {
+                const int64_t dd = j0 + col;
+                const int par = col & 1;
+                auto value = [&](int i, int k) -> T {
+                    const T x = rows[i * stride + k];
+                    return (pending && k > col) ? x - multipliers[i] * pivot_row[k] : x;
+                };
                 T best = T(-1);
                 int64_t row = LLONG_MAX;
 #pragma unroll
                 for (int o = 0; o < kOwned; ++o) {
                     const int i = t + o * kPanelThreads;
-                    if (i < mine && first + i >= d) better(best, row, T(__builtin_fabs(rows[i * stride + c])), first + i);
+                    if (i < mine && first + i >= dd) better(best, row, T(__builtin_fabs(rows[i * stride + col])), first + i);
                 }
                 wave_best(best, row);
                 if (t == 0) {
-                    put(&pub_value[parity * G + b], best);
-                    put(&pub_row[parity * G + b], row);
+                    put(&pub_value[par * G + b], best);
+                    put(&pub_row[par * G + b], row);
                 }
                 if (row != LLONG_MAX)
                     for (int k = t; k < w; k += kPanelThreads)
-                        put(&pub_rows[((int64_t)parity * G + b) * kPanel + k], rows[(row - first) * stride + k]);
-                if (d >= first && d < first + mine)
-                    for (int k = t; k < w; k += kPanelThreads)
-                        put(&diagonal[parity * kPanel + k], rows[(d - first) * stride + k]);
+                        put(&pub_rows[((int64_t)par * G + b) * kPanel + k], value((int)(row - first), k));
+                if (dd >= first && dd < first + mine)
+                    for (int k = t; k < w; k += kPanelThreads) put(&diagonal[par * kPanel + k], value((int)(dd - first), k));
                 __syncthreads(); // every publication has completed before the flag that announces it
-                if (t == 0) __hip_atomic_store(&arrive[b * kLine], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                // Every workgroup gathers: lane g polls workgroup g's arrival, then the candidates are
+                if (t == 0)
+                    __hip_atomic_store(&arrive[b * kLine], (unsigned)col + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            };
+
+            publish(0, false);
+            for (int c = 0; c < w; ++c) {
+                const int64_t d = j0 + c;
+                const int parity = c & 1;
+                const unsigned epoch = (unsigned)c + 1;
+                // A: every workgroup gathers: lane g polls workgroup g's arrival, then the candidates are
                 // folded in the same order everywhere, so all workgroups pick the same pivot.
                 for (int g = t; g < G; g += kPanelThreads) {
                     unsigned spins = 0;
@@ -183,7 +195,7 @@ namespace scihip {
                 wave_best(v, p);
                 if (p == LLONG_MAX) p = d; // only after a failed wait: keep every index in range
 
-                // B: swap and eliminate.
+                // B: swap.
                 const int owner = (int)((p - j0) / kPanelRows); // the workgroup that published row p
                 for (int k = t; k < w; k += kPanelThreads) {
                     pivot_row[k] = get(&pub_rows[((int64_t)parity * G + owner) * kPanel + k]);
@@ -198,11 +210,12 @@ namespace scihip {
                 }
                 if (b == 0 && t == 0) pivots[d] = p;
                 __syncthreads();
+
+                // C: multipliers; column c + 1 and the next candidate first (look-ahead), then the rest of
+                // the rank-one update while the other workgroups' candidates travel.
                 const T pivot = pivot_row[c];
-                if (pivot != T(0)) { // a zero pivot skips the column (decompose.hpp:44-45)
-                    // Multipliers by row owner, then the rank-one update by column: lane t updates
-                    // columns c + 1 + t, c + 65 + t of every row below the diagonal, whose iterations
-                    // are independent (unrolled, many LDS accesses in flight).
+                const bool eliminate = pivot != T(0); // a zero pivot skips the column (decompose.hpp:44-45)
+                if (eliminate) {
 #pragma unroll
                     for (int o = 0; o < kOwned; ++o) {
                         const int i = t + o * kPanelThreads;
@@ -210,16 +223,22 @@ namespace scihip {
                         if (i < mine && first + i > d) {
                             l = rows[i * stride + c] / pivot;
                             rows[i * stride + c] = l;
+                            if (c + 1 < w) rows[i * stride + c + 1] = rows[i * stride + c + 1] - l * pivot_row[c + 1];
                         }
-                        multipliers[i] = l;
+                        if (i < kPanelRows) multipliers[i] = l;
                     }
                     __syncthreads();
+                }
+                if (c + 1 < w) publish(c + 1, eliminate);
+                if (eliminate) {
+                    // Lane t updates columns c + 2 + t, c + 66 + t of every row below the diagonal; the
+                    // iterations are independent (eight LDS loads in flight before any store).
                     const int lo = (int)max((int64_t)0, d + 1 - first);
-                    for (int k = c + 1 + t; k < w; k += kPanelThreads) {
+                    for (int k = c + 2 + t; k < w; k += kPanelThreads) {
                         const T u = pivot_row[k];
                         T *column = rows + k;
                         int i = lo;
-                        for (; i + 8 <= mine; i += 8) { // eight loads in flight before any update
+                        for (; i + 8 <= mine; i += 8) {
                             T x[8], m[8];
 #pragma unroll
                             for (int j = 0; j < 8; ++j) {
@@ -237,35 +256,6 @@ namespace scihip {
 
             for (int i = 0; i < mine; ++i)
                 for (int k = t; k < w; k += kPanelThreads) a[(first + i) * n + j0 + k] = rows[i * stride + k];
-            if (b != 0) return;
-            // inverse(L11) in place in the LDS tile, which holds L11 strictly below its diagonal: with
-            // M = L11^-1, M L11 = I gives, for i > j, M[i][j] = -(L[i][j] + sum_{j<k<i} M[i][k] L[k][j]);
-            // columns are done from the last to the first, so M[i][k] (k > j) is already in place
-            // and column j still holds L. Each lane computes its rows into registers before any lane
-            // overwrites column j.
-            for (int j = w - 2; j >= 0; --j) {
-                T value[kOwned];
-#pragma unroll
-                for (int o = 0; o < kOwned; ++o) {
-                    const int i = t + o * kPanelThreads;
-                    value[o] = T(0);
-                    if (i > j && i < w) {
-                        T s = rows[i * stride + j];
-                        for (int k = j + 1; k < i; ++k) s = s + rows[i * stride + k] * rows[k * stride + j];
-                        value[o] = -s;
-                    }
-                }
-                __syncthreads();
-#pragma unroll
-                for (int o = 0; o < kOwned; ++o) {
-                    const int i = t + o * kPanelThreads;
-                    if (i > j && i < w) rows[i * stride + j] = value[o];
-                }
-                __syncthreads();
-            }
-            for (int r = 0; r < w; ++r)
-                for (int c = t; c < w; c += kPanelThreads)
-                    inverse[(int64_t)r * kPanel + c] = c < r ? rows[r * stride + c] : (c == r ? T(1) : T(0));
         }
 
         // Applies the panel's swaps to every column outside it, in step order.
@@ -364,14 +354,15 @@ namespace scihip {
         // N[i][j] = -(sum_{i<=k<j} N[i][k] U[k][j]) / U[j][j], columns from the first. Column j is
         // overwritten only after every thread has read it.
         template <typename T, bool upper>
-        __global__ __launch_bounds__(256) void invert_diagonal(const T *f, int64_t n, T *out) {
+        __global__ __launch_bounds__(256) void invert_diagonal(const T *f, int64_t n, int size, T *out) {
             constexpr int stride = kBlock + 1;
             __shared__ T m[kBlock * stride];
             const int64_t r0 = (int64_t)blockIdx.x * kBlock;
             const int t = threadIdx.x;
+            // A block of `size` < kBlock rows is completed by the identity, whose inverse it keeps.
             for (int idx = t; idx < kBlock * kBlock; idx += blockDim.x) {
                 const int i = idx / kBlock, k = idx % kBlock;
-                m[i * stride + k] = f[(r0 + i) * n + r0 + k];
+                m[i * stride + k] = (i < size && k < size) ? f[(r0 + i) * n + r0 + k] : (i == k ? T(1) : T(0));
             }
             __syncthreads();
             const int i = t; // threads below kBlock each carry one row of the inverse
@@ -526,9 +517,13 @@ namespace scihip {
             hipError_t e = hipMemsetAsync(flags, 0, (size_t)(groups + 1) * kLine * sizeof(unsigned), s);
             if (e != hipSuccess) return e;
             void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &pub_value, &pub_row, &pub_rows, &diagonal,
-                            &flags, &ws.error, &inverse};
-            return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
-                                              dim3(kPanelThreads), args, 0, s);
+                            &flags, &ws.error};
+            e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
+                                           dim3(kPanelThreads), args, 0, s);
+            if (e != hipSuccess) return e;
+            // L11^-1, transposed (inverse[k][r] = L11^-1[r][k]), for U12 = L11^-1 A12.
+            hipLaunchKernelGGL((invert_diagonal<T, false>), dim3(1), dim3(256), 0, s, a + j0 * n + j0, n, w, inverse);
+            return hipGetLastError();
         };
         const size_t eb = (size_t)element_bytes(mode);
         hipError_t e = launch_panel(0, stream);
@@ -549,7 +544,7 @@ namespace scihip {
                 // U12 = L11^-1 A12: the packed copy of A12 is the product's operand, so the result can
                 // overwrite A12 in place.
                 e = pack(own, a + j0 * n + j0 + w, wide, rest, w, n, true, ub_hi, nullptr, kp, stream);
-                if (e == hipSuccess) e = pack(own, inverse, wide, w, w, kPanel, false, inverse_packed, nullptr, kp, stream);
+                if (e == hipSuccess) e = pack(own, inverse, wide, w, w, kPanel, true, inverse_packed, nullptr, kp, stream);
                 if (e != hipSuccess) return (int)e;
                 ProductArgs solve{inverse_packed, nullptr, ub_hi, nullptr, a + j0 * n + j0 + w, w, rest, (int64_t)kp,
                                   (int64_t)kp, (int64_t)kp, n, 1.0, 0.0};
@@ -590,8 +585,8 @@ namespace scihip {
     template <typename T> hipError_t invert_blocks(const T *f, int64_t n, T *inverses, hipStream_t stream) {
         if (!blocked_substitution(n)) return hipErrorInvalidValue;
         const unsigned blocks = (unsigned)(n / kBlock);
-        hipLaunchKernelGGL((invert_diagonal<T, false>), dim3(blocks), dim3(256), 0, stream, f, n, inverses);
-        hipLaunchKernelGGL((invert_diagonal<T, true>), dim3(blocks), dim3(256), 0, stream, f, n,
+        hipLaunchKernelGGL((invert_diagonal<T, false>), dim3(blocks), dim3(256), 0, stream, f, n, kBlock, inverses);
+        hipLaunchKernelGGL((invert_diagonal<T, true>), dim3(blocks), dim3(256), 0, stream, f, n, kBlock,
                            inverses + n * kBlock);
         return hipGetLastError();
     }

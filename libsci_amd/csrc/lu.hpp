@@ -6,8 +6,8 @@
 //   1. the panel is factored by one cooperative kernel that keeps each workgroup's rows in LDS: per
 //      column every workgroup publishes its best pivot candidate (first maximum of |a|, the
 //      reference's rule) with the candidate's panel row, every workgroup gathers them all and picks
-//      the same pivot, and all workgroups swap and eliminate; workgroup 0 then inverts the unit lower
-//      L11;
+//      the same pivot, and all workgroups swap and eliminate; then one workgroup inverts the unit
+//      lower L11;
 //   2. the panel's row swaps are applied to the columns outside it;
 //   3. U12 = L11^-1 A12 on the matrix cores, in the matrix's own precision;
 //   4. A22 -= L21 U12 on the matrix cores (fp64, or fp32 / bf16x3 / bf16 for a float matrix), in two
@@ -28,7 +28,7 @@
 namespace scihip {
 
     constexpr int kPanel = 128;       // panel width (columns factored per cooperative launch)
-    constexpr int kPanelRows = 128;   // rows per workgroup in the panel kernel (== kPanel)
+    constexpr int kPanelRows = 64;    // rows per workgroup (one wave) in the panel kernel
     constexpr int kSolveRows = 48;    // rows per workgroup in the triangular solves
     constexpr int kSolveColumns = 32; // right-hand sides per pass of the solve kernel
     constexpr int kBlock = 128;       // diagonal block of the blocked substitution
