@@ -104,29 +104,61 @@ namespace scihip {
             }
         }
 
-        constexpr int kLine = 32;          // unsigned flags per 128-byte line: one line per workgroup's flag
+        // The largest 64-bit value across the wave; every lane ends with it.
+        __device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
+            auto step = [&](unsigned long long o) { x = o > x ? o : x; };
+            step((unsigned long long)dpp<0xB1>((unsigned)(x >> 32)) << 32 | dpp<0xB1>((unsigned)x));
+            step((unsigned long long)dpp<0x4E>((unsigned)(x >> 32)) << 32 | dpp<0x4E>((unsigned)x));
+            step((unsigned long long)dpp<0x141>((unsigned)(x >> 32)) << 32 | dpp<0x141>((unsigned)x));
+            step((unsigned long long)dpp<0x140>((unsigned)(x >> 32)) << 32 | dpp<0x140>((unsigned)x));
+            step(__shfl_xor(x, 16, 64));
+            step(__shfl_xor(x, 32, 64));
+            return x;
+        }
+
+        // A workgroup's candidate for one column as one 64-bit word, which is also its arrival flag:
+        // bits 63-32 the float bits of |a| (monotonic in |a| for a non-negative value; exact for float
+        // matrices, rounded for double ones), bits 31-8 the complement of the row's offset from the
+        // panel's first row (so that the larger word has the lower row on equal keys), bits 7-0 the
+        // column number + 1. A workgroup without a candidate publishes key 0 and the largest offset.
+        template <typename T>
+        __device__ __forceinline__ unsigned long long candidate_word(T best, int64_t row, int64_t j0, unsigned epoch) {
+            unsigned long long word = epoch & 0xFFu;
+            if (row != LLONG_MAX) {
+                const float key = (float)best;
+                const unsigned bits = key == key ? __builtin_bit_cast(unsigned, key) : 0u; // NaN never wins
+                word |= (unsigned long long)bits << 32 | (unsigned long long)(0xFFFFFFu - (unsigned)(row - j0)) << 8;
+            }
+            return word;
+        }
+        __device__ __forceinline__ int64_t row_of(unsigned long long word, int64_t j0) {
+            return j0 + (int64_t)(0xFFFFFFu - (unsigned)((word >> 8) & 0xFFFFFFu));
+        }
+
+        constexpr int kLine = 16;          // 64-bit words per 128-byte line: one line per candidate word
         constexpr int kPanelThreads = 64;  // one wave per panel workgroup: its barriers cost nothing
         constexpr int kOwned = (kPanelRows + kPanelThreads - 1) / kPanelThreads; // panel rows per lane, at most
 
         // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b is one wave
         // and owns rows [j0 + b * kPanelRows, ...); it keeps their panel part in LDS for the whole
         // launch, lane l searching rows l, l + 64, ... Per column, every workgroup publishes its candidate
-        // (first maximum of |a| among its rows at or below the diagonal) with that row's panel values
-        // and raises its own arrival flag; every workgroup waits for all flags, folds all candidates to
-        // the same pivot, swaps and eliminates; the next column's candidate is published before the
-        // bulk of the elimination, which then overlaps the flags' round trip. Publications have two
-        // parities (column c uses c & 1): a workgroup overwrites parity p only after every flag of the
-        // column in between has risen, that is after every workgroup has finished reading parity p. value[2][G], row[2][G], rows[2][G][kPanel],
-        // diagonal[2][kPanel].
+        // (first maximum of |a| among its rows at or below the diagonal) with that row's panel values,
+        // then its candidate word (candidate_word), which is its arrival flag and carries the candidate,
+        // so one round trip gathers every candidate; every workgroup takes the largest word, swaps and
+        // eliminates; the next column's candidate is published before the bulk of the elimination,
+        // which then overlaps the words' round trip. Publications have two parities (column c uses
+        // c & 1): a workgroup overwrites parity p only after every word of the column in between has
+        // arrived, that is after every workgroup has finished reading parity p. value[2][G] (exact
+        // |a|, for ties of the float key), rows[2][G][kPanel], diagonal[2][kPanel], words[2][G] (one
+        // 128-byte line each).
         template <typename T>
         __global__ __launch_bounds__(kPanelThreads) void panel(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
-                                                                T *pub_value, int64_t *pub_row, T *pub_rows,
-                                                                T *diagonal, unsigned *flags, unsigned *error) {
+                                                                T *pub_value, T *pub_rows, T *diagonal,
+                                                                unsigned long long *words, unsigned *error) {
             constexpr int stride = kPanel + 1;
             const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
             const int64_t first = j0 + (int64_t)b * kPanelRows;
             const int mine = (int)min((int64_t)kPanelRows, n - first); // rows this workgroup owns
-            unsigned *arrive = flags;                                  // flags[g * kLine]
             __shared__ T rows[kPanelRows * stride];
             __shared__ T pivot_row[kPanel];
             __shared__ T diagonal_row[kPanel];
@@ -156,18 +188,17 @@ namespace scihip {
                     if (i < mine && first + i >= dd) better(best, row, T(__builtin_fabs(rows[i * stride + col])), first + i);
                 }
                 wave_best(best, row);
-                if (t == 0) {
-                    put(&pub_value[par * G + b], best);
-                    put(&pub_row[par * G + b], row);
-                }
+                if (t == 0) put(&pub_value[par * G + b], best); // the exact value, for ties of the float key
                 if (row != LLONG_MAX)
                     for (int k = t; k < w; k += kPanelThreads)
                         put(&pub_rows[((int64_t)par * G + b) * kPanel + k], value((int)(row - first), k));
                 if (dd >= first && dd < first + mine)
                     for (int k = t; k < w; k += kPanelThreads) put(&diagonal[par * kPanel + k], value((int)(dd - first), k));
-                __syncthreads(); // every publication has completed before the flag that announces it
+                __syncthreads(); // every publication has completed before the word that announces it
                 if (t == 0)
-                    __hip_atomic_store(&arrive[b * kLine], (unsigned)col + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&words[((int64_t)par * G + b) * kLine],
+                                       candidate_word<T>(best, row, j0, (unsigned)col + 1), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             };
 
             publish(0, false);
@@ -175,11 +206,17 @@ namespace scihip {
                 const int64_t d = j0 + c;
                 const int parity = c & 1;
                 const unsigned epoch = (unsigned)c + 1;
-                // A: every workgroup gathers: lane g polls workgroup g's arrival, then the candidates are
-                // folded in the same order everywhere, so all workgroups pick the same pivot.
+                // A: every workgroup gathers: lane g polls workgroup g's candidate word until it carries
+                // this column, and the wave takes the largest word, so all workgroups pick the same pivot.
+                // Equal float keys of different doubles are settled with the exact values.
+                unsigned long long mine_best = 0; // this lane's largest word, and how many share its key
+                int sharing = 0;
                 for (int g = t; g < G; g += kPanelThreads) {
+                    unsigned long long *word = &words[((int64_t)parity * G + g) * kLine];
                     unsigned spins = 0;
-                    while (__hip_atomic_load(&arrive[g * kLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+                    unsigned long long x;
+                    while (((x = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 0xFFu) !=
+                           (epoch & 0xFFu)) {
                         if (++spins > kSpinLimit) {
                             __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
@@ -187,13 +224,34 @@ namespace scihip {
                         if ((spins & 63u) == 0 && relaxed(error)) break;
                         __builtin_amdgcn_s_sleep(1);
                     }
+                    if (sharing == 0 || (x >> 32) > (mine_best >> 32)) {
+                        mine_best = x;
+                        sharing = 1;
+                    } else if ((x >> 32) == (mine_best >> 32)) {
+                        if ((x >> 8) > (mine_best >> 8)) mine_best = x;
+                        ++sharing;
+                    }
                 }
-                T v = T(-1);
-                int64_t p = LLONG_MAX;
-                for (int g = t; g < G; g += kPanelThreads)
-                    better(v, p, get(&pub_value[parity * G + g]), get(&pub_row[parity * G + g]));
-                wave_best(v, p);
-                if (p == LLONG_MAX) p = d; // only after a failed wait: keep every index in range
+                const unsigned long long top = wave_max(sharing ? mine_best >> 8 : 0); // key << 24 | ~offset
+                int64_t p = row_of(top << 8, j0);
+                if constexpr (sizeof(T) == 8) {
+                    const bool holds = sharing > 0 && (mine_best >> 32) == (top >> 24);
+                    const unsigned long long holders = __ballot(holds);
+                    if (__ballot(holds && sharing > 1) || (holders & (holders - 1))) {
+                        // Two or more candidates share the float key: compare the exact values. The words
+                        // of this parity are not rewritten before this workgroup publishes again.
+                        T v = T(-1);
+                        p = LLONG_MAX;
+                        for (int g = t; g < G; g += kPanelThreads) {
+                            const unsigned long long x = __hip_atomic_load(&words[((int64_t)parity * G + g) * kLine],
+                                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if ((x >> 32) == (top >> 24) && (x & 0xFFu) == (epoch & 0xFFu))
+                                better(v, p, get(&pub_value[parity * G + g]), row_of(x, j0));
+                        }
+                        wave_best(v, p);
+                    }
+                }
+                if (p < d || p >= n) p = d; // only after a failed wait: keep every index in range
 
                 // B: swap.
                 const int owner = (int)((p - j0) / kPanelRows); // the workgroup that published row p
@@ -443,8 +501,8 @@ namespace scihip {
         (void)mode;
         const size_t kp = (size_t)kPanel;
         // L21 and U12 packed (up to 8 bytes, split pairs), L11^-1 and its packed copy, the panel's
-        // arrival flags.
-        return 4 * (size_t)order * kp * 8 + 2 * kp * kp * 8 + ((size_t)order / kPanelRows + 2) * 128 + 1024;
+        // candidate words (two parities, one 128-byte line each).
+        return 4 * (size_t)order * kp * 8 + 2 * kp * kp * 8 + 2 * ((size_t)order / kPanelRows + 2) * 128 + 1024;
     }
 
     namespace {
@@ -505,19 +563,19 @@ namespace scihip {
         unsigned *flags = reinterpret_cast<unsigned *>(base + 4 * operand + 2 * kp * kp * 8);
         const int max_groups = (int)((n + kPanelRows - 1) / kPanelRows);
         T *pub_value = static_cast<T *>(ws.publish);
-        int64_t *pub_row = reinterpret_cast<int64_t *>(pub_value + 2 * max_groups + 2);
-        T *pub_rows = reinterpret_cast<T *>(pub_row + 2 * max_groups);
+        T *pub_rows = pub_value + 2 * max_groups + 2;
         T *diagonal = pub_rows + (size_t)2 * max_groups * kPanel;
+        unsigned long long *words = reinterpret_cast<unsigned long long *>(flags);
 
         StageTimer timer(stream);
         // Factors the panel at column j0 on stream `s`.
         auto launch_panel = [&](int64_t j0, hipStream_t s) -> hipError_t {
             int w = (int)std::min<int64_t>(kPanel, n - j0);
             const int groups = (int)((n - j0 + kPanelRows - 1) / kPanelRows);
-            hipError_t e = hipMemsetAsync(flags, 0, (size_t)(groups + 1) * kLine * sizeof(unsigned), s);
+            hipError_t e = hipMemsetAsync(words, 0, (size_t)2 * (groups + 1) * kLine * sizeof(unsigned long long), s);
             if (e != hipSuccess) return e;
-            void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &pub_value, &pub_row, &pub_rows, &diagonal,
-                            &flags, &ws.error};
+            void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &pub_value, &pub_rows, &diagonal, &words,
+                            &ws.error};
             e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
                                            dim3(kPanelThreads), args, 0, s);
             if (e != hipSuccess) return e;
