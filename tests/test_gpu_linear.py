@@ -137,3 +137,29 @@ def test_refined_solve(device, mode, graded):
     reference = scipy.linalg.solve(a, b)
     forward = np.abs(x - reference).max() / np.abs(reference).max()
     assert forward < (1e-10 if graded else 1e-13)
+
+
+@pytest.mark.parametrize("case", ["rounded_key_tie", "exact_tie", "float_exact_tie"])
+def test_pivot_ties_across_workgroups(device, case):
+    # Candidates in different panel workgroups whose |a| round to the same float (the larger double
+    # must win) or are equal (the lowest row must win), as the reference's first-maximum rule picks.
+    n = 200
+    rng = np.random.default_rng(11)
+    a = rng.uniform(-0.4, 0.4, (n, n)) + 0.0
+    if case == "rounded_key_tie":
+        a[10, 0], a[150, 0] = 1.0, 1.0 + 2.0 ** -40
+        a[30, 5], a[190, 5] = -3.0, 3.0 + 2.0 ** -45
+    else:
+        a[20, 0], a[100, 0] = -2.0, 2.0
+        a[5, 1], a[170, 1] = 0.75, -0.75
+    mode = "fp32" if case.startswith("float") else "fp64"
+    matrix = a.astype(np.float32 if mode == "fp32" else np.float64)
+    _, rows = gpu_factor(device, matrix, mode)
+    _, want = oracle.factor(matrix)
+    assert (rows == want).all()
+    _, piv = scipy.linalg.lu_factor(matrix)
+    lapack_rows = np.arange(n)
+    for i, p in enumerate(piv):
+        lapack_rows[[i, p]] = lapack_rows[[p, i]]
+    assert (rows == lapack_rows).all()
+    assert rows[0] == (150 if case == "rounded_key_tie" else 20)
