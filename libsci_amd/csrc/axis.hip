@@ -199,9 +199,14 @@ namespace scihip {
             else if (registers == kMedium) hipLaunchKernelGGL((fold_columns<T, W, kMedium>), grid, dim3(256), 0, stream, launch);
             else hipLaunchKernelGGL((fold_columns<T, 1, kLarge>), grid, dim3(256), 0, stream, launch);
             if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
-            hipLaunchKernelGGL((finish_columns<T>), dim3(unsigned((launch.columns + 255) / 256)), dim3(256), 0, stream,
-                               launch.partials, launch.output, launch.columns, slices, launch.p.fold);
+            return finish_axis<T>(launch, slices, stream);
         }
+        return hipGetLastError();
+    }
+
+    template <typename T> hipError_t finish_axis(const AxisLaunch<T> &launch, uint32_t slices, hipStream_t stream) {
+        hipLaunchKernelGGL((finish_columns<T>), dim3(unsigned((launch.columns + 255) / 256)), dim3(256), 0, stream,
+                           launch.partials, launch.output, launch.columns, slices, launch.p.fold);
         return hipGetLastError();
     }
 
@@ -209,6 +214,8 @@ namespace scihip {
     template hipError_t transpose<double>(const double *, double *, int64_t, int64_t, hipStream_t);
     template hipError_t fold_axis<float>(const AxisLaunch<float> &, int, int, uint32_t, hipStream_t);
     template hipError_t fold_axis<double>(const AxisLaunch<double> &, int, int, uint32_t, hipStream_t);
+    template hipError_t finish_axis<float>(const AxisLaunch<float> &, uint32_t, hipStream_t);
+    template hipError_t finish_axis<double>(const AxisLaunch<double> &, uint32_t, hipStream_t);
     template hipError_t fill_value<float>(float *, uint64_t, float, hipStream_t);
     template hipError_t fill_value<double>(double *, uint64_t, double, hipStream_t);
 

@@ -87,6 +87,119 @@ namespace scihip {
         return o.str();
     }
 
+    std::string Compiler::generate_axis(const scihip_chunk &chunk, int axis, bool contract) {
+        const bool single = chunk.precision == SCIHIP_SINGLE;
+        const char *type = single ? "float" : "double";
+        const int width = single ? 4 : 2;
+        std::ostringstream o;
+        o << "extern \"C\" __global__ __launch_bounds__(256) void scihip_specialised(const scihip::AxisLaunch<" << type
+          << "> a) {\n"
+          << "    using namespace scihip;\n"
+          << "    typedef " << type << " T;\n"
+          << "    constexpr int W = " << width << ";\n"
+          << "    typedef Lanes<T, W> L;\n"
+          << "    const Program<T> &p = a.p;\n";
+        for (int i = 0; i < chunk.literals; ++i)
+            o << "    const L k" << i << " = splat<T, W>(p.constants[" << i << "]);\n";
+        const int steps = contract ? chunk.length - 1 : chunk.length;
+        // The chunk's instructions for vector u, as straight-line code over named values.
+        auto body = [&](int u, const char *indent) {
+            std::string temporaries[65];
+            const std::string suffix = "_" + std::to_string(u);
+            auto name = [&](uint8_t id) -> std::string {
+                const uint8_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3F;
+                if (tag == SCIHIP_SLOT_INPUT) return "i" + std::to_string(index) + suffix;
+                if (tag == SCIHIP_SLOT_LITERAL) return "k" + std::to_string(index);
+                if (tag == SCIHIP_SLOT_TEMPORARY) return temporaries[index];
+                return temporaries[64];
+            };
+            for (int s = 0; s < steps; ++s) {
+                const scihip_instruction &in = chunk.instructions[s];
+                const std::string first = name(in.sources[0]);
+                const std::string second = in.sources[1] == SCIHIP_SLOT_NONE ? first : name(in.sources[1]);
+                const std::string third = in.sources[2] == SCIHIP_SLOT_NONE ? first : name(in.sources[2]);
+                const std::string target = "s" + std::to_string(s) + suffix;
+                o << indent << "const L " << target << " = lanes<" << unsigned(in.code) << "u, T, W>(" << first << ", "
+                  << second << ", " << third << ");\n";
+                if (in.target == SCIHIP_SLOT_OUTPUT) temporaries[64] = target;
+                else temporaries[in.target & 0x3F] = target;
+            }
+            if (contract) {
+                const scihip_instruction &last = chunk.instructions[chunk.length - 1];
+                return std::make_pair(name(last.sources[0]), name(last.sources[1]));
+            }
+            return std::make_pair(name(chunk.result), std::string());
+        };
+        if (axis == 1) {
+            o << "    double total = identity(p.fold);\n    uint32_t poisoned = 0;\n"
+              << "    const uint64_t first = (uint64_t)blockIdx.x * a.columns, end = first + a.columns;\n"
+              << "    const uint64_t vectors = (a.columns + W - 1) / W;\n"
+              << "    for (uint64_t base = threadIdx.x; base < vectors; base += 256 * " << kUnroll << ") {\n";
+            for (int u = 0; u < kUnroll; ++u)
+                o << "        const uint64_t v" << u << " = base + " << 256 * u << ";\n"
+                  << "        const bool live" << u << " = v" << u << " < vectors;\n"
+                  << "        const uint64_t at" << u << " = first + v" << u << " * W;\n"
+                  << "        const bool full" << u << " = at" << u << " + W <= end;\n";
+            for (int u = 0; u < kUnroll; ++u)
+                for (int i = 0; i < chunk.arguments; ++i)
+                    o << "        L i" << i << "_" << u << "{}; if (live" << u << ") i" << i << "_" << u
+                      << " = fetch<T, W>(p, " << i << ", at" << u << ", full" << u << ");\n";
+            for (int u = 0; u < kUnroll; ++u) {
+                o << "        if (live" << u << ") {\n";
+                const auto result = body(u, "            ");
+                if (contract)
+                    o << "            contract<T, W>(total, " << result.first << ", " << result.second << ", at" << u
+                      << ", full" << u << ", end);\n";
+                else
+                    o << "            consume<T, W>(p.fold, total, poisoned, " << result.first << ", at" << u << ", full"
+                      << u << ", end);\n";
+                o << "        }\n";
+            }
+            o << "    }\n"
+              << "    const Partial folded = fold_block(p.fold, total, poisoned);\n"
+              << "    if (threadIdx.x == 0) a.output[blockIdx.x] = folded.poisoned ? T(__builtin_nan(\"\")) : T(folded.value);\n";
+        } else {
+            o << "    const uint64_t column = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * W;\n"
+              << "    if (column >= a.columns) return;\n"
+              << "    const uint64_t per_slice = (a.rows + gridDim.y - 1) / gridDim.y;\n"
+              << "    const uint64_t begin = (uint64_t)blockIdx.y * per_slice;\n"
+              << "    const uint64_t end = begin + per_slice < a.rows ? begin + per_slice : a.rows;\n"
+              << "    const bool full = column + W <= a.columns;\n"
+              << "    const uint32_t fold = p.fold;\n"
+              << "    double total[W];\n    uint32_t poisoned[W];\n"
+              << "#pragma unroll\n    for (int k = 0; k < W; ++k) { total[k] = identity(fold); poisoned[k] = 0; }\n"
+              << "    for (uint64_t r = begin; r < end; r += " << kUnroll << ") {\n";
+            for (int u = 0; u < kUnroll; ++u)
+                o << "        const bool live" << u << " = r + " << u << " < end;\n"
+                  << "        const uint64_t at" << u << " = (r + " << u << ") * a.columns + column;\n";
+            for (int u = 0; u < kUnroll; ++u)
+                for (int i = 0; i < chunk.arguments; ++i)
+                    o << "        L i" << i << "_" << u << "{}; if (live" << u << ") i" << i << "_" << u
+                      << " = fetch<T, W>(p, " << i << ", at" << u << ", full);\n";
+            for (int u = 0; u < kUnroll; ++u) {
+                o << "        if (live" << u << ") {\n";
+                const auto result = body(u, "            ");
+                if (contract)
+                    o << "#pragma unroll\n            for (int k = 0; k < W; ++k) total[k] = fused((double)" << result.first
+                      << ".v[k], (double)" << result.second << ".v[k], total[k]);\n";
+                else
+                    o << "#pragma unroll\n            for (int k = 0; k < W; ++k) {\n"
+                      << "                const double x = (double)" << result.first << ".v[k];\n"
+                      << "                if (fold == 0) total[k] += x;\n"
+                      << "                else if (unordered(x)) poisoned[k] = 1;\n"
+                      << "                else total[k] = combine(fold, total[k], x);\n"
+                      << "            }\n";
+                o << "        }\n";
+            }
+            o << "    }\n"
+              << "    Partial *slice = a.partials + (uint64_t)blockIdx.y * a.columns;\n"
+              << "#pragma unroll\n    for (int k = 0; k < W; ++k)\n"
+              << "        if (full || column + k < a.columns) slice[column + k] = Partial{total[k], poisoned[k], 0u};\n";
+        }
+        o << "}\n";
+        return o.str();
+    }
+
     Compiler::Compiler(int ordinal) : ordinal(ordinal) {
         if (const char *text = std::getenv("SCIHIP_JIT")) trace = std::strcmp(text, "trace") == 0;
         worker = std::thread([this] { loop(); });

@@ -204,6 +204,16 @@ namespace scihip {
         double *result;    // reduce: the result
     };
 
+    // A chunk folded along one axis of a rows x columns row-major matrix (every input of the chunk is
+    // such a matrix): output[r] folds row r (axis 1), output[c] folds column c (axis 0); one argument
+    // for the interpreter and specialised kernels alike.
+    template <typename T> struct AxisLaunch {
+        Program<T> p;         // p.count = rows * columns; p.aligned: every row starts 16-byte aligned
+        T *output;
+        Partial *partials;    // axis 0: slices x columns
+        uint64_t rows, columns;
+    };
+
     __device__ __forceinline__ double combine(uint32_t fold, double a, double b) {
         return fold == 0 ? a + b : (fold == 1 ? (b < a ? b : a) : (b > a ? b : a));
     }

@@ -471,17 +471,41 @@ namespace scihip {
         a.rows = rows;
         a.columns = columns;
         a.partials = nullptr;
+        // The chunk's specialised axis kernel when ready (or forced), else the interpreter.
+        const bool contract = contracted(chunk, fold);
+        const Kernel *kernel = nullptr;
+        if (device->policy != 1) {
+            kernel = device->compiler->find(key_of(chunk, 3 + int(axis), contract),
+                                            [&] { return Compiler::generate_axis(chunk, int(axis), contract); },
+                                            device->policy == 2);
+            if (!kernel && device->policy == 2) {
+                fail("specialised kernel failed to compile");
+                return SCIHIP_ECOMPILE;
+            }
+        }
         uint32_t slices = 1;
         int rc = SCIHIP_OK;
         std::unique_ptr<Scratch> partials;
         if (axis == 0) {
-            const int w = registers == kLarge ? 1 : Traits<T>::width;
+            const int w = (registers == kLarge && !kernel) ? 1 : Traits<T>::width;
             const size_t tiles = (columns + 256 * size_t(w) - 1) / (256 * size_t(w));
             const size_t wanted = (size_t(device->units) * 8 + tiles - 1) / tiles;
             slices = uint32_t(std::max<size_t>(1, std::min<size_t>({wanted, rows, 65535})));
             partials = std::make_unique<Scratch>(device, size_t(slices) * columns * sizeof(Partial), rc);
             if (rc) return rc;
             a.partials = partials->as<Partial>();
+        }
+        if (kernel) {
+            ++device->specialised;
+            const unsigned grid_x = axis == 1 ? unsigned(rows)
+                                              : unsigned((columns + 256 * size_t(Traits<T>::width) - 1) /
+                                                         (256 * size_t(Traits<T>::width)));
+            void *parameters[] = {&a};
+            if ((rc = status(hipModuleLaunchKernel(kernel->function, grid_x, axis == 1 ? 1 : slices, 1, 256, 1, 1, 0,
+                                                   device->stream, parameters, nullptr),
+                             "specialised axis launch")))
+                return rc;
+            return axis == 0 ? status(finish_axis<T>(a, slices, device->stream), "axis reduction") : SCIHIP_OK;
         }
         ++device->interpreted;
         return status(fold_axis<T>(a, registers, int(axis), slices, device->stream), "axis reduction");

@@ -63,10 +63,11 @@ def expression_of(program, arrays):
     return arrays[0] * arrays[1]
 
 
+@pytest.mark.parametrize("policy", [1, 2], ids=["interpreter", "specialised"])
 @pytest.mark.parametrize("name", ["plain", "fused", "dot"])
 @pytest.mark.parametrize("axis", [0, 1])
 @pytest.mark.parametrize("shape", [(300, 257), (64, 1000), (1000, 64), (1, 5), (5, 1)], ids=str)
-def test_axis_folds_match_oracle(device, name, axis, shape):
+def test_axis_folds_match_oracle(device, name, axis, shape, policy):
     program = {"plain": PLAIN, "fused": FUSED, "dot": DOT}[name]
     rng = np.random.default_rng(shape[0] * 7 + shape[1] + axis)
     matrices = [rng.uniform(-4, 4, shape).astype(np.float32) for _ in range(2)]
@@ -75,7 +76,11 @@ def test_axis_folds_match_oracle(device, name, axis, shape):
     arrays = [sci.Array.from_numpy(m, device) for m in matrices]
     folds = [Fold.sum] if program is DOT else [Fold.sum, Fold.minimum, Fold.maximum]
     for fold in folds:
-        got = sci.reduce_axis(expression_of(program, arrays), fold, axis).host()
+        device.set_policy(policy)
+        try:
+            got = sci.reduce_axis(expression_of(program, arrays), fold, axis).host()
+        finally:
+            device.set_policy(0)
         lined = [np.ascontiguousarray(m.T) if axis == 0 else m for m in matrices]
         want = oracle_rows(program, lined, fold, Precision.single)
         assert np.array_equal(np.isnan(got), np.isnan(want)), (fold, axis)
