@@ -17,6 +17,7 @@
 // unchanged.
 #include "product.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace scihip {
@@ -161,15 +162,18 @@ namespace scihip {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
-            const int64_t tiles_k = p.kp / S::k;
+            // k-tiles [k_begin, k_begin + tiles_k): all of them, or this workgroup's slice of them.
+            const int64_t tiles_all = p.kp / S::k;
+            const int64_t k_begin = tiles_all * blockIdx.y / gridDim.y;
+            const int64_t tiles_k = tiles_all * (blockIdx.y + 1) / gridDim.y - k_begin;
             if (tiles_k > 0) {
-                fetch(0);
+                fetch(k_begin * S::row);
                 deposit(0);
                 __syncthreads();
             }
             for (int64_t kt = 0; kt < tiles_k; ++kt) {
                 const int stage = (int)(kt & 1);
-                if (kt + 1 < tiles_k) fetch((kt + 1) * S::row);
+                if (kt + 1 < tiles_k) fetch((k_begin + kt + 1) * S::row);
                 const unsigned char *base = lds + stage * stage_bytes;
 #pragma unroll
                 for (int s = 0; s < S::steps; ++s) {
@@ -200,7 +204,7 @@ namespace scihip {
             // so their latencies overlap (stores to the same array would otherwise order every load
             // after the store before it).
             typedef typename std::conditional<M == kFloat64, double, float>::type CT;
-            CT *const c_base = static_cast<CT *>(p.c);
+            CT *const c_base = static_cast<CT *>(p.c) + (int64_t)blockIdx.y * p.slice_stride;
             const CT alpha = (CT)p.alpha, beta = (CT)p.beta;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -267,27 +271,78 @@ namespace scihip {
 
         template <int M> constexpr int lds_bytes() { return 2 * Shape<M>::operands * kTile * Shape<M>::stride; }
 
+        // C[r][c] = alpha * sum over slices s (in order) of partials[s][r][c] + beta * C[r][c].
+        template <typename CT>
+        __global__ __launch_bounds__(256) void fold_slices(const CT *partials, CT *c, int64_t m, int64_t n,
+                                                           int64_t ldc, int slices, CT alpha, CT beta) {
+            const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+            if (idx >= m * n) return;
+            const int64_t r = idx / n, col = idx % n;
+            CT sum = partials[idx];
+            for (int s = 1; s < slices; ++s) sum += partials[(int64_t)s * m * n + idx];
+            CT *target = c + r * ldc + col;
+            *target = beta == CT(0) ? alpha * sum : alpha * sum + beta * *target;
+        }
+
     } // namespace
 
-    hipError_t product(int mode, const ProductArgs &args, hipStream_t stream) {
-        const int64_t tiles = ((args.m + kTile - 1) / kTile) * ((args.n + kTile - 1) / kTile);
-        if (tiles == 0) return hipSuccess;
-        const dim3 grid((unsigned)tiles), block(kThreads);
-        switch (mode) {
-        case kFloat32:
-            hipLaunchKernelGGL(gemm<kFloat32>, grid, block, lds_bytes<kFloat32>(), stream, args);
-            break;
-        case kBfloat16:
-            hipLaunchKernelGGL(gemm<kBfloat16>, grid, block, lds_bytes<kBfloat16>(), stream, args);
-            break;
-        case kSplit:
-            hipLaunchKernelGGL(gemm<kSplit>, grid, block, lds_bytes<kSplit>(), stream, args);
-            break;
-        default:
-            hipLaunchKernelGGL(gemm<kFloat64>, grid, block, lds_bytes<kFloat64>(), stream, args);
-            break;
+    namespace {
+        hipError_t launch_gemm(int mode, const ProductArgs &args, int slices, hipStream_t stream) {
+            const int64_t tiles = ((args.m + kTile - 1) / kTile) * ((args.n + kTile - 1) / kTile);
+            if (tiles == 0) return hipSuccess;
+            const dim3 grid((unsigned)tiles, (unsigned)slices), block(kThreads);
+            switch (mode) {
+            case kFloat32:
+                hipLaunchKernelGGL(gemm<kFloat32>, grid, block, lds_bytes<kFloat32>(), stream, args);
+                break;
+            case kBfloat16:
+                hipLaunchKernelGGL(gemm<kBfloat16>, grid, block, lds_bytes<kBfloat16>(), stream, args);
+                break;
+            case kSplit:
+                hipLaunchKernelGGL(gemm<kSplit>, grid, block, lds_bytes<kSplit>(), stream, args);
+                break;
+            default:
+                hipLaunchKernelGGL(gemm<kFloat64>, grid, block, lds_bytes<kFloat64>(), stream, args);
+                break;
+            }
+            return hipGetLastError();
         }
+    } // namespace
+
+    hipError_t product_split(int mode, const ProductArgs &args, int slices, void *partials, hipStream_t stream) {
+        if (slices <= 1) return product(mode, args, stream);
+        if (args.m == 0 || args.n == 0) return hipSuccess;
+        ProductArgs sliced = args;
+        sliced.c = partials;
+        sliced.ldc = args.n;
+        sliced.alpha = 1.0;
+        sliced.beta = 0.0;
+        sliced.slice_stride = args.m * args.n;
+        hipError_t e = launch_gemm(mode, sliced, slices, stream);
+        if (e != hipSuccess) return e;
+        const unsigned blocks = (unsigned)((args.m * args.n + 255) / 256);
+        if (mode == kFloat64)
+            hipLaunchKernelGGL(fold_slices<double>, dim3(blocks), dim3(256), 0, stream,
+                               static_cast<const double *>(partials), static_cast<double *>(args.c), args.m, args.n,
+                               args.ldc, slices, args.alpha, args.beta);
+        else
+            hipLaunchKernelGGL(fold_slices<float>, dim3(blocks), dim3(256), 0, stream,
+                               static_cast<const float *>(partials), static_cast<float *>(args.c), args.m, args.n,
+                               args.ldc, slices, (float)args.alpha, (float)args.beta);
         return hipGetLastError();
+    }
+
+    int split_slices(int mode, int64_t m, int64_t n, int64_t kp) {
+        const int64_t tiles = ((m + kTile - 1) / kTile) * ((n + kTile - 1) / kTile);
+        if (tiles == 0 || tiles >= 256) return 1;
+        const int64_t by_depth = kp / k_tile(mode) / 8, by_chip = (1024 + tiles - 1) / tiles;
+        return (int)std::max<int64_t>(1, std::min<int64_t>({16, by_depth, by_chip}));
+    }
+
+    hipError_t product(int mode, const ProductArgs &args, hipStream_t stream) {
+        ProductArgs whole = args;
+        whole.slice_stride = 0;
+        return launch_gemm(mode, whole, 1, stream);
     }
 
     hipError_t pack(int mode, const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,

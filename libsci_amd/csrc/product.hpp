@@ -26,6 +26,7 @@ namespace scihip {
         int64_t m, n, kp;
         int64_t lda, ldb, ldc;
         double alpha, beta;
+        int64_t slice_stride; // split-K launches: elements between consecutive slices' outputs
     };
 
     // Bytes of one k-tile row in LDS and the k extent it holds, per mode.
@@ -34,6 +35,17 @@ namespace scihip {
 
     // Launches the product of packed operands. Kp must be a multiple of k_tile(mode).
     hipError_t product(int mode, const ProductArgs &args, hipStream_t stream);
+
+    // The same product with the k-tiles split into `slices` ranges run by separate workgroups (for
+    // products with few output tiles and a long shared dimension): each slice writes its partial
+    // product to `partials` (slices x m x n, in the output type), then a second kernel folds the
+    // slices in order into C with alpha and beta, so the result does not depend on timing.
+    hipError_t product_split(int mode, const ProductArgs &args, int slices, void *partials, hipStream_t stream);
+
+    // How many slices product_split should use for an m x n output over kp: 1 unless the output has
+    // fewer tiles than the chip has CUs; then enough for about four workgroups per CU, each slice
+    // keeping at least 8 k-tiles, at most 16 slices.
+    int split_slices(int mode, int64_t m, int64_t n, int64_t kp);
 
     // Packs a row-major source into the product's operand layout: out[i][k] = src[i][k] (transpose
     // false) or src[k][i] (transpose true), for i < rows and k < depth, zero for depth <= k < kp.

@@ -334,6 +334,9 @@ namespace scihip {
         if (!rc) rc = device->allocator->allocate(b_bytes, &b_hi);
         if (!rc && split) rc = device->allocator->allocate(a_bytes, &a_lo);
         if (!rc && split) rc = device->allocator->allocate(b_bytes, &b_lo);
+        const int slices = split_slices(mode, rows, columns, kp);
+        void *partials = nullptr;
+        if (!rc && slices > 1) rc = device->allocator->allocate(size_t(slices) * rows * columns * (wide ? 8 : 4), &partials);
         if (!rc)
             rc = status(pack(mode, left, wide, rows, depth, left_ld, left_transposed, a_hi, a_lo, kp, device->stream),
                         "pack");
@@ -343,9 +346,9 @@ namespace scihip {
                         "pack");
         if (!rc) {
             ProductArgs args{a_hi, a_lo, b_hi, b_lo, output, rows, columns, kp, kp, kp, columns, 1.0, 0.0};
-            rc = status(product(mode, args, device->stream), "product");
+            rc = status(product_split(mode, args, slices, partials, device->stream), "product");
         }
-        for (void *block : {a_hi, a_lo, b_hi, b_lo})
+        for (void *block : {a_hi, a_lo, b_hi, b_lo, partials})
             if (block) device->allocator->dispose(block);
         return rc;
     }
@@ -883,6 +886,9 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     const bool blocked = blocked_substitution(n);
     Scratch inverses(device, blocked ? 2 * order * kBlock * eb : 16, rc);
     Scratch packed(device, blocked ? width * kBlock * eb : 16, rc);
+    // The residual product has few output tiles (n / 128 x 1) and a long shared dimension.
+    const int slices = split_slices(kFloat64, n, w, kp);
+    Scratch residual_slices(device, slices > 1 ? size_t(slices) * order * width * 8 : 16, rc);
     if (rc) return rc;
     if ((rc = ensure_lu(device, n, eb))) return rc;
 
@@ -958,7 +964,8 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
                          "pack")))
             return rc;
         ProductArgs args{a_operand, nullptr, packed_x.pointer, nullptr, residual.pointer, n, w, kp, kp, kp, w, -1.0, 1.0};
-        if ((rc = status(product(kFloat64, args, device->stream), "residual"))) return rc;
+        if ((rc = status(product_split(kFloat64, args, slices, residual_slices.pointer, device->stream), "residual")))
+            return rc;
         double *partials = partial.as<double>();
         if ((rc = status(column_max(residual.as<double>(), n, w, partials, device->stream), "norm"))) return rc;
         if ((rc = status(column_max(solution, n, w, partials + chunks * w, device->stream), "norm"))) return rc;

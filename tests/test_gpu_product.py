@@ -42,7 +42,10 @@ def test_reference_kat(device, mode):
     assert np.abs(got - want).max() < 1e-3  # the entries are exact in bf16, so every mode is exact here
 
 
-@pytest.mark.parametrize("shape", [(1, 1, 1), (5, 3, 7), (129, 257, 130), (300, 64, 200), (512, 512, 512)])
+# (64, 4096, 64) and (200, 3000, 130) have few output tiles and a long shared dimension, so their
+# shared dimension is split across workgroups and the slices folded afterwards.
+@pytest.mark.parametrize("shape", [(1, 1, 1), (5, 3, 7), (129, 257, 130), (300, 64, 200), (512, 512, 512),
+                                   (64, 4096, 64), (200, 3000, 130)])
 @pytest.mark.parametrize("mode", list(MODES))
 def test_against_float64(device, mode, shape):
     m, k, n = shape
@@ -55,6 +58,16 @@ def test_against_float64(device, mode, shape):
     want = left.astype(np.float64) @ right.astype(np.float64)
     bound = 2 * k * UNIT[mode] * (np.abs(left).astype(np.float64) @ np.abs(right).astype(np.float64))
     assert (np.abs(got - want) <= bound + 1e-300).all(), np.abs(got - want).max()
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_split_depth_is_deterministic(device, mode):
+    """A split product folds its slices in a fixed order: repeated calls agree bit for bit."""
+    left = oracle.uniform(96 * 8192, 5, -1.0, 1.0, dtype=np.float64).reshape(96, 8192)
+    right = oracle.uniform(8192 * 40, 6, -1.0, 1.0, dtype=np.float64).reshape(8192, 40)
+    first = gpu_product(device, left, right, mode)
+    for _ in range(3):
+        assert np.array_equal(gpu_product(device, left, right, mode), first)
 
 
 def test_zero_depth_writes_zeros(device):
