@@ -135,88 +135,294 @@ namespace scihip {
             return j0 + (int64_t)(0xFFFFFFu - (unsigned)((word >> 8) & 0xFFFFFFu));
         }
 
-        constexpr int kLine = 16;          // 64-bit words per 128-byte line: one line per candidate word
-        constexpr int kPanelThreads = 64;  // one wave per panel workgroup: its barriers cost nothing
-        constexpr int kOwned = (kPanelRows + kPanelThreads - 1) / kPanelThreads; // panel rows per lane, at most
+        __device__ __forceinline__ bool has_candidate(unsigned long long word) { return ((word >> 8) & 0xFFFFFFu) != 0; }
 
-        // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b is one wave
-        // and owns rows [j0 + b * kPanelRows, ...); it keeps their panel part in LDS for the whole
-        // launch, lane l searching rows l, l + 64, ... Per column, every workgroup publishes its candidate
-        // (first maximum of |a| among its rows at or below the diagonal) with that row's panel values,
-        // then its candidate word (candidate_word), which is its arrival flag and carries the candidate,
-        // so one round trip gathers every candidate; every workgroup takes the largest word, swaps and
-        // eliminates; the next column's candidate is published before the bulk of the elimination,
-        // which then overlaps the words' round trip. Publications have two parities (column c uses
-        // c & 1): a workgroup overwrites parity p only after every word of the column in between has
-        // arrived, that is after every workgroup has finished reading parity p. value[2][G] (exact
-        // |a|, for ties of the float key), rows[2][G][kPanel], diagonal[2][kPanel], words[2][G] (one
-        // 128-byte line each).
+        constexpr int kLine = 16;          // 64-bit slots per 128-byte line: one line per workgroup and column
+        constexpr int kParities = 4;       // publication buffers, used in turn by consecutive columns
+        constexpr int kPanelThreads = 256; // four waves per panel workgroup
+        constexpr int kWaves = kPanelThreads / 64;
+        constexpr int kRowWaves = kPanelRows / 64;          // the waves whose threads own a row each
+        constexpr int kParts = kPanelThreads / kPanelRows;  // threads sharing a row in the bulk update
+        constexpr int kPoll = 2;           // lines a thread polls at once
+        static_assert(kPanelRows % 64 == 0 && kPanelThreads % kPanelRows == 0 && kPanelThreads == 2 * kPanel,
+                      "panel layout: a thread per row, two threads per panel column");
+
+        // Values that panel workgroups read from each other travel in 64-bit slots: the upper half
+        // holds 32 bits of the value, the lower half a tag (launch << 8 | column + 1). Each slot is
+        // stored with one 64-bit atomic store and read whole (a 64-bit atomic load, or one half of a
+        // 16-byte load), so a reader that finds the expected tag in every slot of a value holds the
+        // current value, in whatever order the writer's stores land.
+        template <typename T> constexpr int kSlots = (int)(sizeof(T) / 4);
+
+        template <typename T> __device__ __forceinline__ void put_tagged(unsigned long long *slots, T v, unsigned tag) {
+            auto half = [&](unsigned long long *slot, unsigned bits) {
+                __hip_atomic_store(slot, (unsigned long long)bits << 32 | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            };
+            if constexpr (sizeof(T) == 8) {
+                const unsigned long long bits = __builtin_bit_cast(unsigned long long, v);
+                half(slots, (unsigned)(bits >> 32));
+                half(slots + 1, (unsigned)bits);
+            } else {
+                half(slots, __builtin_bit_cast(unsigned, v));
+            }
+        }
+        template <typename T> __device__ __forceinline__ bool current(const unsigned long long *slot, unsigned tag) {
+            bool ok = true;
+#pragma unroll
+            for (int i = 0; i < kSlots<T>; ++i) ok = ok && (unsigned)slot[i] == tag;
+            return ok;
+        }
+        template <typename T> __device__ __forceinline__ T value_of(const unsigned long long *slot) {
+            if constexpr (sizeof(T) == 8)
+                return __builtin_bit_cast(T, (slot[0] & 0xFFFFFFFF00000000ull) | (slot[1] >> 32));
+            else
+                return __builtin_bit_cast(T, (unsigned)(slot[0] >> 32));
+        }
+
+        // 16-byte loads of two slots that bypass the caches that are not coherent across the device
+        // (the agent-scope cache policy), through a buffer resource over a publication buffer.
+        constexpr int kBufferFlags = 0x00020000; // raw buffer, 32-bit elements
+        constexpr int kCoherent = 16;            // cache policy: device (agent) scope
+        __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const void *base) {
+            return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0x7FFFFFFF, kBufferFlags);
+        }
+        __device__ __forceinline__ void load_pair(__amdgpu_buffer_rsrc_t buffer, unsigned byte, unsigned long long *slot) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(buffer, byte, 0, kCoherent);
+            slot[0] = (unsigned long long)v[1] << 32 | v[0];
+            slot[1] = (unsigned long long)v[3] << 32 | v[2];
+        }
+
+        // A bounded wait for a tagged value (after a failed wait the value is garbage and *error is set).
+        template <typename T> __device__ T take(const unsigned long long *slots, unsigned tag, unsigned *error) {
+            unsigned long long x[kSlots<T>];
+            unsigned spins = 0;
+            for (;;) {
+#pragma unroll
+                for (int i = 0; i < kSlots<T>; ++i)
+                    x[i] = __hip_atomic_load(slots + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (current<T>(x, tag)) break;
+                if (++spins > kSpinLimit) {
+                    __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                if ((spins & 63u) == 0 && relaxed(error)) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            return value_of<T>(x);
+        }
+
+        // x - m u with one rounding: every place of the panel that applies an elimination computes it
+        // this way, so a value rebuilt from a publication equals the one kept in place.
+        template <typename T> __device__ __forceinline__ T eliminated(T x, T m, T u) {
+            if constexpr (sizeof(T) == 8) return __builtin_fma(-m, u, x);
+            else return __builtin_fmaf(-m, u, x);
+        }
+
+        // Fields of a candidate line after slot 0 (the candidate word), kSlots<T> slots each: the
+        // candidate's value in the column, its value in the next column as stored when published,
+        // and the multiplier of the update the next column still lacks then (0 if none); the same
+        // three for the diagonal row, in the line of the workgroup that owns it.
+        enum : int { kValue, kNext, kFactor, kDiagonal, kDiagonalNext, kDiagonalFactor };
+        template <typename T> __device__ __forceinline__ constexpr int field(int f) { return 1 + f * kSlots<T>; }
+        template <typename T> constexpr int kMainPairs = (field<T>(kDiagonal) + 1) / 2; // word + candidate fields
+        template <typename T> constexpr int kAllPairs = (field<T>(kDiagonalFactor + 1) + 1) / 2;
+
+        // Workgroup barrier for LDS traffic only: the publications (global stores) in flight are not
+        // waited for, nothing in the panel needs them ordered.
+        __device__ __forceinline__ void settle() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        }
+
+        __device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+            for (int offset = 1; offset < 64; offset <<= 1) x += __shfl_xor(x, offset, 64);
+            return x;
+        }
+
+        // Factors columns [j0, j0 + w) of rows [j0, n) (see lu.hpp, step 1). Workgroup b (four waves)
+        // owns rows [j0 + b * kPanelRows, ...) and keeps their panel part in LDS for the whole launch;
+        // thread t < kPanelRows owns row t of them.
+        //
+        // Per column c, every workgroup publishes its candidate (first maximum of |a| among its rows at
+        // or below the diagonal): the candidate's panel row, the diagonal row if it owns it, and a line
+        // holding the candidate word (candidate_word, also the arrival flag) and the fields the next
+        // step needs at once. Every workgroup polls every line together with its fields, so one round
+        // trip gives the pivot p, the pivot value and the pivot row's next entry, and the diagonal
+        // row's two entries: it swaps and eliminates column c, updates column c + 1 and publishes its
+        // candidate for c + 1 before it reads the rest of the pivot row and of the diagonal row, whose
+        // round trip overlaps that of the next column's lines.
+        //
+        // Publications are made as the rows stand in LDS, whose columns beyond c + 1 still lack column
+        // c's update then; the reader applies it (eliminated() with the published multiplier and the
+        // previous pivot row, which every workgroup keeps). The workgroup that receives the old
+        // diagonal row in place of the pivot row has only its columns c and c + 1 at that point; if
+        // that row is its next candidate or the next diagonal row, it reads the rest first.
+        //
+        // Buffers (kParities of each, column c uses c % kParities; a buffer is rewritten three
+        // columns later, after every workgroup has read it): lines[4][G][kLine], rows[4][G][kPanel *
+        // kSlots] and diagonal[4][kPanel * kSlots], in tagged slots. Lines are zeroed before each
+        // launch; rows and diagonal are told apart by the launch number in their tags.
         template <typename T>
         __global__ __launch_bounds__(kPanelThreads) void panel(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
-                                                                T *pub_value, T *pub_rows, T *diagonal,
-                                                                unsigned long long *words, unsigned *error) {
+                                                                unsigned long long *lines,
+                                                                unsigned long long *published,
+                                                                unsigned long long *diagonal, unsigned launch,
+                                                                unsigned *error) {
             constexpr int stride = kPanel + 1;
+            constexpr int S = kSlots<T>;
             const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+            const int lane = t & 63, wave = t >> 6;
+            const int k_own = t & (kPanel - 1), part = t / kPanel; // a panel column and which half of the threads
             const int64_t first = j0 + (int64_t)b * kPanelRows;
             const int mine = (int)min((int64_t)kPanelRows, n - first); // rows this workgroup owns
             __shared__ T rows[kPanelRows * stride];
-            __shared__ T pivot_row[kPanel];
-            __shared__ T diagonal_row[kPanel];
+            __shared__ T pivot_rows[2][kPanel]; // the pivot row of column c (c & 1) and of column c - 1
             __shared__ T multipliers[kPanelRows];
+            __shared__ unsigned long long wave_word[kWaves]; // gather: each wave's largest word
+            __shared__ int wave_sharing[kWaves];             // and how many lines carry its key
+            __shared__ T wave_fields[kWaves][3];             // and that word's fields
+            __shared__ T diagonal_fields[3];
+            __shared__ T search_value[kWaves];
+            __shared__ int64_t search_row[kWaves];
 
-            for (int i = 0; i < mine; ++i)
-                for (int k = t; k < w; k += kPanelThreads) rows[i * stride + k] = a[(first + i) * n + j0 + k];
-            __syncthreads();
+            const __amdgpu_buffer_rsrc_t line_buffer = buffer_over(lines);
+            const __amdgpu_buffer_rsrc_t row_buffer = buffer_over(published);
+            const __amdgpu_buffer_rsrc_t diagonal_buffer = buffer_over(diagonal);
 
-            // Publishes this workgroup's candidate for column `col` (the first maximum of |a| among its
-            // rows at or below the diagonal) with the candidate's panel row, and the diagonal row if it
-            // owns it; then raises its arrival flag for `col`. Column `col` of the rows is up to date;
-            // with `pending`, their columns beyond it still lack the previous column's update, which
-            // is applied to the published values (x - l * u, as the elimination will store it).
-            auto publish = [&](int col, bool pending) {
+            for (int i = part; i < mine; i += kParts)
+                if (k_own < w) rows[i * stride + k_own] = a[(first + i) * n + j0 + k_own];
+            if (t < kPanelRows) multipliers[t] = T(0);
+            settle();
+
+            auto tag_of = [&](int col) { return launch << 8 | (unsigned)(col + 1); };
+            auto line_index = [&](int col, int g) { return ((int64_t)(col & (kParities - 1)) * G + g) * kLine; };
+            auto row_index = [&](int col, int g) { return ((int64_t)(col & (kParities - 1)) * G + g) * kPanel * S; };
+            auto diagonal_index = [&](int col) { return (int64_t)(col & (kParities - 1)) * kPanel * S; };
+
+            // This workgroup's candidate row for column `col` (the same in every thread; LLONG_MAX if none).
+            auto search = [&](int col) -> int64_t {
                 const int64_t dd = j0 + col;
-                const int par = col & 1;
-                auto value = [&](int i, int k) -> T {
-                    const T x = rows[i * stride + k];
-                    return (pending && k > col) ? x - multipliers[i] * pivot_row[k] : x;
-                };
                 T best = T(-1);
                 int64_t row = LLONG_MAX;
-#pragma unroll
-                for (int o = 0; o < kOwned; ++o) {
-                    const int i = t + o * kPanelThreads;
-                    if (i < mine && first + i >= dd) better(best, row, T(__builtin_fabs(rows[i * stride + col])), first + i);
+                if (t < mine && first + t >= dd) {
+                    best = T(__builtin_fabs(rows[t * stride + col]));
+                    row = first + t;
                 }
                 wave_best(best, row);
-                if (t == 0) put(&pub_value[par * G + b], best); // the exact value, for ties of the float key
-                if (row != LLONG_MAX)
-                    for (int k = t; k < w; k += kPanelThreads)
-                        put(&pub_rows[((int64_t)par * G + b) * kPanel + k], value((int)(row - first), k));
-                if (dd >= first && dd < first + mine)
-                    for (int k = t; k < w; k += kPanelThreads) put(&diagonal[par * kPanel + k], value((int)(dd - first), k));
-                __syncthreads(); // every publication has completed before the word that announces it
-                if (t == 0)
-                    __hip_atomic_store(&words[((int64_t)par * G + b) * kLine],
-                                       candidate_word<T>(best, row, j0, (unsigned)col + 1), __ATOMIC_RELAXED,
+                if (lane == 0) {
+                    search_value[wave] = best;
+                    search_row[wave] = row;
+                }
+                settle();
+                best = search_value[0];
+                row = search_row[0];
+#pragma unroll
+                for (int q = 1; q < kRowWaves; ++q) better(best, row, search_value[q], search_row[q]);
+                return row;
+            };
+            // Publishes the candidate `row` for column `col` (threads of part 0), and the diagonal row if
+            // this workgroup owns it (part 1); `pending`: the rows' columns beyond col still lack the
+            // update of column col - 1, whose multipliers are in `multipliers`.
+            auto publish = [&](int col, int64_t row, bool pending) {
+                const int64_t dd = j0 + col;
+                const unsigned tag = tag_of(col);
+                unsigned long long *line = lines + line_index(col, b);
+                auto send = [&](int i, unsigned long long *target, int fields) {
+                    if (k_own < w) put_tagged(target + k_own * S, rows[i * stride + k_own], tag);
+                    if (k_own == 0) {
+                        put_tagged(line + field<T>(fields), rows[i * stride + col], tag);
+                        if (col + 1 < w) put_tagged(line + field<T>(fields + 1), rows[i * stride + col + 1], tag);
+                        put_tagged(line + field<T>(fields + 2), pending ? multipliers[i] : T(0), tag);
+                    }
+                };
+                if (part == 0 && row != LLONG_MAX) send((int)(row - first), published + row_index(col, b), kValue);
+                if (part == 1 && dd >= first && dd < first + mine)
+                    send((int)(dd - first), diagonal + diagonal_index(col), kDiagonal);
+                if (t == 0) {
+                    const T best = row != LLONG_MAX ? T(__builtin_fabs(rows[(row - first) * stride + col])) : T(-1);
+                    __hip_atomic_store(line, candidate_word<T>(best, row, j0, (unsigned)col + 1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
+                }
             };
 
-            publish(0, false);
+            publish(0, search(0), false);
+            bool previous = false; // whether column c - 1 eliminated (its update is pending in publications)
             for (int c = 0; c < w; ++c) {
                 const int64_t d = j0 + c;
-                const int parity = c & 1;
-                const unsigned epoch = (unsigned)c + 1;
-                // A: every workgroup gathers: lane g polls workgroup g's candidate word until it carries
-                // this column, and the wave takes the largest word, so all workgroups pick the same pivot.
-                // Equal float keys of different doubles are settled with the exact values.
-                unsigned long long mine_best = 0; // this lane's largest word, and how many share its key
+                const unsigned tag = tag_of(c);
+                const unsigned epoch = ((unsigned)c + 1) & 0xFFu;
+                const int owner_d = (int)((d - j0) / kPanelRows);
+                const bool more = c + 1 < w;
+                T *const u = pivot_rows[c & 1];
+                const T *const u_before = pivot_rows[(c + 1) & 1];
+
+                // A: gather. Thread g polls the lines of workgroups g, g + 256, ... (kPoll at once) with their
+                // fields until each carries this column, and keeps the largest word with its fields; the
+                // workgroup takes the largest word, so all workgroups pick the same pivot. Equal float keys
+                // of different doubles are settled with the exact values.
+                unsigned long long mine_best = 0; // this thread's largest word, and how many share its key
                 int sharing = 0;
-                for (int g = t; g < G; g += kPanelThreads) {
-                    unsigned long long *word = &words[((int64_t)parity * G + g) * kLine];
+                T value = T(0), next = T(0), factor = T(0); // the fields of that word's line
+                for (int g0 = t; g0 < G; g0 += kPoll * kPanelThreads) {
+                    unsigned waiting = 0;
+#pragma unroll
+                    for (int q = 0; q < kPoll; ++q)
+                        if (g0 + q * kPanelThreads < G) waiting |= 1u << q;
                     unsigned spins = 0;
-                    unsigned long long x;
-                    while (((x = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 0xFFu) !=
-                           (epoch & 0xFFu)) {
+                    while (waiting) {
+                        unsigned long long slot[kPoll][2 * kAllPairs<T>];
+#pragma unroll
+                        for (int q = 0; q < kPoll; ++q) {
+                            if (!(waiting >> q & 1u)) continue;
+                            const int g = g0 + q * kPanelThreads;
+                            const unsigned base = (unsigned)(line_index(c, g) * 8);
+#pragma unroll
+                            for (int e = 0; e < kMainPairs<T>; ++e) load_pair(line_buffer, base + 16 * e, slot[q] + 2 * e);
+                            if (g == owner_d)
+#pragma unroll
+                                for (int e = kMainPairs<T>; e < kAllPairs<T>; ++e)
+                                    load_pair(line_buffer, base + 16 * e, slot[q] + 2 * e);
+                        }
+#pragma unroll
+                        for (int q = 0; q < kPoll; ++q) {
+                            if (!(waiting >> q & 1u)) continue;
+                            const int g = g0 + q * kPanelThreads;
+                            const unsigned long long *s = slot[q];
+                            const unsigned long long x = s[0];
+                            bool ready = (x & 0xFFu) == epoch;
+                            if (ready && has_candidate(x))
+                                ready = current<T>(s + field<T>(kValue), tag) &&
+                                        (!more || current<T>(s + field<T>(kNext), tag)) &&
+                                        current<T>(s + field<T>(kFactor), tag);
+                            if (ready && g == owner_d)
+                                ready = current<T>(s + field<T>(kDiagonal), tag) &&
+                                        (!more || current<T>(s + field<T>(kDiagonalNext), tag)) &&
+                                        current<T>(s + field<T>(kDiagonalFactor), tag);
+                            if (!ready) continue;
+                            waiting &= ~(1u << q);
+                            if (g == owner_d) {
+                                diagonal_fields[0] = value_of<T>(s + field<T>(kDiagonal));
+                                diagonal_fields[1] = value_of<T>(s + field<T>(kDiagonalNext));
+                                diagonal_fields[2] = value_of<T>(s + field<T>(kDiagonalFactor));
+                            }
+                            bool take_it = false;
+                            if (sharing == 0 || (x >> 32) > (mine_best >> 32)) {
+                                take_it = true;
+                                sharing = 1;
+                            } else if ((x >> 32) == (mine_best >> 32)) {
+                                take_it = (x >> 8) > (mine_best >> 8);
+                                ++sharing;
+                            }
+                            if (take_it) {
+                                mine_best = x;
+                                value = value_of<T>(s + field<T>(kValue));
+                                next = value_of<T>(s + field<T>(kNext));
+                                factor = value_of<T>(s + field<T>(kFactor));
+                            }
+                        }
+                        if (!waiting) break;
                         if (++spins > kSpinLimit) {
                             __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
@@ -224,96 +430,194 @@ namespace scihip {
                         if ((spins & 63u) == 0 && relaxed(error)) break;
                         __builtin_amdgcn_s_sleep(1);
                     }
-                    if (sharing == 0 || (x >> 32) > (mine_best >> 32)) {
-                        mine_best = x;
-                        sharing = 1;
-                    } else if ((x >> 32) == (mine_best >> 32)) {
-                        if ((x >> 8) > (mine_best >> 8)) mine_best = x;
-                        ++sharing;
+                }
+                // The workgroup's largest word: each wave's, with its fields and how many lines of the
+                // wave carry its key, then the largest of the waves'.
+                {
+                    const unsigned long long top = wave_max(sharing ? mine_best >> 8 : 0); // key << 24 | ~offset
+                    const bool holder = sharing > 0 && (mine_best >> 8) == top;
+                    const int carrying = wave_sum(sharing > 0 && (mine_best >> 32) == (top >> 24) ? sharing : 0);
+                    if (holder) {
+                        wave_fields[wave][0] = value;
+                        wave_fields[wave][1] = next;
+                        wave_fields[wave][2] = factor;
+                    }
+                    if (lane == 0) {
+                        wave_word[wave] = top;
+                        wave_sharing[wave] = carrying;
                     }
                 }
-                const unsigned long long top = wave_max(sharing ? mine_best >> 8 : 0); // key << 24 | ~offset
+                settle();
+                unsigned long long top = wave_word[0];
+                int from = 0;
+#pragma unroll
+                for (int q = 1; q < kWaves; ++q)
+                    if (wave_word[q] > top) {
+                        top = wave_word[q];
+                        from = q;
+                    }
                 int64_t p = row_of(top << 8, j0);
+                T pivot = wave_fields[from][0], pivot_next = wave_fields[from][1], pivot_factor = wave_fields[from][2];
                 if constexpr (sizeof(T) == 8) {
-                    const bool holds = sharing > 0 && (mine_best >> 32) == (top >> 24);
-                    const unsigned long long holders = __ballot(holds);
-                    if (__ballot(holds && sharing > 1) || (holders & (holders - 1))) {
-                        // Two or more candidates share the float key: compare the exact values. The words
-                        // of this parity are not rewritten before this workgroup publishes again.
+                    int carrying = 0;
+#pragma unroll
+                    for (int q = 0; q < kWaves; ++q)
+                        if ((wave_word[q] >> 24) == (top >> 24)) carrying += wave_sharing[q];
+                    if (carrying > 1) {
+                        // Two or more candidates share the float key: compare the exact values. The lines
+                        // of this column are not rewritten before this workgroup publishes again.
                         T v = T(-1);
                         p = LLONG_MAX;
                         for (int g = t; g < G; g += kPanelThreads) {
-                            const unsigned long long x = __hip_atomic_load(&words[((int64_t)parity * G + g) * kLine],
-                                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            if ((x >> 32) == (top >> 24) && (x & 0xFFu) == (epoch & 0xFFu))
-                                better(v, p, get(&pub_value[parity * G + g]), row_of(x, j0));
+                            const unsigned long long *line = lines + line_index(c, g);
+                            const unsigned long long x = __hip_atomic_load(line, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if ((x >> 32) == (top >> 24) && (x & 0xFFu) == epoch && has_candidate(x))
+                                better(v, p, T(__builtin_fabs(take<T>(line + field<T>(kValue), tag, error))), row_of(x, j0));
                         }
                         wave_best(v, p);
+                        if (lane == 0) {
+                            search_value[wave] = v;
+                            search_row[wave] = p;
+                        }
+                        settle();
+                        v = search_value[0];
+                        p = search_row[0];
+#pragma unroll
+                        for (int q = 1; q < kWaves; ++q) better(v, p, search_value[q], search_row[q]);
+                        if (p >= j0 && p < n) {
+                            const unsigned long long *line = lines + line_index(c, (int)((p - j0) / kPanelRows));
+                            pivot = take<T>(line + field<T>(kValue), tag, error);
+                            pivot_next = more ? take<T>(line + field<T>(kNext), tag, error) : T(0);
+                            pivot_factor = take<T>(line + field<T>(kFactor), tag, error);
+                        }
                     }
                 }
                 if (p < d || p >= n) p = d; // only after a failed wait: keep every index in range
+                const int owner_p = (int)((p - j0) / kPanelRows);
+                const T diagonal_value = diagonal_fields[0], diagonal_next = diagonal_fields[1],
+                        diagonal_factor = diagonal_fields[2];
+                // entry c + 1 of the pivot row and of the diagonal row, with column c - 1's update
+                const T u1 = !more ? T(0) : previous ? eliminated(pivot_next, pivot_factor, u_before[c + 1]) : pivot_next;
+                const T e1 = !more ? T(0) : previous ? eliminated(diagonal_next, diagonal_factor, u_before[c + 1]) : diagonal_next;
 
-                // B: swap.
-                const int owner = (int)((p - j0) / kPanelRows); // the workgroup that published row p
-                for (int k = t; k < w; k += kPanelThreads) {
-                    pivot_row[k] = get(&pub_rows[((int64_t)parity * G + owner) * kPanel + k]);
-                    diagonal_row[k] = get(&diagonal[parity * kPanel + k]);
-                }
-                __syncthreads();
-                if (p != d) {
-                    if (p >= first && p < first + mine)
-                        for (int k = t; k < w; k += kPanelThreads) rows[(p - first) * stride + k] = diagonal_row[k];
-                    if (d >= first && d < first + mine)
-                        for (int k = t; k < w; k += kPanelThreads) rows[(d - first) * stride + k] = pivot_row[k];
+                // B: swap, columns c and c + 1 for now (by the thread that owns row p, which eliminates it
+                // next). Workgroup owner_p puts the diagonal row in place of row p; workgroup owner_d
+                // receives the pivot row in row d after publication.
+                const bool swapped = p != d;
+                const bool receiver = swapped && b == owner_p;
+                if (receiver && t == (int)(p - first)) {
+                    rows[t * stride + c] = diagonal_value;
+                    if (more) rows[t * stride + c + 1] = e1;
                 }
                 if (b == 0 && t == 0) pivots[d] = p;
-                __syncthreads();
 
-                // C: multipliers; column c + 1 and the next candidate first (look-ahead), then the rest of
-                // the rank-one update while the other workgroups' candidates travel.
-                const T pivot = pivot_row[c];
+                // C: multipliers and column c + 1 (look-ahead), each thread on its row.
                 const bool eliminate = pivot != T(0); // a zero pivot skips the column (decompose.hpp:44-45)
-                if (eliminate) {
-#pragma unroll
-                    for (int o = 0; o < kOwned; ++o) {
-                        const int i = t + o * kPanelThreads;
-                        T l = T(0);
-                        if (i < mine && first + i > d) {
-                            l = rows[i * stride + c] / pivot;
-                            rows[i * stride + c] = l;
-                            if (c + 1 < w) rows[i * stride + c + 1] = rows[i * stride + c + 1] - l * pivot_row[c + 1];
-                        }
-                        if (i < kPanelRows) multipliers[i] = l;
+                if (eliminate && t < kPanelRows) {
+                    T l = T(0);
+                    if (t < mine && first + t > d) {
+                        l = rows[t * stride + c] / pivot;
+                        rows[t * stride + c] = l;
+                        if (more) rows[t * stride + c + 1] = eliminated(rows[t * stride + c + 1], l, u1);
                     }
-                    __syncthreads();
+                    multipliers[t] = l;
                 }
-                if (c + 1 < w) publish(c + 1, eliminate);
+
+                // Entry k_own of a published row (not columns c and c + 1; `slots` is the row's first slot
+                // in `buffer`'s buffer `base`); entries beyond c + 1 get column c - 1's update (multiplier m).
+                auto receive = [&](__amdgpu_buffer_rsrc_t buffer, const unsigned long long *base, int64_t slots, T m,
+                                   T *out) {
+                    const int k = k_own;
+                    if (k >= w || k == c || k == c + 1) return;
+                    const int64_t at = slots + (int64_t)k * S;
+                    unsigned long long x[2];
+                    unsigned spins = 0;
+                    for (;;) {
+                        if constexpr (S == 2)
+                            load_pair(buffer, (unsigned)(at * 8), x);
+                        else
+                            x[0] = __hip_atomic_load(base + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (current<T>(x, tag)) break;
+                        if (++spins > kSpinLimit) {
+                            __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        if ((spins & 63u) == 0 && relaxed(error)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    T v = value_of<T>(x);
+                    if (k > c + 1 && previous) v = eliminated(v, m, u_before[k]);
+                    out[k] = v;
+                };
+                bool received = false;
+
+                // D: the candidate for column c + 1 goes out before the rest of this column's work.
+                if (more) {
+                    const int64_t row = search(c + 1); // its barrier also orders B and C before publication
+                    if (receiver && (row == p || d + 1 == p)) {
+                        // the swapped-in row is needed whole now
+                        if (part == 1) receive(diagonal_buffer, diagonal, diagonal_index(c), diagonal_factor, rows + (p - first) * stride);
+                        received = true;
+                        settle();
+                    }
+                    publish(c + 1, row, eliminate);
+                } else {
+                    settle();
+                }
+
+                // E: the rest of the pivot row (part 0) and, at workgroup owner_p, of the diagonal row
+                // (part 1); then row d at workgroup owner_d.
+                if (part == 0) {
+                    receive(row_buffer, published, row_index(c, owner_p), pivot_factor, u);
+                    if (t == 0) {
+                        u[c] = pivot;
+                        if (more) u[c + 1] = u1;
+                    }
+                } else if (receiver && !received) {
+                    receive(diagonal_buffer, diagonal, diagonal_index(c), diagonal_factor, rows + (p - first) * stride);
+                }
+                settle();
+                if (swapped && b == owner_d && part == 0 && k_own < w) rows[(d - first) * stride + k_own] = u[k_own];
+
+                // F: the rest of the rank-one update: the two threads of a row split its columns c + 2
+                // onwards, 16 per batch.
                 if (eliminate) {
-                    // Lane t updates columns c + 2 + t, c + 66 + t of every row below the diagonal; the
-                    // iterations are independent (eight LDS loads in flight before any store).
-                    const int lo = (int)max((int64_t)0, d + 1 - first);
-                    for (int k = c + 2 + t; k < w; k += kPanelThreads) {
-                        const T u = pivot_row[k];
-                        T *column = rows + k;
-                        int i = lo;
-                        for (; i + 8 <= mine; i += 8) {
-                            T x[8], m[8];
+                    const int i = t % kPanelRows;
+                    if (i < mine && first + i > d) {
+                        const T l = multipliers[i];
+                        T *row = rows + i * stride;
+                        const int span = w - (c + 2);
+                        int k = c + 2 + span * part / kParts;
+                        const int end = c + 2 + span * (part + 1) / kParts;
+                        for (; k + 16 <= end; k += 16) {
+                            T x[16], y[16];
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) {
-                                x[j] = column[(i + j) * stride];
-                                m[j] = multipliers[i + j];
+                            for (int j = 0; j < 16; ++j) {
+                                x[j] = row[k + j];
+                                y[j] = u[k + j];
                             }
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) column[(i + j) * stride] = x[j] - m[j] * u;
+                            for (int j = 0; j < 16; ++j) row[k + j] = eliminated(x[j], l, y[j]);
                         }
-                        for (; i < mine; ++i) column[i * stride] = column[i * stride] - multipliers[i] * u;
+                        for (; k + 4 <= end; k += 4) {
+                            T x[4], y[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                x[j] = row[k + j];
+                                y[j] = u[k + j];
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) row[k + j] = eliminated(x[j], l, y[j]);
+                        }
+                        for (; k < end; ++k) row[k] = eliminated(row[k], l, u[k]);
                     }
                 }
-                __syncthreads();
+                settle();
+                previous = eliminate;
             }
 
-            for (int i = 0; i < mine; ++i)
-                for (int k = t; k < w; k += kPanelThreads) a[(first + i) * n + j0 + k] = rows[i * stride + k];
+            for (int i = part; i < mine; i += kParts)
+                if (k_own < w) a[(first + i) * n + j0 + k_own] = rows[i * stride + k_own];
         }
 
         // Applies the panel's swaps to every column outside it, in step order.
@@ -501,8 +805,14 @@ namespace scihip {
         (void)mode;
         const size_t kp = (size_t)kPanel;
         // L21 and U12 packed (up to 8 bytes, split pairs), L11^-1 and its packed copy, the panel's
-        // candidate words (two parities, one 128-byte line each).
-        return 4 * (size_t)order * kp * 8 + 2 * kp * kp * 8 + 2 * ((size_t)order / kPanelRows + 2) * 128 + 1024;
+        // candidate lines (kParities buffers, one 128-byte line per workgroup).
+        return 4 * (size_t)order * kp * 8 + 2 * kp * kp * 8 + kParities * ((size_t)order / kPanelRows + 2) * 128 + 1024;
+    }
+
+    size_t panel_publish_bytes(int64_t order, size_t element) {
+        const size_t groups = (size_t)((order + kPanelRows - 1) / kPanelRows);
+        const size_t slots = element / 4; // tagged 64-bit slots per value
+        return kParities * (groups + 1) * kPanel * slots * 8;
     }
 
     namespace {
@@ -560,21 +870,31 @@ namespace scihip {
         void *ub_hi = base + 2 * operand, *ub_lo = base + 3 * operand;
         T *inverse = reinterpret_cast<T *>(base + 4 * operand);
         void *inverse_packed = base + 4 * operand + kp * kp * 8;
-        unsigned *flags = reinterpret_cast<unsigned *>(base + 4 * operand + 2 * kp * kp * 8);
+        unsigned long long *lines = reinterpret_cast<unsigned long long *>(base + 4 * operand + 2 * kp * kp * 8);
         const int max_groups = (int)((n + kPanelRows - 1) / kPanelRows);
-        T *pub_value = static_cast<T *>(ws.publish);
-        T *pub_rows = pub_value + 2 * max_groups + 2;
-        T *diagonal = pub_rows + (size_t)2 * max_groups * kPanel;
-        unsigned long long *words = reinterpret_cast<unsigned long long *>(flags);
+        if (ws.publish_bytes < panel_publish_bytes(n, sizeof(T))) {
+            *error = "LU publication buffer too small";
+            return -1;
+        }
+        unsigned long long *published = static_cast<unsigned long long *>(ws.publish);
+        unsigned long long *diagonal = published + (size_t)kParities * max_groups * kPanel * kSlots<T>;
 
         StageTimer timer(stream);
         // Factors the panel at column j0 on stream `s`.
         auto launch_panel = [&](int64_t j0, hipStream_t s) -> hipError_t {
             int w = (int)std::min<int64_t>(kPanel, n - j0);
             const int groups = (int)((n - j0 + kPanelRows - 1) / kPanelRows);
-            hipError_t e = hipMemsetAsync(words, 0, (size_t)2 * (groups + 1) * kLine * sizeof(unsigned long long), s);
+            hipError_t e = hipMemsetAsync(lines, 0, (size_t)kParities * groups * kLine * sizeof(unsigned long long), s);
             if (e != hipSuccess) return e;
-            void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &pub_value, &pub_rows, &diagonal, &words,
+            // The launch number goes into the publications' tags (24 bits); when it wraps, the buffers
+            // are cleared so that no slot of an earlier launch carries a tag that is current again.
+            if (++ws.launches > 0xFFFFFFu) {
+                e = hipMemsetAsync(ws.publish, 0, ws.publish_bytes, s);
+                if (e != hipSuccess) return e;
+                ws.launches = 1;
+            }
+            unsigned launch = ws.launches;
+            void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &lines, &published, &diagonal, &launch,
                             &ws.error};
             e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
                                            dim3(kPanelThreads), args, 0, s);

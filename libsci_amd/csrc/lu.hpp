@@ -28,7 +28,7 @@
 namespace scihip {
 
     constexpr int kPanel = 128;       // panel width (columns factored per cooperative launch)
-    constexpr int kPanelRows = 64;    // rows per workgroup (one wave) in the panel kernel
+    constexpr int kPanelRows = 128;   // rows per workgroup (four waves, a thread per row) in the panel kernel
     constexpr int kSolveRows = 48;    // rows per workgroup in the triangular solves
     constexpr int kSolveColumns = 32; // right-hand sides per pass of the solve kernel
     constexpr int kBlock = 128;       // diagonal block of the blocked substitution
@@ -41,8 +41,9 @@ namespace scihip {
         unsigned *counter = nullptr; // grid barrier arrivals (zeroed before each launch)
         unsigned *error = nullptr;   // set when a bounded spin gave up
         unsigned *flags = nullptr;   // solve: per-block ready epochs
-        void *publish = nullptr;     // panel: candidate values, rows and panel rows, two parities
+        void *publish = nullptr;     // panel: published candidate and diagonal rows (panel_publish_bytes)
         size_t publish_bytes = 0;
+        unsigned launches = 0;       // panel launches so far, for the publications' tags
         size_t flag_count = 0;
         hipStream_t side = nullptr;         // the look-ahead panels run here (default priority)
         hipEvent_t columns_ready = nullptr; // main stream: the next panel's columns are up to date
@@ -58,6 +59,10 @@ namespace scihip {
 
     // Bytes of scratch factor() needs.
     size_t factor_scratch(int64_t order, int mode);
+
+    // Bytes of Workspace::publish the panel kernel needs for an order (element: 4 or 8 bytes). The
+    // buffer is zeroed when allocated.
+    size_t panel_publish_bytes(int64_t order, size_t element);
 
     // Solves L U x = P b for `width` right-hand sides (row-major order x width), in place in `x`,
     // which holds P b on entry. `epoch` is the device's running launch count for the ready flags;

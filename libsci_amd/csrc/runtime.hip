@@ -397,15 +397,17 @@ namespace scihip {
             ws.flags = static_cast<unsigned *>(block);
             ws.flag_count = count;
         }
-        const size_t groups = size_t((order + kPanelRows - 1) / kPanelRows);
-        const size_t bytes = (2 * groups + 2) * (element + 8 + kPanel * element) + 2 * kPanel * element + 256;
+        const size_t bytes = std::max(panel_publish_bytes(order, 8), panel_publish_bytes(order, element));
         if (bytes > ws.publish_bytes) {
             if (ws.publish) {
                 (void)hipStreamSynchronize(device->stream);
+                (void)hipStreamSynchronize(ws.side);
                 (void)hipFree(ws.publish);
                 ws.publish = nullptr;
+                ws.publish_bytes = 0;
             }
             if (int rc = status(hipMalloc(&ws.publish, bytes), "hipMalloc")) return rc;
+            if (int rc = status(hipMemset(ws.publish, 0, bytes), "hipMemset")) return rc;
             ws.publish_bytes = bytes;
         }
         return SCIHIP_OK;
