@@ -235,6 +235,15 @@ namespace scihip {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         }
 
+        // A pivot candidate's rank: the bits of |x| plus one, which order non-negative values as the
+        // values do; NaN ranks 0 with an absent candidate, below every number (it never wins,
+        // decompose.hpp:35-38 compares with >).
+        template <typename T> __device__ __forceinline__ unsigned long long rank_of(T x) {
+            if (x != x) return 0;
+            if constexpr (sizeof(T) == 8) return __builtin_bit_cast(unsigned long long, T(__builtin_fabs(x))) + 1;
+            else return (unsigned long long)__builtin_bit_cast(unsigned, T(__builtin_fabsf(x))) + 1;
+        }
+
         __device__ __forceinline__ int wave_sum(int x) {
 #pragma unroll
             for (int offset = 1; offset < 64; offset <<= 1) x += __shfl_xor(x, offset, 64);
@@ -285,6 +294,7 @@ namespace scihip {
             __shared__ T wave_fields[kWaves][3];             // and that word's fields
             __shared__ T diagonal_fields[3];
             __shared__ T search_value[kWaves];
+            __shared__ unsigned long long search_rank[kWaves];
             __shared__ int64_t search_row[kWaves];
 
             const __amdgpu_buffer_rsrc_t line_buffer = buffer_over(lines);
@@ -301,25 +311,27 @@ namespace scihip {
             auto row_index = [&](int col, int g) { return ((int64_t)(col & (kParities - 1)) * G + g) * kPanel * S; };
             auto diagonal_index = [&](int col) { return (int64_t)(col & (kParities - 1)) * kPanel * S; };
 
-            // This workgroup's candidate row for column `col` (the same in every thread; LLONG_MAX if none).
+            // This workgroup's candidate row for column `col` (the same in every thread; LLONG_MAX if none):
+            // the largest rank, on equal ranks the lowest row (rows follow the threads).
             auto search = [&](int col) -> int64_t {
                 const int64_t dd = j0 + col;
-                T best = T(-1);
-                int64_t row = LLONG_MAX;
-                if (t < mine && first + t >= dd) {
-                    best = T(__builtin_fabs(rows[t * stride + col]));
-                    row = first + t;
-                }
-                wave_best(best, row);
+                const unsigned long long rank =
+                        t < mine && first + t >= dd ? rank_of(rows[t * stride + col]) : 0ull;
+                const unsigned long long top = wave_max(rank);
+                const unsigned long long hits = __ballot(rank == top && top != 0);
                 if (lane == 0) {
-                    search_value[wave] = best;
-                    search_row[wave] = row;
+                    search_rank[wave] = top;
+                    search_row[wave] = hits ? first + wave * 64 + (__ffsll((long long)hits) - 1) : LLONG_MAX;
                 }
                 settle();
-                best = search_value[0];
-                row = search_row[0];
+                unsigned long long best = search_rank[0];
+                int64_t row = search_row[0];
 #pragma unroll
-                for (int q = 1; q < kRowWaves; ++q) better(best, row, search_value[q], search_row[q]);
+                for (int q = 1; q < kRowWaves; ++q)
+                    if (search_rank[q] > best) {
+                        best = search_rank[q];
+                        row = search_row[q];
+                    }
                 return row;
             };
             // Publishes the candidate `row` for column `col` (threads of part 0), and the diagonal row if
@@ -492,10 +504,20 @@ namespace scihip {
                         }
                     }
                 }
-                if (p < d || p >= n) p = d; // only after a failed wait: keep every index in range
-                const int owner_p = (int)((p - j0) / kPanelRows);
                 const T diagonal_value = diagonal_fields[0], diagonal_next = diagonal_fields[1],
                         diagonal_factor = diagonal_fields[2];
+                // A NaN diagonal stays the pivot: the reference starts from the diagonal and takes a row
+                // only if |a| is larger (decompose.hpp:35-37), which nothing is than NaN.
+                if (diagonal_value != diagonal_value) p = d;
+                if (p < d || p >= n) p = d; // only after a failed wait: keep every index in range
+                // With p = d the pivot row is the diagonal row, which is always published.
+                const bool at_diagonal = p == d;
+                if (at_diagonal) {
+                    pivot = diagonal_value;
+                    pivot_next = diagonal_next;
+                    pivot_factor = diagonal_factor;
+                }
+                const int owner_p = (int)((p - j0) / kPanelRows);
                 // entry c + 1 of the pivot row and of the diagonal row, with column c - 1's update
                 const T u1 = !more ? T(0) : previous ? eliminated(pivot_next, pivot_factor, u_before[c + 1]) : pivot_next;
                 const T e1 = !more ? T(0) : previous ? eliminated(diagonal_next, diagonal_factor, u_before[c + 1]) : diagonal_next;
@@ -568,7 +590,8 @@ namespace scihip {
                 // E: the rest of the pivot row (part 0) and, at workgroup owner_p, of the diagonal row
                 // (part 1); then row d at workgroup owner_d.
                 if (part == 0) {
-                    receive(row_buffer, published, row_index(c, owner_p), pivot_factor, u);
+                    if (at_diagonal) receive(diagonal_buffer, diagonal, diagonal_index(c), pivot_factor, u);
+                    else receive(row_buffer, published, row_index(c, owner_p), pivot_factor, u);
                     if (t == 0) {
                         u[c] = pivot;
                         if (more) u[c + 1] = u1;
@@ -618,6 +641,32 @@ namespace scihip {
 
             for (int i = part; i < mine; i += kParts)
                 if (k_own < w) a[(first + i) * n + j0 + k_own] = rows[i * stride + k_own];
+        }
+
+        // U12 = L11^-1 A12 through the inverse multiplies the zeros above the inverse's diagonal with
+        // A12's entries, so a NaN or an infinity in a row of A12 would spread to the rows above it,
+        // which the reference's elimination leaves alone (each row subtracts multiples of the pivot rows
+        // above it only, decompose.hpp:47-53). A column of U12 (columns [j0 + w, n) of rows [j0, j0 +
+        // w)) that came out with a NaN is solved again by substitution, in the reference's order, from
+        // A12's packed copy (column k at a12[k * kp]). Thread k takes column k.
+        // `nan_seen` is raised by the product when it stored a NaN; otherwise there is nothing to redo.
+        template <typename T>
+        __global__ __launch_bounds__(256) void redo_nan_columns(T *a, int64_t n, int64_t j0, int w, const T *a12,
+                                                                int64_t kp, int64_t rest, const unsigned *nan_seen) {
+            if (*nan_seen == 0) return;
+            const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+            if (k >= rest) return;
+            T *u = a + j0 * n + j0 + w + k; // column k of U12, rows w apart by n
+            bool nan = false;
+            for (int r = 0; r < w; ++r) nan |= u[(int64_t)r * n] != u[(int64_t)r * n];
+            if (!nan) return;
+            const T *l = a + j0 * n + j0; // L11 (unit lower)
+            const T *source = a12 + k * kp;
+            for (int r = 0; r < w; ++r) {
+                T x = source[r];
+                for (int s = 0; s < r; ++s) x = eliminated(x, l[(int64_t)r * n + s], u[(int64_t)s * n]);
+                u[(int64_t)r * n] = x;
+            }
         }
 
         // Applies the panel's swaps to every column outside it, in step order.
@@ -806,7 +855,8 @@ namespace scihip {
         const size_t kp = (size_t)kPanel;
         // L21 and U12 packed (up to 8 bytes, split pairs), L11^-1 and its packed copy, the panel's
         // candidate lines (kParities buffers, one 128-byte line per workgroup).
-        return 4 * (size_t)order * kp * 8 + 2 * kp * kp * 8 + kParities * ((size_t)order / kPanelRows + 2) * 128 + 1024;
+        return 4 * (size_t)order * kp * 8 + 2 * kp * kp * 8 + kParities * ((size_t)order / kPanelRows + 2) * 128 +
+               ((size_t)order / kPanel + 1) * 4 + 1024;
     }
 
     size_t panel_publish_bytes(int64_t order, size_t element) {
@@ -871,6 +921,9 @@ namespace scihip {
         T *inverse = reinterpret_cast<T *>(base + 4 * operand);
         void *inverse_packed = base + 4 * operand + kp * kp * 8;
         unsigned long long *lines = reinterpret_cast<unsigned long long *>(base + 4 * operand + 2 * kp * kp * 8);
+        // one NaN flag per panel, after the lines
+        unsigned *nan_flags = reinterpret_cast<unsigned *>(
+                base + 4 * operand + 2 * kp * kp * 8 + (size_t)kParities * ((size_t)n / kPanelRows + 2) * 128);
         const int max_groups = (int)((n + kPanelRows - 1) / kPanelRows);
         if (ws.publish_bytes < panel_publish_bytes(n, sizeof(T))) {
             *error = "LU publication buffer too small";
@@ -879,6 +932,10 @@ namespace scihip {
         unsigned long long *published = static_cast<unsigned long long *>(ws.publish);
         unsigned long long *diagonal = published + (size_t)kParities * max_groups * kPanel * kSlots<T>;
 
+        {
+            const hipError_t e = hipMemsetAsync(nan_flags, 0, ((size_t)n / kPanel + 1) * 4, stream);
+            if (e != hipSuccess) return (int)e;
+        }
         StageTimer timer(stream);
         // Factors the panel at column j0 on stream `s`.
         auto launch_panel = [&](int64_t j0, hipStream_t s) -> hipError_t {
@@ -924,10 +981,13 @@ namespace scihip {
                 e = pack(own, a + j0 * n + j0 + w, wide, rest, w, n, true, ub_hi, nullptr, kp, stream);
                 if (e == hipSuccess) e = pack(own, inverse, wide, w, w, kPanel, true, inverse_packed, nullptr, kp, stream);
                 if (e != hipSuccess) return (int)e;
+                unsigned *nan_seen = nan_flags + j0 / kPanel;
                 ProductArgs solve{inverse_packed, nullptr, ub_hi, nullptr, a + j0 * n + j0 + w, w, rest, (int64_t)kp,
-                                  (int64_t)kp, (int64_t)kp, n, 1.0, 0.0};
+                                  (int64_t)kp, (int64_t)kp, n, 1.0, 0.0, 0, nan_seen};
                 e = product(own, solve, stream);
                 if (e != hipSuccess) return (int)e;
+                hipLaunchKernelGGL(redo_nan_columns<T>, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, stream, a, n,
+                                   j0, w, static_cast<const T *>(ub_hi), (int64_t)kp, rest, (const unsigned *)nan_seen);
                 timer.stage(2);
                 // A22 -= L21 U12 on the matrix cores, in the requested mode: the next panel's columns
                 // first, then (while that panel is factored on the side stream) the rest.

@@ -150,7 +150,7 @@ def test_pivot_ties_across_workgroups(device, case):
         a[10, 0], a[150, 0] = 1.0, 1.0 + 2.0 ** -40
         a[30, 5], a[190, 5] = -3.0, 3.0 + 2.0 ** -45
     else:
-        a[20, 0], a[100, 0] = -2.0, 2.0
+        a[20, 0], a[140, 0] = -2.0, 2.0
         a[5, 1], a[170, 1] = 0.75, -0.75
     mode = "fp32" if case.startswith("float") else "fp64"
     matrix = a.astype(np.float32 if mode == "fp32" else np.float64)
@@ -163,3 +163,22 @@ def test_pivot_ties_across_workgroups(device, case):
         lapack_rows[[i, p]] = lapack_rows[[p, i]]
     assert (rows == lapack_rows).all()
     assert rows[0] == (150 if case == "rounded_key_tie" else 20)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_nan_pivots_follow_reference(device, mode):
+    """A NaN below the diagonal never becomes the pivot and a NaN on the diagonal stays the pivot, as
+    the reference's search, which starts from the diagonal and takes a row only for a larger |a|
+    (include/linear/decompose.hpp:34-37), decides; the oracle restates it. Row 200 turns NaN from
+    column 3 on, so it is passed over until it reaches the diagonal at step 200, after which every
+    column is NaN and no workgroup has a candidate."""
+    n = 300
+    a = oracle.uniform(n * n, 21, -1.0, 1.0, dtype=np.float64).reshape(n, n) + 0.5 * np.eye(n)
+    a[200, 3] = np.nan
+    matrix = a.astype(np.float32 if mode == "fp32" else np.float64)
+    lu, rows = gpu_factor(device, matrix, mode)
+    ref_lu, want = oracle.factor(matrix)
+    assert (rows == want).all()
+    assert (np.isnan(lu) == np.isnan(ref_lu)).all()
+    assert np.isnan(lu[200:, 200:]).all() and np.isfinite(lu[:200, :200]).all()
+
