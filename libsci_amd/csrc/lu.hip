@@ -643,30 +643,58 @@ namespace scihip {
                 if (k_own < w) a[(first + i) * n + j0 + k_own] = rows[i * stride + k_own];
         }
 
-        // U12 = L11^-1 A12 through the inverse multiplies the zeros above the inverse's diagonal with
-        // A12's entries, so a NaN or an infinity in a row of A12 would spread to the rows above it,
-        // which the reference's elimination leaves alone (each row subtracts multiples of the pivot rows
-        // above it only, decompose.hpp:47-53). A column of U12 (columns [j0 + w, n) of rows [j0, j0 +
-        // w)) that came out with a NaN is solved again by substitution, in the reference's order, from
-        // A12's packed copy (column k at a12[k * kp]). Thread k takes column k.
-        // `nan_seen` is raised by the product when it stored a NaN; otherwise there is nothing to redo.
+        // U12 = L11^-1 A12 in place (rows [j0, j0 + kPanel) of columns [j0 + kPanel, n)) by forward
+        // substitution with the unit lower L11. Row r subtracts l[r][s] times row s for s = 0, 1, ...,
+        // r - 1 in that order, which is the order in which the reference's elimination reaches A12
+        // (decompose.hpp:47-53), so NaN and infinity reach exactly the entries it reaches. Each
+        // workgroup copies L11 to LDS, where every lane reads the same entry at once; thread k owns
+        // column k of A12, kept in registers in two halves of 64 rows: the first half is solved and
+        // stored; the second half receives the first half's updates 16 rows at a time, then is solved
+        // itself. `l` is L11 and `u` A12 (row stride n).
+        constexpr int kSolveU12Threads = 256;
         template <typename T>
-        __global__ __launch_bounds__(256) void redo_nan_columns(T *a, int64_t n, int64_t j0, int w, const T *a12,
-                                                                int64_t kp, int64_t rest, const unsigned *nan_seen) {
-            if (*nan_seen == 0) return;
-            const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        __global__ __launch_bounds__(kSolveU12Threads) void solve_u12(const T *__restrict__ l, T *__restrict__ u,
+                                                                      int64_t n, int64_t rest) {
+            constexpr int H = kPanel / 2;
+            __shared__ T lower[kPanel * kPanel];
+#pragma unroll 16
+            for (int e = threadIdx.x; e < kPanel * kPanel; e += kSolveU12Threads)
+                lower[e] = l[(int64_t)(e / kPanel) * n + e % kPanel];
+            __syncthreads();
+            const int64_t k = (int64_t)blockIdx.x * kSolveU12Threads + threadIdx.x;
             if (k >= rest) return;
-            T *u = a + j0 * n + j0 + w + k; // column k of U12, rows w apart by n
-            bool nan = false;
-            for (int r = 0; r < w; ++r) nan |= u[(int64_t)r * n] != u[(int64_t)r * n];
-            if (!nan) return;
-            const T *l = a + j0 * n + j0; // L11 (unit lower)
-            const T *source = a12 + k * kp;
-            for (int r = 0; r < w; ++r) {
-                T x = source[r];
-                for (int s = 0; s < r; ++s) x = eliminated(x, l[(int64_t)r * n + s], u[(int64_t)s * n]);
-                u[(int64_t)r * n] = x;
+            T *__restrict__ column = u + k;
+            T x[H];
+            // rows 0 .. H - 1
+#pragma unroll
+            for (int r = 0; r < H; ++r) x[r] = column[(int64_t)r * n];
+#pragma unroll
+            for (int r = 1; r < H; ++r)
+#pragma unroll
+                for (int s = 0; s < r; ++s) x[r] = eliminated(x[r], lower[r * kPanel + s], x[s]);
+#pragma unroll
+            for (int r = 0; r < H; ++r) column[(int64_t)r * n] = x[r];
+            // rows H .. kPanel - 1: the first half's updates, 16 rows at a time
+            for (int r0 = H; r0 < kPanel; r0 += 16) {
+                T y[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) y[i] = column[(int64_t)(r0 + i) * n];
+#pragma unroll
+                for (int s = 0; s < H; ++s)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) y[i] = eliminated(y[i], lower[(r0 + i) * kPanel + s], x[s]);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) column[(int64_t)(r0 + i) * n] = y[i];
             }
+            // then their own triangle
+#pragma unroll
+            for (int r = 0; r < H; ++r) x[r] = column[(int64_t)(H + r) * n];
+#pragma unroll
+            for (int r = 1; r < H; ++r)
+#pragma unroll
+                for (int s = 0; s < r; ++s) x[r] = eliminated(x[r], lower[(H + r) * kPanel + H + s], x[s]);
+#pragma unroll
+            for (int r = 0; r < H; ++r) column[(int64_t)(H + r) * n] = x[r];
         }
 
         // Applies the panel's swaps to every column outside it, in step order.
@@ -852,11 +880,9 @@ namespace scihip {
 
     size_t factor_scratch(int64_t order, int mode) {
         (void)mode;
-        const size_t kp = (size_t)kPanel;
-        // L21 and U12 packed (up to 8 bytes, split pairs), L11^-1 and its packed copy, the panel's
-        // candidate lines (kParities buffers, one 128-byte line per workgroup).
-        return 4 * (size_t)order * kp * 8 + 2 * kp * kp * 8 + kParities * ((size_t)order / kPanelRows + 2) * 128 +
-               ((size_t)order / kPanel + 1) * 4 + 1024;
+        // L21 and U12 packed (up to 8 bytes, split pairs), then the panel's candidate lines
+        // (kParities buffers, one 128-byte line per workgroup).
+        return 4 * (size_t)order * kPanel * 8 + kParities * ((size_t)order / kPanelRows + 2) * 128 + 1024;
     }
 
     size_t panel_publish_bytes(int64_t order, size_t element) {
@@ -912,18 +938,12 @@ namespace scihip {
             return -1;
         }
         const bool wide = PackMode<T>::wide();
-        const int own = wide ? kFloat64 : kFloat32; // U12 = L11^-1 A12 runs in the matrix's precision
         const size_t kp = kPanel;
         unsigned char *base = static_cast<unsigned char *>(scratch);
         const size_t operand = (size_t)n * kp * 8; // bytes of one packed n x kPanel operand, at most
         void *la_hi = base, *la_lo = base + operand;
         void *ub_hi = base + 2 * operand, *ub_lo = base + 3 * operand;
-        T *inverse = reinterpret_cast<T *>(base + 4 * operand);
-        void *inverse_packed = base + 4 * operand + kp * kp * 8;
-        unsigned long long *lines = reinterpret_cast<unsigned long long *>(base + 4 * operand + 2 * kp * kp * 8);
-        // one NaN flag per panel, after the lines
-        unsigned *nan_flags = reinterpret_cast<unsigned *>(
-                base + 4 * operand + 2 * kp * kp * 8 + (size_t)kParities * ((size_t)n / kPanelRows + 2) * 128);
+        unsigned long long *lines = reinterpret_cast<unsigned long long *>(base + 4 * operand);
         const int max_groups = (int)((n + kPanelRows - 1) / kPanelRows);
         if (ws.publish_bytes < panel_publish_bytes(n, sizeof(T))) {
             *error = "LU publication buffer too small";
@@ -932,11 +952,19 @@ namespace scihip {
         unsigned long long *published = static_cast<unsigned long long *>(ws.publish);
         unsigned long long *diagonal = published + (size_t)kParities * max_groups * kPanel * kSlots<T>;
 
-        {
-            const hipError_t e = hipMemsetAsync(nan_flags, 0, ((size_t)n / kPanel + 1) * 4, stream);
-            if (e != hipSuccess) return (int)e;
-        }
         StageTimer timer(stream);
+        // Panel workgroups the device holds at once.
+        int resident = 0;
+        {
+            int device = 0, units = 0, per_unit = 0;
+            hipError_t e = hipGetDevice(&device);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device);
+            if (e == hipSuccess)
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, reinterpret_cast<const void *>(&panel<T>),
+                                                                 kPanelThreads, 0);
+            if (e != hipSuccess) return (int)e;
+            resident = units * per_unit;
+        }
         // Factors the panel at column j0 on stream `s`.
         auto launch_panel = [&](int64_t j0, hipStream_t s) -> hipError_t {
             int w = (int)std::min<int64_t>(kPanel, n - j0);
@@ -953,12 +981,18 @@ namespace scihip {
             unsigned launch = ws.launches;
             void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &lines, &published, &diagonal, &launch,
                             &ws.error};
-            e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
-                                           dim3(kPanelThreads), args, 0, s);
-            if (e != hipSuccess) return e;
-            // L11^-1, transposed (inverse[k][r] = L11^-1[r][k]), for U12 = L11^-1 A12.
-            hipLaunchKernelGGL((invert_diagonal<T, false>), dim3(1), dim3(256), 0, s, a + j0 * n + j0, n, w, inverse);
-            return hipGetLastError();
+            // The workgroups wait for each other, so all of them must be resident at once. When the grid
+            // fits the device, a plain launch is enough: workgroups queued behind the concurrent update
+            // start as soon as its workgroups finish, which they do without waiting on the panel (and
+            // every wait is bounded). Larger grids take the cooperative launch, which checks residency.
+            if (groups <= resident)
+                hipLaunchKernelGGL((panel<T>), dim3(groups), dim3(kPanelThreads), 0, s, a, n, j0, w, pivots, lines,
+                                   published, diagonal, launch, ws.error);
+            else
+                e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
+                                               dim3(kPanelThreads), args, 0, s);
+            if (e == hipSuccess) e = hipGetLastError();
+            return e;
         };
         const size_t eb = (size_t)element_bytes(mode);
         hipError_t e = launch_panel(0, stream);
@@ -976,18 +1010,12 @@ namespace scihip {
             timer.stage(1);
             const int64_t rest = n - j0 - w;
             if (rest > 0) {
-                // U12 = L11^-1 A12: the packed copy of A12 is the product's operand, so the result can
-                // overwrite A12 in place.
-                e = pack(own, a + j0 * n + j0 + w, wide, rest, w, n, true, ub_hi, nullptr, kp, stream);
-                if (e == hipSuccess) e = pack(own, inverse, wide, w, w, kPanel, true, inverse_packed, nullptr, kp, stream);
+                // U12 = L11^-1 A12 in place, by substitution (rest > 0 means w = kPanel).
+                hipLaunchKernelGGL(solve_u12<T>, dim3((unsigned)((rest + kSolveU12Threads - 1) / kSolveU12Threads)),
+                                   dim3(kSolveU12Threads), 0, stream,
+                                   (const T *)(a + j0 * n + j0), a + j0 * n + j0 + w, n, rest);
+                e = hipGetLastError();
                 if (e != hipSuccess) return (int)e;
-                unsigned *nan_seen = nan_flags + j0 / kPanel;
-                ProductArgs solve{inverse_packed, nullptr, ub_hi, nullptr, a + j0 * n + j0 + w, w, rest, (int64_t)kp,
-                                  (int64_t)kp, (int64_t)kp, n, 1.0, 0.0, 0, nan_seen};
-                e = product(own, solve, stream);
-                if (e != hipSuccess) return (int)e;
-                hipLaunchKernelGGL(redo_nan_columns<T>, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, stream, a, n,
-                                   j0, w, static_cast<const T *>(ub_hi), (int64_t)kp, rest, (const unsigned *)nan_seen);
                 timer.stage(2);
                 // A22 -= L21 U12 on the matrix cores, in the requested mode: the next panel's columns
                 // first, then (while that panel is factored on the side stream) the rest.

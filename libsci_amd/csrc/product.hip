@@ -206,7 +206,6 @@ namespace scihip {
             typedef typename std::conditional<M == kFloat64, double, float>::type CT;
             CT *const c_base = static_cast<CT *>(p.c) + (int64_t)blockIdx.y * p.slice_stride;
             const CT alpha = (CT)p.alpha, beta = (CT)p.beta;
-            bool nan = false;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 CT old[4][4];
@@ -228,12 +227,10 @@ namespace scihip {
                         const int64_t col = col0 + wn * 64 + j * 16 + r;
                         if (row >= p.m || col >= p.n) continue;
                         const CT v = (CT)acc[i][j][e];
-                        const CT out = beta == CT(0) ? (alpha == CT(1) ? v : alpha * v) : alpha * v + beta * old[j][e];
-                        c_base[row * p.ldc + col] = out;
-                        nan |= out != out;
+                        c_base[row * p.ldc + col] = beta == CT(0) ? (alpha == CT(1) ? v : alpha * v)
+                                                                  : alpha * v + beta * old[j][e];
                     }
             }
-            if (p.nan_seen && nan) __hip_atomic_fetch_or(p.nan_seen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
 
         // 64 x 64 tiles through LDS, so that both the plain and the transposed copy read and write

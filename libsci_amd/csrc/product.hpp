@@ -27,7 +27,6 @@ namespace scihip {
         int64_t lda, ldb, ldc;
         double alpha, beta;
         int64_t slice_stride; // split-K launches: elements between consecutive slices' outputs
-        unsigned *nan_seen;   // if set, raised (to 1) when a stored element is NaN
     };
 
     // Bytes of one k-tile row in LDS and the k extent it holds, per mode.
