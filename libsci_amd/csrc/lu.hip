@@ -1039,7 +1039,7 @@ namespace scihip {
                             ub_lo ? static_cast<const unsigned char *>(ub_lo) + (size_t)next * kp * eb : nullptr;
                     ProductArgs wide_update{la_hi, la_lo, b_hi, mode == kSplit ? b_lo : nullptr, c + next, rest,
                                             rest - next, (int64_t)kp, (int64_t)kp, (int64_t)kp, n, -1.0, 1.0};
-                    e = product(mode, wide_update, stream);
+                    e = product_alongside(mode, wide_update, stream); // the next panel runs meanwhile
                     if (e != hipSuccess) return (int)e;
                 }
                 timer.stage(4);

@@ -94,7 +94,8 @@ namespace scihip {
             tn = within / rows;
         }
 
-        template <int M> __global__ __launch_bounds__(kThreads) void gemm(const ProductArgs p) {
+        // Waves: waves per SIMD the registers are sized for (2: two workgroups per compute unit).
+        template <int M, int Waves = 2> __global__ __launch_bounds__(kThreads, Waves) void gemm(const ProductArgs p) {
             typedef Shape<M> S;
             typedef typename Accumulator<M>::type Acc;
             extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
@@ -287,7 +288,7 @@ namespace scihip {
     } // namespace
 
     namespace {
-        hipError_t launch_gemm(int mode, const ProductArgs &args, int slices, hipStream_t stream) {
+        hipError_t launch_gemm(int mode, const ProductArgs &args, int slices, hipStream_t stream, bool alongside = false) {
             const int64_t tiles = ((args.m + kTile - 1) / kTile) * ((args.n + kTile - 1) / kTile);
             if (tiles == 0) return hipSuccess;
             const dim3 grid((unsigned)tiles, (unsigned)slices), block(kThreads);
@@ -302,7 +303,8 @@ namespace scihip {
                 hipLaunchKernelGGL(gemm<kSplit>, grid, block, lds_bytes<kSplit>(), stream, args);
                 break;
             default:
-                hipLaunchKernelGGL(gemm<kFloat64>, grid, block, lds_bytes<kFloat64>(), stream, args);
+                if (alongside) hipLaunchKernelGGL((gemm<kFloat64, 1>), grid, block, lds_bytes<kFloat64>(), stream, args);
+                else hipLaunchKernelGGL((gemm<kFloat64, 2>), grid, block, lds_bytes<kFloat64>(), stream, args);
                 break;
             }
             return hipGetLastError();
@@ -343,6 +345,12 @@ namespace scihip {
         ProductArgs whole = args;
         whole.slice_stride = 0;
         return launch_gemm(mode, whole, 1, stream);
+    }
+
+    hipError_t product_alongside(int mode, const ProductArgs &args, hipStream_t stream) {
+        ProductArgs whole = args;
+        whole.slice_stride = 0;
+        return launch_gemm(mode, whole, 1, stream, true);
     }
 
     hipError_t pack(int mode, const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,

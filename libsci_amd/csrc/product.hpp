@@ -36,6 +36,11 @@ namespace scihip {
     // Launches the product of packed operands. Kp must be a multiple of k_tile(mode).
     hipError_t product(int mode, const ProductArgs &args, hipStream_t stream);
 
+    // The same, with the fp64 kernel sized for one workgroup per compute unit instead of two, for a
+    // product that runs beside a kernel whose workgroups each need most of a compute unit's LDS (the
+    // fp64 LU panel): as the product's workgroups finish, whole compute units come free for it.
+    hipError_t product_alongside(int mode, const ProductArgs &args, hipStream_t stream);
+
     // The same product with the k-tiles split into `slices` ranges run by separate workgroups (for
     // products with few output tiles and a long shared dimension): each slice writes its partial
     // product to `partials` (slices x m x n, in the output type), then a second kernel folds the
