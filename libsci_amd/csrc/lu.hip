@@ -647,54 +647,66 @@ namespace scihip {
         // substitution with the unit lower L11. Row r subtracts l[r][s] times row s for s = 0, 1, ...,
         // r - 1 in that order, which is the order in which the reference's elimination reaches A12
         // (decompose.hpp:47-53), so NaN and infinity reach exactly the entries it reaches. Each
-        // workgroup copies L11 to LDS, where every lane reads the same entry at once; thread k owns
-        // column k of A12, kept in registers in two halves of 64 rows: the first half is solved and
-        // stored; the second half receives the first half's updates 16 rows at a time, then is solved
-        // itself. `l` is L11 and `u` A12 (row stride n).
-        constexpr int kSolveU12Threads = 256;
+        // workgroup copies L11's strict lower part to LDS, where every lane reads the same entry at
+        // once. Thread k owns column k of A12 and takes its rows a quarter (32 rows) at a
+        // time, in registers: the quarter receives the updates of each solved quarter before it (whose
+        // rows it reads back), then is solved itself and stored. The code is a 32 x 32 update and a
+        // 32-row triangle, small enough for the instruction cache. `l` is L11 and `u` A12 (row stride n).
+        // Threads per workgroup: one wave for fp64, four for fp32 (whose entries take half the LDS
+        // bandwidth), as measured.
+        template <typename T> constexpr int kSolveU12Threads = sizeof(T) == 8 ? 64 : 256;
         template <typename T>
-        __global__ __launch_bounds__(kSolveU12Threads) void solve_u12(const T *__restrict__ l, T *__restrict__ u,
-                                                                      int64_t n, int64_t rest) {
-            constexpr int H = kPanel / 2;
+        __global__ __launch_bounds__(kSolveU12Threads<T>) void solve_u12(const T *__restrict__ l, T *__restrict__ u,
+                                                                         int64_t n, int64_t rest) {
+            constexpr int Q = 32;
+            constexpr int kThreads = kSolveU12Threads<T>;
+            constexpr int kRows = kThreads / 64; // rows of L11 copied per pass, one per wave
             __shared__ T lower[kPanel * kPanel];
-#pragma unroll 16
-            for (int e = threadIdx.x; e < kPanel * kPanel; e += kSolveU12Threads)
-                lower[e] = l[(int64_t)(e / kPanel) * n + e % kPanel];
+            // L11's strict lower part, 32 passes' loads in flight at a time (each lane two entries).
+            const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+            for (int r0 = 0; r0 < kPanel; r0 += 32 * kRows) {
+                T near[32], far[32];
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const int r = r0 + i * kRows + wave;
+                    near[i] = r < kPanel && lane < r ? l[(int64_t)r * n + lane] : T(0);
+                    far[i] = r < kPanel && lane + 64 < r ? l[(int64_t)r * n + lane + 64] : T(0);
+                }
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const int r = r0 + i * kRows + wave;
+                    if (r < kPanel) {
+                        lower[r * kPanel + lane] = near[i];
+                        lower[r * kPanel + lane + 64] = far[i];
+                    }
+                }
+            }
             __syncthreads();
-            const int64_t k = (int64_t)blockIdx.x * kSolveU12Threads + threadIdx.x;
+            const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
             if (k >= rest) return;
             T *__restrict__ column = u + k;
-            T x[H];
-            // rows 0 .. H - 1
+            for (int q = 0; q < kPanel; q += Q) {
+                T y[Q];
 #pragma unroll
-            for (int r = 0; r < H; ++r) x[r] = column[(int64_t)r * n];
+                for (int i = 0; i < Q; ++i) y[i] = column[(int64_t)(q + i) * n];
+                for (int p0 = 0; p0 < q; p0 += Q) {
+                    T x[Q];
 #pragma unroll
-            for (int r = 1; r < H; ++r)
+                    for (int j = 0; j < Q; ++j) x[j] = column[(int64_t)(p0 + j) * n];
+                    const T *block = lower + q * kPanel + p0;
 #pragma unroll
-                for (int s = 0; s < r; ++s) x[r] = eliminated(x[r], lower[r * kPanel + s], x[s]);
+                    for (int j = 0; j < Q; ++j)
 #pragma unroll
-            for (int r = 0; r < H; ++r) column[(int64_t)r * n] = x[r];
-            // rows H .. kPanel - 1: the first half's updates, 16 rows at a time
-            for (int r0 = H; r0 < kPanel; r0 += 16) {
-                T y[16];
+                        for (int i = 0; i < Q; ++i) y[i] = eliminated(y[i], block[i * kPanel + j], x[j]);
+                }
+                const T *block = lower + q * kPanel + q;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) y[i] = column[(int64_t)(r0 + i) * n];
+                for (int i = 1; i < Q; ++i)
 #pragma unroll
-                for (int s = 0; s < H; ++s)
+                    for (int j = 0; j < i; ++j) y[i] = eliminated(y[i], block[i * kPanel + j], y[j]);
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) y[i] = eliminated(y[i], lower[(r0 + i) * kPanel + s], x[s]);
-#pragma unroll
-                for (int i = 0; i < 16; ++i) column[(int64_t)(r0 + i) * n] = y[i];
+                for (int i = 0; i < Q; ++i) column[(int64_t)(q + i) * n] = y[i];
             }
-            // then their own triangle
-#pragma unroll
-            for (int r = 0; r < H; ++r) x[r] = column[(int64_t)(H + r) * n];
-#pragma unroll
-            for (int r = 1; r < H; ++r)
-#pragma unroll
-                for (int s = 0; s < r; ++s) x[r] = eliminated(x[r], lower[(H + r) * kPanel + H + s], x[s]);
-#pragma unroll
-            for (int r = 0; r < H; ++r) column[(int64_t)(H + r) * n] = x[r];
         }
 
         // Applies the panel's swaps to every column outside it, in step order.
@@ -1011,8 +1023,8 @@ namespace scihip {
             const int64_t rest = n - j0 - w;
             if (rest > 0) {
                 // U12 = L11^-1 A12 in place, by substitution (rest > 0 means w = kPanel).
-                hipLaunchKernelGGL(solve_u12<T>, dim3((unsigned)((rest + kSolveU12Threads - 1) / kSolveU12Threads)),
-                                   dim3(kSolveU12Threads), 0, stream,
+                hipLaunchKernelGGL(solve_u12<T>, dim3((unsigned)((rest + kSolveU12Threads<T> - 1) / kSolveU12Threads<T>)),
+                                   dim3(kSolveU12Threads<T>), 0, stream,
                                    (const T *)(a + j0 * n + j0), a + j0 * n + j0 + w, n, rest);
                 e = hipGetLastError();
                 if (e != hipSuccess) return (int)e;
