@@ -849,6 +849,185 @@ namespace scihip {
             }
         }
 
+        // One triangle of the factors for up to kTriangleColumns right-hand sides, all diagonal blocks in
+        // one launch. Workgroup q takes block j (q for the lower triangle, nb - 1 - q for the upper):
+        // for every block i it depends on (i < j lower, i > j upper, in the order they complete) except
+        // the last, it waits for x_i's publication, then subtracts F_ji x_i (F_ji from the factors, 32
+        // columns of it in LDS at a time); it applies inv(D_jj) to the result, and only then waits for
+        // the last block, whose contribution it subtracts through G_j = inv(D_jj) F_j,last (from
+        // invert_blocks): one product after the wait instead of two. x_j is stored into x and
+        // published in tagged slots (tag: this launch's number) at published[((j * kBlock + r) *
+        // kTriangleColumns + c) * kSlots]. A workgroup waits only on workgroups of lower index, which are
+        // dispatched before it. Thread t holds rows 4 (t / 8) .. + 3 and columns 4 (t % 8) .. + 3 of the
+        // block.
+        constexpr int kTriangleColumns = 32;
+        template <typename T, bool upper>
+        __global__ __launch_bounds__(256) void solve_triangle(const T *f, const T *inverses_t, const T *links, T *x,
+                                                              int64_t n, int64_t width, int64_t col0,
+                                                              unsigned long long *published, unsigned tag,
+                                                              unsigned *error) {
+            constexpr int S = kSlots<T>;
+            constexpr int C = kTriangleColumns;
+            __shared__ T xl[kBlock][C + 1]; // x_i, later the block's reduced right-hand side
+            __shared__ T fl[kBlock][33];    // 32 columns of F_ji
+            const int nb = (int)(n / kBlock), q = (int)blockIdx.x, j = upper ? nb - 1 - q : q;
+            const int t = threadIdx.x, rg = t >> 3, cg = t & 7;
+            const int cols = (int)min((int64_t)C, width - col0);
+            const int64_t r0 = (int64_t)j * kBlock;
+            T acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = 4 * cg + b < cols ? x[(r0 + 4 * rg + a) * width + col0 + 4 * cg + b] : T(0);
+
+            // acc[a][b] -= sum_k M[4 rg + a][k] * xl[k][4 cg + b] over k = 0 .. kBlock - 1 (k ascending), for the
+            // 128 x 128 matrix M whose entry (r, k) is at m[r * row + k * column]; M goes through LDS 32
+            // columns at a time, the next chunk's loads in flight while the current one is used.
+            // `chunk` holds the first chunk's values, loaded by the caller with fetch().
+            auto fetch = [&](const T *m, int64_t row, int64_t column, int k0, T *chunk) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int idx = t + 256 * e, r = idx >> 5, k = idx & 31;
+                    chunk[e] = m[r * row + (int64_t)(k0 + k) * column];
+                }
+            };
+            auto subtract = [&](const T *m, int64_t row, int64_t column, T *chunk, T (&into)[4][4], bool add) {
+                for (int k0 = 0; k0 < kBlock; k0 += 32) {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int idx = t + 256 * e;
+                        fl[idx >> 5][idx & 31] = chunk[e];
+                    }
+                    __syncthreads();
+                    if (k0 + 32 < kBlock) fetch(m, row, column, k0 + 32, chunk);
+#pragma unroll 8
+                    for (int k = 0; k < 32; ++k) {
+                        T fv[4], xv[4];
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) fv[a] = fl[4 * rg + a][k];
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) xv[b] = xl[k0 + k][4 * cg + b];
+#pragma unroll
+                        for (int a = 0; a < 4; ++a)
+#pragma unroll
+                            for (int b = 0; b < 4; ++b) {
+                                if (add) {
+                                    if constexpr (sizeof(T) == 8) into[a][b] = __builtin_fma(fv[a], xv[b], into[a][b]);
+                                    else into[a][b] = __builtin_fmaf(fv[a], xv[b], into[a][b]);
+                                } else {
+                                    into[a][b] = eliminated(into[a][b], fv[a], xv[b]);
+                                }
+                            }
+                    }
+                    __syncthreads();
+                }
+            };
+
+            // x_i into xl once published: 16 entries per thread, their loads in flight together.
+            auto await = [&](int i) {
+                unsigned long long slot[16][2];
+                unsigned waiting = 0;
+#pragma unroll
+                for (int e = 0; e < 16; ++e)
+                    if ((t + 256 * e) % C < cols) waiting |= 1u << e;
+                unsigned spins = 0;
+                while (waiting) {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        if (!(waiting >> e & 1u)) continue;
+                        const int idx = t + 256 * e, r = idx / C, c = idx % C;
+                        const unsigned long long *at = published + (((int64_t)i * kBlock + r) * C + c) * S;
+#pragma unroll
+                        for (int h = 0; h < S; ++h)
+                            slot[e][h] = __hip_atomic_load(at + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        if (!(waiting >> e & 1u) || !current<T>(slot[e], tag)) continue;
+                        waiting &= ~(1u << e);
+                        const int idx = t + 256 * e;
+                        xl[idx / C][idx % C] = value_of<T>(slot[e]);
+                    }
+                    if (!waiting) break;
+                    if (++spins > kSpinLimit) {
+                        __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    if ((spins & 63u) == 0 && relaxed(error)) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                for (int idx = t; idx < kBlock * C; idx += 256)
+                    if (idx % C >= cols) xl[idx / C][idx % C] = T(0);
+                __syncthreads();
+            };
+            auto depend = [&](int i) {
+                const T *m = f + r0 * n + (int64_t)i * kBlock; // F_ji, rows n apart
+                T chunk[16];
+                fetch(m, n, 1, 0, chunk); // in flight while x_i is awaited
+                await(i);
+                subtract(m, n, 1, chunk, acc, false);
+            };
+            // Every dependency but the one that completes last (block j - 1, or j + 1 for the upper
+            // triangle), whose contribution goes through links: G_j = inv(D_jj) F_j,last.
+            if (!upper)
+                for (int i = 0; i + 1 < j; ++i) depend(i);
+            else
+                for (int i = nb - 1; i > j + 1; --i) depend(i);
+            const int last = upper ? j + 1 : j - 1;
+            const bool linked = upper ? j + 1 < nb : j > 0;
+
+            // x_j = inv(D_jj) acc; the inverse is given transposed, inv[r][k] at inverses_t[k * kBlock + r]
+            const T *inverse = inverses_t + (int64_t)j * kBlock * kBlock;
+            T chunk[16];
+            fetch(inverse, 1, kBlock, 0, chunk);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) xl[4 * rg + a][4 * cg + b] = acc[a][b];
+            __syncthreads();
+            T out[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) out[a][b] = T(0);
+            subtract(inverse, 1, kBlock, chunk, out, true);
+            if (linked) {
+                const T *g = links + (int64_t)j * kBlock * kBlock; // G_j, row-major
+                fetch(g, kBlock, 1, 0, chunk);
+                await(last);
+                subtract(g, kBlock, 1, chunk, out, false);
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int r = 4 * rg + a, c = 4 * cg + b;
+                    if (c >= cols) continue;
+                    x[(r0 + r) * width + col0 + c] = out[a][b];
+                    put_tagged(published + (((int64_t)j * kBlock + r) * C + c) * S, out[a][b], tag);
+                }
+        }
+
+        // G_j = inv(D_jj) F_j,j-1 (lower) or inv(D_jj) F_j,j+1 (upper), row-major, for solve_triangle;
+        // thread (blockIdx.x * 256 + threadIdx.x) of block j computes entry (r, k) = (idx / kBlock, idx % kBlock).
+        template <typename T, bool upper>
+        __global__ __launch_bounds__(256) void link_blocks(const T *f, int64_t n, const T *inverses_t, T *links) {
+            const int nb = (int)(n / kBlock), j = (int)blockIdx.y;
+            const int neighbour = upper ? j + 1 : j - 1;
+            if (neighbour < 0 || neighbour >= nb) return;
+            const int idx = (int)blockIdx.x * 256 + (int)threadIdx.x, r = idx / kBlock, k = idx % kBlock;
+            const T *inverse = inverses_t + (int64_t)j * kBlock * kBlock;
+            const T *block = f + (int64_t)j * kBlock * n + (int64_t)neighbour * kBlock;
+            T sum = T(0);
+#pragma unroll 16
+            for (int m = 0; m < kBlock; ++m) {
+                if constexpr (sizeof(T) == 8) sum = __builtin_fma(inverse[m * kBlock + r], block[(int64_t)m * n + k], sum);
+                else sum = __builtin_fmaf(inverse[m * kBlock + r], block[(int64_t)m * n + k], sum);
+            }
+            links[(int64_t)j * kBlock * kBlock + (int64_t)r * kBlock + k] = sum;
+        }
+
         // x[r0 + r][c] = sum_k inv[r][k] x[r0 + k][c] over one diagonal block (inverse given transposed,
         // inverse_t[k][r]); one workgroup of kBlock threads per column c. Also writes the packed B
         // operand of the update that follows, packed[c][r] (width x kBlock).
@@ -1066,37 +1245,35 @@ namespace scihip {
         hipLaunchKernelGGL((invert_diagonal<T, false>), dim3(blocks), dim3(256), 0, stream, f, n, kBlock, inverses);
         hipLaunchKernelGGL((invert_diagonal<T, true>), dim3(blocks), dim3(256), 0, stream, f, n, kBlock,
                            inverses + n * kBlock);
+        const dim3 grid(kBlock * kBlock / 256, blocks);
+        hipLaunchKernelGGL((link_blocks<T, false>), grid, dim3(256), 0, stream, f, n, inverses, inverses + 2 * n * kBlock);
+        hipLaunchKernelGGL((link_blocks<T, true>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
+                           inverses + 3 * n * kBlock);
         return hipGetLastError();
     }
 
     template <typename T>
     int substitute(const T *f, int64_t n, T *x, int64_t width, Workspace &ws, unsigned &epoch, hipStream_t stream,
-                   const char **error, const T *inverses, T *packed) {
+                   const char **error, const T *inverses, void *scratch) {
         *error = nullptr;
-        if (inverses && packed && blocked_substitution(n)) {
-            // Block by block: x_j = inv(L_jj) x_j, then the rows below lose L[below, j] x_j (forward);
-            // x_j = inv(U_jj) x_j, then the rows above lose U[above, j] x_j (backward). The updates
-            // read L and U in place (row stride n) on the matrix cores.
-            const int mode = sizeof(T) == 8 ? kFloat64 : kFloat32;
+        if (inverses && scratch && blocked_substitution(n)) {
+            // Each triangle in one launch per 32 right-hand sides (solve_triangle), the forward pass with
+            // the inverses of L's diagonal blocks, the backward pass with U's.
             const int64_t nb = n / kBlock;
             const T *inverse_lower = inverses, *inverse_upper = inverses + n * kBlock;
-            for (int pass = 0; pass < 2; ++pass) {
-                const bool upper = pass == 1;
-                for (int64_t step = 0; step < nb; ++step) {
-                    const int64_t j = upper ? nb - 1 - step : step;
-                    const int64_t r0 = j * kBlock;
-                    hipLaunchKernelGGL(apply_inverse<T>, dim3((unsigned)width), dim3(kBlock), 0, stream,
-                                       (upper ? inverse_upper : inverse_lower) + j * kBlock * kBlock, x, r0, width, packed);
-                    const int64_t rows = upper ? r0 : n - r0 - kBlock;
-                    if (rows == 0) continue;
-                    const T *a_block = upper ? f + r0 : f + (r0 + kBlock) * n + r0;
-                    T *c_block = upper ? x : x + (r0 + kBlock) * width;
-                    ProductArgs update{a_block, nullptr, packed, nullptr, c_block, rows, width, (int64_t)kBlock,
-                                       n, (int64_t)kBlock, width, -1.0, 1.0};
-                    const hipError_t e = product(mode, update, stream);
-                    if (e != hipSuccess) return (int)e;
+            unsigned long long *published = static_cast<unsigned long long *>(scratch);
+            for (int64_t col0 = 0; col0 < width; col0 += kTriangleColumns)
+                for (int pass = 0; pass < 2; ++pass) {
+                    const unsigned tag = ++epoch;
+                    if (pass == 0)
+                        hipLaunchKernelGGL((solve_triangle<T, false>), dim3((unsigned)nb), dim3(256), 0, stream, f,
+                                           inverse_lower, inverses + 2 * n * kBlock, x, n, width, col0, published,
+                                           tag, ws.error);
+                    else
+                        hipLaunchKernelGGL((solve_triangle<T, true>), dim3((unsigned)nb), dim3(256), 0, stream, f,
+                                           inverse_upper, inverses + 3 * n * kBlock, x, n, width, col0, published,
+                                           tag, ws.error);
                 }
-            }
             return hipGetLastError();
         }
         const int blocks = (int)((n + kSolveRows - 1) / kSolveRows);
@@ -1132,9 +1309,11 @@ namespace scihip {
     template int factor<float>(float *, int64_t, int64_t *, int, Workspace &, void *, size_t, hipStream_t,
                                const char **);
     template int substitute<double>(const double *, int64_t, double *, int64_t, Workspace &, unsigned &, hipStream_t,
-                                    const char **, const double *, double *);
+                                    const char **, const double *, void *);
     template int substitute<float>(const float *, int64_t, float *, int64_t, Workspace &, unsigned &, hipStream_t,
-                                   const char **, const float *, float *);
+                                   const char **, const float *, void *);
+
+    size_t substitute_scratch(int64_t order) { return (size_t)order * kTriangleColumns * 2 * 8; }
     template hipError_t invert_blocks<double>(const double *, int64_t, double *, hipStream_t);
     template hipError_t invert_blocks<float>(const float *, int64_t, float *, hipStream_t);
     template hipError_t permute<double, double>(const double *, double *, const int64_t *, int64_t, int64_t,

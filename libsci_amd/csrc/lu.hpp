@@ -67,14 +67,18 @@ namespace scihip {
     // Solves L U x = P b for `width` right-hand sides (row-major order x width), in place in `x`,
     // which holds P b on entry. `epoch` is the device's running launch count for the ready flags;
     // each launch takes the next value, so a flag never satisfies a wait of a later launch early.
-    // With `inverses` (2 * order * kBlock elements from invert_blocks) and `packed` (width * kBlock
-    // elements of scratch) and an order that takes it, the blocked substitution runs.
+    // With `inverses` (4 * order * kBlock elements from invert_blocks) and `scratch`
+    // (substitute_scratch(order) bytes) and an order that takes it, the blocked substitution runs.
     template <typename T>
     int substitute(const T *factors, int64_t order, T *x, int64_t width, Workspace &ws, unsigned &epoch,
-                   hipStream_t stream, const char **error, const T *inverses = nullptr, T *packed = nullptr);
+                   hipStream_t stream, const char **error, const T *inverses = nullptr, void *scratch = nullptr);
+
+    // Bytes of scratch the blocked substitution needs for an order (tagged copies of the solved blocks).
+    size_t substitute_scratch(int64_t order);
 
     // The transposed inverses of the diagonal blocks of L (first order * kBlock elements) and of U
-    // (the next order * kBlock) of packed LU factors, for the blocked substitution.
+    // (the next order * kBlock) of packed LU factors, then the links inv(D_jj) L_j,j-1 and inv(D_jj)
+    // U_j,j+1 (row-major, order * kBlock elements each), for the blocked substitution.
     template <typename T> hipError_t invert_blocks(const T *factors, int64_t order, T *inverses, hipStream_t stream);
 
     // Applies the row order of `pivots` to `source` (order x width) into `target`: target row k is

@@ -547,7 +547,7 @@ namespace scihip {
     // x = U^-1 L^-1 P source (source in S, solved in T), rows on the device.
     template <typename T, typename S>
     int solve_on_device(scihip_device *device, const T *factors, const int64_t *rows, const S *source, T *x,
-                        int64_t order, int64_t width, const T *inverses = nullptr, T *packed = nullptr) {
+                        int64_t order, int64_t width, const T *inverses = nullptr, void *packed = nullptr) {
         if (int rc = status(permute<T, S>(source, x, rows, order, width, device->stream), "permute")) return rc;
         const char *message = nullptr;
         const int e = substitute<T>(factors, order, x, width, device->lu, device->epoch, device->stream, &message,
@@ -810,8 +810,8 @@ int scihip_substitute(scihip_device *device, uint32_t precision, const void *fac
         return rc;
     // The blocked substitution needs the inverses of the diagonal blocks of these factors.
     const bool blocked = blocked_substitution(int64_t(order));
-    Scratch inverses(device, blocked ? 2 * order * kBlock * eb : 16, rc);
-    Scratch packed(device, blocked ? width * kBlock * eb : 16, rc);
+    Scratch inverses(device, blocked ? 4 * order * kBlock * eb : 16, rc);
+    Scratch packed(device, blocked ? substitute_scratch(order) : 16, rc);
     if (rc) return rc;
     if (precision == SCIHIP_WIDE) {
         const double *f = static_cast<const double *>(factors);
@@ -821,7 +821,7 @@ int scihip_substitute(scihip_device *device, uint32_t precision, const void *fac
         rc = solve_on_device<double, double>(device, f, device_rows.as<int64_t>(), static_cast<const double *>(right),
                                              static_cast<double *>(solution), int64_t(order), int64_t(width),
                                              blocked ? inverses.as<double>() : nullptr,
-                                             blocked ? packed.as<double>() : nullptr);
+                                             blocked ? packed.pointer : nullptr);
     } else {
         const float *f = static_cast<const float *>(factors);
         if (blocked && (rc = status(invert_blocks<float>(f, int64_t(order), inverses.as<float>(), device->stream),
@@ -830,7 +830,7 @@ int scihip_substitute(scihip_device *device, uint32_t precision, const void *fac
         rc = solve_on_device<float, float>(device, f, device_rows.as<int64_t>(), static_cast<const float *>(right),
                                            static_cast<float *>(solution), int64_t(order), int64_t(width),
                                            blocked ? inverses.as<float>() : nullptr,
-                                           blocked ? packed.as<float>() : nullptr);
+                                           blocked ? packed.pointer : nullptr);
     }
     if (rc) return rc;
     // The host rows buffer was copied asynchronously from pageable memory: finish before returning it.
@@ -886,8 +886,8 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     const bool aligned = n % 16 == 0;
     Scratch packed_a(device, aligned ? 16 : size_t(n) * size_t(kp) * 8, rc);
     const bool blocked = blocked_substitution(n);
-    Scratch inverses(device, blocked ? 2 * order * kBlock * eb : 16, rc);
-    Scratch packed(device, blocked ? width * kBlock * eb : 16, rc);
+    Scratch inverses(device, blocked ? 4 * order * kBlock * eb : 16, rc);
+    Scratch packed(device, blocked ? substitute_scratch(order) : 16, rc);
     // The residual product has few output tiles (n / 128 x 1) and a long shared dimension.
     const int slices = split_slices(kFloat64, n, w, kp);
     Scratch residual_slices(device, slices > 1 ? size_t(slices) * order * width * 8 : 16, rc);
@@ -934,10 +934,10 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
         if (wide) e = solve_on_device<double, double>(device, factors.as<double>(), device_rows.as<int64_t>(), source,
                                                       correction.as<double>(), n, w,
                                                       blocked ? inverses.as<double>() : nullptr,
-                                                      blocked ? packed.as<double>() : nullptr);
+                                                      blocked ? packed.pointer : nullptr);
         else e = solve_on_device<float, double>(device, factors.as<float>(), device_rows.as<int64_t>(), source,
                                                 correction.as<float>(), n, w, blocked ? inverses.as<float>() : nullptr,
-                                                blocked ? packed.as<float>() : nullptr);
+                                                blocked ? packed.pointer : nullptr);
         if (e) return e;
         if (!add) {
             if (wide) return status(hipMemcpyAsync(solution, correction.pointer, order * width * 8,
