@@ -858,33 +858,53 @@ namespace scihip {
         // invert_blocks): one product after the wait instead of two. x_j is stored into x and
         // published in tagged slots (tag: this launch's number) at published[((j * kBlock + r) *
         // kTriangleColumns + c) * kSlots]. A workgroup waits only on workgroups of lower index, which are
-        // dispatched before it. Thread t holds rows 4 (t / 8) .. + 3 and columns 4 (t % 8) .. + 3 of the
-        // block.
+        // dispatched before it. The products run on the matrix cores, each wave on 32 rows.
         constexpr int kTriangleColumns = 32;
+        template <typename T> struct Quad;
+        template <> struct Quad<float> {
+            typedef float type __attribute__((ext_vector_type(4)));
+        };
+        template <> struct Quad<double> {
+            typedef double type __attribute__((ext_vector_type(4)));
+        };
+        // One 16 x 16 x 4 matrix-core step, acc += a b (lane l: a = A[l % 16][l / 16], b = B[l / 16][l % 16]).
+        template <typename T>
+        __device__ __forceinline__ void mfma(typename Quad<T>::type &acc, T a, T b) {
+            if constexpr (sizeof(T) == 8) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            else acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+        }
+
         template <typename T, bool upper>
         __global__ __launch_bounds__(256) void solve_triangle(const T *f, const T *inverses_t, const T *links, T *x,
                                                               int64_t n, int64_t width, int64_t col0,
                                                               unsigned long long *published, unsigned tag,
                                                               unsigned *error) {
+            typedef typename Quad<T>::type V;
             constexpr int S = kSlots<T>;
             constexpr int C = kTriangleColumns;
             __shared__ T xl[kBlock][C + 1]; // x_i, later the block's reduced right-hand side
-            __shared__ T fl[kBlock][33];    // 32 columns of F_ji
+            __shared__ T fl[kBlock][33];    // 32 columns of the matrix being applied
             const int nb = (int)(n / kBlock), q = (int)blockIdx.x, j = upper ? nb - 1 - q : q;
-            const int t = threadIdx.x, rg = t >> 3, cg = t & 7;
+            const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, c16 = lane & 15;
             const int cols = (int)min((int64_t)C, width - col0);
             const int64_t r0 = (int64_t)j * kBlock;
-            T acc[4][4];
+            // Wave w holds rows 32 w .. 32 w + 31 as 2 x 2 tiles of 16 x 16 (matrix-core accumulators):
+            // element e of tile (i, h) is row 32 w + 16 i + (fp32: 4 g + e; fp64: g + 4 e), column 16 h + c16.
+            auto row_of = [&](int i, int e) { return 32 * wave + 16 * i + (sizeof(T) == 8 ? g + 4 * e : 4 * g + e); };
+            V acc[2][2];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    acc[a][b] = 4 * cg + b < cols ? x[(r0 + 4 * rg + a) * width + col0 + 4 * cg + b] : T(0);
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int c = 16 * h + c16;
+                        acc[i][h][e] = c < cols ? x[(r0 + row_of(i, e)) * width + col0 + c] : T(0);
+                    }
 
-            // acc[a][b] -= sum_k M[4 rg + a][k] * xl[k][4 cg + b] over k = 0 .. kBlock - 1 (k ascending), for the
-            // 128 x 128 matrix M whose entry (r, k) is at m[r * row + k * column]; M goes through LDS 32
-            // columns at a time, the next chunk's loads in flight while the current one is used.
-            // `chunk` holds the first chunk's values, loaded by the caller with fetch().
+            // into += sign * M xl over k = 0 .. kBlock - 1, for the 128 x 128 matrix M whose entry (r, k) is
+            // at m[r * row + k * column]; M goes through LDS 32 columns at a time, the next chunk's loads
+            // in flight while the current one is used. `chunk` holds the first chunk, loaded with fetch().
             auto fetch = [&](const T *m, int64_t row, int64_t column, int k0, T *chunk) {
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
@@ -892,7 +912,7 @@ namespace scihip {
                     chunk[e] = m[r * row + (int64_t)(k0 + k) * column];
                 }
             };
-            auto subtract = [&](const T *m, int64_t row, int64_t column, T *chunk, T (&into)[4][4], bool add) {
+            auto apply = [&](const T *m, int64_t row, int64_t column, T *chunk, V (&into)[2][2], T sign) {
                 for (int k0 = 0; k0 < kBlock; k0 += 32) {
 #pragma unroll
                     for (int e = 0; e < 16; ++e) {
@@ -901,24 +921,17 @@ namespace scihip {
                     }
                     __syncthreads();
                     if (k0 + 32 < kBlock) fetch(m, row, column, k0 + 32, chunk);
-#pragma unroll 8
-                    for (int k = 0; k < 32; ++k) {
-                        T fv[4], xv[4];
 #pragma unroll
-                        for (int a = 0; a < 4; ++a) fv[a] = fl[4 * rg + a][k];
+                    for (int kk = 0; kk < 32; kk += 4) {
+                        T am[2], bm[2];
 #pragma unroll
-                        for (int b = 0; b < 4; ++b) xv[b] = xl[k0 + k][4 * cg + b];
+                        for (int i = 0; i < 2; ++i) am[i] = sign * fl[32 * wave + 16 * i + c16][kk + g];
 #pragma unroll
-                        for (int a = 0; a < 4; ++a)
+                        for (int h = 0; h < 2; ++h) bm[h] = xl[k0 + kk + g][16 * h + c16];
 #pragma unroll
-                            for (int b = 0; b < 4; ++b) {
-                                if (add) {
-                                    if constexpr (sizeof(T) == 8) into[a][b] = __builtin_fma(fv[a], xv[b], into[a][b]);
-                                    else into[a][b] = __builtin_fmaf(fv[a], xv[b], into[a][b]);
-                                } else {
-                                    into[a][b] = eliminated(into[a][b], fv[a], xv[b]);
-                                }
-                            }
+                        for (int i = 0; i < 2; ++i)
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) mfma<T>(into[i][h], am[i], bm[h]);
                     }
                     __syncthreads();
                 }
@@ -966,8 +979,9 @@ namespace scihip {
                 T chunk[16];
                 fetch(m, n, 1, 0, chunk); // in flight while x_i is awaited
                 await(i);
-                subtract(m, n, 1, chunk, acc, false);
+                apply(m, n, 1, chunk, acc, T(-1));
             };
+
             // Every dependency but the one that completes last (block j - 1, or j + 1 for the upper
             // triangle), whose contribution goes through links: G_j = inv(D_jj) F_j,last.
             if (!upper)
@@ -977,36 +991,41 @@ namespace scihip {
             const int last = upper ? j + 1 : j - 1;
             const bool linked = upper ? j + 1 < nb : j > 0;
 
-            // x_j = inv(D_jj) acc; the inverse is given transposed, inv[r][k] at inverses_t[k * kBlock + r]
+            // out = inv(D_jj) acc (the inverse given transposed, inv[r][k] at inverses_t[k * kBlock + r]),
+            // then out -= G_j x_last
             const T *inverse = inverses_t + (int64_t)j * kBlock * kBlock;
             T chunk[16];
             fetch(inverse, 1, kBlock, 0, chunk);
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) xl[4 * rg + a][4 * cg + b] = acc[a][b];
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) xl[row_of(i, e)][16 * h + c16] = acc[i][h][e];
             __syncthreads();
-            T out[4][4];
+            V out[2][2];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) out[a][b] = T(0);
-            subtract(inverse, 1, kBlock, chunk, out, true);
+                for (int h = 0; h < 2; ++h) out[i][h] = V{T(0), T(0), T(0), T(0)};
+            apply(inverse, 1, kBlock, chunk, out, T(1));
             if (linked) {
-                const T *g = links + (int64_t)j * kBlock * kBlock; // G_j, row-major
-                fetch(g, kBlock, 1, 0, chunk);
+                const T *link = links + (int64_t)j * kBlock * kBlock; // G_j, row-major
+                fetch(link, kBlock, 1, 0, chunk);
                 await(last);
-                subtract(g, kBlock, 1, chunk, out, false);
+                apply(link, kBlock, 1, chunk, out, T(-1));
             }
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int r = 4 * rg + a, c = 4 * cg + b;
-                    if (c >= cols) continue;
-                    x[(r0 + r) * width + col0 + c] = out[a][b];
-                    put_tagged(published + (((int64_t)j * kBlock + r) * C + c) * S, out[a][b], tag);
-                }
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int r = row_of(i, e), c = 16 * h + c16;
+                        if (c >= cols) continue;
+                        x[(r0 + r) * width + col0 + c] = out[i][h][e];
+                        put_tagged(published + (((int64_t)j * kBlock + r) * C + c) * S, out[i][h][e], tag);
+                    }
         }
 
         // G_j = inv(D_jj) F_j,j-1 (lower) or inv(D_jj) F_j,j+1 (upper), row-major, for solve_triangle;
