@@ -1139,6 +1139,11 @@ namespace scihip {
 
     } // namespace
 
+    // Measured on MI355X at n = 16384: an fp64 panel takes about 0.6 ms, and the fp64 update beside it
+    // runs at about 25 TFLOP/s. They size the share of each fp64 update that overlaps the next panel.
+    constexpr double kPanelMsFp64 = 0.6;
+    constexpr double kAlongsideRateFp64 = 25e12;
+
     template <typename T>
     int factor(T *a, int64_t n, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
                hipStream_t stream, const char **error) {
@@ -1244,12 +1249,34 @@ namespace scihip {
                 if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
                 if (e != hipSuccess) return (int)e;
                 if (rest > next) {
-                    const unsigned char *b_hi = static_cast<const unsigned char *>(ub_hi) + (size_t)next * kp * eb;
-                    const unsigned char *b_lo =
-                            ub_lo ? static_cast<const unsigned char *>(ub_lo) + (size_t)next * kp * eb : nullptr;
-                    ProductArgs wide_update{la_hi, la_lo, b_hi, mode == kSplit ? b_lo : nullptr, c + next, rest,
-                                            rest - next, (int64_t)kp, (int64_t)kp, (int64_t)kp, n, -1.0, 1.0};
-                    e = product_alongside(mode, wide_update, stream); // the next panel runs meanwhile
+                    // The rest of the update, columns [next, rest). For fp64, the product beside the panel is
+                    // sized for one workgroup per compute unit so that the panel gets whole compute units;
+                    // only about the panel's duration of it runs that way (a share of the columns from the
+                    // measured rates), the remainder after the panel at two workgroups per compute unit.
+                    const int64_t columns = rest - next;
+                    int64_t beside = columns;
+                    if (wide) {
+                        const double flops = 2.0 * (double)rest * (double)columns * (double)w;
+                        const double share = kPanelMsFp64 * 1e-3 * kAlongsideRateFp64 / flops;
+                        if (share < 1.0)
+                            beside = std::max<int64_t>(kPanel, (int64_t)(share * (double)columns) / kPanel * kPanel);
+                        beside = std::min(beside, columns);
+                    }
+                    auto update = [&](int64_t from, int64_t count, bool alongside) -> hipError_t {
+                        const unsigned char *b_hi =
+                                static_cast<const unsigned char *>(ub_hi) + (size_t)(next + from) * kp * eb;
+                        const unsigned char *b_lo =
+                                ub_lo ? static_cast<const unsigned char *>(ub_lo) + (size_t)(next + from) * kp * eb
+                                      : nullptr;
+                        ProductArgs part{la_hi, la_lo, b_hi, mode == kSplit ? b_lo : nullptr, c + next + from, rest,
+                                         count, (int64_t)kp, (int64_t)kp, (int64_t)kp, n, -1.0, 1.0};
+                        return alongside ? product_alongside(mode, part, stream) : product(mode, part, stream);
+                    };
+                    e = update(0, beside, true); // the next panel runs meanwhile
+                    if (e == hipSuccess && beside < columns) {
+                        e = hipStreamWaitEvent(stream, ws.panel_done, 0);
+                        if (e == hipSuccess) e = update(beside, columns - beside, false);
+                    }
                     if (e != hipSuccess) return (int)e;
                 }
                 timer.stage(4);
