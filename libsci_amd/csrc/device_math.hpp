@@ -150,11 +150,14 @@ namespace scihip {
         return (unordered(l) || unordered(r)) ? l + r : (r > l ? r : l);
     }
 
-    // e^x (exponential.hpp:319-344).
+    // e^x (exponential.hpp:319-344). The clamp to [lower, upper] is two compare-selects that let a NaN
+    // through unchanged, which gives the same values as the CPU's NaN-propagating minimum / maximum
+    // (a NaN then flows through the arithmetic to the result).
     template <typename T> __device__ __forceinline__ T exponential(T x) {
         typedef Limits<T> L;
         constexpr Series<T, L::exp_terms> series = factorials<T, L::exp_terms>();
-        const T clamped = greater(lesser(x, L::upper), L::lower);
+        const T below = x > L::upper ? L::upper : x;
+        const T clamped = below < L::lower ? L::lower : below;
         const T k = fused(clamped, L::binary, L::round) - L::round;
         const T negative = -k;
         T r = fused(negative, L::natural, clamped);
@@ -168,12 +171,27 @@ namespace scihip {
                value(bits(T(rest + L::exponent)) << L::mantissa);
     }
 
-    // ln x (logarithm.hpp:90-126).
-    template <typename T> __device__ __forceinline__ T logarithm(T x) {
+    // f / d for the logarithm's f / (f + 2), f in [sqrt(1/2) - 1, sqrt(2) - 1]. For float this is the
+    // correctly rounded division's own Newton-Raphson sequence (hardware reciprocal, one refinement
+    // of it, two corrections of the quotient) without its range scaling and special-case fix-up, which
+    // change nothing for operands in this range: the same bits as f / d with two fewer steps.
+    template <typename T> __device__ __forceinline__ T ratio(T f, T d) { return f / d; }
+    template <> __device__ __forceinline__ float ratio(float f, float d) {
+        float y = __builtin_amdgcn_rcpf(d);
+        y = fused(fused(-d, y, 1.0f), y, y);
+        float q = f * y;
+        q = fused(fused(-d, q, f), y, q);
+        return fused(fused(-d, q, f), y, q);
+    }
+
+    // ln x (logarithm.hpp:90-126). `ordinary`: x is a positive normal number or NaN, so the CPU's
+    // scaling of subnormals and its special cases (0, +inf, negative) do not apply, and a NaN is
+    // returned as it came.
+    template <bool ordinary, typename T> __device__ __forceinline__ T logarithm_of(T x) {
         typedef Limits<T> L;
         typedef typename L::U U;
         constexpr Series<T, L::log_terms> series = reciprocals<T, L::log_terms>();
-        const bool small = x < L::tiny;
+        const bool small = !ordinary && x < L::tiny;
         const T scaled = small ? x * L::lift : x;
         const U pattern = bits(scaled);
         T exponent = value((U)((pattern >> L::mantissa) | bits(L::magnitude))) - L::exponent;
@@ -183,17 +201,33 @@ namespace scihip {
         exponent = large ? exponent + T(1) : exponent;
         exponent = small ? exponent - L::shift : exponent;
         const T f = mantissa - T(1);
-        const T s = f / (f + T(2));
+        const T s = ordinary ? ratio(f, f + T(2)) : f / (f + T(2));
         const T z = s * s;
         const T tail = z * horner<0>(series, z);
         const T half = (f * f) * T(0.5);
         const T inner = fused(s, half + tail, exponent * L::remainder);
         T result = exponent * L::natural - ((half - inner) - f);
-        const T infinity = T(__builtin_huge_val());
-        result = x == infinity ? x : result;
-        result = x == T(0) ? -infinity : result;
-        result = x < T(0) ? T(__builtin_nan("")) : result;
+        if (!ordinary) {
+            const T infinity = T(__builtin_huge_val());
+            result = x == infinity ? x : result;
+            result = x == T(0) ? -infinity : result;
+            result = x < T(0) ? T(__builtin_nan("")) : result;
+        }
         return unordered(x) ? x : result;
+    }
+
+    // Floating-point classes for __builtin_amdgcn_class: signalling and quiet NaN, positive normal.
+    constexpr int kNanOrPositiveNormal = 0x001 | 0x002 | 0x100;
+    __device__ __forceinline__ bool nan_or_positive_normal(float x) {
+        return __builtin_amdgcn_classf(x, kNanOrPositiveNormal);
+    }
+    __device__ __forceinline__ bool nan_or_positive_normal(double x) {
+        return __builtin_amdgcn_class(x, kNanOrPositiveNormal);
+    }
+
+    template <typename T> __device__ __forceinline__ T logarithm(T x) {
+        if (__builtin_expect(nan_or_positive_normal(x), 1)) return logarithm_of<true>(x);
+        return logarithm_of<false>(x);
     }
 
     template <typename T> __device__ __forceinline__ T library_sin(T x);
