@@ -16,7 +16,7 @@ namespace scihip {
 #include "embedded.inc"
 
 
-    std::string Compiler::generate(const scihip_chunk &chunk, int mode, bool contract) {
+    std::string Compiler::generate(const scihip_chunk &chunk, int mode, bool contract, uint32_t fold) {
         const bool single = chunk.precision == SCIHIP_SINGLE;
         const char *type = single ? "float" : "double";
         const int width = single ? 4 : 2;
@@ -31,7 +31,9 @@ namespace scihip {
           << "    const Program<T> &p = a.p;\n";
         for (int i = 0; i < chunk.literals; ++i)
             o << "    const L k" << i << " = splat<T, W>(p.constants[" << i << "]);\n";
-        if (mode == 1) o << "    double total = identity(p.fold);\n    uint32_t poisoned = 0;\n";
+        // The fold is a compile-time constant of the kernel, so each element's fold is straight-line code.
+        if (mode == 1)
+            o << "    constexpr uint32_t F = " << fold << ";\n    double total = identity(F);\n    uint32_t poisoned = 0;\n";
         // Each thread takes U vectors 256 apart per sweep; all of their loads are issued before any
         // of them is computed, so U * inputs 16-byte loads are in flight per thread.
         o << "    const uint64_t steps = (p.count + W - 1) / W;\n"
@@ -76,7 +78,7 @@ namespace scihip {
                 o << "            contract<T, W>(total, " << name(last.sources[0]) << ", " << name(last.sources[1])
                   << ", at" << u << ", full" << u << ", p.count);\n";
             } else {
-                o << "            consume<T, W>(p.fold, total, poisoned, " << name(chunk.result) << ", at" << u
+                o << "            consume<T, W>(F, total, poisoned, " << name(chunk.result) << ", at" << u
                   << ", full" << u << ", p.count);\n";
             }
             o << "        }\n";
@@ -87,7 +89,7 @@ namespace scihip {
         return o.str();
     }
 
-    std::string Compiler::generate_axis(const scihip_chunk &chunk, int axis, bool contract) {
+    std::string Compiler::generate_axis(const scihip_chunk &chunk, int axis, bool contract, uint32_t fold) {
         const bool single = chunk.precision == SCIHIP_SINGLE;
         const char *type = single ? "float" : "double";
         const int width = single ? 4 : 2;
@@ -98,7 +100,8 @@ namespace scihip {
           << "    typedef " << type << " T;\n"
           << "    constexpr int W = " << width << ";\n"
           << "    typedef Lanes<T, W> L;\n"
-          << "    const Program<T> &p = a.p;\n";
+          << "    const Program<T> &p = a.p;\n"
+          << "    constexpr uint32_t F = " << fold << ";\n";
         for (int i = 0; i < chunk.literals; ++i)
             o << "    const L k" << i << " = splat<T, W>(p.constants[" << i << "]);\n";
         const int steps = contract ? chunk.length - 1 : chunk.length;
@@ -131,7 +134,7 @@ namespace scihip {
             return std::make_pair(name(chunk.result), std::string());
         };
         if (axis == 1) {
-            o << "    double total = identity(p.fold);\n    uint32_t poisoned = 0;\n"
+            o << "    double total = identity(F);\n    uint32_t poisoned = 0;\n"
               << "    const uint64_t first = (uint64_t)blockIdx.x * a.columns, end = first + a.columns;\n"
               << "    const uint64_t vectors = (a.columns + W - 1) / W;\n"
               << "    for (uint64_t base = threadIdx.x; base < vectors; base += 256 * " << kUnroll << ") {\n";
@@ -151,21 +154,21 @@ namespace scihip {
                     o << "            contract<T, W>(total, " << result.first << ", " << result.second << ", at" << u
                       << ", full" << u << ", end);\n";
                 else
-                    o << "            consume<T, W>(p.fold, total, poisoned, " << result.first << ", at" << u << ", full"
+                    o << "            consume<T, W>(F, total, poisoned, " << result.first << ", at" << u << ", full"
                       << u << ", end);\n";
                 o << "        }\n";
             }
             o << "    }\n"
-              << "    const Partial folded = fold_block(p.fold, total, poisoned);\n"
+              << "    const Partial folded = fold_block(F, total, poisoned);\n"
               << "    if (threadIdx.x == 0) a.output[blockIdx.x] = folded.poisoned ? T(__builtin_nan(\"\")) : T(folded.value);\n";
         } else {
-            o << "    const uint64_t column = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * W;\n"
+            o << "    const uint64_t column = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * W;\n"
               << "    if (column >= a.columns) return;\n"
               << "    const uint64_t per_slice = (a.rows + gridDim.y - 1) / gridDim.y;\n"
               << "    const uint64_t begin = (uint64_t)blockIdx.y * per_slice;\n"
               << "    const uint64_t end = begin + per_slice < a.rows ? begin + per_slice : a.rows;\n"
               << "    const bool full = column + W <= a.columns;\n"
-              << "    const uint32_t fold = p.fold;\n"
+              << "    constexpr uint32_t fold = F;\n"
               << "    double total[W];\n    uint32_t poisoned[W];\n"
               << "#pragma unroll\n    for (int k = 0; k < W; ++k) { total[k] = identity(fold); poisoned[k] = 0; }\n"
               << "    for (uint64_t r = begin; r < end; r += " << kUnroll << ") {\n";
@@ -183,12 +186,8 @@ namespace scihip {
                     o << "#pragma unroll\n            for (int k = 0; k < W; ++k) total[k] = fused((double)" << result.first
                       << ".v[k], (double)" << result.second << ".v[k], total[k]);\n";
                 else
-                    o << "#pragma unroll\n            for (int k = 0; k < W; ++k) {\n"
-                      << "                const double x = (double)" << result.first << ".v[k];\n"
-                      << "                if (fold == 0) total[k] += x;\n"
-                      << "                else if (unordered(x)) poisoned[k] = 1;\n"
-                      << "                else total[k] = combine(fold, total[k], x);\n"
-                      << "            }\n";
+                    o << "#pragma unroll\n            for (int k = 0; k < W; ++k) fold_in(fold, total[k], poisoned[k], (double)"
+                      << result.first << ".v[k]);\n";
                 o << "        }\n";
             }
             o << "    }\n"

@@ -42,11 +42,11 @@ namespace scihip {
 
         // Straight-line source for a chunk. `mode` 0 evaluates, 1 reduces; `contract` marks a sum
         // whose last multiply is a dot product.
-        static std::string generate(const scihip_chunk &chunk, int mode, bool contract);
+        static std::string generate(const scihip_chunk &chunk, int mode, bool contract, uint32_t fold = 0);
 
         // Straight-line source folding a chunk along one axis (axis 1: a workgroup per row; axis 0:
         // W columns per thread over a slice of the rows), with kUnroll vectors in flight per thread.
-        static std::string generate_axis(const scihip_chunk &chunk, int axis, bool contract);
+        static std::string generate_axis(const scihip_chunk &chunk, int axis, bool contract, uint32_t fold = 0);
 
         static constexpr size_t limit = 1024; // distinct chunks compiled at most (vulkan/compiler.cpp:27)
         static constexpr int kUnroll = 4;     // vectors per thread per sweep of a specialised kernel

@@ -222,23 +222,27 @@ namespace scihip {
         return fold == 0 ? 0.0 : (fold == 1 ? __builtin_huge_val() : -__builtin_huge_val());
     }
 
-    // Folds W lanes of a result into a thread's running sum / extreme. Sums propagate NaN by
-    // arithmetic; minimum and maximum skip NaN and raise the flag, which makes the result NaN.
+    // Folds one value into a running sum / extreme. Sums propagate NaN by arithmetic; minimum and
+    // maximum skip NaN and raise the flag, which makes the result NaN. `fold` is wave-uniform; the NaN
+    // test is a select, not a branch, so lanes never diverge.
+    __device__ __forceinline__ void fold_in(uint32_t fold, double &total, uint32_t &poisoned, double x) {
+        if (fold == 0) {
+            total += x;
+            return;
+        }
+        const bool nan = unordered(x);
+        poisoned |= nan ? 1u : 0u;
+        const double y = nan ? total : x;
+        total = fold == 1 ? (y < total ? y : total) : (y > total ? y : total);
+    }
+
+    // Folds W lanes of a result into a thread's running sum / extreme.
     template <typename T, int W>
     __device__ __forceinline__ void consume(uint32_t fold, double &total, uint32_t &poisoned, const Lanes<T, W> &r,
                                             uint64_t at, bool full, uint64_t count) {
 #pragma unroll
-        for (int k = 0; k < W; ++k) {
-            if (!(full || at + k < count)) continue;
-            const double x = (double)r.v[k];
-            if (fold == 0) {
-                total += x;
-            } else if (unordered(x)) {
-                poisoned = 1;
-            } else {
-                total = combine(fold, total, x);
-            }
-        }
+        for (int k = 0; k < W; ++k)
+            if (full || at + k < count) fold_in(fold, total, poisoned, (double)r.v[k]);
     }
 
     // A contracted sum: sum(a * b) with each product exact in double (two floats' product always is).

@@ -236,11 +236,12 @@ namespace scihip {
         return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>({wanted, fill, kMaxBlocks})));
     }
 
-    std::string key_of(const scihip_chunk &chunk, int mode, bool contract) {
+    std::string key_of(const scihip_chunk &chunk, int mode, bool contract, uint32_t fold = 0) {
         std::string key;
-        key.reserve(8 + 5 * chunk.length);
+        key.reserve(9 + 5 * chunk.length);
         key += char(mode);
         key += char(contract);
+        key += char(fold);
         key += char(chunk.precision);
         key += char(chunk.arguments);
         key += char(chunk.literals);
@@ -264,8 +265,9 @@ namespace scihip {
         const size_t count = arguments.p.count;
         const Kernel *kernel = nullptr;
         if (device->policy != 1) {
-            const std::string key = key_of(chunk, mode, contract);
-            kernel = device->compiler->find(key, [&] { return Compiler::generate(chunk, mode, contract); },
+            const uint32_t fold = mode == 1 && !contract ? arguments.p.fold : 0;
+            const std::string key = key_of(chunk, mode, contract, fold);
+            kernel = device->compiler->find(key, [&] { return Compiler::generate(chunk, mode, contract, fold); },
                                             device->policy == 2);
             if (!kernel && device->policy == 2) {
                 fail("specialised kernel failed to compile");
@@ -480,8 +482,10 @@ namespace scihip {
         const bool contract = contracted(chunk, fold);
         const Kernel *kernel = nullptr;
         if (device->policy != 1) {
-            kernel = device->compiler->find(key_of(chunk, 3 + int(axis), contract),
-                                            [&] { return Compiler::generate_axis(chunk, int(axis), contract); },
+            const uint32_t kernel_fold = contract ? 0 : launch.p.fold;
+            kernel = device->compiler->find(key_of(chunk, 3 + int(axis), contract, kernel_fold), [&] {
+                return Compiler::generate_axis(chunk, int(axis), contract, kernel_fold);
+            },
                                             device->policy == 2);
             if (!kernel && device->policy == 2) {
                 fail("specialised kernel failed to compile");
@@ -491,10 +495,14 @@ namespace scihip {
         uint32_t slices = 1;
         int rc = SCIHIP_OK;
         std::unique_ptr<Scratch> partials;
+        // Specialised column folds run one-wave workgroups, four times as many as the device holds at
+        // once, so that workgroups finishing early on one XCD leave no compute units idle (measured
+        // 4.7 -> 5.1 TB/s at 16384^2 against four-wave workgroups that all start together).
+        const unsigned threads = kernel && axis == 0 ? 64 : 256;
         if (axis == 0) {
             const int w = (registers == kLarge && !kernel) ? 1 : Traits<T>::width;
-            const size_t tiles = (columns + 256 * size_t(w) - 1) / (256 * size_t(w));
-            const size_t wanted = (size_t(device->units) * 8 + tiles - 1) / tiles;
+            const size_t tiles = (columns + threads * size_t(w) - 1) / (threads * size_t(w));
+            const size_t wanted = (size_t(device->units) * 8 * (256 / threads) + tiles - 1) / tiles;
             slices = uint32_t(std::max<size_t>(1, std::min<size_t>({wanted, rows, 65535})));
             partials = std::make_unique<Scratch>(device, size_t(slices) * columns * sizeof(Partial), rc);
             if (rc) return rc;
@@ -503,10 +511,10 @@ namespace scihip {
         if (kernel) {
             ++device->specialised;
             const unsigned grid_x = axis == 1 ? unsigned(rows)
-                                              : unsigned((columns + 256 * size_t(Traits<T>::width) - 1) /
-                                                         (256 * size_t(Traits<T>::width)));
+                                              : unsigned((columns + threads * size_t(Traits<T>::width) - 1) /
+                                                         (threads * size_t(Traits<T>::width)));
             void *parameters[] = {&a};
-            if ((rc = status(hipModuleLaunchKernel(kernel->function, grid_x, axis == 1 ? 1 : slices, 1, 256, 1, 1, 0,
+            if ((rc = status(hipModuleLaunchKernel(kernel->function, grid_x, axis == 1 ? 1 : slices, 1, threads, 1, 1, 0,
                                                    device->stream, parameters, nullptr),
                              "specialised axis launch")))
                 return rc;
