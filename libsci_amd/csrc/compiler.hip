@@ -36,9 +36,58 @@ namespace scihip {
             o << "    constexpr uint32_t F = " << fold << ";\n    double total = identity(F);\n    uint32_t poisoned = 0;\n";
         // Each thread takes U vectors 256 apart per sweep; all of their loads are issued before any
         // of them is computed, so U * inputs 16-byte loads are in flight per thread.
+        const int steps = contract ? chunk.length - 1 : chunk.length;
+        // The chunk's instructions on vector u; `bulk`: the vector is inside the arrays, which are
+        // aligned (no checks).
+        auto body = [&](int u, bool bulk, const char *indent) {
+            std::string temporaries[65];
+            const std::string suffix = "_" + std::to_string(u);
+            const std::string full = bulk ? "true" : "full" + std::to_string(u);
+            auto name = [&](uint8_t id) -> std::string {
+                const uint8_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3F;
+                if (tag == SCIHIP_SLOT_INPUT) return "i" + std::to_string(index) + suffix;
+                if (tag == SCIHIP_SLOT_LITERAL) return "k" + std::to_string(index);
+                if (tag == SCIHIP_SLOT_TEMPORARY) return temporaries[index];
+                return temporaries[64];
+            };
+            for (int s = 0; s < steps; ++s) {
+                const scihip_instruction &in = chunk.instructions[s];
+                const std::string first = name(in.sources[0]);
+                const std::string second = in.sources[1] == SCIHIP_SLOT_NONE ? first : name(in.sources[1]);
+                const std::string third = in.sources[2] == SCIHIP_SLOT_NONE ? first : name(in.sources[2]);
+                const std::string target = "s" + std::to_string(s) + suffix;
+                o << indent << "const L " << target << " = lanes<" << unsigned(in.code) << "u, T, W>(" << first << ", "
+                  << second << ", " << third << ");\n";
+                if (in.target == SCIHIP_SLOT_OUTPUT) temporaries[64] = target;
+                else temporaries[in.target & 0x3F] = target;
+            }
+            if (mode == 0) {
+                if (bulk) o << indent << "store_vector<T, W>(p, at" << u << ", " << name(SCIHIP_SLOT_OUTPUT) << ");\n";
+                else o << indent << "store<T, W>(p, at" << u << ", " << full << ", " << name(SCIHIP_SLOT_OUTPUT) << ");\n";
+            } else if (contract) {
+                const scihip_instruction &last = chunk.instructions[chunk.length - 1];
+                o << indent << "contract<T, W>(total, " << name(last.sources[0]) << ", " << name(last.sources[1])
+                  << ", at" << u << ", " << full << ", p.count);\n";
+            } else {
+                o << indent << "consume<T, W>(F, total, poisoned, " << name(chunk.result) << ", at" << u << ", " << full
+                  << ", p.count);\n";
+            }
+        };
         o << "    const uint64_t steps = (p.count + W - 1) / W;\n"
           << "    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * U;\n"
-          << "    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * U + threadIdx.x; base < steps; base += stride) {\n";
+          << "    uint64_t base = (uint64_t)blockIdx.x * blockDim.x * U + threadIdx.x;\n";
+        // Sweeps whose U vectors all lie inside aligned arrays: unconditional 16-byte loads and stores,
+        // so no branch separates the loads from one another. The rest of the sweeps follow, checked.
+        o << "    if (p.aligned) {\n"
+          << "        const uint64_t whole = p.count / W;\n"
+          << "        for (; base + " << 256 * (kUnroll - 1) << " < whole; base += stride) {\n";
+        for (int u = 0; u < kUnroll; ++u) o << "            const uint64_t at" << u << " = (base + " << 256 * u << ") * W;\n";
+        for (int u = 0; u < kUnroll; ++u)
+            for (int i = 0; i < chunk.arguments; ++i)
+                o << "            const L i" << i << "_" << u << " = fetch_vector<T, W>(p, " << i << ", at" << u << ");\n";
+        for (int u = 0; u < kUnroll; ++u) body(u, true, "            ");
+        o << "        }\n    }\n";
+        o << "    for (; base < steps; base += stride) {\n";
         for (int u = 0; u < kUnroll; ++u)
             o << "        const uint64_t v" << u << " = base + " << 256 * u << ";\n"
               << "        const bool live" << u << " = v" << u << " < steps;\n"
@@ -48,39 +97,9 @@ namespace scihip {
             for (int i = 0; i < chunk.arguments; ++i)
                 o << "        L i" << i << "_" << u << "{}; if (live" << u << ") i" << i << "_" << u
                   << " = fetch<T, W>(p, " << i << ", at" << u << ", full" << u << ");\n";
-        const int steps = contract ? chunk.length - 1 : chunk.length;
         for (int u = 0; u < kUnroll; ++u) {
-            std::string temporaries[65];
-            const std::string suffix = "_" + std::to_string(u);
-            auto name = [&](uint8_t id) -> std::string {
-                const uint8_t tag = id & SCIHIP_SLOT_TYPES, index = id & 0x3F;
-                if (tag == SCIHIP_SLOT_INPUT) return "i" + std::to_string(index) + suffix;
-                if (tag == SCIHIP_SLOT_LITERAL) return "k" + std::to_string(index);
-                if (tag == SCIHIP_SLOT_TEMPORARY) return temporaries[index];
-                return temporaries[64];
-            };
             o << "        if (live" << u << ") {\n";
-            for (int s = 0; s < steps; ++s) {
-                const scihip_instruction &in = chunk.instructions[s];
-                const std::string first = name(in.sources[0]);
-                const std::string second = in.sources[1] == SCIHIP_SLOT_NONE ? first : name(in.sources[1]);
-                const std::string third = in.sources[2] == SCIHIP_SLOT_NONE ? first : name(in.sources[2]);
-                const std::string target = "s" + std::to_string(s) + suffix;
-                o << "            const L " << target << " = lanes<" << unsigned(in.code) << "u, T, W>(" << first << ", "
-                  << second << ", " << third << ");\n";
-                if (in.target == SCIHIP_SLOT_OUTPUT) temporaries[64] = target;
-                else temporaries[in.target & 0x3F] = target;
-            }
-            if (mode == 0) {
-                o << "            store<T, W>(p, at" << u << ", full" << u << ", " << name(SCIHIP_SLOT_OUTPUT) << ");\n";
-            } else if (contract) {
-                const scihip_instruction &last = chunk.instructions[chunk.length - 1];
-                o << "            contract<T, W>(total, " << name(last.sources[0]) << ", " << name(last.sources[1])
-                  << ", at" << u << ", full" << u << ", p.count);\n";
-            } else {
-                o << "            consume<T, W>(F, total, poisoned, " << name(chunk.result) << ", at" << u
-                  << ", full" << u << ", p.count);\n";
-            }
+            body(u, false, "            ");
             o << "        }\n";
         }
         o << "    }\n";

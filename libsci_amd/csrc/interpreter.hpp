@@ -131,6 +131,17 @@ namespace scihip {
         return l;
     }
 
+    // The same as one 16-byte load, for W elements that lie inside an aligned input.
+    template <typename T, int W, typename P>
+    __device__ __forceinline__ Lanes<T, W> fetch_vector(const P &p, uint32_t i, uint64_t at) {
+        static_assert(W * sizeof(T) == 16, "one 16-byte vector");
+        typedef typename Vector<T, W>::type V;
+        const V x = __builtin_nontemporal_load(reinterpret_cast<const V *>(p.inputs[i] + at));
+        Lanes<T, W> l;
+        __builtin_memcpy(&l, &x, sizeof(l));
+        return l;
+    }
+
     template <typename T, int W> __device__ __forceinline__ Lanes<T, W> splat(T value) {
         Lanes<T, W> l;
 #pragma unroll
@@ -187,6 +198,16 @@ namespace scihip {
     template <typename T, int W>
     __device__ __forceinline__ void store(const Program<T> &p, uint64_t at, bool full, const Lanes<T, W> &out) {
         store_to<T, W>(p.output, p.count, p.aligned != 0, at, full, out);
+    }
+
+    // The same as one 16-byte store, for W elements that lie inside an aligned output.
+    template <typename T, int W>
+    __device__ __forceinline__ void store_vector(const Program<T> &p, uint64_t at, const Lanes<T, W> &out) {
+        static_assert(W * sizeof(T) == 16, "one 16-byte vector");
+        typedef typename Vector<T, W>::type V;
+        V x;
+        __builtin_memcpy(&x, &out, sizeof(x));
+        __builtin_nontemporal_store(x, reinterpret_cast<V *>(p.output + at));
     }
 
     // A reduction partial: the sum, or the extreme of the non-NaN values, and whether a NaN was seen.
