@@ -44,9 +44,7 @@ namespace scihip {
 
     template <typename T> struct AdjointLaunch {
         Adjoint<T> p;
-        Partial *partials; // loss: one per workgroup
-        uint32_t *done;    // loss: workgroups finished; zero on entry and on exit
-        double *result;    // loss: the result
+        Partial *partials; // loss: one per workgroup (folded by fold_partials)
     };
 
     // The adjoint interpreter: the register file of R values per lane stays in VGPRs for R * sizeof(T)
@@ -71,7 +69,7 @@ namespace scihip {
             for (uint32_t g = 0; g < p.gradients; ++g)
                 if (p.outputs[g]) store_to<T, W>(p.outputs[g], p.count, p.aligned != 0, at, full, regs.get(p.slot[g]));
         }
-        if (p.loss) finish_fold(0, a.partials, a.done, a.result, total, poisoned);
+        if (p.loss) leave_partial(0, a.partials, total, poisoned);
     }
 
 } // namespace scihip

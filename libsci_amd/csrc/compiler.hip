@@ -103,7 +103,7 @@ namespace scihip {
             o << "        }\n";
         }
         o << "    }\n";
-        if (mode == 1) o << "    finish<T>(a, total, poisoned);\n";
+        if (mode == 1) o << "    leave_partial(F, a.partials, total, poisoned);\n";
         o << "}\n";
         return o.str();
     }

@@ -12,6 +12,7 @@
 #include "adjoint.hpp"
 #include "compiler.hpp"
 #include "differentiate.hpp"
+#include "kernels.hpp"
 #include "product.hpp"
 #include "runtime.hpp"
 
@@ -490,7 +491,7 @@ namespace scihip {
             o << "        }\n";
         }
         o << "    }\n";
-        if (plan.loss) o << "    finish_fold(0, a.partials, a.done, a.result, total, poisoned);\n";
+        if (plan.loss) o << "    leave_partial(0, a.partials, total, poisoned);\n";
         o << "}\n";
         return o.str();
     }
@@ -573,8 +574,6 @@ namespace scihip {
             }
             p.aligned = aligned ? 1u : 0u;
             launch.partials = static_cast<Partial *>(device->partials);
-            launch.done = device->done;
-            launch.result = device->result;
             const int mode = plan.loss ? 1 : 0;
 
             const Kernel *kernel = nullptr;
@@ -586,8 +585,9 @@ namespace scihip {
                 return SCIHIP_ECOMPILE;
             }
             int rc;
+            unsigned blocks = 0;
             if (kernel) {
-                const unsigned blocks = blocks_for(device, count, lanes_of<T>(), kAdjointUnroll, mode);
+                blocks = blocks_for(device, count, lanes_of<T>(), kAdjointUnroll, mode);
                 void *parameters[] = {&launch}; // copied at launch (and into a graph node under capture)
                 ++device->specialised;
                 rc = status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream,
@@ -599,10 +599,11 @@ namespace scihip {
                                                                                                    : kAdjointLarge;
                 const int width = registers == kAdjointLarge ? 1 : lanes_of<T>();
                 ++device->interpreted;
-                rc = status(interpret_adjoint<T>(launch, registers, blocks_for(device, count, width, 1, mode),
-                                                 device->stream),
-                            "adjoint launch");
+                blocks = blocks_for(device, count, width, 1, mode);
+                rc = status(interpret_adjoint<T>(launch, registers, blocks, device->stream), "adjoint launch");
             }
+            if (!rc && plan.loss)
+                rc = status(fold_partials(0, launch.partials, blocks, device->result, device->stream), "loss fold");
             if (rc || !loss) return rc;
             if ((rc = status(hipMemcpyAsync(device->staging, device->result, sizeof(double), hipMemcpyDeviceToHost,
                                             device->stream),

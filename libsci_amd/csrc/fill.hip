@@ -1,4 +1,5 @@
-// Seeded synthetic data generated in place on the device, and small helpers shared by the runtime.
+// Seeded synthetic data generated in place on the device, the fold of reduction partials, and small
+// helpers shared by the runtime.
 #include "kernels.hpp"
 
 namespace scihip {
@@ -9,6 +10,18 @@ namespace scihip {
     const void *interpreter_function(int precision, int registers, int mode) {
         return precision == SCIHIP_SINGLE ? interpreter_function_f32(registers, mode)
                                           : interpreter_function_f64(registers, mode);
+    }
+
+    __global__ __launch_bounds__(256) void fold_partials_kernel(uint32_t fold, const Partial *partials, uint32_t count,
+                                                                double *result) {
+        const Partial folded = fold_partials_block(fold, partials, count);
+        if (threadIdx.x == 0) *result = folded.poisoned ? __builtin_nan("") : folded.value;
+    }
+
+    hipError_t fold_partials(uint32_t fold, const Partial *partials, unsigned count, double *result,
+                             hipStream_t stream) {
+        hipLaunchKernelGGL(fold_partials_kernel, dim3(1), dim3(256), 0, stream, fold, partials, count, result);
+        return hipGetLastError();
     }
 
     // SplitMix64's finaliser of seed * phi + index: a counter-based hash, so any element can be made

@@ -29,6 +29,10 @@ namespace scihip {
     hipError_t interpret(const Launch<float> &launch, int registers, int mode, unsigned blocks, hipStream_t stream);
     hipError_t interpret(const Launch<double> &launch, int registers, int mode, unsigned blocks, hipStream_t stream);
 
+    // Folds the `count` partials a reduction kernel left, in workgroup order, into *result (one
+    // workgroup; NaN when a minimum / maximum saw one).
+    hipError_t fold_partials(uint32_t fold, const Partial *partials, unsigned count, double *result, hipStream_t stream);
+
     // The function of each interpreter variant, for occupancy queries.
     const void *interpreter_function(int precision, int registers, int mode);
 

@@ -214,8 +214,6 @@ namespace scihip {
         p.aligned = aligned ? 1u : 0u;
         p.output = output;
         launch.partials = static_cast<Partial *>(device->partials);
-        launch.done = device->done;
-        launch.result = device->result;
         const uint32_t needed = number.registers();
         if (needed > uint32_t(kLarge)) {
             fail("chunk needs more registers than the interpreter has");
@@ -274,17 +272,23 @@ namespace scihip {
                 return SCIHIP_ECOMPILE;
             }
         }
+        unsigned blocks = 0;
+        int rc = SCIHIP_OK;
         if (kernel) {
-            const unsigned blocks = blocks_for(device, count, Traits<T>::width, Compiler::kUnroll, mode);
+            blocks = blocks_for(device, count, Traits<T>::width, Compiler::kUnroll, mode);
             void *parameters[] = {&arguments}; // copied at launch (and into a graph node under capture)
             ++device->specialised;
-            return status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream,
-                                                parameters, nullptr),
-                          "specialised launch");
+            rc = status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream, parameters,
+                                              nullptr),
+                        "specialised launch");
+        } else {
+            ++device->interpreted;
+            blocks = blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width, 1, mode);
+            rc = status(interpret(arguments, registers, mode, blocks, device->stream), "interpreter launch");
         }
-        ++device->interpreted;
-        const unsigned blocks = blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width, 1, mode);
-        return status(interpret(arguments, registers, mode, blocks, device->stream), "interpreter launch");
+        if (rc || mode != 1) return rc;
+        return status(fold_partials(arguments.p.fold, arguments.partials, blocks, device->result, device->stream),
+                      "reduction fold");
     }
 
     template <typename T>
@@ -594,9 +598,7 @@ int scihip_open(uint32_t ordinal, scihip_device **out) {
     if (int rc = status(hipStreamCreateWithFlags(&device->stream, hipStreamNonBlocking), "hipStreamCreate")) return rc;
     device->allocator = std::make_unique<Allocator>(device->stream);
     if (int rc = status(hipMalloc(&device->partials, kMaxBlocks * sizeof(Partial)), "hipMalloc")) return rc;
-    if (int rc = status(hipMalloc(reinterpret_cast<void **>(&device->done), 256), "hipMalloc")) return rc;
-    if (int rc = status(hipMemset(device->done, 0, 256), "hipMemset")) return rc;
-    device->result = reinterpret_cast<double *>(reinterpret_cast<char *>(device->done) + 128);
+    if (int rc = status(hipMalloc(reinterpret_cast<void **>(&device->result), 256), "hipMalloc")) return rc;
     if (int rc = status(hipHostMalloc(reinterpret_cast<void **>(&device->staging), sizeof(double)), "hipHostMalloc"))
         return rc;
     if (const char *text = std::getenv("SCIHIP_JIT")) {
@@ -625,7 +627,7 @@ int scihip_close(scihip_device *device) {
         release_lu(device);
         device->allocator.reset();
         (void)hipFree(device->partials);
-        (void)hipFree(device->done);
+        (void)hipFree(device->result);
         (void)hipHostFree(device->staging);
         (void)hipStreamDestroy(device->stream);
     }

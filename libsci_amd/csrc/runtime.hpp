@@ -87,7 +87,6 @@ struct scihip_device {
     std::unique_ptr<scihip::Compiler> compiler;
     int units = 0;              // compute units
     void *partials = nullptr;   // reduction partials, one per workgroup
-    uint32_t *done = nullptr;   // reduction completion counter (zero between launches)
     double *result = nullptr;   // reduction result on the device
     double *staging = nullptr;  // pinned host copy of it
     int policy = 0;             // 0 automatic, 1 interpreter only, 2 specialised only
