@@ -215,6 +215,59 @@ def measure_two_dimensional(sci, _abi, device, events, n=16384, steps=10) -> dic
     return result
 
 
+def measure_wide_store(sci, _abi, device, events, count: int, steps: int) -> dict:
+    """The C2 store expression over float64 arrays of `count` elements (the same bytes as the float32
+    workload at twice the count): the precision the reference's GPU path refuses and hands to the CPU."""
+    import numpy as np
+
+    a = sci.uniform(count, device, 11, -4.0, 4.0, dtype=np.float64)
+    b = sci.uniform(count, device, 12, -4.0, 4.0, dtype=np.float64, nan_seed=13, nan_fraction=0.01)
+    c = sci.uniform(count, device, 14, -4.0, 4.0, dtype=np.float64)
+    out = sci.Array(device, (count,), np.float64)
+    expression = sci.exponential(a) * 0.5 + sci.logarithm(sci.absolute(b) + 1.0) * c
+    chunk = sci.record(expression)
+    device.prepare(expression)
+    device.set_policy(2)
+
+    def store():
+        _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), out.pointer, count))
+
+    _, ms = time_launches(device, events, store, steps, 2)
+    device.set_policy(0)
+    return {"workload": f"exp(a)*0.5 + log(|b|+1)*c, N={count} float64", "ms": round(ms, 4),
+            "GBps": round(32 * count / (ms * 1e-3) / 1e9, 1)}
+
+
+def measure_small_arrays(sci, _abi, device, events, steps: int = 50) -> dict:
+    """Latency of small chunks, the sizes below the reference's CPU hand-off threshold
+    (src/core/drivers/device.cpp:100-104): a store (asynchronous; device time per launch) and a
+    sum (synchronous; wall time per call, result on the host) of sin(a) * b."""
+    out = {}
+    for count in (1_000, 100_000, 1_000_000):
+        a = sci.uniform(count, device, 21, -1.0, 1.0)
+        b = sci.uniform(count, device, 22, -1.0, 1.0)
+        target = sci.Array(device, (count,), "float32")
+        expression = sci.sin(a) * b
+        store_chunk = sci.record(expression)
+        sum_chunk = sci.record(expression, reduce=True)
+        device.prepare(expression)
+        device.prepare(expression, reduce=True)
+        total = ctypes.c_double()
+
+        def store():
+            _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(store_chunk.struct), target.pointer,
+                                                 count))
+
+        def total_of():
+            _abi.check(_abi.lib().scihip_reduce(device.handle, ctypes.byref(sum_chunk.struct), 0, count,
+                                                ctypes.byref(total)))
+
+        _, store_ms = time_launches(device, events, store, steps, 3)
+        sum_wall, _ = time_launches(device, events, total_of, steps, 3)
+        out[str(count)] = {"store_us": round(store_ms * 1e3, 2), "sum_call_us": round(sum_wall * 1e3, 2)}
+    return out
+
+
 def measure_graph(sci, _abi, device, events, count=1_000_000, chain=32, steps=20) -> dict:
     """The launch-bound case: `chain` dependent chunk stores over 1e6 float32 elements (the CPU-runnable
     workload's size: c = a*b + 2, then alternately sin / cos of the previous result times b), issued
@@ -402,6 +455,8 @@ def main() -> int:
         dot_call = reduction(a * c)
         w, _ = time_launches(device, events, dot_call, max(args.steps // 2, 5), 2)
         extra["c2_dot_GBps"] = round(8 * n / (w * 1e-3) / 1e9, 1)
+        extra["fp64"] = measure_wide_store(sci, _abi, device, events, n // 2, max(args.steps // 2, 3))
+        extra["small_arrays"] = measure_small_arrays(sci, _abi, device, events)
 
     if args.store_only:
         copy_chunk = sci.record(sci.copy(a))

@@ -9,11 +9,13 @@
 //    plus the linear-algebra and adjoint entry points of the backend.
 //  - Array<T> is a device array; Expr<T> a lazy expression over arrays and scalars that records one
 //    Chunk when it is assigned, evaluated or reduced (include/array/expression.hpp:158-222).
+//    Array<T>::save / load read and write .npy files (sci/npy.hpp).
 //
 // Errors throw sci::hip::Error (sci::hip::Declined when the backend cannot run the request as given).
 #pragma once
 
 #include "scihip.h"
+#include "sci/npy.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -260,6 +262,17 @@ namespace sci::hip {
             std::vector<T> values(count);
             owner->download(values.data(), data(), count * sizeof(T));
             return values;
+        }
+
+        // Writes the array to a .npy file (its shape and elements; see sci/npy.hpp).
+        void save(const std::string &path) const { npy::write(path, host(), dims); }
+
+        // An array on `device` holding a .npy file's elements, which must be of type T.
+        static Array load(Device &device, const std::string &path) {
+            std::vector<size_t> shape;
+            const std::vector<T> values = npy::read<T>(path, shape);
+            if (shape.empty()) shape = {1};
+            return from(device, values, shape);
         }
 
         T *data() const noexcept { return static_cast<T *>(storage.get()); }

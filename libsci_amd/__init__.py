@@ -13,6 +13,6 @@ from .array import (  # noqa: F401
     copy, negate, absolute, root, exponential, logarithm, sin, cos, tan, asin, acos, atan, csc, sec, cot,
     acsc, asec, acot, sinh, cosh, tanh, asinh, acosh, atanh, csch, sech, coth, acsch, asech, acoth,
     add, subtract, multiply, divide,
-    linspace, full, ones, zeros, uniform,
+    linspace, full, ones, zeros, uniform, save, load,
 )
 from ._abi import BackendError, Declined  # noqa: F401

@@ -456,6 +456,20 @@ def maximum(left, right=None, axis=None):
     return _binary(Code.maximum, left, right)
 
 
+def save(path, array: Array) -> None:
+    """Writes a device array to a .npy file (numpy's format, which include/sci/npy.hpp also reads and
+    writes; the reference plans .npy I/O, README.md:155)."""
+    np.save(path, array.host(), allow_pickle=False)
+
+
+def load(path, device: Device) -> Array:
+    """A device array holding a .npy file's float32 or float64 elements (nothing in the file is executed)."""
+    values = np.load(path, allow_pickle=False)
+    if values.dtype not in (np.float32, np.float64):
+        raise TypeError(f"{path}: elements are {values.dtype}, not float32 or float64")
+    return Array.from_numpy(values.reshape(values.shape or (1,)), device)
+
+
 def transpose(array: Array) -> Array:
     """A new device array holding array^T (scihip_transpose; the reference's transpose,
     include/operations/manipulations.hpp:36-62, remaps on the host)."""

@@ -149,7 +149,13 @@ namespace {
         for (size_t i = 0; i < right.size(); ++i) right[i] = float(int(i % 5) - 2) * 0.5f;
         Array<float> l = Array<float>::from(device, left, {64, 40});
         Array<float> r = Array<float>::from(device, right, {40, 48});
-        write(directory + "/product.f32", product(l, r, Mode::float32).host());
+        const Array<float> p = product(l, r, Mode::float32);
+        write(directory + "/product.f32", p.host());
+        // .npy out and back in (sci/npy.hpp): the file numpy reads, and the array loaded from it
+        p.save(directory + "/product.npy");
+        const Array<float> reloaded = Array<float>::load(device, directory + "/product.npy");
+        std::printf("reloaded shape %zu x %zu\n", reloaded.shape()[0], reloaded.shape()[1]);
+        write(directory + "/reloaded.f32", reloaded.host());
 
         // include/linear tests: A x = b with x = [1, -2, 3] (tests/linear/test_solve.cpp:8-25)
         Array<double> A = Array<double>::from(device, {4, -2, 1, -2, 4, -2, 1, -2, 4}, {3, 3});

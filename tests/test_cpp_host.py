@@ -3,7 +3,7 @@
 CPU: the mirror's Chunk records the workload expressions exactly as the Python recorder does, which
 tests/test_oracle.py pins to the reference's own Chunk and to the hand-derived bytecode.
 GPU: the program's small workload (c = a*b + 2 over linspace / ones, sum(sin(c)), a fused gradient,
-a product and the 3 x 3 solve KAT) agrees with the oracle: c bit for bit, the sum within the
+a product, saved to .npy and loaded back, and the 3 x 3 solve KAT) agrees with the oracle: c bit for bit, the sum within the
 reordering bound, the gradients within the adjoint tests' bound, the product exactly (its values are
 exact in float) and x = [1, -2, 3] within 1e-12.
 """
@@ -97,6 +97,10 @@ def test_cpp_workload_on_the_gpu(tmp_path):
     right = ((np.arange(40 * 48) % 5) - 2).astype(np.float32).reshape(40, 48) * 0.5
     product = np.fromfile(tmp_path / "product.f32", dtype=np.float32).reshape(64, 48)
     assert (product == (left.astype(np.float64) @ right.astype(np.float64))).all()
+    saved = np.load(tmp_path / "product.npy", allow_pickle=False)  # Array::save
+    assert saved.dtype == np.float32 and saved.shape == (64, 48) and (saved == product).all()
+    assert "reloaded shape 64 x 48" in result.stdout  # Array::load
+    assert (np.fromfile(tmp_path / "reloaded.f32", dtype=np.float32).reshape(64, 48) == product).all()
 
     x = np.fromfile(tmp_path / "solve.f64", dtype=np.float64)
     assert np.abs(x - [1.0, -2.0, 3.0]).max() < 1e-12
