@@ -1180,12 +1180,15 @@ namespace scihip {
             if (e != hipSuccess) return (int)e;
             resident = units * per_unit;
         }
-        // Factors the panel at column j0 on stream `s`.
+        // Clears the candidate lines for the next panel launch on stream `s` (each launch expects them
+        // zero; the clear after a look-ahead panel runs on its stream while the main stream works).
+        const size_t line_bytes = (size_t)kParities * max_groups * kLine * sizeof(unsigned long long);
+        auto clear_lines = [&](hipStream_t s) { return hipMemsetAsync(lines, 0, line_bytes, s); };
+        // Factors the panel at column j0 on stream `s` (its candidate lines already clear).
         auto launch_panel = [&](int64_t j0, hipStream_t s) -> hipError_t {
             int w = (int)std::min<int64_t>(kPanel, n - j0);
             const int groups = (int)((n - j0 + kPanelRows - 1) / kPanelRows);
-            hipError_t e = hipMemsetAsync(lines, 0, (size_t)kParities * groups * kLine * sizeof(unsigned long long), s);
-            if (e != hipSuccess) return e;
+            hipError_t e = hipSuccess;
             // The launch number goes into the publications' tags (24 bits); when it wraps, the buffers
             // are cleared so that no slot of an earlier launch carries a tag that is current again.
             if (++ws.launches > 0xFFFFFFu) {
@@ -1210,7 +1213,9 @@ namespace scihip {
             return e;
         };
         const size_t eb = (size_t)element_bytes(mode);
-        hipError_t e = launch_panel(0, stream);
+        hipError_t e = clear_lines(stream);
+        if (e == hipSuccess) e = launch_panel(0, stream);
+        if (e == hipSuccess) e = clear_lines(stream);
         if (e != hipSuccess) return (int)e;
         for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
             const int w = (int)std::min<int64_t>(kPanel, n - j0);
@@ -1247,6 +1252,7 @@ namespace scihip {
                 if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
                 if (e == hipSuccess) e = launch_panel(j0 + w, ws.side);
                 if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
+                if (e == hipSuccess) e = clear_lines(ws.side);
                 if (e != hipSuccess) return (int)e;
                 if (rest > next) {
                     // The rest of the update, columns [next, rest). For fp64, the product beside the panel is
