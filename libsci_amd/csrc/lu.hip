@@ -957,32 +957,36 @@ namespace scihip {
                         acc[i][h][e] = c < cols ? x[(r0 + row_of(i, e)) * width + col0 + c] : T(0);
                     }
 
-            // into += sign * M xl over k = 0 .. kBlock - 1, for the 128 x 128 matrix M whose entry (r, k) is
-            // at m[r * row + k * column]; M goes through LDS 32 columns at a time, the next chunk's loads
-            // in flight while the current one is used. `chunk` holds the first chunk, loaded with fetch().
-            auto fetch = [&](const T *m, int64_t row, int64_t column, int k0, T *chunk) {
+            // A 128 x 128 matrix M whose entry (r, k) is at m[r * row + k * column], as four chunks of 32
+            // columns in this thread's registers: all 64 loads are issued together, so a dependency
+            // costs one memory latency, not one per chunk.
+            typedef T Block[4][16];
+            auto fetch = [&](const T *m, int64_t row, int64_t column, Block &block) {
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const int idx = t + 256 * e, r = idx >> 5, k = idx & 31;
-                    chunk[e] = m[r * row + (int64_t)(k0 + k) * column];
-                }
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int idx = t + 256 * e, r = idx >> 5, k = idx & 31;
+                        block[c][e] = m[r * row + (int64_t)(32 * c + k) * column];
+                    }
             };
-            auto apply = [&](const T *m, int64_t row, int64_t column, T *chunk, V (&into)[2][2], T sign) {
-                for (int k0 = 0; k0 < kBlock; k0 += 32) {
+            // into += sign * M xl over k = 0 .. kBlock - 1, M from fetch(), through LDS 32 columns at a time.
+            auto apply = [&](const Block &block, V (&into)[2][2], T sign) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
 #pragma unroll
                     for (int e = 0; e < 16; ++e) {
                         const int idx = t + 256 * e;
-                        fl[idx >> 5][idx & 31] = chunk[e];
+                        fl[idx >> 5][idx & 31] = block[c][e];
                     }
                     __syncthreads();
-                    if (k0 + 32 < kBlock) fetch(m, row, column, k0 + 32, chunk);
 #pragma unroll
                     for (int kk = 0; kk < 32; kk += 4) {
                         T am[2], bm[2];
 #pragma unroll
                         for (int i = 0; i < 2; ++i) am[i] = sign * fl[32 * wave + 16 * i + c16][kk + g];
 #pragma unroll
-                        for (int h = 0; h < 2; ++h) bm[h] = xl[k0 + kk + g][16 * h + c16];
+                        for (int h = 0; h < 2; ++h) bm[h] = xl[32 * c + kk + g][16 * h + c16];
 #pragma unroll
                         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -1030,11 +1034,10 @@ namespace scihip {
                 __syncthreads();
             };
             auto depend = [&](int i) {
-                const T *m = f + r0 * n + (int64_t)i * kBlock; // F_ji, rows n apart
-                T chunk[16];
-                fetch(m, n, 1, 0, chunk); // in flight while x_i is awaited
+                Block block;
+                fetch(f + r0 * n + (int64_t)i * kBlock, n, 1, block); // F_ji, rows n apart; in flight while x_i is awaited
                 await(i);
-                apply(m, n, 1, chunk, acc, T(-1));
+                apply(block, acc, T(-1));
             };
 
             // Every dependency but the one that completes last (block j - 1, or j + 1 for the upper
@@ -1049,8 +1052,8 @@ namespace scihip {
             // out = inv(D_jj) acc (the inverse given transposed, inv[r][k] at inverses_t[k * kBlock + r]),
             // then out -= G_j x_last
             const T *inverse = inverses_t + (int64_t)j * kBlock * kBlock;
-            T chunk[16];
-            fetch(inverse, 1, kBlock, 0, chunk);
+            Block block;
+            fetch(inverse, 1, kBlock, block);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -1063,12 +1066,11 @@ namespace scihip {
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int h = 0; h < 2; ++h) out[i][h] = V{T(0), T(0), T(0), T(0)};
-            apply(inverse, 1, kBlock, chunk, out, T(1));
+            apply(block, out, T(1));
             if (linked) {
-                const T *link = links + (int64_t)j * kBlock * kBlock; // G_j, row-major
-                fetch(link, kBlock, 1, 0, chunk);
+                fetch(links + (int64_t)j * kBlock * kBlock, kBlock, 1, block); // G_j, row-major
                 await(last);
-                apply(link, kBlock, 1, chunk, out, T(-1));
+                apply(block, out, T(-1));
             }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
