@@ -907,10 +907,10 @@ namespace scihip {
         // One triangle of the factors for up to kTriangleColumns right-hand sides, all diagonal blocks in
         // one launch. Workgroup q takes block j (q for the lower triangle, nb - 1 - q for the upper):
         // for every block i it depends on (i < j lower, i > j upper, in the order they complete) except
-        // the last, it waits for x_i's publication, then subtracts F_ji x_i (F_ji from the factors, 32
+        // the last two, it waits for x_i's publication, then subtracts F_ji x_i (F_ji from the factors, 32
         // columns of it in LDS at a time); it applies inv(D_jj) to the result, and only then waits for
-        // the last block, whose contribution it subtracts through G_j = inv(D_jj) F_j,last (from
-        // invert_blocks): one product after the wait instead of two. x_j is stored into x and
+        // the last two blocks, whose contributions it subtracts through H_j = inv(D_jj) F_j,j-2 and
+        // G_j = inv(D_jj) F_j,j-1 (from invert_blocks): one product after each wait. x_j is stored into x and
         // published in tagged slots (tag: this launch's number) at published[((j * kBlock + r) *
         // kTriangleColumns + c) * kSlots]. A workgroup waits only on workgroups of lower index, which are
         // dispatched before it. The products run on the matrix cores, each wave on 32 rows.
@@ -930,7 +930,8 @@ namespace scihip {
         }
 
         template <typename T, bool upper>
-        __global__ __launch_bounds__(256) void solve_triangle(const T *f, const T *inverses_t, const T *links, T *x,
+        __global__ __launch_bounds__(256) void solve_triangle(const T *f, const T *inverses_t, const T *links,
+                                                              const T *links2, T *x,
                                                               int64_t n, int64_t width, int64_t col0,
                                                               unsigned long long *published, unsigned tag,
                                                               unsigned *error) {
@@ -1040,14 +1041,16 @@ namespace scihip {
                 apply(block, acc, T(-1));
             };
 
-            // Every dependency but the one that completes last (block j - 1, or j + 1 for the upper
-            // triangle), whose contribution goes through links: G_j = inv(D_jj) F_j,last.
+            // Every dependency but the two that complete last (blocks j - 2 and j - 1, or j + 2 and j + 1
+            // for the upper triangle), whose contributions go through links: H_j = inv(D_jj) F_j,j-2 and
+            // G_j = inv(D_jj) F_j,j-1. Then the inverse and the first link are applied while block j - 1
+            // is still being solved, and only G_j's product follows its publication.
             if (!upper)
-                for (int i = 0; i + 1 < j; ++i) depend(i);
+                for (int i = 0; i + 2 < j; ++i) depend(i);
             else
-                for (int i = nb - 1; i > j + 1; --i) depend(i);
-            const int last = upper ? j + 1 : j - 1;
-            const bool linked = upper ? j + 1 < nb : j > 0;
+                for (int i = nb - 1; i > j + 2; --i) depend(i);
+            const int last = upper ? j + 1 : j - 1, second = upper ? j + 2 : j - 2;
+            const bool linked = upper ? j + 1 < nb : j > 0, linked2 = upper ? j + 2 < nb : j > 1;
 
             // out = inv(D_jj) acc (the inverse given transposed, inv[r][k] at inverses_t[k * kBlock + r]),
             // then out -= G_j x_last
@@ -1067,6 +1070,11 @@ namespace scihip {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) out[i][h] = V{T(0), T(0), T(0), T(0)};
             apply(block, out, T(1));
+            if (linked2) {
+                fetch(links2 + (int64_t)j * kBlock * kBlock, kBlock, 1, block); // H_j, row-major
+                await(second);
+                apply(block, out, T(-1));
+            }
             if (linked) {
                 fetch(links + (int64_t)j * kBlock * kBlock, kBlock, 1, block); // G_j, row-major
                 await(last);
@@ -1085,23 +1093,44 @@ namespace scihip {
                     }
         }
 
-        // G_j = inv(D_jj) F_j,j-1 (lower) or inv(D_jj) F_j,j+1 (upper), row-major, for solve_triangle;
-        // thread (blockIdx.x * 256 + threadIdx.x) of block j computes entry (r, k) = (idx / kBlock, idx % kBlock).
-        template <typename T, bool upper>
+        // inv(D_jj) F_j,j-d (lower) or inv(D_jj) F_j,j+d (upper) for d = `distance` (1: G_j, 2: H_j),
+        // row-major, for solve_triangle; workgroup j computes block j's 128 x 128 entries, thread t
+        // column t % kBlock of rows [64 (t / kBlock), + 64), over 32-row slices of both operands staged
+        // in LDS (the inverse read where all lanes share an entry). Each entry is the fma chain over
+        // m = 0, 1, ..., kBlock - 1.
+        template <typename T, bool upper, int distance>
         __global__ __launch_bounds__(256) void link_blocks(const T *f, int64_t n, const T *inverses_t, T *links) {
-            const int nb = (int)(n / kBlock), j = (int)blockIdx.y;
-            const int neighbour = upper ? j + 1 : j - 1;
+            constexpr int M = 32;
+            __shared__ T inverse_s[M][kBlock];
+            __shared__ T block_s[M][kBlock];
+            const int nb = (int)(n / kBlock), j = (int)blockIdx.x;
+            const int neighbour = upper ? j + distance : j - distance;
             if (neighbour < 0 || neighbour >= nb) return;
-            const int idx = (int)blockIdx.x * 256 + (int)threadIdx.x, r = idx / kBlock, k = idx % kBlock;
-            const T *inverse = inverses_t + (int64_t)j * kBlock * kBlock;
+            const int t = (int)threadIdx.x, k = t % kBlock, r0 = 64 * (t / kBlock);
+            const T *inverse = inverses_t + (int64_t)j * kBlock * kBlock; // inverse[m * kBlock + r] = inv[r][m]
             const T *block = f + (int64_t)j * kBlock * n + (int64_t)neighbour * kBlock;
-            T sum = T(0);
-#pragma unroll 16
-            for (int m = 0; m < kBlock; ++m) {
-                if constexpr (sizeof(T) == 8) sum = __builtin_fma(inverse[m * kBlock + r], block[(int64_t)m * n + k], sum);
-                else sum = __builtin_fmaf(inverse[m * kBlock + r], block[(int64_t)m * n + k], sum);
+            T acc[64];
+#pragma unroll
+            for (int i = 0; i < 64; ++i) acc[i] = T(0);
+            for (int m0 = 0; m0 < kBlock; m0 += M) {
+                for (int e = t; e < M * kBlock; e += 256) {
+                    const int mm = e / kBlock, c = e % kBlock;
+                    inverse_s[mm][c] = inverse[(int64_t)(m0 + mm) * kBlock + c];
+                    block_s[mm][c] = block[(int64_t)(m0 + mm) * n + c];
+                }
+                __syncthreads();
+                for (int mm = 0; mm < M; ++mm) {
+                    const T b = block_s[mm][k];
+#pragma unroll
+                    for (int i = 0; i < 64; ++i) {
+                        if constexpr (sizeof(T) == 8) acc[i] = __builtin_fma(inverse_s[mm][r0 + i], b, acc[i]);
+                        else acc[i] = __builtin_fmaf(inverse_s[mm][r0 + i], b, acc[i]);
+                    }
+                }
+                __syncthreads();
             }
-            links[(int64_t)j * kBlock * kBlock + (int64_t)r * kBlock + k] = sum;
+#pragma unroll
+            for (int i = 0; i < 64; ++i) links[(int64_t)j * kBlock * kBlock + (int64_t)(r0 + i) * kBlock + k] = acc[i];
         }
 
         // x[r0 + r][c] = sum_k inv[r][k] x[r0 + k][c] over one diagonal block (inverse given transposed,
@@ -1379,10 +1408,15 @@ namespace scihip {
         hipLaunchKernelGGL((invert_diagonal<T, false>), dim3(blocks), dim3(256), 0, stream, f, n, kBlock, inverses);
         hipLaunchKernelGGL((invert_diagonal<T, true>), dim3(blocks), dim3(256), 0, stream, f, n, kBlock,
                            inverses + n * kBlock);
-        const dim3 grid(kBlock * kBlock / 256, blocks);
-        hipLaunchKernelGGL((link_blocks<T, false>), grid, dim3(256), 0, stream, f, n, inverses, inverses + 2 * n * kBlock);
-        hipLaunchKernelGGL((link_blocks<T, true>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
+        const dim3 grid(blocks);
+        hipLaunchKernelGGL((link_blocks<T, false, 1>), grid, dim3(256), 0, stream, f, n, inverses,
+                           inverses + 2 * n * kBlock);
+        hipLaunchKernelGGL((link_blocks<T, true, 1>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
                            inverses + 3 * n * kBlock);
+        hipLaunchKernelGGL((link_blocks<T, false, 2>), grid, dim3(256), 0, stream, f, n, inverses,
+                           inverses + 4 * n * kBlock);
+        hipLaunchKernelGGL((link_blocks<T, true, 2>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
+                           inverses + 5 * n * kBlock);
         return hipGetLastError();
     }
 
@@ -1401,12 +1435,12 @@ namespace scihip {
                     const unsigned tag = ++epoch;
                     if (pass == 0)
                         hipLaunchKernelGGL((solve_triangle<T, false>), dim3((unsigned)nb), dim3(256), 0, stream, f,
-                                           inverse_lower, inverses + 2 * n * kBlock, x, n, width, col0, published,
-                                           tag, ws.error);
+                                           inverse_lower, inverses + 2 * n * kBlock, inverses + 4 * n * kBlock, x, n,
+                                           width, col0, published, tag, ws.error);
                     else
                         hipLaunchKernelGGL((solve_triangle<T, true>), dim3((unsigned)nb), dim3(256), 0, stream, f,
-                                           inverse_upper, inverses + 3 * n * kBlock, x, n, width, col0, published,
-                                           tag, ws.error);
+                                           inverse_upper, inverses + 3 * n * kBlock, inverses + 5 * n * kBlock, x, n,
+                                           width, col0, published, tag, ws.error);
                 }
             return hipGetLastError();
         }

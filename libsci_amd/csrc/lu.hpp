@@ -67,7 +67,7 @@ namespace scihip {
     // Solves L U x = P b for `width` right-hand sides (row-major order x width), in place in `x`,
     // which holds P b on entry. `epoch` is the device's running launch count for the ready flags;
     // each launch takes the next value, so a flag never satisfies a wait of a later launch early.
-    // With `inverses` (4 * order * kBlock elements from invert_blocks) and `scratch`
+    // With `inverses` (6 * order * kBlock elements from invert_blocks) and `scratch`
     // (substitute_scratch(order) bytes) and an order that takes it, the blocked substitution runs.
     template <typename T>
     int substitute(const T *factors, int64_t order, T *x, int64_t width, Workspace &ws, unsigned &epoch,
@@ -78,7 +78,8 @@ namespace scihip {
 
     // The transposed inverses of the diagonal blocks of L (first order * kBlock elements) and of U
     // (the next order * kBlock) of packed LU factors, then the links inv(D_jj) L_j,j-1 and inv(D_jj)
-    // U_j,j+1 (row-major, order * kBlock elements each), for the blocked substitution.
+    // U_j,j+1, then inv(D_jj) L_j,j-2 and inv(D_jj) U_j,j+2 (row-major, order * kBlock elements each),
+    // for the blocked substitution: 6 * order * kBlock elements.
     template <typename T> hipError_t invert_blocks(const T *factors, int64_t order, T *inverses, hipStream_t stream);
 
     // Applies the row order of `pivots` to `source` (order x width) into `target`: target row k is

@@ -820,7 +820,7 @@ int scihip_substitute(scihip_device *device, uint32_t precision, const void *fac
         return rc;
     // The blocked substitution needs the inverses of the diagonal blocks of these factors.
     const bool blocked = blocked_substitution(int64_t(order));
-    Scratch inverses(device, blocked ? 4 * order * kBlock * eb : 16, rc);
+    Scratch inverses(device, blocked ? 6 * order * kBlock * eb : 16, rc);
     Scratch packed(device, blocked ? substitute_scratch(order) : 16, rc);
     if (rc) return rc;
     if (precision == SCIHIP_WIDE) {
@@ -896,7 +896,7 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     const bool aligned = n % 16 == 0;
     Scratch packed_a(device, aligned ? 16 : size_t(n) * size_t(kp) * 8, rc);
     const bool blocked = blocked_substitution(n);
-    Scratch inverses(device, blocked ? 4 * order * kBlock * eb : 16, rc);
+    Scratch inverses(device, blocked ? 6 * order * kBlock * eb : 16, rc);
     Scratch packed(device, blocked ? substitute_scratch(order) : 16, rc);
     // The residual product has few output tiles (n / 128 x 1) and a long shared dimension.
     const int slices = split_slices(kFloat64, n, w, kp);
