@@ -97,7 +97,9 @@ namespace scihip {
         return hipGetLastError();
     }
 
-    int64_t column_max_chunks(int64_t rows) { return std::max<int64_t>(1, std::min<int64_t>(256, (rows + 1023) / 1024)); }
+    // Chunks of about 64 rows (16 per thread row), at most 512: enough workgroups that the loads of a
+    // 16384-row column set are a few latencies deep (1024-row chunks took 96 us for 4 MB).
+    int64_t column_max_chunks(int64_t rows) { return std::max<int64_t>(1, std::min<int64_t>(512, (rows + 63) / 64)); }
 
     hipError_t column_max(const double *x, int64_t rows, int64_t width, double *partial, hipStream_t stream) {
         if (rows == 0 || width == 0) return hipSuccess;

@@ -1162,6 +1162,104 @@ namespace scihip {
             }
         }
 
+        // Slice blockIdx.y of R = A X for a narrow X (n x w, w <= 32, row-major) and an order that is a
+        // multiple of kBlock: workgroup (b, s) multiplies rows [128 b, 128 b + 128) of A by X over the
+        // s-th share of the kBlock-wide column blocks of A, on the fp64 matrix cores (each wave 32 rows
+        // x 32 columns, as in solve_triangle), and writes its partial product to partials[s] (n x w).
+        // A is read in place, row-major, once: the residual of a refinement is bound by that read,
+        // where a general product's 128-column tile would spend four times the matrix-core work on
+        // the padding of X's 32 columns. Each 32-column chunk of the next A block is loaded as soon as
+        // the same chunk of the current one is in LDS, and X's next rows while the current are used.
+        __global__ __launch_bounds__(256) void residual_slices(const double *a, int64_t n, const double *x, int64_t w,
+                                                               double *partials) {
+            typedef typename Quad<double>::type V;
+            __shared__ double xl[kBlock][33];
+            __shared__ double fl[kBlock][33];
+            const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, c16 = lane & 15;
+            const int64_t r0 = (int64_t)blockIdx.x * kBlock;
+            const int64_t blocks = n / kBlock, first = blocks * blockIdx.y / gridDim.y,
+                          end = blocks * (blockIdx.y + 1) / gridDim.y;
+            const int cols = (int)min((int64_t)32, w);
+            auto row_of = [&](int i, int e) { return 32 * wave + 16 * i + g + 4 * e; };
+            V acc[2][2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) acc[i][h] = V{0.0, 0.0, 0.0, 0.0};
+            double block[4][16], xs[16];
+            auto fetch_chunk = [&](int64_t kb, int c) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int idx = t + 256 * e, r = idx >> 5, k = idx & 31;
+                    block[c][e] = a[(r0 + r) * n + kb * kBlock + 32 * c + k];
+                }
+            };
+            auto fetch_x = [&](int64_t kb) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int idx = t + 256 * e, k = idx >> 5, c = idx & 31;
+                    xs[e] = c < cols ? x[(kb * kBlock + k) * w + c] : 0.0;
+                }
+            };
+            if (first < end) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) fetch_chunk(first, c);
+                fetch_x(first);
+            }
+            for (int64_t kb = first; kb < end; ++kb) {
+                const bool more = kb + 1 < end;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int idx = t + 256 * e;
+                    xl[idx >> 5][idx & 31] = xs[e];
+                }
+                if (more) fetch_x(kb + 1);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int idx = t + 256 * e;
+                        fl[idx >> 5][idx & 31] = block[c][e];
+                    }
+                    __syncthreads();
+                    if (more) fetch_chunk(kb + 1, c);
+#pragma unroll
+                    for (int kk = 0; kk < 32; kk += 4) {
+                        double am[2], bm[2];
+#pragma unroll
+                        for (int i = 0; i < 2; ++i) am[i] = fl[32 * wave + 16 * i + c16][kk + g];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) bm[h] = xl[32 * c + kk + g][16 * h + c16];
+#pragma unroll
+                        for (int i = 0; i < 2; ++i)
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) mfma<double>(acc[i][h], am[i], bm[h]);
+                    }
+                    __syncthreads();
+                }
+            }
+            double *out = partials + (int64_t)blockIdx.y * n * w;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int c = 16 * h + c16;
+                        if (c < cols) out[(r0 + row_of(i, e)) * w + c] = acc[i][h][e];
+                    }
+        }
+
+        // r -= the sum of the slices' partial products, in slice order.
+        __global__ __launch_bounds__(256) void subtract_slices(const double *partials, double *r, int64_t count,
+                                                               int slices) {
+            const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+            if (idx >= count) return;
+            double sum = partials[idx];
+            for (int s = 1; s < slices; ++s) sum += partials[(int64_t)s * count + idx];
+            r[idx] = -sum + r[idx];
+        }
+
         const char *kSpinFailure = "a bounded wait in the LU kernels gave up";
 
         template <typename T> struct PackMode;
@@ -1399,6 +1497,18 @@ namespace scihip {
                 timer.stage(4);
             }
         }
+        return hipGetLastError();
+    }
+
+    bool narrow_residual_fits(int64_t order, int64_t width) { return blocked_substitution(order) && width <= 32; }
+
+    hipError_t narrow_residual(const double *a, int64_t n, const double *x, int64_t w, double *r, double *partials,
+                               hipStream_t stream) {
+        if (!narrow_residual_fits(n, w)) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(residual_slices, dim3((unsigned)(n / kBlock), kResidualSlices), dim3(256), 0, stream, a, n, x,
+                           w, partials);
+        hipLaunchKernelGGL(subtract_slices, dim3((unsigned)((n * w + 255) / 256)), dim3(256), 0, stream,
+                           (const double *)partials, r, n * w, kResidualSlices);
         return hipGetLastError();
     }
 

@@ -82,6 +82,15 @@ namespace scihip {
     // for the blocked substitution: 6 * order * kBlock elements.
     template <typename T> hipError_t invert_blocks(const T *factors, int64_t order, T *inverses, hipStream_t stream);
 
+    // r -= A x for a narrow x (order x width, row-major, width <= 32) and an order that takes the
+    // blocked substitution: A (order x order, row-major fp64) is read in place once, on the fp64 matrix
+    // cores, over kResidualSlices slices of its columns; `partials` holds kResidualSlices * order *
+    // width doubles.
+    constexpr int kResidualSlices = 8;
+    bool narrow_residual_fits(int64_t order, int64_t width);
+    hipError_t narrow_residual(const double *a, int64_t order, const double *x, int64_t width, double *r,
+                               double *partials, hipStream_t stream);
+
     // Applies the row order of `pivots` to `source` (order x width) into `target`: target row k is
     // source row rows[k], where rows is the identity permuted by the swaps pivots[0..order).
     template <typename T, typename S>

@@ -900,7 +900,12 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     Scratch packed(device, blocked ? substitute_scratch(order) : 16, rc);
     // The residual product has few output tiles (n / 128 x 1) and a long shared dimension.
     const int slices = split_slices(kFloat64, n, w, kp);
-    Scratch residual_slices(device, slices > 1 ? size_t(slices) * order * width * 8 : 16, rc);
+    // The residual: the narrow kernel that reads A in place when the shape allows, else the general
+    // product over packed operands, its shared dimension split.
+    const bool narrow = narrow_residual_fits(n, w);
+    const size_t slice_bytes = narrow ? size_t(kResidualSlices) * order * width * 8
+                                      : (slices > 1 ? size_t(slices) * order * width * 8 : 16);
+    Scratch residual_slices(device, slice_bytes, rc);
     if (rc) return rc;
     if ((rc = ensure_lu(device, n, eb))) return rc;
 
@@ -972,12 +977,22 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
         if ((rc = status(hipMemcpyAsync(residual.pointer, right, order * width * 8, hipMemcpyDeviceToDevice,
                                         device->stream), "copy")))
             return rc;
-        if ((rc = status(pack(kFloat64, solution, true, w, n, w, true, packed_x.pointer, nullptr, kp, device->stream),
-                         "pack")))
-            return rc;
-        ProductArgs args{a_operand, nullptr, packed_x.pointer, nullptr, residual.pointer, n, w, kp, kp, kp, w, -1.0, 1.0};
-        if ((rc = status(product_split(kFloat64, args, slices, residual_slices.pointer, device->stream), "residual")))
-            return rc;
+        if (narrow) {
+            if ((rc = status(narrow_residual(matrix, n, solution, w, residual.as<double>(),
+                                             residual_slices.as<double>(), device->stream),
+                             "residual")))
+                return rc;
+        } else {
+            if ((rc = status(pack(kFloat64, solution, true, w, n, w, true, packed_x.pointer, nullptr, kp,
+                                  device->stream),
+                             "pack")))
+                return rc;
+            ProductArgs args{a_operand, nullptr, packed_x.pointer, nullptr, residual.pointer, n, w, kp, kp, kp, w,
+                             -1.0, 1.0};
+            if ((rc = status(product_split(kFloat64, args, slices, residual_slices.pointer, device->stream),
+                             "residual")))
+                return rc;
+        }
         double *partials = partial.as<double>();
         if ((rc = status(column_max(residual.as<double>(), n, w, partials, device->stream), "norm"))) return rc;
         if ((rc = status(column_max(solution, n, w, partials + chunks * w, device->stream), "norm"))) return rc;
