@@ -383,8 +383,8 @@ def main() -> int:
                         help="also time the 8192^3 products and the n = 16384 solves in every mode")
     args = parser.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU design is described in DESIGN.md "
-                                   "and not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this backend runs on one GPU; sharding across GPUs is designed but not built",
+                          "n_gpus": args.gpus}))
         return 2
 
     import libsci_amd as sci
