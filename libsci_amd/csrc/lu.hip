@@ -1435,12 +1435,19 @@ namespace scihip {
                 timer.stage(2);
                 // A22 -= L21 U12 on the matrix cores, in the requested mode: the next panel's columns
                 // first, then (while that panel is factored on the side stream) the rest.
-                e = pack(mode, a + (j0 + w) * n + j0, wide, rest, w, n, false, la_hi, la_lo, kp, stream);
+                // L21 is already in the product's A layout (rows, the shared dimension contiguous, row
+                // stride n) when the product takes the matrix's own type: it is read in place. The
+                // bf16 modes pack (and round) it.
+                const bool in_place = mode == kFloat64 || mode == kFloat32;
+                const void *l21 = in_place ? static_cast<const void *>(a + (j0 + w) * n + j0) : la_hi;
+                const int64_t l21_ld = in_place ? n : (int64_t)kp;
+                e = in_place ? hipSuccess
+                             : pack(mode, a + (j0 + w) * n + j0, wide, rest, w, n, false, la_hi, la_lo, kp, stream);
                 if (e == hipSuccess) e = pack(mode, a12, wide, first, w, n, true, ub_hi, ub_lo, kp, stream);
                 if (e != hipSuccess) return (int)e;
                 timer.stage(3);
                 T *c = a + (j0 + w) * n + (j0 + w);
-                ProductArgs narrow{la_hi, la_lo, ub_hi, ub_lo, c, rest, next, (int64_t)kp, (int64_t)kp, (int64_t)kp, n,
+                ProductArgs narrow{l21, la_lo, ub_hi, ub_lo, c, rest, next, (int64_t)kp, l21_ld, (int64_t)kp, n,
                                    -1.0, 1.0};
                 e = product(mode, narrow, stream);
                 if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
@@ -1483,8 +1490,8 @@ namespace scihip {
                         const unsigned char *b_lo =
                                 ub_lo ? static_cast<const unsigned char *>(ub_lo) + (size_t)(next + from) * kp * eb
                                       : nullptr;
-                        ProductArgs part{la_hi, la_lo, b_hi, mode == kSplit ? b_lo : nullptr, c + next + from, rest,
-                                         count, (int64_t)kp, (int64_t)kp, (int64_t)kp, n, -1.0, 1.0};
+                        ProductArgs part{l21, la_lo, b_hi, mode == kSplit ? b_lo : nullptr, c + next + from, rest,
+                                         count, (int64_t)kp, l21_ld, (int64_t)kp, n, -1.0, 1.0};
                         return alongside ? product_alongside(mode, part, stream) : product(mode, part, stream);
                     };
                     e = update(0, beside, true); // the next panel runs meanwhile
