@@ -139,6 +139,20 @@ def test_refined_solve(device, mode, graded):
     assert forward < (1e-10 if graded else 1e-13)
 
 
+@pytest.mark.parametrize("n,width", [(384, 5), (512, 1), (512, 40), (300, 3)])
+def test_refined_solve_widths(device, n, width):
+    """The refinement's residual at other shapes: the narrow in-place kernel (order a multiple of 128,
+    at most 32 right-hand sides, fewer than 32 here) and the general product otherwise."""
+    a, b = system(n, False, seed=5, width=width)
+    print(f"[test] solve n={n} width={width}", flush=True)
+    x, stats = gpu_solve(device, a, b, "fp32")
+    assert stats.converged == 1, stats.backward_error
+    residual = np.abs(b - a @ x).max(axis=0)
+    backward = residual / (np.abs(a).sum(axis=1).max() * np.abs(x).max(axis=0) + np.abs(b).max(axis=0))
+    assert backward.max() <= 1e-14 * 1.01
+    assert np.abs(x - scipy.linalg.solve(a, b)).max() / np.abs(x).max() < 1e-13
+
+
 @pytest.mark.parametrize("case", ["rounded_key_tie", "exact_tie", "float_exact_tie"])
 def test_pivot_ties_across_workgroups(device, case):
     # Candidates in different panel workgroups whose |a| round to the same float (the larger double
