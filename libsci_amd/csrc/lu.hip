@@ -877,18 +877,23 @@ namespace scihip {
                 T v = T(0);
                 const bool active = i < kBlock && (upper ? i <= j : i > j);
                 if (active) {
+                    // The dot product over k in [from, to) as four interleaved partial sums, so that four
+                    // independent fma chains (and their LDS reads) are in flight instead of one.
+                    auto dot = [&](int from, int to) {
+                        T part[4] = {T(0), T(0), T(0), T(0)};
+                        int k = from;
+                        for (; k + 4 <= to; k += 4)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+                                part[q] = __builtin_fma(m[i * stride + k + q], m[(k + q) * stride + j], part[q]);
+                        for (; k < to; ++k) part[0] = __builtin_fma(m[i * stride + k], m[k * stride + j], part[0]);
+                        return (part[0] + part[1]) + (part[2] + part[3]);
+                    };
                     if (upper) {
-                        if (i == j) {
-                            v = T(1) / m[j * stride + j];
-                        } else {
-                            T sum = T(0);
-                            for (int k = i; k < j; ++k) sum = __builtin_fma(m[i * stride + k], m[k * stride + j], sum);
-                            v = -sum / m[j * stride + j];
-                        }
+                        if (i == j) v = T(1) / m[j * stride + j];
+                        else v = -dot(i, j) / m[j * stride + j];
                     } else {
-                        T sum = m[i * stride + j];
-                        for (int k = j + 1; k < i; ++k) sum = __builtin_fma(m[i * stride + k], m[k * stride + j], sum);
-                        v = -sum;
+                        v = -(m[i * stride + j] + dot(j + 1, i));
                     }
                 }
                 __syncthreads();
