@@ -1402,6 +1402,93 @@ namespace scihip {
             return e;
         };
         const size_t eb = (size_t)element_bytes(mode);
+
+        // fp64 steps in pairs. The trailing update of a product of depth 128 reads and writes all of C
+        // for 2 x 128 flops per element, which on MI355X is bound by HBM, not by the matrix cores. So a
+        // step leaves most of its update undone ("carried"): it updates the next panel's columns, and,
+        // beside that panel, a panel's time worth of the last columns; the columns between wait. The
+        // next step brings its own panel rows up to date with the carried update, solves its U12, and
+        // updates the waiting columns with both steps' updates as one product of depth 256 - [L21 of
+        // both] (adjacent columns of the matrix, read in place) times [U12 of both] (the two halves of
+        // 256-element packed rows, the two steps' columns aligned) - for half the C traffic; the
+        // columns already done get its own update alone. The row swaps in between act on the carried
+        // L21's rows and on C alike, so the pair stays consistent.
+        bool carry = false;
+        int64_t carried_end = 0; // carried: columns [next panel's end, carried_end) wait for the update
+        auto step_pair = [&](int64_t j0, int w, int64_t rest) -> hipError_t {
+            constexpr int64_t kPair = 2 * kPanel; // elements per packed U12 row
+            T *ub = static_cast<T *>(ub_hi); // rest x kPair; spans the ub_hi and ub_lo regions (fp64 only)
+            const int64_t next = std::min<int64_t>(kPanel, rest);
+            const T *l11 = a + j0 * n + j0;
+            T *a12 = a + j0 * n + j0 + w;
+            T *c = a + (j0 + w) * n + (j0 + w);
+            const T *l21 = a + (j0 + w) * n + j0;          // this step's L21, in place
+            const int64_t waiting = carry ? carried_end - (j0 + w) : 0; // carried columns [0, waiting) of this step
+            hipError_t e = hipSuccess;
+            if (carry) { // this panel's rows, carried columns: the previous step's update
+                ProductArgs slab{a + j0 * n + (j0 - kPanel), nullptr, ub + kPanel * kPair, nullptr, a12, w, waiting,
+                                 kPanel, n, kPair, n, -1.0, 1.0};
+                e = product(kFloat64, slab, stream);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL(solve_u12<T>, dim3((unsigned)((rest + kSolveU12Threads<T> - 1) / kSolveU12Threads<T>)),
+                               dim3(kSolveU12Threads<T>), 0, stream, l11, a12, n, rest);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            timer.stage(2);
+            // This step's U12 column t goes to packed row t (first half), or, after a carry, to row
+            // kPanel + t (second half), beside the previous step's U12 of the same matrix column.
+            T *u = carry ? ub + kPanel * kPair : ub; // packed row of this step's first column
+            e = pack(kFloat64, a12, true, rest, w, n, true, carry ? u + kPanel : u, nullptr, kPanel, stream, kPair);
+            if (e != hipSuccess) return e;
+            timer.stage(3);
+            // A product over this step's columns [from, from + count): depth 256 (both steps) or 128
+            // (this step's update alone).
+            auto update = [&](int64_t from, int64_t count, bool both, bool alongside) -> hipError_t {
+                if (count <= 0) return hipSuccess;
+                const T *left = both ? l21 - kPanel : l21;
+                const T *right = u + from * kPair + (carry && !both ? kPanel : 0);
+                ProductArgs part{left, nullptr, right, nullptr, c + from, rest, count, both ? kPair : kPanel, n,
+                                 kPair, n, -1.0, 1.0};
+                return alongside ? product_alongside(kFloat64, part, stream) : product(kFloat64, part, stream);
+            };
+            e = update(0, next, carry, false); // the next panel's columns (carried ones, after a carry)
+            if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
+            if (e == hipSuccess) e = launch_panel(j0 + w, ws.side);
+            if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
+            if (e == hipSuccess) e = clear_lines(ws.side);
+            if (e != hipSuccess) return e;
+            // Beside the panel, at one workgroup per compute unit, about a panel's time of depth-128
+            // work (a share of the columns from the measured rates).
+            auto panel_share = [&](int64_t columns) -> int64_t {
+                const double flops = 2.0 * (double)rest * (double)columns * (double)kPanel;
+                const double share = kPanelMsFp64 * 1e-3 * kAlongsideRateFp64 / flops;
+                if (share >= 1.0) return columns;
+                return std::min(columns, std::max<int64_t>(kPanel, (int64_t)(share * (double)columns) / kPanel * kPanel));
+            };
+            if (carry) {
+                // Columns already carrying the previous update: this step's alone, beside the panel; then
+                // the waiting ones with both steps', at two workgroups per compute unit.
+                e = update(waiting, rest - waiting, false, true);
+                if (e == hipSuccess && waiting > next) {
+                    e = hipStreamWaitEvent(stream, ws.panel_done, 0);
+                    if (e == hipSuccess) e = update(next, waiting - next, true, false);
+                }
+                carry = false;
+            } else if (rest - next >= kPanel) {
+                // Carry: the last columns beside the panel, leaving at least the following panel's
+                // columns (and all between) to the next step.
+                const int64_t beside = std::min(panel_share(rest - next), rest - next - kPanel);
+                e = update(rest - beside, beside, false, true);
+                carry = true;
+                carried_end = j0 + w + rest - beside;
+            } else if (rest > next) {
+                e = update(next, rest - next, false, true); // a last short update
+            }
+            return e;
+        };
+
         hipError_t e = clear_lines(stream);
         if (e == hipSuccess) e = launch_panel(0, stream);
         if (e == hipSuccess) e = clear_lines(stream);
@@ -1418,6 +1505,14 @@ namespace scihip {
                                    w, (const int64_t *)pivots);
             timer.stage(1);
             const int64_t rest = n - j0 - w;
+            if constexpr (sizeof(T) == 8) {
+                if (rest > 0) {
+                    e = step_pair(j0, w, rest);
+                    if (e != hipSuccess) return (int)e;
+                    timer.stage(4);
+                }
+                continue;
+            }
             if (rest > 0) {
                 // U12 = L11^-1 A12 in place, by substitution (rest > 0 means w = kPanel). For float
                 // matrices, whose factorisation waits on the chain of panels, the next panel's columns

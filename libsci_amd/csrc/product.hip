@@ -238,7 +238,7 @@ namespace scihip {
         // global memory along rows.
         template <typename S, typename D, bool split>
         __global__ __launch_bounds__(256) void pack_tile(const S *source, int64_t rows, int64_t depth, int64_t ld,
-                                                         bool transpose, D *hi, D *lo, int64_t kp) {
+                                                         bool transpose, D *hi, D *lo, int64_t kp, int64_t stride) {
             __shared__ S tile[64][65];
             const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
             const int64_t k0 = (int64_t)blockIdx.x * 64, i0 = (int64_t)blockIdx.y * 64;
@@ -259,13 +259,13 @@ namespace scihip {
                 if constexpr (sizeof(D) == 2) {
                     const float f = (float)x;
                     const __bf16 h = (__bf16)f;
-                    hi[i * kp + k] = h;
+                    hi[i * stride + k] = h;
                     if constexpr (split) {
                         const float rest = __builtin_isfinite(f) ? f - (float)h : 0.0f;
-                        lo[i * kp + k] = (__bf16)rest;
+                        lo[i * stride + k] = (__bf16)rest;
                     }
                 } else {
-                    hi[i * kp + k] = (D)x;
+                    hi[i * stride + k] = (D)x;
                 }
             }
         }
@@ -354,41 +354,42 @@ namespace scihip {
     }
 
     hipError_t pack(int mode, const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,
-                    void *hi, void *lo, int64_t kp, hipStream_t stream) {
+                    void *hi, void *lo, int64_t kp, hipStream_t stream, int64_t stride) {
         if (rows == 0 || kp == 0) return hipSuccess;
+        if (stride == 0) stride = kp;
         const dim3 grid((unsigned)((kp + 63) / 64), (unsigned)((rows + 63) / 64)), block(256);
         if (mode == kFloat64) {
             if (!wide) return hipErrorInvalidValue;
             hipLaunchKernelGGL((pack_tile<double, double, false>), grid, block, 0, stream,
                                static_cast<const double *>(source), rows, depth, ld, transpose,
-                               static_cast<double *>(hi), nullptr, kp);
+                               static_cast<double *>(hi), nullptr, kp, stride);
         } else if (mode == kFloat32) {
             if (wide)
                 hipLaunchKernelGGL((pack_tile<double, float, false>), grid, block, 0, stream,
                                    static_cast<const double *>(source), rows, depth, ld, transpose,
-                                   static_cast<float *>(hi), nullptr, kp);
+                                   static_cast<float *>(hi), nullptr, kp, stride);
             else
                 hipLaunchKernelGGL((pack_tile<float, float, false>), grid, block, 0, stream,
                                    static_cast<const float *>(source), rows, depth, ld, transpose,
-                                   static_cast<float *>(hi), nullptr, kp);
+                                   static_cast<float *>(hi), nullptr, kp, stride);
         } else if (mode == kBfloat16) {
             if (wide)
                 hipLaunchKernelGGL((pack_tile<double, __bf16, false>), grid, block, 0, stream,
                                    static_cast<const double *>(source), rows, depth, ld, transpose,
-                                   static_cast<__bf16 *>(hi), nullptr, kp);
+                                   static_cast<__bf16 *>(hi), nullptr, kp, stride);
             else
                 hipLaunchKernelGGL((pack_tile<float, __bf16, false>), grid, block, 0, stream,
                                    static_cast<const float *>(source), rows, depth, ld, transpose,
-                                   static_cast<__bf16 *>(hi), nullptr, kp);
+                                   static_cast<__bf16 *>(hi), nullptr, kp, stride);
         } else {
             if (wide)
                 hipLaunchKernelGGL((pack_tile<double, __bf16, true>), grid, block, 0, stream,
                                    static_cast<const double *>(source), rows, depth, ld, transpose,
-                                   static_cast<__bf16 *>(hi), static_cast<__bf16 *>(lo), kp);
+                                   static_cast<__bf16 *>(hi), static_cast<__bf16 *>(lo), kp, stride);
             else
                 hipLaunchKernelGGL((pack_tile<float, __bf16, true>), grid, block, 0, stream,
                                    static_cast<const float *>(source), rows, depth, ld, transpose,
-                                   static_cast<__bf16 *>(hi), static_cast<__bf16 *>(lo), kp);
+                                   static_cast<__bf16 *>(hi), static_cast<__bf16 *>(lo), kp, stride);
         }
         return hipGetLastError();
     }

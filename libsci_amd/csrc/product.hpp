@@ -55,8 +55,9 @@ namespace scihip {
     // Packs a row-major source into the product's operand layout: out[i][k] = src[i][k] (transpose
     // false) or src[k][i] (transpose true), for i < rows and k < depth, zero for depth <= k < kp.
     // `source` is float or double (by `wide`); the output is in the mode's input type; kSplit also
-    // writes the low parts to `lo`.
+    // writes the low parts to `lo`. Output rows are `stride` elements apart (0: kp), so that two
+    // packs can fill the two halves of wider rows.
     hipError_t pack(int mode, const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,
-                    void *hi, void *lo, int64_t kp, hipStream_t stream);
+                    void *hi, void *lo, int64_t kp, hipStream_t stream, int64_t stride = 0);
 
 } // namespace scihip
