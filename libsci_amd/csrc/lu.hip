@@ -1403,8 +1403,10 @@ namespace scihip {
         };
         const size_t eb = (size_t)element_bytes(mode);
 
-        // fp64 steps in pairs. The trailing update of a product of depth 128 reads and writes all of C
-        // for 2 x 128 flops per element, which on MI355X is bound by HBM, not by the matrix cores. So a
+        // fp64 steps in pairs. The trailing update, a product of depth 128, reads and writes all of C
+        // for 2 x 128 flops per element; each product workgroup loads and stores its C tile around a
+        // loop of only 32 matrix-core steps, so the C traffic adds to the math rather than hiding
+        // under it, and a product of depth 256 does twice the work per C pass. So a
         // step leaves most of its update undone ("carried"): it updates the next panel's columns, and,
         // beside that panel, a panel's time worth of the last columns; the columns between wait. The
         // next step brings its own panel rows up to date with the carried update, solves its U12, and
