@@ -910,16 +910,24 @@ namespace scihip {
         }
 
         // One triangle of the factors for up to kTriangleColumns right-hand sides, all diagonal blocks in
-        // one launch. Workgroup q takes block j (q for the lower triangle, nb - 1 - q for the upper):
-        // for every block i it depends on (i < j lower, i > j upper, in the order they complete) except
-        // the last two, it waits for x_i's publication, then subtracts F_ji x_i (F_ji from the factors, 32
-        // columns of it in LDS at a time); it applies inv(D_jj) to the result, and only then waits for
-        // the last two blocks, whose contributions it subtracts through H_j = inv(D_jj) F_j,j-2 and
-        // G_j = inv(D_jj) F_j,j-1 (from invert_blocks): one product after each wait. x_j is stored into x and
-        // published in tagged slots (tag: this launch's number) at published[((j * kBlock + r) *
-        // kTriangleColumns + c) * kSlots]. A workgroup waits only on workgroups of lower index, which are
-        // dispatched before it. The products run on the matrix cores, each wave on 32 rows.
+        // one launch. Four workgroups share each diagonal block, one per quarter (32 rows) of it, so that
+        // the product on the chain from block to block is spread over four compute units. Workgroups 4 q
+        // .. 4 q + 3 take block j (q for the lower triangle, nb - 1 - q for the upper):
+        // - for every block i the block depends on (i < j lower, i > j upper, in the order they complete)
+        //   except the last two, the workgroup waits for x_i's publication, then subtracts F_ji x_i over
+        //   its quarter's rows;
+        // - it publishes its quarter of the result and gathers the other three (published at about the
+        //   same time by its siblings), and applies its rows of inv(D_jj);
+        // - only then it waits for the last two blocks, whose contributions it subtracts through its rows
+        //   of H_j = inv(D_jj) F_j,j-2 and G_j = inv(D_jj) F_j,j-1 (from invert_blocks): one 32-row
+        //   product after each wait.
+        // Its rows of x_j are stored into x and published in tagged slots (tag: this launch's number) at
+        // published[((j * kBlock + r) * kTriangleColumns + c) * kSlots]; the quarters before inv(D_jj)
+        // go to the same place in `partial`. A workgroup waits on workgroups of lower index, which are
+        // dispatched before it, and on its three siblings, which are dispatched with it. Each output
+        // element is the same sequence of matrix-core steps as when one workgroup took the whole block.
         constexpr int kTriangleColumns = 32;
+        constexpr int kQuarter = kBlock / 4; // rows of a diagonal block per workgroup
         template <typename T> struct Quad;
         template <> struct Quad<float> {
             typedef float type __attribute__((ext_vector_type(4)));
@@ -934,88 +942,78 @@ namespace scihip {
             else acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
         }
 
+        // Two workgroups per compute unit for float, so that the 4 x 128 workgroups of order 16384 are
+        // resident at once; double keeps one (its registers are full), and its later workgroups start as
+        // earlier ones finish.
         template <typename T, bool upper>
-        __global__ __launch_bounds__(256) void solve_triangle(const T *f, const T *inverses_t, const T *links,
-                                                              const T *links2, T *x,
-                                                              int64_t n, int64_t width, int64_t col0,
-                                                              unsigned long long *published, unsigned tag,
-                                                              unsigned *error) {
+        __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void solve_triangle(
+            const T *f, const T *inverses_t, const T *links, const T *links2, T *x, int64_t n, int64_t width,
+            int64_t col0, unsigned long long *published, unsigned long long *partial, unsigned tag, unsigned *error) {
             typedef typename Quad<T>::type V;
             constexpr int S = kSlots<T>;
             constexpr int C = kTriangleColumns;
-            __shared__ T xl[kBlock][C + 1]; // x_i, later the block's reduced right-hand side
-            __shared__ T fl[kBlock][33];    // 32 columns of the matrix being applied
-            const int nb = (int)(n / kBlock), q = (int)blockIdx.x, j = upper ? nb - 1 - q : q;
+            constexpr int kStride = kBlock + 4; // LDS row of fl: 16 lanes of consecutive rows on distinct banks
+            __shared__ T xl[kBlock][C + 1];      // x_i, later the block's reduced right-hand side
+            __shared__ T fl[kQuarter * kStride]; // this quarter's rows of the 128 x 128 matrix being applied
+            const int nb = (int)(n / kBlock), q = (int)(blockIdx.x >> 2), s = (int)(blockIdx.x & 3);
+            const int j = upper ? nb - 1 - q : q;
             const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, c16 = lane & 15;
             const int cols = (int)min((int64_t)C, width - col0);
-            const int64_t r0 = (int64_t)j * kBlock;
-            // Wave w holds rows 32 w .. 32 w + 31 as 2 x 2 tiles of 16 x 16 (matrix-core accumulators):
-            // element e of tile (i, h) is row 32 w + 16 i + (fp32: 4 g + e; fp64: g + 4 e), column 16 h + c16.
-            auto row_of = [&](int i, int e) { return 32 * wave + 16 * i + (sizeof(T) == 8 ? g + 4 * e : 4 * g + e); };
-            V acc[2][2];
+            const int64_t r0 = (int64_t)j * kBlock + kQuarter * s; // first row of this quarter
+            // Wave w holds one 16 x 16 tile of the quarter's 32 x 32 result (a matrix-core accumulator):
+            // rows 16 (w / 2) .., columns 16 (w % 2) ..; element e is row 16 (w / 2) + (fp32: 4 g + e;
+            // fp64: g + 4 e), column 16 (w % 2) + c16.
+            const int tile_row = 16 * (wave >> 1), col = 16 * (wave & 1) + c16;
+            auto row_of = [&](int e) { return tile_row + (sizeof(T) == 8 ? g + 4 * e : 4 * g + e); };
+            V acc;
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int c = 16 * h + c16;
-                        acc[i][h][e] = c < cols ? x[(r0 + row_of(i, e)) * width + col0 + c] : T(0);
-                    }
+            for (int e = 0; e < 4; ++e) acc[e] = col < cols ? x[(r0 + row_of(e)) * width + col0 + col] : T(0);
 
-            // A 128 x 128 matrix M whose entry (r, k) is at m[r * row + k * column], as four chunks of 32
-            // columns in this thread's registers: all 64 loads are issued together, so a dependency
-            // costs one memory latency, not one per chunk.
-            typedef T Block[4][16];
+            // The quarter's rows of a 128 x 128 matrix M whose entry (r, k) is at m[r * row + k * column]
+            // (m pointing at the quarter's first row), 16 entries per thread, all loads issued together
+            // so that a dependency costs one memory latency. Consecutive threads take consecutive
+            // addresses: along k when `row` is the long stride, along r otherwise.
+            typedef T Block[16];
             auto fetch = [&](const T *m, int64_t row, int64_t column, Block &block) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-#pragma unroll
-                    for (int e = 0; e < 16; ++e) {
-                        const int idx = t + 256 * e, r = idx >> 5, k = idx & 31;
-                        block[c][e] = m[r * row + (int64_t)(32 * c + k) * column];
-                    }
-            };
-            // into += sign * M xl over k = 0 .. kBlock - 1, M from fetch(), through LDS 32 columns at a time.
-            auto apply = [&](const Block &block, V (&into)[2][2], T sign) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-#pragma unroll
-                    for (int e = 0; e < 16; ++e) {
-                        const int idx = t + 256 * e;
-                        fl[idx >> 5][idx & 31] = block[c][e];
-                    }
-                    __syncthreads();
-#pragma unroll
-                    for (int kk = 0; kk < 32; kk += 4) {
-                        T am[2], bm[2];
-#pragma unroll
-                        for (int i = 0; i < 2; ++i) am[i] = sign * fl[32 * wave + 16 * i + c16][kk + g];
-#pragma unroll
-                        for (int h = 0; h < 2; ++h) bm[h] = xl[32 * c + kk + g][16 * h + c16];
-#pragma unroll
-                        for (int i = 0; i < 2; ++i)
-#pragma unroll
-                            for (int h = 0; h < 2; ++h) mfma<T>(into[i][h], am[i], bm[h]);
-                    }
-                    __syncthreads();
+                for (int e = 0; e < 16; ++e) {
+                    const int idx = t + 256 * e;
+                    const int r = column == 1 ? idx >> 7 : idx & (kQuarter - 1);
+                    const int k = column == 1 ? idx & (kBlock - 1) : idx >> 5;
+                    block[e] = m[r * row + (int64_t)k * column];
                 }
             };
+            // into += sign * M xl over k = 0 .. kBlock - 1, M from fetch() with the same strides.
+            auto apply = [&](const Block &block, int64_t column, V &into, T sign) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int idx = t + 256 * e;
+                    const int r = column == 1 ? idx >> 7 : idx & (kQuarter - 1);
+                    const int k = column == 1 ? idx & (kBlock - 1) : idx >> 5;
+                    fl[r * kStride + k] = block[e];
+                }
+                __syncthreads();
+#pragma unroll 8
+                for (int kk = 0; kk < kBlock; kk += 4)
+                    mfma<T>(into, sign * fl[(tile_row + c16) * kStride + kk + g], xl[kk + g][col]);
+                __syncthreads();
+            };
 
-            // x_i into xl once published: 16 entries per thread, their loads in flight together.
-            auto await = [&](int i) {
+            // The 128 x 32 block i of `from` into xl once published, 16 entries per thread with their
+            // loads in flight together; entries of rows in quarter `own` (0-3) are left to the caller.
+            auto gather = [&](const unsigned long long *from, int i, int own) {
                 unsigned long long slot[16][2];
                 unsigned waiting = 0;
 #pragma unroll
-                for (int e = 0; e < 16; ++e)
-                    if ((t + 256 * e) % C < cols) waiting |= 1u << e;
+                for (int e = 0; e < 16; ++e) // entry e is row (t + 256 e) / C: quarter e / 4
+                    if ((t + 256 * e) % C < cols && (e >> 2) != own) waiting |= 1u << e;
                 unsigned spins = 0;
                 while (waiting) {
 #pragma unroll
                     for (int e = 0; e < 16; ++e) {
                         if (!(waiting >> e & 1u)) continue;
                         const int idx = t + 256 * e, r = idx / C, c = idx % C;
-                        const unsigned long long *at = published + (((int64_t)i * kBlock + r) * C + c) * S;
+                        const unsigned long long *at = from + (((int64_t)i * kBlock + r) * C + c) * S;
 #pragma unroll
                         for (int h = 0; h < S; ++h)
                             slot[e][h] = __hip_atomic_load(at + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1042,8 +1040,8 @@ namespace scihip {
             auto depend = [&](int i) {
                 Block block;
                 fetch(f + r0 * n + (int64_t)i * kBlock, n, 1, block); // F_ji, rows n apart; in flight while x_i is awaited
-                await(i);
-                apply(block, acc, T(-1));
+                gather(published, i, -1);
+                apply(block, 1, acc, T(-1));
             };
 
             // Every dependency but the two that complete last (blocks j - 2 and j - 1, or j + 2 and j + 1
@@ -1057,45 +1055,38 @@ namespace scihip {
             const int last = upper ? j + 1 : j - 1, second = upper ? j + 2 : j - 2;
             const bool linked = upper ? j + 1 < nb : j > 0, linked2 = upper ? j + 2 < nb : j > 1;
 
-            // out = inv(D_jj) acc (the inverse given transposed, inv[r][k] at inverses_t[k * kBlock + r]),
-            // then out -= G_j x_last
-            const T *inverse = inverses_t + (int64_t)j * kBlock * kBlock;
+            // The block's reduced right-hand side: this quarter from acc (published for the siblings),
+            // the others gathered from theirs.
             Block block;
-            fetch(inverse, 1, kBlock, block);
+            // this quarter's rows of inv(D_jj) (given transposed, inv[r][k] at inverses_t[k * kBlock + r])
+            fetch(inverses_t + (int64_t)j * kBlock * kBlock + kQuarter * s, 1, kBlock, block);
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) xl[row_of(i, e)][16 * h + c16] = acc[i][h][e];
-            __syncthreads();
-            V out[2][2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) out[i][h] = V{T(0), T(0), T(0), T(0)};
-            apply(block, out, T(1));
+            for (int e = 0; e < 4; ++e) {
+                const int r = row_of(e);
+                xl[kQuarter * s + r][col] = acc[e];
+                if (col < cols) put_tagged(partial + ((r0 + r) * C + col) * S, acc[e], tag);
+            }
+            gather(partial, j, s);
+            // out = inv(D_jj) acc, then out -= H_j x_second, then out -= G_j x_last (row-major links)
+            V out = V{T(0), T(0), T(0), T(0)};
+            apply(block, kBlock, out, T(1));
             if (linked2) {
-                fetch(links2 + (int64_t)j * kBlock * kBlock, kBlock, 1, block); // H_j, row-major
-                await(second);
-                apply(block, out, T(-1));
+                fetch(links2 + ((int64_t)j * kBlock + kQuarter * s) * kBlock, kBlock, 1, block);
+                gather(published, second, -1);
+                apply(block, 1, out, T(-1));
             }
             if (linked) {
-                fetch(links + (int64_t)j * kBlock * kBlock, kBlock, 1, block); // G_j, row-major
-                await(last);
-                apply(block, out, T(-1));
+                fetch(links + ((int64_t)j * kBlock + kQuarter * s) * kBlock, kBlock, 1, block);
+                gather(published, last, -1);
+                apply(block, 1, out, T(-1));
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int r = row_of(i, e), c = 16 * h + c16;
-                        if (c >= cols) continue;
-                        x[(r0 + r) * width + col0 + c] = out[i][h][e];
-                        put_tagged(published + (((int64_t)j * kBlock + r) * C + c) * S, out[i][h][e], tag);
-                    }
+            for (int e = 0; e < 4; ++e) {
+                const int r = row_of(e);
+                if (col >= cols) continue;
+                x[(r0 + r) * width + col0 + col] = out[e];
+                put_tagged(published + ((r0 + r) * C + col) * S, out[e], tag);
+            }
         }
 
         // inv(D_jj) F_j,j-d (lower) or inv(D_jj) F_j,j+d (upper) for d = `distance` (1: G_j, 2: H_j),
@@ -1649,17 +1640,18 @@ namespace scihip {
             const int64_t nb = n / kBlock;
             const T *inverse_lower = inverses, *inverse_upper = inverses + n * kBlock;
             unsigned long long *published = static_cast<unsigned long long *>(scratch);
+            unsigned long long *partial = published + n * kTriangleColumns * 2; // the quarters before inv(D_jj)
             for (int64_t col0 = 0; col0 < width; col0 += kTriangleColumns)
                 for (int pass = 0; pass < 2; ++pass) {
                     const unsigned tag = ++epoch;
                     if (pass == 0)
-                        hipLaunchKernelGGL((solve_triangle<T, false>), dim3((unsigned)nb), dim3(256), 0, stream, f,
-                                           inverse_lower, inverses + 2 * n * kBlock, inverses + 4 * n * kBlock, x, n,
-                                           width, col0, published, tag, ws.error);
+                        hipLaunchKernelGGL((solve_triangle<T, false>), dim3((unsigned)(4 * nb)), dim3(256), 0, stream,
+                                           f, inverse_lower, inverses + 2 * n * kBlock, inverses + 4 * n * kBlock, x,
+                                           n, width, col0, published, partial, tag, ws.error);
                     else
-                        hipLaunchKernelGGL((solve_triangle<T, true>), dim3((unsigned)nb), dim3(256), 0, stream, f,
-                                           inverse_upper, inverses + 3 * n * kBlock, inverses + 5 * n * kBlock, x, n,
-                                           width, col0, published, tag, ws.error);
+                        hipLaunchKernelGGL((solve_triangle<T, true>), dim3((unsigned)(4 * nb)), dim3(256), 0, stream,
+                                           f, inverse_upper, inverses + 3 * n * kBlock, inverses + 5 * n * kBlock, x,
+                                           n, width, col0, published, partial, tag, ws.error);
                 }
             return hipGetLastError();
         }
@@ -1700,7 +1692,7 @@ namespace scihip {
     template int substitute<float>(const float *, int64_t, float *, int64_t, Workspace &, unsigned &, hipStream_t,
                                    const char **, const float *, void *);
 
-    size_t substitute_scratch(int64_t order) { return (size_t)order * kTriangleColumns * 2 * 8; }
+    size_t substitute_scratch(int64_t order) { return 2 * (size_t)order * kTriangleColumns * 2 * 8; }
     template hipError_t invert_blocks<double>(const double *, int64_t, double *, hipStream_t);
     template hipError_t invert_blocks<float>(const float *, int64_t, float *, hipStream_t);
     template hipError_t permute<double, double>(const double *, double *, const int64_t *, int64_t, int64_t,

@@ -73,7 +73,8 @@ namespace scihip {
     int substitute(const T *factors, int64_t order, T *x, int64_t width, Workspace &ws, unsigned &epoch,
                    hipStream_t stream, const char **error, const T *inverses = nullptr, void *scratch = nullptr);
 
-    // Bytes of scratch the blocked substitution needs for an order (tagged copies of the solved blocks).
+    // Bytes of scratch the blocked substitution needs for an order (tagged copies of the solved blocks,
+    // and of the blocks' reduced right-hand sides that the four workgroups of a block exchange).
     size_t substitute_scratch(int64_t order);
 
     // The transposed inverses of the diagonal blocks of L (first order * kBlock elements) and of U
