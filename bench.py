@@ -374,6 +374,8 @@ def main() -> int:
     parser.add_argument("--extra", action="store_true", help="also time reductions and the interpreter")
     parser.add_argument("--solve-only", type=int, default=0, metavar="ORDER",
                         help="only run one fp64 and one fp32-refined solve of this order (for profiling)")
+    parser.add_argument("--modes", default="fp64,fp32",
+                        help="with --solve-only: the factorisations to time (fp64, fp32, bf16x3, bf16)")
     parser.add_argument("--store-only", action="store_true",
                         help="only the headline store, then a plain copy chunk of the same size (8 B per element) "
                              "that calibrates hardware byte counters; for profiling")
@@ -393,8 +395,10 @@ def main() -> int:
 
     if args.solve_only:
         device = sci.Device(0)
-        print(json.dumps(measure_solves(sci, _abi, device, args.solve_only, graded_cases=(False,),
-                                        modes=(("fp64", 0), ("fp32", 1)))), flush=True)
+        codes = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16": 3}
+        modes = tuple((name, codes[name]) for name in args.modes.split(","))
+        print(json.dumps(measure_solves(sci, _abi, device, args.solve_only, graded_cases=(False,), modes=modes)),
+              flush=True)
         device.close()
         return 0
 

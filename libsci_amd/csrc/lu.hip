@@ -1270,9 +1270,9 @@ namespace scihip {
 
     size_t factor_scratch(int64_t order, int mode) {
         (void)mode;
-        // L21 and U12 packed (up to 8 bytes, split pairs), then the panel's candidate lines
+        // L21 and U12 packed (up to 8 bytes, split pairs), then two sets of the panel's candidate lines
         // (kParities buffers, one 128-byte line per workgroup).
-        return 4 * (size_t)order * kPanel * 8 + kParities * ((size_t)order / kPanelRows + 2) * 128 + 1024;
+        return 4 * (size_t)order * kPanel * 8 + 2 * kParities * ((size_t)order / kPanelRows + 2) * 128 + 1024;
     }
 
     size_t panel_publish_bytes(int64_t order, size_t element) {
@@ -1360,12 +1360,16 @@ namespace scihip {
             if (e != hipSuccess) return (int)e;
             resident = units * per_unit;
         }
-        // Clears the candidate lines for the next panel launch on stream `s` (each launch expects them
-        // zero; the clear after a look-ahead panel runs on its stream while the main stream works).
+        // Clears candidate line set `set` on stream `s` (each launch expects its lines zero). fp64
+        // panels, which run on the side stream, all take set 0, cleared after each panel on its stream;
+        // float panels, which run on the main stream, alternate between the sets, the set of the panel
+        // after next being cleared on the side stream meanwhile.
         const size_t line_bytes = (size_t)kParities * max_groups * kLine * sizeof(unsigned long long);
-        auto clear_lines = [&](hipStream_t s) { return hipMemsetAsync(lines, 0, line_bytes, s); };
-        // Factors the panel at column j0 on stream `s` (its candidate lines already clear).
-        auto launch_panel = [&](int64_t j0, hipStream_t s) -> hipError_t {
+        unsigned long long *line_sets[2] = {lines, lines + line_bytes / sizeof(unsigned long long)};
+        auto clear_lines = [&](hipStream_t s, int set) { return hipMemsetAsync(line_sets[set], 0, line_bytes, s); };
+        // Factors the panel at column j0 on stream `s` with candidate line set `set` (already clear).
+        auto launch_panel = [&](int64_t j0, hipStream_t s, int set) -> hipError_t {
+            unsigned long long *lines = line_sets[set];
             int w = (int)std::min<int64_t>(kPanel, n - j0);
             const int groups = (int)((n - j0 + kPanelRows - 1) / kPanelRows);
             hipError_t e = hipSuccess;
@@ -1408,7 +1412,7 @@ namespace scihip {
         // L21's rows and on C alike, so the pair stays consistent.
         bool carry = false;
         int64_t carried_end = 0; // carried: columns [next panel's end, carried_end) wait for the update
-        auto step_pair = [&](int64_t j0, int w, int64_t rest) -> hipError_t {
+        auto step_pair = [&](auto j0, int w, int64_t rest) -> hipError_t { // generic: compiled for double only
             constexpr int64_t kPair = 2 * kPanel; // elements per packed U12 row
             T *ub = static_cast<T *>(ub_hi); // rest x kPair; spans the ub_hi and ub_lo regions (fp64 only)
             const int64_t next = std::min<int64_t>(kPanel, rest);
@@ -1448,9 +1452,9 @@ namespace scihip {
             e = update(0, next, carry, false); // the next panel's columns (carried ones, after a carry)
             if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
-            if (e == hipSuccess) e = launch_panel(j0 + w, ws.side);
+            if (e == hipSuccess) e = launch_panel(j0 + w, ws.side, 0);
             if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
-            if (e == hipSuccess) e = clear_lines(ws.side);
+            if (e == hipSuccess) e = clear_lines(ws.side, 0);
             if (e != hipSuccess) return e;
             // Beside the panel, at one workgroup per compute unit, about a panel's time of depth-128
             // work (a share of the columns from the measured rates).
@@ -1482,13 +1486,103 @@ namespace scihip {
             return e;
         };
 
-        hipError_t e = clear_lines(stream);
-        if (e == hipSuccess) e = launch_panel(0, stream);
-        if (e == hipSuccess) e = clear_lines(stream);
+        // Float steps in pairs too. A float factorisation waits on the chain of panels, except in its
+        // first steps, where each trailing update (all of C read and written) outlasts the panel beside
+        // it. So a fresh step updates only the next panel's columns, which lets that panel run alone,
+        // and carries the rest; the next step brings its panel rows up to date (a 128-row product),
+        // then updates all its columns with both steps' updates as products of depth 256 beside its
+        // panel: [L21 of both] (in place for fp32; packed after that step's row swaps otherwise, with
+        // the panel rows first for the 128-row product) times [U12 of both] (packed as for fp64). In
+        // either step the next panel's U12 columns are solved first (the row-parallel kernel), and the
+        // rest while that panel is factored. A fresh step with fewer than two panels of columns left
+        // updates them all itself.
+        auto step_float = [&](auto j0, int w, int64_t rest) -> hipError_t { // generic: compiled for float only
+            constexpr int64_t kPair = 2 * kPanel; // elements per packed row of a pair
+            const int64_t next = std::min<int64_t>(kPanel, rest);
+            const T *l11 = a + j0 * n + j0;
+            T *a12 = a + j0 * n + j0 + w;
+            T *c = a + (j0 + w) * n + (j0 + w);
+            const bool in_place = mode == kFloat32;
+            const bool split = mode == kSplit;
+            // Element `index` of a packed buffer (nullptr stays nullptr).
+            auto at = [&](void *base, int64_t index) -> void * {
+                return base ? static_cast<void *>(static_cast<unsigned char *>(base) + (size_t)index * eb) : nullptr;
+            };
+            const bool pair = carry;
+            hipError_t e = hipSuccess;
+            if (pair) {
+                // L21 of both steps, rows from this panel's first (A's columns j0 - kPanel .. j0 + kPanel).
+                if (!in_place)
+                    e = pack(mode, a + j0 * n + (j0 - kPanel), wide, rest + w, kPair, n, false, la_hi, la_lo, kPair,
+                             stream);
+                // This panel's rows, all this step's columns: the previous step's update.
+                if (e == hipSuccess) {
+                    ProductArgs slab{in_place ? static_cast<const void *>(a + j0 * n + (j0 - kPanel)) : la_hi,
+                                     split ? la_lo : nullptr, at(ub_hi, kPanel * kPair), split ? at(ub_lo, kPanel * kPair) : nullptr,
+                                     a12, w, rest, kPanel, in_place ? n : kPair, kPair, n, -1.0, 1.0};
+                    e = product(mode, slab, stream);
+                }
+                if (e != hipSuccess) return e;
+            } else if (!in_place) {
+                e = pack(mode, a + (j0 + w) * n + j0, wide, rest, w, n, false, la_hi, la_lo, kp, stream);
+                if (e != hipSuccess) return e;
+            }
+            // The left operand over C's rows: both steps' L21 (depth 256) or this step's (depth 128).
+            const void *left = in_place ? static_cast<const void *>(a + (j0 + w) * n + (pair ? j0 - kPanel : j0))
+                                        : pair ? at(la_hi, w * kPair) : la_hi;
+            const void *left_lo = split ? (pair ? at(la_lo, w * kPair) : la_lo) : nullptr;
+            const int64_t left_ld = in_place ? n : pair ? kPair : (int64_t)kp;
+            const int64_t depth = pair ? kPair : kPanel;
+            // This step's U12 column t goes to packed row t (first half), or, in a pair, to row kPanel + t
+            // (second half), beside the previous step's U12 of the same matrix column.
+            const int64_t row0 = pair ? kPanel : 0, half = pair ? kPanel : 0;
+            auto pack_u12 = [&](int64_t from, int64_t count, hipStream_t s) -> hipError_t {
+                const int64_t index = (row0 + from) * kPair + half;
+                return pack(mode, a12 + from, wide, count, w, n, true, at(ub_hi, index), split ? at(ub_lo, index) : nullptr,
+                            kPanel, s, kPair);
+            };
+            // C's columns [from, from + count) -= left times the packed U12 rows of those columns.
+            auto update = [&](int64_t from, int64_t count, hipStream_t s) -> hipError_t {
+                const int64_t index = (row0 + from) * kPair;
+                ProductArgs part{left, left_lo, at(ub_hi, index), split ? at(ub_lo, index) : nullptr, c + from, rest,
+                                 count, depth, left_ld, kPair, n, -1.0, 1.0};
+                return from == 0 ? product(mode, part, s) : product_alongside(mode, part, s);
+            };
+            hipLaunchKernelGGL(solve_u12_narrow<T>, dim3(1), dim3(kU12NarrowThreads), 0, stream, l11, a12, n, (int)next);
+            e = hipGetLastError();
+            timer.stage(2);
+            if (e == hipSuccess) e = pack_u12(0, next, stream);
+            timer.stage(3);
+            if (e == hipSuccess) e = update(0, next, stream); // the next panel's columns
+            // The next panel follows on the main stream at once; the rest of the step runs beside it on
+            // the side stream, whose end the next step waits for (panel_done marks it).
+            const int set = (int)((j0 + w) / kPanel) & 1;
+            if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
+            if (e == hipSuccess) e = launch_panel(j0 + w, stream, set);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
+            if (e == hipSuccess) e = clear_lines(ws.side, set ^ 1); // for the panel after next
+            if (e != hipSuccess) return e;
+            if (rest > next) { // U12's other columns, and their packed copy
+                const int64_t others = rest - next;
+                hipLaunchKernelGGL(solve_u12<T>, dim3((unsigned)((others + kSolveU12Threads<T> - 1) / kSolveU12Threads<T>)),
+                                   dim3(kSolveU12Threads<T>), 0, ws.side, l11, a12 + next, n, others);
+                e = hipGetLastError();
+                if (e == hipSuccess) e = pack_u12(next, others, ws.side);
+                if (e != hipSuccess) return e;
+            }
+            carry = !pair && rest - next >= kPanel;
+            if (rest > next && !carry) e = update(next, rest - next, ws.side); // beside the panel
+            if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
+            return e;
+        };
+
+        hipError_t e = hipMemsetAsync(lines, 0, 2 * line_bytes, stream); // both sets
+        if (e == hipSuccess) e = launch_panel(0, stream, 0);
+        if (e == hipSuccess && sizeof(T) == 8) e = clear_lines(stream, 0);
         if (e != hipSuccess) return (int)e;
         for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
             const int w = (int)std::min<int64_t>(kPanel, n - j0);
-            if (j0 > 0) { // this panel ran on the side stream, during the previous update
+            if (j0 > 0) { // fp64: this panel, run on the side stream; float: the previous step's side work
                 e = hipStreamWaitEvent(stream, ws.panel_done, 0);
                 if (e != hipSuccess) return (int)e;
             }
@@ -1498,102 +1592,10 @@ namespace scihip {
                                    w, (const int64_t *)pivots);
             timer.stage(1);
             const int64_t rest = n - j0 - w;
-            if constexpr (sizeof(T) == 8) {
-                if (rest > 0) {
-                    e = step_pair(j0, w, rest);
-                    if (e != hipSuccess) return (int)e;
-                    timer.stage(4);
-                }
-                continue;
-            }
             if (rest > 0) {
-                // U12 = L11^-1 A12 in place, by substitution (rest > 0 means w = kPanel). For float
-                // matrices, whose factorisation waits on the chain of panels, the next panel's columns
-                // come first, by the row-parallel kernel, and the rest while that panel is factored
-                // (measured 1.5 ms faster at n = 16384). For double ones, whose early steps wait on the
-                // update instead, one launch does all columns (the split measured 1.5 ms slower).
-                const int64_t next = std::min<int64_t>(kPanel, rest);
-                const T *l11 = a + j0 * n + j0;
-                T *a12 = a + j0 * n + j0 + w;
-                const int64_t first = sizeof(T) == 8 ? rest : next; // U12 columns solved before the next panel
-                if constexpr (sizeof(T) == 8)
-                    hipLaunchKernelGGL(solve_u12<T>,
-                                       dim3((unsigned)((rest + kSolveU12Threads<T> - 1) / kSolveU12Threads<T>)),
-                                       dim3(kSolveU12Threads<T>), 0, stream, l11, a12, n, rest);
-                else
-                    hipLaunchKernelGGL(solve_u12_narrow<T>, dim3(1), dim3(kU12NarrowThreads), 0, stream, l11, a12, n,
-                                       (int)next);
-                e = hipGetLastError();
+                if constexpr (sizeof(T) == 8) e = step_pair(j0, w, rest);
+                else e = step_float(j0, w, rest);
                 if (e != hipSuccess) return (int)e;
-                timer.stage(2);
-                // A22 -= L21 U12 on the matrix cores, in the requested mode: the next panel's columns
-                // first, then (while that panel is factored on the side stream) the rest.
-                // L21 is already in the product's A layout (rows, the shared dimension contiguous, row
-                // stride n) when the product takes the matrix's own type: it is read in place. The
-                // bf16 modes pack (and round) it.
-                const bool in_place = mode == kFloat64 || mode == kFloat32;
-                const void *l21 = in_place ? static_cast<const void *>(a + (j0 + w) * n + j0) : la_hi;
-                const int64_t l21_ld = in_place ? n : (int64_t)kp;
-                e = in_place ? hipSuccess
-                             : pack(mode, a + (j0 + w) * n + j0, wide, rest, w, n, false, la_hi, la_lo, kp, stream);
-                if (e == hipSuccess) e = pack(mode, a12, wide, first, w, n, true, ub_hi, ub_lo, kp, stream);
-                if (e != hipSuccess) return (int)e;
-                timer.stage(3);
-                T *c = a + (j0 + w) * n + (j0 + w);
-                ProductArgs narrow{l21, la_lo, ub_hi, ub_lo, c, rest, next, (int64_t)kp, l21_ld, (int64_t)kp, n,
-                                   -1.0, 1.0};
-                e = product(mode, narrow, stream);
-                if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
-                if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
-                if (e == hipSuccess) e = launch_panel(j0 + w, ws.side);
-                if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
-                if (e == hipSuccess) e = clear_lines(ws.side);
-                if (e != hipSuccess) return (int)e;
-                if (rest > first) {
-                    // U12's other columns, and their packed copy.
-                    const int64_t others = rest - next;
-                    hipLaunchKernelGGL(solve_u12<T>,
-                                       dim3((unsigned)((others + kSolveU12Threads<T> - 1) / kSolveU12Threads<T>)),
-                                       dim3(kSolveU12Threads<T>), 0, stream, l11, a12 + next, n, others);
-                    e = hipGetLastError();
-                    if (e == hipSuccess)
-                        e = pack(mode, a12 + next, wide, others, w, n, true,
-                                 static_cast<unsigned char *>(ub_hi) + (size_t)next * kp * eb,
-                                 ub_lo ? static_cast<unsigned char *>(ub_lo) + (size_t)next * kp * eb : nullptr, kp,
-                                 stream);
-                    if (e != hipSuccess) return (int)e;
-                }
-                if (rest > next) {
-                    // The rest of the update, columns [next, rest). For fp64, the product beside the panel is
-                    // sized for one workgroup per compute unit so that the panel gets whole compute units;
-                    // only about the panel's duration of it runs that way (a share of the columns from the
-                    // measured rates), the remainder after the panel at two workgroups per compute unit.
-                    const int64_t columns = rest - next;
-                    int64_t beside = columns;
-                    if (wide) {
-                        const double flops = 2.0 * (double)rest * (double)columns * (double)w;
-                        const double share = kPanelMsFp64 * 1e-3 * kAlongsideRateFp64 / flops;
-                        if (share < 1.0)
-                            beside = std::max<int64_t>(kPanel, (int64_t)(share * (double)columns) / kPanel * kPanel);
-                        beside = std::min(beside, columns);
-                    }
-                    auto update = [&](int64_t from, int64_t count, bool alongside) -> hipError_t {
-                        const unsigned char *b_hi =
-                                static_cast<const unsigned char *>(ub_hi) + (size_t)(next + from) * kp * eb;
-                        const unsigned char *b_lo =
-                                ub_lo ? static_cast<const unsigned char *>(ub_lo) + (size_t)(next + from) * kp * eb
-                                      : nullptr;
-                        ProductArgs part{l21, la_lo, b_hi, mode == kSplit ? b_lo : nullptr, c + next + from, rest,
-                                         count, (int64_t)kp, l21_ld, (int64_t)kp, n, -1.0, 1.0};
-                        return alongside ? product_alongside(mode, part, stream) : product(mode, part, stream);
-                    };
-                    e = update(0, beside, true); // the next panel runs meanwhile
-                    if (e == hipSuccess && beside < columns) {
-                        e = hipStreamWaitEvent(stream, ws.panel_done, 0);
-                        if (e == hipSuccess) e = update(beside, columns - beside, false);
-                    }
-                    if (e != hipSuccess) return (int)e;
-                }
                 timer.stage(4);
             }
         }

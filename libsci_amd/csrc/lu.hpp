@@ -47,7 +47,8 @@ namespace scihip {
         size_t flag_count = 0;
         hipStream_t side = nullptr;         // the look-ahead panels run here (default priority)
         hipEvent_t columns_ready = nullptr; // main stream: the next panel's columns are up to date
-        hipEvent_t panel_done = nullptr;    // side stream: that panel is factored
+        hipEvent_t panel_done = nullptr;    // side stream: that panel is factored (fp64), or the
+                                            // step's side work is done (float)
     };
 
     // Factors the order x order row-major matrix in place (T = float or double); pivots[k] is the
