@@ -709,9 +709,10 @@ namespace scihip {
             }
         }
 
-        // The same U12 = L11^-1 A12 for at most kPanel columns (the next panel's), with the rows split
-        // over kU12Blocks blocks of kU12Rows so that eight waves per column pair share the work: thread
-        // (q, k) holds rows [q R, q R + R) of column k. Block s is solved by its own threads, which leave
+        // The same U12 = L11^-1 A12, kPanel columns per workgroup, with the rows split over kU12Blocks
+        // blocks of kU12Rows so that eight waves per column pair share the work: thread (q, k) holds rows
+        // [q R, q R + R) of column k. With sixteen waves per compute unit instead of one, memory and LDS
+        // latencies overlap (fp64 at n = 16384, all of a step's columns: 65 -> 47-53 us). Block s is solved by its own threads, which leave
         // the solved values in LDS; then every later block subtracts their contribution. Each entry
         // receives exactly solve_u12's operations in solve_u12's order (row r subtracts l[r][s] times
         // row s for s = 0, 1, ..., r - 1), so the results are the same bits; the longest chain of
@@ -721,7 +722,7 @@ namespace scihip {
         constexpr int kU12NarrowThreads = kPanel * kU12Blocks; // 1024
         template <typename T>
         __global__ __launch_bounds__(kU12NarrowThreads) void solve_u12_narrow(const T *__restrict__ l, T *__restrict__ u,
-                                                                               int64_t n, int columns) {
+                                                                               int64_t n, int64_t columns) {
             constexpr int R = kU12Rows;
             __shared__ T lower[kPanel * kPanel];
             __shared__ T solved[R][kPanel];
@@ -731,8 +732,8 @@ namespace scihip {
                 lower[e] = c < r ? l[(int64_t)r * n + c] : T(0);
             }
             const int k = t % kPanel, q = t / kPanel; // q is the same in a whole wave
-            const bool live = k < columns;
-            T *__restrict__ column = u + k;
+            const bool live = (int64_t)blockIdx.x * kPanel + k < columns;
+            T *__restrict__ column = u + (int64_t)blockIdx.x * kPanel + k;
             T y[R];
 #pragma unroll
             for (int i = 0; i < R; ++i) y[i] = live ? column[(int64_t)(q * R + i) * n] : T(0);
@@ -1360,10 +1361,9 @@ namespace scihip {
             if (e != hipSuccess) return (int)e;
             resident = units * per_unit;
         }
-        // Clears candidate line set `set` on stream `s` (each launch expects its lines zero). fp64
-        // panels, which run on the side stream, all take set 0, cleared after each panel on its stream;
-        // float panels, which run on the main stream, alternate between the sets, the set of the panel
-        // after next being cleared on the side stream meanwhile.
+        // Clears candidate line set `set` on stream `s` (each launch expects its lines zero). Panels
+        // run on the main stream and alternate between the sets; the set of the panel after next is
+        // cleared on the side stream meanwhile.
         const size_t line_bytes = (size_t)kParities * max_groups * kLine * sizeof(unsigned long long);
         unsigned long long *line_sets[2] = {lines, lines + line_bytes / sizeof(unsigned long long)};
         auto clear_lines = [&](hipStream_t s, int set) { return hipMemsetAsync(line_sets[set], 0, line_bytes, s); };
@@ -1428,8 +1428,8 @@ namespace scihip {
                 e = product(kFloat64, slab, stream);
                 if (e != hipSuccess) return e;
             }
-            hipLaunchKernelGGL(solve_u12<T>, dim3((unsigned)((rest + kSolveU12Threads<T> - 1) / kSolveU12Threads<T>)),
-                               dim3(kSolveU12Threads<T>), 0, stream, l11, a12, n, rest);
+            hipLaunchKernelGGL(solve_u12_narrow<T>, dim3((unsigned)((rest + kPanel - 1) / kPanel)), dim3(kU12NarrowThreads),
+                               0, stream, l11, a12, n, rest);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
             timer.stage(2);
@@ -1441,20 +1441,24 @@ namespace scihip {
             timer.stage(3);
             // A product over this step's columns [from, from + count): depth 256 (both steps) or 128
             // (this step's update alone).
+            // Beside the panel (`alongside`, on the side stream) it is sized for one workgroup per compute
+            // unit; after the panel, on the main stream, for two.
             auto update = [&](int64_t from, int64_t count, bool both, bool alongside) -> hipError_t {
                 if (count <= 0) return hipSuccess;
                 const T *left = both ? l21 - kPanel : l21;
                 const T *right = u + from * kPair + (carry && !both ? kPanel : 0);
                 ProductArgs part{left, nullptr, right, nullptr, c + from, rest, count, both ? kPair : kPanel, n,
                                  kPair, n, -1.0, 1.0};
-                return alongside ? product_alongside(kFloat64, part, stream) : product(kFloat64, part, stream);
+                return alongside ? product_alongside(kFloat64, part, ws.side) : product(kFloat64, part, stream);
             };
             e = update(0, next, carry, false); // the next panel's columns (carried ones, after a carry)
+            // The next panel follows on the main stream at once; the update beside it runs on the side
+            // stream, whose end the next step waits for (panel_done marks it).
+            const int set = (int)((j0 + w) / kPanel) & 1;
             if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
+            if (e == hipSuccess) e = launch_panel(j0 + w, stream, set);
             if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
-            if (e == hipSuccess) e = launch_panel(j0 + w, ws.side, 0);
-            if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
-            if (e == hipSuccess) e = clear_lines(ws.side, 0);
+            if (e == hipSuccess) e = clear_lines(ws.side, set ^ 1); // for the panel after next
             if (e != hipSuccess) return e;
             // Beside the panel, at one workgroup per compute unit, about a panel's time of depth-128
             // work (a share of the columns from the measured rates).
@@ -1466,12 +1470,9 @@ namespace scihip {
             };
             if (carry) {
                 // Columns already carrying the previous update: this step's alone, beside the panel; then
-                // the waiting ones with both steps', at two workgroups per compute unit.
+                // the waiting ones with both steps', after the panel at two workgroups per compute unit.
                 e = update(waiting, rest - waiting, false, true);
-                if (e == hipSuccess && waiting > next) {
-                    e = hipStreamWaitEvent(stream, ws.panel_done, 0);
-                    if (e == hipSuccess) e = update(next, waiting - next, true, false);
-                }
+                if (e == hipSuccess && waiting > next) e = update(next, waiting - next, true, false);
                 carry = false;
             } else if (rest - next >= kPanel) {
                 // Carry: the last columns beside the panel, leaving at least the following panel's
@@ -1483,6 +1484,7 @@ namespace scihip {
             } else if (rest > next) {
                 e = update(next, rest - next, false, true); // a last short update
             }
+            if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
             return e;
         };
 
@@ -1548,7 +1550,7 @@ namespace scihip {
                                  count, depth, left_ld, kPair, n, -1.0, 1.0};
                 return from == 0 ? product(mode, part, s) : product_alongside(mode, part, s);
             };
-            hipLaunchKernelGGL(solve_u12_narrow<T>, dim3(1), dim3(kU12NarrowThreads), 0, stream, l11, a12, n, (int)next);
+            hipLaunchKernelGGL(solve_u12_narrow<T>, dim3(1), dim3(kU12NarrowThreads), 0, stream, l11, a12, n, next);
             e = hipGetLastError();
             timer.stage(2);
             if (e == hipSuccess) e = pack_u12(0, next, stream);
@@ -1578,11 +1580,10 @@ namespace scihip {
 
         hipError_t e = hipMemsetAsync(lines, 0, 2 * line_bytes, stream); // both sets
         if (e == hipSuccess) e = launch_panel(0, stream, 0);
-        if (e == hipSuccess && sizeof(T) == 8) e = clear_lines(stream, 0);
         if (e != hipSuccess) return (int)e;
         for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
             const int w = (int)std::min<int64_t>(kPanel, n - j0);
-            if (j0 > 0) { // fp64: this panel, run on the side stream; float: the previous step's side work
+            if (j0 > 0) { // the previous step's work on the side stream
                 e = hipStreamWaitEvent(stream, ws.panel_done, 0);
                 if (e != hipSuccess) return (int)e;
             }

@@ -3,20 +3,21 @@
 // The reference factors and solves on the host only (include/linear/decompose.hpp:29-72,
 // include/linear/solve.hpp:25-76). The same row-major, in-place algorithm runs here blocked:
 // for each panel of kPanel columns,
-//   1. the panel is factored by one cooperative kernel that keeps each workgroup's rows in LDS: per
-//      column every workgroup publishes its best pivot candidate (first maximum of |a|, the
-//      reference's rule) with the candidate's panel row, every workgroup gathers them all and picks
-//      the same pivot, and all workgroups swap and eliminate; then one workgroup inverts the unit
-//      lower L11;
+//   1. the panel is factored by one kernel that keeps each workgroup's rows in LDS: per column every
+//      workgroup publishes its best pivot candidate (first maximum of |a|, the reference's rule)
+//      with the candidate's panel row, every workgroup gathers them all and picks the same pivot,
+//      and all workgroups swap and eliminate;
 //   2. the panel's row swaps are applied to the columns outside it;
-//   3. U12 = L11^-1 A12 on the matrix cores, in the matrix's own precision;
-//   4. A22 -= L21 U12 on the matrix cores (fp64, or fp32 / bf16x3 / bf16 for a float matrix), in two
-//      products: first the next panel's columns, then the rest. Once the first is done, the next
-//      panel is factored on a second stream while the rest of the update runs (look-ahead), and the
-//      next step waits for it.
+//   3. U12 = L11^-1 A12 by substitution in the reference's order, in the matrix's own precision;
+//   4. A22 -= L21 U12 on the matrix cores (fp64, or fp32 / bf16x3 / bf16 for a float matrix): first
+//      the next panel's columns, then that panel is factored on the main stream while the rest of
+//      the update runs on a second stream (look-ahead), which the next step waits for. Steps go in
+//      pairs: a step carries most of its update into the next one, which applies both steps'
+//      updates as products of depth 2 kPanel, for half the traffic of C.
 // Solves of an order that is a multiple of kBlock invert the kBlock x kBlock diagonal blocks of L and
-// U once (all blocks in parallel), then walk the blocks: x_j = inv(D_jj) x_j and a matrix-core
-// update of the remaining rows that reads the factors in place. Other orders run a sync-free blocked
+// U once (all blocks in parallel) and link each block to the blocks one and two away, then solve each
+// triangle in one launch that walks the blocks, four workgroups per block, each waiting on the blocks
+// it depends on as they are published. Other orders run a sync-free blocked
 // substitution: one workgroup per kSolveRows-row block, each waiting on the blocks before it with a
 // bounded spin. Every spin in this file is bounded: past kSpinLimit the kernel records an error and
 // exits, and the host reports it.
@@ -45,10 +46,9 @@ namespace scihip {
         size_t publish_bytes = 0;
         unsigned launches = 0;       // panel launches so far, for the publications' tags
         size_t flag_count = 0;
-        hipStream_t side = nullptr;         // the look-ahead panels run here (default priority)
+        hipStream_t side = nullptr;         // updates beside the look-ahead panels (default priority)
         hipEvent_t columns_ready = nullptr; // main stream: the next panel's columns are up to date
-        hipEvent_t panel_done = nullptr;    // side stream: that panel is factored (fp64), or the
-                                            // step's side work is done (float)
+        hipEvent_t panel_done = nullptr;    // side stream: the step's work there is done
     };
 
     // Factors the order x order row-major matrix in place (T = float or double); pivots[k] is the
