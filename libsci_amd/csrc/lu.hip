@@ -712,7 +712,10 @@ namespace scihip {
         // The same U12 = L11^-1 A12, kPanel columns per workgroup, with the rows split over kU12Blocks
         // blocks of kU12Rows so that eight waves per column pair share the work: thread (q, k) holds rows
         // [q R, q R + R) of column k. With sixteen waves per compute unit instead of one, memory and LDS
-        // latencies overlap (fp64 at n = 16384, all of a step's columns: 65 -> 47-53 us). Block s is solved by its own threads, which leave
+        // latencies overlap (fp64 at n = 16384, all of a step's columns: 65 -> 47-53 us). Column c is
+        // also stored as row c of the product's packed B operand at hi / lo (rows `stride` elements
+        // apart, its kPanel entries first; hi == nullptr stores none), which saves a packing launch on
+        // the chain of panels. Block s is solved by its own threads, which leave
         // the solved values in LDS; then every later block subtracts their contribution. Each entry
         // receives exactly solve_u12's operations in solve_u12's order (row r subtracts l[r][s] times
         // row s for s = 0, 1, ..., r - 1), so the results are the same bits; the longest chain of
@@ -720,9 +723,10 @@ namespace scihip {
         constexpr int kU12Rows = 16;
         constexpr int kU12Blocks = kPanel / kU12Rows;
         constexpr int kU12NarrowThreads = kPanel * kU12Blocks; // 1024
-        template <typename T>
+        template <typename T, typename D, bool split>
         __global__ __launch_bounds__(kU12NarrowThreads) void solve_u12_narrow(const T *__restrict__ l, T *__restrict__ u,
-                                                                               int64_t n, int64_t columns) {
+                                                                               int64_t n, int64_t columns, D *hi, D *lo,
+                                                                               int64_t stride) {
             constexpr int R = kU12Rows;
             __shared__ T lower[kPanel * kPanel];
             __shared__ T solved[R][kPanel];
@@ -732,8 +736,9 @@ namespace scihip {
                 lower[e] = c < r ? l[(int64_t)r * n + c] : T(0);
             }
             const int k = t % kPanel, q = t / kPanel; // q is the same in a whole wave
-            const bool live = (int64_t)blockIdx.x * kPanel + k < columns;
-            T *__restrict__ column = u + (int64_t)blockIdx.x * kPanel + k;
+            const int64_t index = (int64_t)blockIdx.x * kPanel + k; // the column
+            const bool live = index < columns;
+            T *__restrict__ column = u + index;
             T y[R];
 #pragma unroll
             for (int i = 0; i < R; ++i) y[i] = live ? column[(int64_t)(q * R + i) * n] : T(0);
@@ -747,22 +752,65 @@ namespace scihip {
                         for (int j = 0; j < i; ++j) y[i] = eliminated(y[i], block[i * kPanel + j], y[j]);
 #pragma unroll
                     for (int i = 0; i < R; ++i) solved[i][k] = y[i];
-                    if (live)
+                    if (live) {
 #pragma unroll
                         for (int i = 0; i < R; ++i) column[(int64_t)(s * R + i) * n] = y[i];
+                        if (hi)
+#pragma unroll
+                            for (int i = 0; i < R; ++i) pack_element<D, split>(y[i], hi, lo, index * stride + s * R + i);
+                    }
                 }
                 __syncthreads();
                 if (q > s) {
+                    // fp64 reads four entries of a row of L at once (at n = 16384, with the packed copy
+                    // stored too: 53 -> 38 us per step); float reads one (the four-entry form measured
+                    // 24 -> 33 us per step). Each y[i] takes j = 0, 1, ... in order either way.
                     const T *block = lower + (q * R) * kPanel + s * R;
+                    if constexpr (sizeof(T) == 8) {
+                        typedef T Four __attribute__((ext_vector_type(4)));
 #pragma unroll
-                    for (int j = 0; j < R; ++j) {
-                        const T x = solved[j][k];
+                        for (int j = 0; j < R; j += 4) {
+                            T x[4];
 #pragma unroll
-                        for (int i = 0; i < R; ++i) y[i] = eliminated(y[i], block[i * kPanel + j], x);
+                            for (int h = 0; h < 4; ++h) x[h] = solved[j + h][k];
+#pragma unroll
+                            for (int i = 0; i < R; ++i) {
+                                const Four m = *reinterpret_cast<const Four *>(block + i * kPanel + j);
+#pragma unroll
+                                for (int h = 0; h < 4; ++h) y[i] = eliminated(y[i], m[h], x[h]);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < R; ++j) {
+                            const T x = solved[j][k];
+#pragma unroll
+                            for (int i = 0; i < R; ++i) y[i] = eliminated(y[i], block[i * kPanel + j], x);
+                        }
                     }
                 }
                 __syncthreads();
             }
+        }
+
+        // U12 = L11^-1 A12 for `columns` columns by solve_u12_narrow, packed in mode `mode`'s input type
+        // at hi / lo (see there; hi == nullptr: not packed).
+        template <typename T>
+        hipError_t solve_and_pack_u12(const T *l11, T *a12, int64_t n, int64_t columns, int mode, void *hi, void *lo,
+                                      int64_t stride, hipStream_t s) {
+            const dim3 grid((unsigned)((columns + kPanel - 1) / kPanel)), block(kU12NarrowThreads);
+            if (sizeof(T) == 8 || mode == kFloat32 || !hi) {
+                hipLaunchKernelGGL((solve_u12_narrow<T, T, false>), grid, block, 0, s, l11, a12, n, columns,
+                                   static_cast<T *>(hi), nullptr, stride);
+            } else if constexpr (sizeof(T) == 8) {
+                return hipErrorInvalidValue; // fp64 factors pack as fp64 only
+            } else if (mode == kBfloat16)
+                hipLaunchKernelGGL((solve_u12_narrow<T, __bf16, false>), grid, block, 0, s, l11, a12, n, columns,
+                                   static_cast<__bf16 *>(hi), nullptr, stride);
+            else
+                hipLaunchKernelGGL((solve_u12_narrow<T, __bf16, true>), grid, block, 0, s, l11, a12, n, columns,
+                                   static_cast<__bf16 *>(hi), static_cast<__bf16 *>(lo), stride);
+            return hipGetLastError();
         }
 
         // Applies the panel's swaps to every column outside it, in step order.
@@ -1428,17 +1476,13 @@ namespace scihip {
                 e = product(kFloat64, slab, stream);
                 if (e != hipSuccess) return e;
             }
-            hipLaunchKernelGGL(solve_u12_narrow<T>, dim3((unsigned)((rest + kPanel - 1) / kPanel)), dim3(kU12NarrowThreads),
-                               0, stream, l11, a12, n, rest);
-            e = hipGetLastError();
+            // U12, solved in place and packed: this step's column t goes to packed row t (first half),
+            // or, after a carry, to row kPanel + t (second half), beside the previous step's U12 of the
+            // same matrix column.
+            T *u = carry ? ub + kPanel * kPair : ub; // packed row of this step's first column
+            e = solve_and_pack_u12(l11, a12, n, rest, kFloat64, carry ? u + kPanel : u, nullptr, kPair, stream);
             if (e != hipSuccess) return e;
             timer.stage(2);
-            // This step's U12 column t goes to packed row t (first half), or, after a carry, to row
-            // kPanel + t (second half), beside the previous step's U12 of the same matrix column.
-            T *u = carry ? ub + kPanel * kPair : ub; // packed row of this step's first column
-            e = pack(kFloat64, a12, true, rest, w, n, true, carry ? u + kPanel : u, nullptr, kPanel, stream, kPair);
-            if (e != hipSuccess) return e;
-            timer.stage(3);
             // A product over this step's columns [from, from + count): depth 256 (both steps) or 128
             // (this step's update alone).
             // Beside the panel (`alongside`, on the side stream) it is sized for one workgroup per compute
@@ -1550,11 +1594,10 @@ namespace scihip {
                                  count, depth, left_ld, kPair, n, -1.0, 1.0};
                 return from == 0 ? product(mode, part, s) : product_alongside(mode, part, s);
             };
-            hipLaunchKernelGGL(solve_u12_narrow<T>, dim3(1), dim3(kU12NarrowThreads), 0, stream, l11, a12, n, next);
-            e = hipGetLastError();
+            // The next panel's U12 columns, solved in place and packed.
+            e = solve_and_pack_u12(l11, a12, n, next, mode, at(ub_hi, row0 * kPair + half),
+                                   split ? at(ub_lo, row0 * kPair + half) : nullptr, kPair, stream);
             timer.stage(2);
-            if (e == hipSuccess) e = pack_u12(0, next, stream);
-            timer.stage(3);
             if (e == hipSuccess) e = update(0, next, stream); // the next panel's columns
             // The next panel follows on the main stream at once; the rest of the step runs beside it on
             // the side stream, whose end the next step waits for (panel_done marks it).

@@ -255,18 +255,7 @@ namespace scihip {
             for (int y = ty; y < 64; y += 4) {
                 const int64_t i = i0 + y, k = k0 + tx;
                 if (i >= rows || k >= kp) continue;
-                const S x = tile[y][tx];
-                if constexpr (sizeof(D) == 2) {
-                    const float f = (float)x;
-                    const __bf16 h = (__bf16)f;
-                    hi[i * stride + k] = h;
-                    if constexpr (split) {
-                        const float rest = __builtin_isfinite(f) ? f - (float)h : 0.0f;
-                        lo[i * stride + k] = (__bf16)rest;
-                    }
-                } else {
-                    hi[i * stride + k] = (D)x;
-                }
+                pack_element<D, split>(tile[y][tx], hi, lo, i * stride + k);
             }
         }
 

@@ -52,6 +52,24 @@ namespace scihip {
     // keeping at least 8 k-tiles, at most 16 slices.
     int split_slices(int mode, int64_t m, int64_t n, int64_t kp);
 
+    // Stores one element of a packed operand at hi[at] (and, for the split form, lo[at]): D is the
+    // matrix-core input type; a bf16 high part is the value rounded to float and then to bf16, the
+    // low part the rounded remainder, zero where the value is not finite.
+    template <typename D, bool split, typename S>
+    __device__ __forceinline__ void pack_element(S x, D *hi, D *lo, int64_t at) {
+        if constexpr (sizeof(D) == 2) {
+            const float f = (float)x;
+            const __bf16 h = (__bf16)f;
+            hi[at] = h;
+            if constexpr (split) {
+                const float rest = __builtin_isfinite(f) ? f - (float)h : 0.0f;
+                lo[at] = (__bf16)rest;
+            }
+        } else {
+            hi[at] = (D)x;
+        }
+    }
+
     // Packs a row-major source into the product's operand layout: out[i][k] = src[i][k] (transpose
     // false) or src[k][i] (transpose true), for i < rows and k < depth, zero for depth <= k < kp.
     // `source` is float or double (by `wide`); the output is in the mode's input type; kSplit also
