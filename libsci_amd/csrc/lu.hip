@@ -1368,9 +1368,12 @@ namespace scihip {
 
     } // namespace
 
-    // Measured on MI355X at n = 16384: an fp64 panel takes about 0.6 ms, and the fp64 update beside it
-    // runs at about 25 TFLOP/s. They size the share of each fp64 update that overlaps the next panel.
-    constexpr double kPanelMsFp64 = 0.6;
+    // The share of each fp64 update that runs beside the next panel: kPanelMsFp64 milliseconds of work
+    // at kAlongsideRateFp64, the rate measured for an fp64 update beside a panel on MI355X. An fp64
+    // panel takes about 0.6 ms at n = 16384, but the fp64 solve there measured best with 0.4 ms of
+    // work beside it (0.1 to 1.1 ms swept: 158, 153, 148, 146, 150, 155, 161 ms at 0.1, 0.2, 0.3,
+    // 0.4, 0.6, 0.8, 1.1): the update beside the panel slows it down.
+    constexpr double kPanelMsFp64 = 0.4;
     constexpr double kAlongsideRateFp64 = 25e12;
 
     template <typename T>
