@@ -1375,6 +1375,12 @@ namespace scihip {
     // 0.4, 0.6, 0.8, 1.1): the update beside the panel slows it down.
     constexpr double kPanelMsFp64 = 0.4;
     constexpr double kAlongsideRateFp64 = 25e12;
+    // Float steps pair while this many columns or more remain (SCIHIP_LU_PAIR overrides it, for
+    // tests at small orders): in later steps the update hides
+    // beside the panel anyway, and a pair only adds its repacking and 128-row product to the chain of
+    // panels. At n = 16384 (threshold 0 / 4096 / 8192 / 12288) the fp32 solve measured 104.2 / 103.8 /
+    // 103.3 / 104.4 ms and the bf16 one 94.0 / 92.6 / 92.6 / 92.8 ms (mean of two runs on one box).
+    constexpr int64_t kFloatPairColumns = 8192;
 
     template <typename T>
     int factor(T *a, int64_t n, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
@@ -1535,6 +1541,8 @@ namespace scihip {
             return e;
         };
 
+        int64_t pair_columns = kFloatPairColumns;
+        if (const char *text = std::getenv("SCIHIP_LU_PAIR")) pair_columns = std::strtoll(text, nullptr, 10);
         // Float steps in pairs too. A float factorisation waits on the chain of panels, except in its
         // first steps, where each trailing update (all of C read and written) outlasts the panel beside
         // it. So a fresh step updates only the next panel's columns, which lets that panel run alone,
@@ -1618,7 +1626,7 @@ namespace scihip {
                 if (e == hipSuccess) e = pack_u12(next, others, ws.side);
                 if (e != hipSuccess) return e;
             }
-            carry = !pair && rest - next >= kPanel;
+            carry = !pair && rest - next >= kPanel && rest >= pair_columns;
             if (rest > next && !carry) e = update(next, rest - next, ws.side); // beside the panel
             if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
             return e;

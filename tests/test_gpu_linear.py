@@ -90,11 +90,14 @@ def test_factor_matches_oracle(device, n):
     assert np.abs(a[rows] - lower @ upper).max() < 1e-12 * n
 
 
-@pytest.mark.parametrize("n", [300, 640, 1000])
+@pytest.mark.parametrize("n,pair", [(300, None), (640, "0"), (1000, "0"), (1000, None)])
 @pytest.mark.parametrize("mode", ["fp32", "bf16x3", "bf16"])
-def test_float_factor_reconstructs(device, mode, n):
-    # n = 640 and 1000 take steps in pairs (one step's trailing update carried into the next one's)
-    print(f"[test] float factor n={n} {mode}", flush=True)
+def test_float_factor_reconstructs(device, mode, n, pair, monkeypatch):
+    # SCIHIP_LU_PAIR=0: every step that can pairs with the next (one step's trailing update carried
+    # into the next one's), as the first steps of large orders do by default
+    if pair is not None:
+        monkeypatch.setenv("SCIHIP_LU_PAIR", pair)
+    print(f"[test] float factor n={n} {mode} pair={pair}", flush=True)
     a = (oracle.uniform(n * n, 5, -1.0, 1.0, dtype=np.float64).reshape(n, n) + (n / 4) * np.eye(n)).astype(np.float32)
     lu, rows = gpu_factor(device, a, mode)
     assert (rows == np.arange(n)).all()  # a shifted diagonal keeps the identity order
