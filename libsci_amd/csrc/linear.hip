@@ -21,12 +21,30 @@ namespace scihip {
                 target[i] = target[i] + (double)delta[i];
         }
 
-        // One workgroup per row: sum of |a_ij| over the row.
-        __global__ __launch_bounds__(256) void row_sums_kernel(const double *a, int64_t n, int64_t columns,
-                                                               double *sums) {
+        // One workgroup per row: the row copied to target (converted to D) and the sum of |a_ij| over
+        // it, two elements per thread per 16-byte load where the rows are 16-byte aligned.
+        template <typename D>
+        __global__ __launch_bounds__(256) void copy_row_sums_kernel(const double *a, int64_t columns, D *target,
+                                                                    double *sums) {
             const int64_t row = blockIdx.x;
+            const double *source = a + row * columns;
+            D *out = target + row * columns;
             double s = 0.0;
-            for (int64_t j = threadIdx.x; j < columns; j += blockDim.x) s += __builtin_fabs(a[row * columns + j]);
+            if (columns % 2 == 0 && reinterpret_cast<uintptr_t>(a) % 16 == 0) {
+                typedef double Two __attribute__((ext_vector_type(2)));
+                for (int64_t j = 2 * (int64_t)threadIdx.x; j < columns; j += 2 * blockDim.x) {
+                    const Two v = *reinterpret_cast<const Two *>(source + j);
+                    out[j] = (D)v[0];
+                    out[j + 1] = (D)v[1];
+                    s += __builtin_fabs(v[0]);
+                    s += __builtin_fabs(v[1]);
+                }
+            } else {
+                for (int64_t j = threadIdx.x; j < columns; j += blockDim.x) {
+                    out[j] = (D)source[j];
+                    s += __builtin_fabs(source[j]);
+                }
+            }
 #pragma unroll
             for (int offset = 32; offset > 0; offset >>= 1) s += __shfl_xor(s, offset, 64);
             __shared__ double waves[4];
@@ -91,9 +109,19 @@ namespace scihip {
         return hipGetLastError();
     }
 
-    hipError_t row_sums(const double *a, int64_t rows, int64_t columns, double *sums, hipStream_t stream) {
+    hipError_t copy_with_row_sums(const double *a, int64_t rows, int64_t columns, double *target, double *sums,
+                                  hipStream_t stream) {
         if (rows == 0) return hipSuccess;
-        hipLaunchKernelGGL(row_sums_kernel, dim3((unsigned)rows), dim3(256), 0, stream, a, rows, columns, sums);
+        hipLaunchKernelGGL(copy_row_sums_kernel<double>, dim3((unsigned)rows), dim3(256), 0, stream, a, columns, target,
+                           sums);
+        return hipGetLastError();
+    }
+
+    hipError_t copy_with_row_sums(const double *a, int64_t rows, int64_t columns, float *target, double *sums,
+                                  hipStream_t stream) {
+        if (rows == 0) return hipSuccess;
+        hipLaunchKernelGGL(copy_row_sums_kernel<float>, dim3((unsigned)rows), dim3(256), 0, stream, a, columns, target,
+                           sums);
         return hipGetLastError();
     }
 

@@ -11,8 +11,12 @@ namespace scihip {
     // target += delta, in double.
     hipError_t accumulate(double *target, const float *delta, int64_t count, hipStream_t stream);
     hipError_t accumulate(double *target, const double *delta, int64_t count, hipStream_t stream);
-    // sums[i] = sum_j |a_ij| (the infinity norm is their maximum).
-    hipError_t row_sums(const double *a, int64_t rows, int64_t columns, double *sums, hipStream_t stream);
+    // target = a (row-major, converted to the target's type) and sums[i] = sum_j |a_ij| (the infinity
+    // norm is their maximum), reading a once for both.
+    hipError_t copy_with_row_sums(const double *a, int64_t rows, int64_t columns, double *target, double *sums,
+                                  hipStream_t stream);
+    hipError_t copy_with_row_sums(const double *a, int64_t rows, int64_t columns, float *target, double *sums,
+                                  hipStream_t stream);
     // Partial column maxima of |x|: column_max_chunks(rows) rows of `width` values.
     int64_t column_max_chunks(int64_t rows);
     hipError_t column_max(const double *x, int64_t rows, int64_t width, double *partial, hipStream_t stream);

@@ -909,15 +909,17 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     if (rc) return rc;
     if ((rc = ensure_lu(device, n, eb))) return rc;
 
-    // Factor a copy of A in the chosen precision.
+    // Factor a copy of A in the chosen precision; the copy also gives |A|'s row sums.
     std::vector<int64_t> pivots;
     if (wide) {
-        if ((rc = status(hipMemcpyAsync(factors.pointer, matrix, order * order * 8, hipMemcpyDeviceToDevice,
-                                        device->stream), "copy")))
+        if ((rc = status(copy_with_row_sums(matrix, n, n, factors.as<double>(), sums.as<double>(), device->stream),
+                         "copy")))
             return rc;
         rc = factor_on_device<double>(device, factors.as<double>(), n, kFloat64, pivots);
     } else {
-        if ((rc = status(convert(matrix, factors.as<float>(), n * n, device->stream), "convert"))) return rc;
+        if ((rc = status(copy_with_row_sums(matrix, n, n, factors.as<float>(), sums.as<double>(), device->stream),
+                         "convert")))
+            return rc;
         const int product_mode = mode == SCIHIP_FACTOR_FLOAT32 ? kFloat32 : mode == SCIHIP_FACTOR_SPLIT ? kSplit : kBfloat16;
         rc = factor_on_device<float>(device, factors.as<float>(), n, product_mode, pivots);
     }
@@ -936,7 +938,6 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
 
     // |A|_inf, once.
     std::vector<double> host(size_t(std::max<int64_t>(n, chunks * w * 3)));
-    if ((rc = status(row_sums(matrix, n, n, sums.as<double>(), device->stream), "row sums"))) return rc;
     if ((rc = status(hipMemcpyAsync(host.data(), sums.pointer, order * 8, hipMemcpyDeviceToHost, device->stream), "copy")))
         return rc;
     if ((rc = status(hipStreamSynchronize(device->stream), "sync"))) return rc;

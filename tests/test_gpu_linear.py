@@ -144,7 +144,7 @@ def test_refined_solve(device, mode, graded):
     assert forward < (1e-10 if graded else 1e-13)
 
 
-@pytest.mark.parametrize("n,width", [(384, 5), (512, 1), (512, 40), (300, 3)])
+@pytest.mark.parametrize("n,width", [(384, 5), (512, 1), (512, 40), (300, 3), (301, 2)])
 def test_refined_solve_widths(device, n, width):
     """The refinement's residual at other shapes: the narrow in-place kernel (order a multiple of 128,
     at most 32 right-hand sides, fewer than 32 here) and the general product otherwise."""
