@@ -1032,8 +1032,8 @@ namespace scihip {
                     block[e] = m[r * row + (int64_t)k * column];
                 }
             };
-            // into += sign * M xl over k = 0 .. kBlock - 1, M from fetch() with the same strides.
-            auto apply = [&](const Block &block, int64_t column, V &into, T sign) {
+            // M from fetch() (the same strides) into fl, then into += sign * M xl over k = 0 .. kBlock - 1.
+            auto stage = [&](const Block &block, int64_t column) {
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
                     const int idx = t + 256 * e;
@@ -1042,10 +1042,16 @@ namespace scihip {
                     fl[r * kStride + k] = block[e];
                 }
                 __syncthreads();
+            };
+            auto multiply = [&](V &into, T sign) {
 #pragma unroll 8
                 for (int kk = 0; kk < kBlock; kk += 4)
                     mfma<T>(into, sign * fl[(tile_row + c16) * kStride + kk + g], xl[kk + g][col]);
                 __syncthreads();
+            };
+            auto apply = [&](const Block &block, int64_t column, V &into, T sign) {
+                stage(block, column);
+                multiply(into, sign);
             };
 
             // The 128 x 32 block i of `from` into xl once published, 16 entries per thread with their
@@ -1082,8 +1088,9 @@ namespace scihip {
                     if ((spins & 63u) == 0 && relaxed(error)) break;
                     __builtin_amdgcn_s_sleep(1);
                 }
-                for (int idx = t; idx < kBlock * C; idx += 256)
-                    if (idx % C >= cols) xl[idx / C][idx % C] = T(0);
+                if (cols < C)
+                    for (int idx = t; idx < kBlock * C; idx += 256)
+                        if (idx % C >= cols) xl[idx / C][idx % C] = T(0);
                 __syncthreads();
             };
             auto depend = [&](int i) {
@@ -1119,15 +1126,19 @@ namespace scihip {
             // out = inv(D_jj) acc, then out -= H_j x_second, then out -= G_j x_last (row-major links)
             V out = V{T(0), T(0), T(0), T(0)};
             apply(block, kBlock, out, T(1));
+            // The links are staged in LDS before their blocks are awaited, so that only the product
+            // follows the last wait.
             if (linked2) {
                 fetch(links2 + ((int64_t)j * kBlock + kQuarter * s) * kBlock, kBlock, 1, block);
+                stage(block, 1);
                 gather(published, second, -1);
-                apply(block, 1, out, T(-1));
+                multiply(out, T(-1));
             }
             if (linked) {
                 fetch(links + ((int64_t)j * kBlock + kQuarter * s) * kBlock, kBlock, 1, block);
+                stage(block, 1);
                 gather(published, last, -1);
-                apply(block, 1, out, T(-1));
+                multiply(out, T(-1));
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
