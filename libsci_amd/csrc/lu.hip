@@ -1538,7 +1538,7 @@ namespace scihip {
                 e = update(waiting, rest - waiting, false, true);
                 if (e == hipSuccess && waiting > next) e = update(next, waiting - next, true, false);
                 carry = false;
-            } else if (rest - next >= kPanel) {
+            } else if (rest - next >= kPanel && panel_share(rest - next) < rest - next) {
                 // Carry: the last columns beside the panel, leaving at least the following panel's
                 // columns (and all between) to the next step.
                 const int64_t beside = std::min(panel_share(rest - next), rest - next - kPanel);
@@ -1546,7 +1546,8 @@ namespace scihip {
                 carry = true;
                 carried_end = j0 + w + rest - beside;
             } else if (rest > next) {
-                e = update(next, rest - next, false, true); // a last short update
+                // All of it fits beside the panel (late steps), or too few columns are left to carry.
+                e = update(next, rest - next, false, true);
             }
             if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
             return e;
