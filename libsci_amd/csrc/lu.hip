@@ -1386,8 +1386,7 @@ namespace scihip {
     // 0.4, 0.6, 0.8, 1.1): the update beside the panel slows it down.
     constexpr double kPanelMsFp64 = 0.4;
     constexpr double kAlongsideRateFp64 = 25e12;
-    // Float steps pair while this many columns or more remain (SCIHIP_LU_PAIR overrides it, for
-    // tests at small orders): in later steps the update hides
+    // Float steps pair while this many columns or more remain: in later steps the update hides
     // beside the panel anyway, and a pair only adds its repacking and 128-row product to the chain of
     // panels. At n = 16384 (threshold 0 / 4096 / 8192 / 12288) the fp32 solve measured 104.2 / 103.8 /
     // 103.3 / 104.4 ms and the bf16 one 94.0 / 92.6 / 92.6 / 92.8 ms (mean of two runs on one box).
@@ -1466,6 +1465,12 @@ namespace scihip {
         };
         const size_t eb = (size_t)element_bytes(mode);
 
+        // SCIHIP_LU_PAIR=c (for tests at small orders): every step pairs while c or more columns remain,
+        // and an fp64 step leaves at least half its columns to the next one.
+        int64_t pair_columns = kFloatPairColumns;
+        const char *pair_text = std::getenv("SCIHIP_LU_PAIR");
+        const bool forced = pair_text != nullptr;
+        if (forced) pair_columns = std::strtoll(pair_text, nullptr, 10);
         // fp64 steps in pairs. The trailing update, a product of depth 128, reads and writes all of C
         // for 2 x 128 flops per element; each product workgroup loads and stores its C tile around a
         // loop of only 32 matrix-core steps, so the C traffic adds to the math rather than hiding
@@ -1538,10 +1543,12 @@ namespace scihip {
                 e = update(waiting, rest - waiting, false, true);
                 if (e == hipSuccess && waiting > next) e = update(next, waiting - next, true, false);
                 carry = false;
-            } else if (rest - next >= kPanel && panel_share(rest - next) < rest - next) {
+            } else if (rest - next >= kPanel &&
+                       (forced ? rest >= pair_columns : panel_share(rest - next) < rest - next)) {
                 // Carry: the last columns beside the panel, leaving at least the following panel's
                 // columns (and all between) to the next step.
-                const int64_t beside = std::min(panel_share(rest - next), rest - next - kPanel);
+                int64_t beside = std::min(panel_share(rest - next), rest - next - kPanel);
+                if (forced) beside = std::min(beside, (rest - next - kPanel) / 2 / kPanel * kPanel);
                 e = update(rest - beside, beside, false, true);
                 carry = true;
                 carried_end = j0 + w + rest - beside;
@@ -1553,8 +1560,6 @@ namespace scihip {
             return e;
         };
 
-        int64_t pair_columns = kFloatPairColumns;
-        if (const char *text = std::getenv("SCIHIP_LU_PAIR")) pair_columns = std::strtoll(text, nullptr, 10);
         // Float steps in pairs too. A float factorisation waits on the chain of panels, except in its
         // first steps, where each trailing update (all of C read and written) outlasts the panel beside
         // it. So a fresh step updates only the next panel's columns, which lets that panel run alone,

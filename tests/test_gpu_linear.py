@@ -73,9 +73,14 @@ def test_solve_kat(device):
     assert np.abs(a @ x - np.array([[11, 1], [-16, 0], [17, 2]])).max() < 1e-9
 
 
-@pytest.mark.parametrize("n", [7, 100, 129, 257, 600])
-def test_factor_matches_oracle(device, n):
-    print(f"[test] factor n={n}", flush=True)
+@pytest.mark.parametrize("n,pair", [(7, None), (100, None), (129, None), (257, None), (600, None), (600, "0"),
+                                    (1000, "0")])
+def test_factor_matches_oracle(device, n, pair, monkeypatch):
+    # SCIHIP_LU_PAIR=0: steps in pairs, part of each update carried into the next step's, as the
+    # first steps of large orders do by default
+    if pair is not None:
+        monkeypatch.setenv("SCIHIP_LU_PAIR", pair)
+    print(f"[test] factor n={n} pair={pair}", flush=True)
     a = oracle.uniform(n * n, n, -1.0, 1.0, dtype=np.float64).reshape(n, n)
     lu, rows = gpu_factor(device, a)
     ref_lu, ref_rows = oracle.factor(a)
