@@ -280,10 +280,28 @@ namespace scihip {
         }
     }
 
+    // The kernel's symbol: its kind (the key's first byte: 0 store, 1 reduce, 2 adjoint, 3 column
+    // folds, 4 row folds) and a 64-bit FNV-1a hash of the whole key, so that a profile tells the
+    // kernels of different chunks apart.
+    std::string Compiler::symbol(const std::string &key) {
+        static const char *const kinds[] = {"store", "reduce", "adjoint", "columns", "rows"};
+        const unsigned kind = key.empty() ? 0u : static_cast<unsigned char>(key[0]);
+        uint64_t hash = 0xcbf29ce484222325ull;
+        for (const char c : key) {
+            hash ^= static_cast<unsigned char>(c);
+            hash *= 0x100000001b3ull;
+        }
+        char text[64];
+        std::snprintf(text, sizeof(text), "chunk_%s_%016llx", kind < 5 ? kinds[kind] : "kernel",
+                      static_cast<unsigned long long>(hash));
+        return text;
+    }
+
     void Compiler::build(const std::string &key, const std::string &source, Kernel &kernel) {
-        (void)key;
         (void)hipSetDevice(ordinal);
-        const std::string text = std::string(kEmbeddedSource) + "\n" + source;
+        // Every generated source names its kernel scihip_specialised; the macro renames it.
+        const std::string name = symbol(key);
+        const std::string text = std::string(kEmbeddedSource) + "\n#define scihip_specialised " + name + "\n" + source;
         hiprtcProgram program;
         if (hiprtcCreateProgram(&program, text.c_str(), "chunk.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
             kernel.failed = true;
@@ -309,12 +327,12 @@ namespace scihip {
         hiprtcGetCode(program, code.data());
         hiprtcDestroyProgram(&program);
         if (hipModuleLoadData(&kernel.module, code.data()) != hipSuccess ||
-            hipModuleGetFunction(&kernel.function, kernel.module, "scihip_specialised") != hipSuccess) {
+            hipModuleGetFunction(&kernel.function, kernel.module, name.c_str()) != hipSuccess) {
             kernel.failed = true;
             return;
         }
         ++done;
-        if (trace) std::fprintf(stderr, "[scihip] compiled a specialised kernel (%zu bytes)\n", code.size());
+        if (trace) std::fprintf(stderr, "[scihip] compiled %s (%zu bytes)\n", name.c_str(), code.size());
     }
 
 } // namespace scihip

@@ -38,6 +38,9 @@ namespace scihip {
         // compiles in the foreground instead.
         const Kernel *find(const std::string &key, const std::function<std::string()> &generate, bool wait);
 
+        // The symbol a key's kernel is compiled under: chunk_<kind>_<hash of the key>.
+        static std::string symbol(const std::string &key);
+
         uint64_t compiled() const { return done; }
 
         // Straight-line source for a chunk. `mode` 0 evaluates, 1 reduces; `contract` marks a sum

@@ -696,7 +696,7 @@ int scihip_copy(scihip_device *device, void *destination, const void *source, si
 }
 
 int scihip_execute(scihip_device *device, const scihip_chunk *chunk, void *output, size_t count) {
-    if (!device || !chunk) return SCIHIP_EINVAL;
+    if (!device || !chunk || chunk->precision > SCIHIP_WIDE) return SCIHIP_EINVAL;
     if (count == 0) return SCIHIP_OK;
     std::lock_guard<std::mutex> lock(device->guard);
     return chunk->precision == SCIHIP_SINGLE ? execute<float>(device, *chunk, output, count)
@@ -704,7 +704,8 @@ int scihip_execute(scihip_device *device, const scihip_chunk *chunk, void *outpu
 }
 
 int scihip_reduce(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t count, double *result) {
-    if (!device || !chunk || !result || fold > SCIHIP_FOLD_MAXIMUM) return SCIHIP_EINVAL;
+    if (!device || !chunk || !result || fold > SCIHIP_FOLD_MAXIMUM || chunk->precision > SCIHIP_WIDE)
+        return SCIHIP_EINVAL;
     if (count == 0) { // device.cpp:266-268
         *result = fold == SCIHIP_FOLD_SUM ? 0.0 : std::numeric_limits<double>::quiet_NaN();
         return SCIHIP_OK;
@@ -731,11 +732,15 @@ int scihip_prepare(scihip_device *device, const scihip_chunk *chunk) {
     std::lock_guard<std::mutex> lock(device->guard);
     const bool stored = chunk->result == SCIHIP_SLOT_OUTPUT;
     const int mode = stored ? 0 : 1;
+    // A stored chunk has one kernel. A reduced one has a kernel per fold (the fold is a constant of
+    // each), plus the dot-product kernel of a contracted sum.
     const bool contract = !stored && contracted(*chunk, SCIHIP_FOLD_SUM);
-    if (!device->compiler->find(key_of(*chunk, mode, false), [&] { return Compiler::generate(*chunk, mode, false); },
-                                true)) {
-        fail("specialised kernel failed to compile");
-        return SCIHIP_ECOMPILE;
+    for (uint32_t fold = 0; fold <= (stored ? 0u : uint32_t(SCIHIP_FOLD_MAXIMUM)); ++fold) {
+        if (!device->compiler->find(key_of(*chunk, mode, false, fold),
+                                    [&] { return Compiler::generate(*chunk, mode, false, fold); }, true)) {
+            fail("specialised kernel failed to compile");
+            return SCIHIP_ECOMPILE;
+        }
     }
     if (contract &&
         !device->compiler->find(key_of(*chunk, mode, true), [&] { return Compiler::generate(*chunk, mode, true); },

@@ -87,3 +87,34 @@ def test_large_fp32_and_split(device):
         relative = np.linalg.norm(got - want) / np.linalg.norm(want)
         print(f"[test] 2048^3 {mode}: relative Frobenius error {relative:.3e}", flush=True)
         assert relative < {"fp32": 1e-6, "bf16x3": 1e-4, "bf16": 1e-2}[mode]
+
+
+@pytest.mark.parametrize("shape", [(8192, 8192, 8192), (4096, 16384, 1024)], ids=["8192^3", "4096x16384x1024"])
+def test_metric_sizes(device, shape):
+    """The products the measurement names: 8192^3 and 4096 x 16384 times 16384 x 1024, entries seeded
+    uniform[-1, 1] (generated on the device; the oracle's generator gives the same bits on the host).
+    Against a float64 product of 64 seeded rows: every entry within the per-mode bound
+    2k u (|A||B|), and the relative Frobenius error of those rows stated and bounded."""
+    m, k, n = shape
+    print(f"[test] product at {shape}", flush=True)
+    left = sci.uniform(m * k, device, 31, -1.0, 1.0).reshape(m, k)
+    right = sci.uniform(k * n, device, 32, -1.0, 1.0).reshape(k, n)
+    output = sci.Array(device, (m, n), np.float32)
+    left_h = oracle.uniform(m * k, 31, -1.0, 1.0).reshape(m, k)
+    right_h = oracle.uniform(k * n, 32, -1.0, 1.0).reshape(k, n)
+    rows = np.sort(np.random.default_rng(m + n).choice(m, 64, replace=False))
+    assert np.array_equal(left.host()[rows], left_h[rows])
+    right64 = right_h.astype(np.float64)
+    want = left_h[rows].astype(np.float64) @ right64
+    magnitude = np.abs(left_h[rows]).astype(np.float64) @ np.abs(right64)
+    frobenius = {"fp32": 1e-5, "bf16x3": 1e-4, "bf16": 1e-2}
+    for mode in ("fp32", "bf16x3", "bf16"):
+        _abi.check(_abi.lib().scihip_product(device.handle, MODES[mode], left.pointer, right.pointer,
+                                             output.pointer, m, n, k), "scihip_product")
+        got = output.host()[rows].astype(np.float64)
+        error = np.abs(got - want)
+        relative = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+        print(f"[test] {shape} {mode}: max abs error {error.max():.3e}, relative Frobenius {relative:.3e}",
+              flush=True)
+        assert (error <= 2 * k * UNIT[mode] * magnitude).all()
+        assert relative < frobenius[mode]

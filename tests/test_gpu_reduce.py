@@ -142,7 +142,15 @@ def test_front_end_reductions(device):
 
 
 def test_c2_reduce_full_size(device):
-    """N = 2^28: sum / min / max / mean of sin(a)*cos(c) + tanh(b), with and without b's NaNs."""
+    """N = 2^28: sum / min / max / mean of sin(a)*cos(c) + tanh(b) and the contracted sum of
+    sin(a)*cos(c), with and without b's NaNs. Every link is checked against something independent:
+    sin and cos against the oracle bit for bit, tanh (glibc on the CPU, the device library on the GPU)
+    against glibc within LIBRARY_ULPS, the terms the GPU stores bit for bit against the same float
+    operations on those values, and each GPU fold against the exact sum (or the exact extreme) of
+    the terms with the tight double-accumulation bound n * 2^-53 * sum |x|, below 1% of |sum| (0.07% for
+    the sum, 0.7% for the dot, whose terms nearly cancel)."""
+    from helpers import exact_sum, ulps32
+
     n = 1 << 28
     print("[test] c2 reduce at 2^28: generating on the device", flush=True)
     a = sci.uniform(n, device, 1, -4.0, 4.0)
@@ -154,16 +162,33 @@ def test_c2_reduce_full_size(device):
         assert math.isnan(sci.reduce_raw(expr_nan, fold))
     expr = sci.sin(a) * sci.cos(c) + sci.tanh(b)
     got = {fold: sci.reduce_raw(expr, fold) for fold in Fold}
+    dot = sci.reduce_raw(sci.sin(a) * sci.cos(c), Fold.sum)
+    product_extremes = [sci.reduce_raw(sci.sin(a) * sci.cos(c), fold) for fold in (Fold.minimum, Fold.maximum)]
+    terms = sci.evaluate(expr).host()
+    tanh_gpu = sci.evaluate(sci.tanh(b)).host()
     print("[test] c2 reduce at 2^28: oracle", flush=True)
     ah = oracle.uniform(n, 1, -4.0, 4.0)
     bh = oracle.uniform(n, 2, -4.0, 4.0)
     ch = oracle.uniform(n, 3, -4.0, 4.0)
-    chunk = record_postfix(C2_REDUCE, Precision.single, "reduce", [ah.ctypes.data, bh.ctypes.data, ch.ctypes.data])
-    for fold in (Fold.minimum, Fold.maximum):
-        want = oracle.reduce(chunk, fold, n, threads=16)
-        assert abs(got[fold] - want) <= LIBRARY_ULPS * 2 ** -24 * (abs(want) + 1)  # tanh: glibc vs device
-    cpu = oracle.reduce(chunk, Fold.sum, n, threads=16)
-    # sum |terms| <= 2n: the CPU's float blocks are within 1024 * 2^-24 * 2n of the exact sum, and the
-    # GPU's terms within LIBRARY_ULPS * 2^-24 * (|term| + 1) of the CPU's
-    assert abs(got[Fold.sum] - cpu) <= (1024 + 2 * LIBRARY_ULPS) * 2 ** -24 * 2 * n
+    sines = oracle.map_f32(Code.sin, ah, threads=16)
+    cosines = oracle.map_f32(Code.cos, ch, threads=16)
+    tanh_cpu = oracle.map_f32(Code.tanh, bh, threads=16)
+    tanh_ulps = int(ulps32(tanh_gpu, tanh_cpu).max())
+    print(f"[test] tanh over 2^28 inputs in [-4, 4]: max {tanh_ulps} ulp between the GPU and glibc", flush=True)
+    assert tanh_ulps <= LIBRARY_ULPS
+    products = sines * cosines  # float32 multiply, correctly rounded, as the chunk's multiply
+    assert (terms.view(np.uint32) == (products + tanh_gpu).view(np.uint32)).all()
+    wide = terms.astype(np.float64)
+    bound = n * 2 ** -53 * float(np.abs(wide).sum())
+    exact = exact_sum(wide)
+    print(f"[test] sum {got[Fold.sum]!r} exact {exact!r} bound {bound:.3g}", flush=True)
+    assert abs(got[Fold.sum] - exact) <= bound and bound < 1e-2 * abs(exact)
+    assert got[Fold.minimum] == float(terms.min()) and got[Fold.maximum] == float(terms.max())
     assert sci.mean(expr) == np.float32(np.float32(got[Fold.sum]) / np.float32(n))
+    # no library code: sin(a)*cos(c) summed as exact products of the oracle's own sines and cosines
+    exact_products = sines.astype(np.float64) * cosines.astype(np.float64)
+    bound = n * 2 ** -53 * float(np.abs(exact_products).sum())
+    exact = exact_sum(exact_products)
+    print(f"[test] dot {dot!r} exact {exact!r} bound {bound:.3g}", flush=True)
+    assert abs(dot - exact) <= bound and bound < 1e-2 * abs(exact)
+    assert product_extremes == [float(products.min()), float(products.max())]
