@@ -145,6 +145,20 @@ int scihip_execute(scihip_device *device, const scihip_chunk *chunk, void *outpu
  * (include/core/bytecode/chunk.hpp:151-154). */
 int scihip_reduce(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t count, double *result);
 
+/* scihip_reduce with the result left in device memory, for callers that cannot allocate: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. No counterpart in
+ * the reference. It runs the same kernels as scihip_reduce but calls no hipMalloc, hipFree or other
+ * allocating routine, never compiles (a specialised kernel runs only when one is already loaded,
+ * e.g. by scihip_prepare; otherwise the interpreter runs), never waits and copies nothing to the
+ * host: `result` is device memory of this device (8 bytes) and receives the value asynchronously.
+ * The per-workgroup partials go to a static scratch array of 4096 partials (64 KiB) on each GPU, held
+ * until the process ends; a launch that would need more partials is refused with SCIHIP_EINVAL
+ * (MI355X launches at most 4 per compute unit, 1024). The scratch keeps the last call's contents and
+ * nothing in it needs to start at zero. Every call on a GPU shares that GPU's scratch: calls on one
+ * device are ordered on its stream, but calls through two devices opened on one GPU race. */
+int scihip_reduce_static(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t count,
+                         double *result);
+
 /* Product modes. */
 enum scihip_product_mode {
     SCIHIP_PRODUCT_FLOAT32 = 0,  /* float operands and result, fp32 matrix cores */

@@ -33,6 +33,7 @@ SIGNATURES = {
     "scihip_copy": (_i, [_vp, _vp, _vp, _sz]),
     "scihip_execute": (_i, [_vp, _vp, _vp, _sz]),
     "scihip_reduce": (_i, [_vp, _vp, _u32, _sz, ctypes.POINTER(_d)]),
+    "scihip_reduce_static": (_i, [_vp, _vp, _u32, _sz, _vp]),
     "scihip_set_policy": (_i, [_vp, _i]),
     "scihip_prepare": (_i, [_vp, _vp]),
     "scihip_statistics": (_i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),

@@ -297,6 +297,13 @@ namespace scihip {
         return text;
     }
 
+    const Kernel *Compiler::loaded(const std::string &key) {
+        std::lock_guard<std::mutex> lock(guard);
+        const auto found = kernels.find(key);
+        if (found == kernels.end() || !ready[key] || found->second->failed) return nullptr;
+        return found->second.get();
+    }
+
     void Compiler::build(const std::string &key, const std::string &source, Kernel &kernel) {
         (void)hipSetDevice(ordinal);
         // Every generated source names its kernel scihip_specialised; the macro renames it.

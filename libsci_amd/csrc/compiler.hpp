@@ -38,6 +38,9 @@ namespace scihip {
         // compiles in the foreground instead.
         const Kernel *find(const std::string &key, const std::function<std::string()> &generate, bool wait);
 
+        // The kernel for `key` when it is compiled and loaded, else null. Never queues a compile.
+        const Kernel *loaded(const std::string &key);
+
         // The symbol a key's kernel is compiled under: chunk_<kind>_<hash of the key>.
         static std::string symbol(const std::string &key);
 

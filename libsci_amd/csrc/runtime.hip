@@ -15,6 +15,15 @@
 #include <cstring>
 #include <limits>
 
+// The reduction partials of scihip_reduce_static, the one entry point that allocates nothing: one
+// Partial per workgroup of its launch, folded into the caller's result by a second kernel. It is
+// the only device memory that entry point uses beyond the caller's inputs and result. Each GPU holds
+// its own copy (64 KiB) until the process ends; host code reaches it only through the address that
+// scihip_open takes with hipGetSymbolAddress (scihip_device::static_partials), never by this name.
+// Every call on a GPU shares it, so such calls on two devices opened on the same GPU (two streams)
+// race; calls on one device are ordered on its stream.
+static __device__ __align__(256) scihip::Partial scratch[scihip::kStaticPartials];
+
 namespace scihip {
 
     thread_local std::string last_error;
@@ -256,26 +265,43 @@ namespace scihip {
         return key;
     }
 
-    // Runs a chunk: its specialised kernel when ready (or forced), else the interpreter.
+    // Where a reduction leaves its partials and its result, and whether the call may compile.
+    struct Sink {
+        double *result;      // device memory for the folded result
+        unsigned capacity;   // partials arguments.partials holds
+        bool compile;        // false: only a kernel already loaded runs (loading a module allocates)
+    };
+
+    // Runs a chunk: its specialised kernel when ready (or forced), else the interpreter. A reduction
+    // folds its partials into sink.result.
     template <typename T>
     int launch(scihip_device *device, const scihip_chunk &chunk, Launch<T> &arguments, int registers, int mode,
-               bool contract) {
+               bool contract, const Sink &sink) {
         const size_t count = arguments.p.count;
         const Kernel *kernel = nullptr;
         if (device->policy != 1) {
             const uint32_t fold = mode == 1 && !contract ? arguments.p.fold : 0;
             const std::string key = key_of(chunk, mode, contract, fold);
-            kernel = device->compiler->find(key, [&] { return Compiler::generate(chunk, mode, contract, fold); },
-                                            device->policy == 2);
-            if (!kernel && device->policy == 2) {
-                fail("specialised kernel failed to compile");
-                return SCIHIP_ECOMPILE;
+            if (sink.compile) {
+                kernel = device->compiler->find(key, [&] { return Compiler::generate(chunk, mode, contract, fold); },
+                                                device->policy == 2);
+                if (!kernel && device->policy == 2) {
+                    fail("specialised kernel failed to compile");
+                    return SCIHIP_ECOMPILE;
+                }
+            } else {
+                kernel = device->compiler->loaded(key);
             }
         }
-        unsigned blocks = 0;
+        const unsigned blocks = kernel ? blocks_for(device, count, Traits<T>::width, Compiler::kUnroll, mode)
+                                       : blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width, 1, mode);
+        if (mode == 1 && blocks > sink.capacity) {
+            fail("the reduction needs " + std::to_string(blocks) + " partials; its buffer holds " +
+                 std::to_string(sink.capacity));
+            return SCIHIP_EINVAL;
+        }
         int rc = SCIHIP_OK;
         if (kernel) {
-            blocks = blocks_for(device, count, Traits<T>::width, Compiler::kUnroll, mode);
             void *parameters[] = {&arguments}; // copied at launch (and into a graph node under capture)
             ++device->specialised;
             rc = status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream, parameters,
@@ -283,11 +309,10 @@ namespace scihip {
                         "specialised launch");
         } else {
             ++device->interpreted;
-            blocks = blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width, 1, mode);
             rc = status(interpret(arguments, registers, mode, blocks, device->stream), "interpreter launch");
         }
         if (rc || mode != 1) return rc;
-        return status(fold_partials(arguments.p.fold, arguments.partials, blocks, device->result, device->stream),
+        return status(fold_partials(arguments.p.fold, arguments.partials, blocks, sink.result, device->stream),
                       "reduction fold");
     }
 
@@ -305,7 +330,7 @@ namespace scihip {
             fail("a stored chunk ends in its output");
             return SCIHIP_EINVAL;
         }
-        return launch<T>(device, chunk, arguments, registers, 0, false);
+        return launch<T>(device, chunk, arguments, registers, 0, false, Sink{nullptr, 0, true});
     }
 
     template <typename T>
@@ -314,7 +339,9 @@ namespace scihip {
         int registers = 0;
         if (const int rc = encode<T>(device, chunk, nullptr, count, fold, true, arguments, registers)) return rc;
         const bool contract = contracted(chunk, fold);
-        if (const int rc = launch<T>(device, chunk, arguments, registers, 1, contract)) return rc;
+        if (const int rc = launch<T>(device, chunk, arguments, registers, 1, contract,
+                                     Sink{device->result, kMaxBlocks, true}))
+            return rc;
         if (const int rc = status(hipMemcpyAsync(device->staging, device->result, sizeof(double),
                                                  hipMemcpyDeviceToHost, device->stream),
                                   "result copy"))
@@ -322,6 +349,18 @@ namespace scihip {
         if (const int rc = status(hipStreamSynchronize(device->stream), "reduce")) return rc;
         *result = *device->staging;
         return SCIHIP_OK;
+    }
+
+    // scihip_reduce's kernels with the partials in the static scratch and the result left in device
+    // memory: no allocation, no compile, no wait.
+    template <typename T>
+    int reduce_static(scihip_device *device, const scihip_chunk &chunk, uint32_t fold, size_t count, double *result) {
+        Launch<T> arguments;
+        int registers = 0;
+        if (const int rc = encode<T>(device, chunk, nullptr, count, fold, true, arguments, registers)) return rc;
+        arguments.partials = device->static_partials;
+        return launch<T>(device, chunk, arguments, registers, 1, contracted(chunk, fold),
+                         Sink{result, kStaticPartials, false});
     }
 
 
@@ -540,10 +579,10 @@ namespace scihip {
     int factor_on_device(scihip_device *device, T *matrix, int64_t order, int mode, std::vector<int64_t> &pivots) {
         int rc = ensure_lu(device, order, sizeof(T));
         Scratch pivot_block(device, size_t(order) * sizeof(int64_t), rc);
-        Scratch scratch(device, factor_scratch(order, mode), rc);
+        Scratch work(device, factor_scratch(order, mode), rc);
         if (rc) return rc;
         const char *message = nullptr;
-        const int e = factor<T>(matrix, order, pivot_block.as<int64_t>(), mode, device->lu, scratch.pointer,
+        const int e = factor<T>(matrix, order, pivot_block.as<int64_t>(), mode, device->lu, work.pointer,
                                 factor_scratch(order, mode), device->stream, &message);
         if (e) {
             fail(message ? message : hipGetErrorString(hipError_t(e)));
@@ -601,6 +640,10 @@ int scihip_open(uint32_t ordinal, scihip_device **out) {
     if (int rc = status(hipMalloc(reinterpret_cast<void **>(&device->result), 256), "hipMalloc")) return rc;
     if (int rc = status(hipHostMalloc(reinterpret_cast<void **>(&device->staging), sizeof(double)), "hipHostMalloc"))
         return rc;
+    // This GPU's copy of the static scratch (hipSetDevice above made it current).
+    void *symbol = nullptr;
+    if (int rc = status(hipGetSymbolAddress(&symbol, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return rc;
+    device->static_partials = static_cast<Partial *>(symbol);
     if (const char *text = std::getenv("SCIHIP_JIT")) {
         if (std::strcmp(text, "0") == 0) device->policy = 1;
         if (std::strcmp(text, "force") == 0) device->policy = 2;
@@ -714,6 +757,27 @@ int scihip_reduce(scihip_device *device, const scihip_chunk *chunk, uint32_t fol
     if (int rc = refuse_while_capturing(device, "scihip_reduce")) return rc;
     return chunk->precision == SCIHIP_SINGLE ? reduce<float>(device, *chunk, fold, count, result)
                                              : reduce<double>(device, *chunk, fold, count, result);
+}
+
+int scihip_reduce_static(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t count,
+                         double *result) {
+    if (!device || !chunk || !result || fold > SCIHIP_FOLD_MAXIMUM || chunk->precision > SCIHIP_WIDE)
+        return SCIHIP_EINVAL;
+    std::lock_guard<std::mutex> lock(device->guard);
+    if (!device->allocator->locate(result, sizeof(double))) {
+        fail("the result of scihip_reduce_static is not device memory of this device");
+        return SCIHIP_DECLINED;
+    }
+    if (!device->static_partials) {
+        fail("the static reduction scratch has no device address");
+        return SCIHIP_EDEVICE;
+    }
+    if (count == 0) { // device.cpp:266-268, written by a fill kernel (no memset: callers may capture this)
+        const double empty = fold == SCIHIP_FOLD_SUM ? 0.0 : std::numeric_limits<double>::quiet_NaN();
+        return status(fill_value<double>(result, 1, empty, device->stream), "fill");
+    }
+    return chunk->precision == SCIHIP_SINGLE ? reduce_static<float>(device, *chunk, fold, count, result)
+                                             : reduce_static<double>(device, *chunk, fold, count, result);
 }
 
 int scihip_set_policy(scihip_device *device, int policy) {

@@ -26,7 +26,8 @@ namespace scihip {
     bool wellformed(const scihip_chunk &chunk);
     bool contracted(const scihip_chunk &chunk, uint32_t fold);
 
-    constexpr unsigned kMaxBlocks = 16384; // reduction partials buffer capacity
+    constexpr unsigned kMaxBlocks = 16384;     // reduction partials buffer capacity
+    constexpr unsigned kStaticPartials = 4096; // partials of scihip_reduce_static's static scratch
 
     // Device memory with the reference allocator's contract (src/core/drivers/vulkan/allocator.cpp:16-136):
     // blocks are cached and reused when one at most 25% larger fits; `locate` finds the block holding
@@ -74,7 +75,8 @@ namespace scihip {
                  const void *right, int64_t right_ld, bool right_transposed, void *output, int64_t rows,
                  int64_t columns, int64_t depth);
 
-    struct Kernel; // a specialised, loaded kernel
+    struct Kernel;  // a specialised, loaded kernel
+    struct Partial; // a reduction partial (interpreter.hpp)
 
 } // namespace scihip
 
@@ -87,6 +89,7 @@ struct scihip_device {
     std::unique_ptr<scihip::Compiler> compiler;
     int units = 0;              // compute units
     void *partials = nullptr;   // reduction partials, one per workgroup
+    scihip::Partial *static_partials = nullptr; // this GPU's static scratch (scihip_reduce_static)
     double *result = nullptr;   // reduction result on the device
     double *staging = nullptr;  // pinned host copy of it
     int policy = 0;             // 0 automatic, 1 interpreter only, 2 specialised only
