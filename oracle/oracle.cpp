@@ -35,8 +35,11 @@
 #include <bit>
 #include <cmath>
 #include <cstdint>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <limits>
+#include <mutex>
 #include <math.h>
 #include <numbers>
 #include <thread>
@@ -299,62 +302,338 @@ namespace {
         }
     }
 
+    // ---------------------------------------------------------------- packs (elementwise.hpp:14-83)
+
+    // The reference runs each code over a block a pack at a time: every full pack through the pack's
+    // operations, the ragged tail element by element through the one-lane pack
+    // (include/core/drivers/processor/kernels/generic/elementwise.hpp:14-83). Here a pack is 64 bytes
+    // (16 floats or 8 doubles, the AVX-512 width) as a clang vector, and the tail runs apply().
+    // A lane's result never depends on its pack: each lane takes the same IEEE operations in the same
+    // order as apply() does, and the two pack-wide branches give every lane the value it would get
+    // alone (the exponential's split scaling multiplies by powers of two whose product is the single
+    // scale, rounding once where the single scale rounds; the trigonometric functions send exactly
+    // the lanes beyond the Cody-Waite reach to the C library). So packs change the speed, not a bit.
+    template <typename T> struct Lanes {
+        static constexpr size_t width = 64 / sizeof(T);
+        typedef T V __attribute__((ext_vector_type(64 / sizeof(T))));
+        typedef Bits<T> B __attribute__((ext_vector_type(64 / sizeof(T))));
+    };
+    template <typename T> using PV = typename Lanes<T>::V;
+    template <typename T> using PB = typename Lanes<T>::B;
+
+#define ORACLE_INLINE inline __attribute__((always_inline))
+
+    template <typename T> ORACLE_INLINE PV<T> vload(const T *p) {
+        PV<T> v;
+        std::memcpy(&v, p, sizeof(v));
+        return v;
+    }
+    template <typename T> ORACLE_INLINE void vstore(T *p, PV<T> v) { std::memcpy(p, &v, sizeof(v)); }
+    template <typename T> ORACLE_INLINE PV<T> vfma(PV<T> a, PV<T> b, PV<T> c) {
+        return __builtin_elementwise_fma(a, b, c);
+    }
+    template <typename T> ORACLE_INLINE PB<T> vraw(PV<T> v) { return __builtin_bit_cast(PB<T>, v); }
+    template <typename T> ORACLE_INLINE PV<T> vcast(PB<T> v) { return __builtin_bit_cast(PV<T>, v); }
+    template <typename T, size_t N, size_t from> ORACLE_INLINE PV<T> vhorner(const Poly<T, N> &p, PV<T> x) {
+        PV<T> r = p.c[N - 1];
+        for (size_t i = N - 1; i-- > from;) r = vfma<T>(r, x, PV<T>(p.c[i]));
+        return r;
+    }
+    // The vector minimum / maximum with a constant: a NaN first operand stays (exponential's clamp).
+    template <typename T> ORACLE_INLINE PV<T> vlesser_vector(PV<T> l, T r) { return l != l ? l : (l < r ? l : PV<T>(r)); }
+    template <typename T> ORACLE_INLINE PV<T> vgreater_vector(PV<T> l, T r) { return l != l ? l : (l > r ? l : PV<T>(r)); }
+
+    template <typename T> ORACLE_INLINE PV<T> vexponential(PV<T> x) { // exponential.hpp:24-49
+        using L = K<T>;
+        using V = PV<T>;
+        static constexpr auto series = Poly<T, L::exponential + 1>::factorials();
+        const V c = vgreater_vector<T>(vlesser_vector<T>(x, L::upper), L::lower);
+        const V k = vfma<T>(c, V(L::binary), V(L::round)) - L::round;
+        const V negative = -k;
+        V r = vfma<T>(negative, V(L::natural), c);
+        r = vfma<T>(negative, V(L::remainder), r);
+        const V significand = vfma<T>(r, vhorner<T, L::exponential + 1, 1>(series, r), V(T{1}));
+        auto scale = [](V e) { return vcast<T>(vraw<T>(e + L::exponent) << L::mantissa); };
+        const V magnitude = vcast<T>(vraw<T>(k) & PB<T>(~(Bits<T>{1} << (sizeof(T) * 8 - 1))));
+        if (!__builtin_reduce_or(magnitude > L::normal)) return significand * scale(k);
+        const V half = vfma<T>(k, V(T{0.5}), V(L::round)) - L::round;
+        const V rest = k - half;
+        return (significand * scale(half)) * scale(rest);
+    }
+
+    template <typename T> ORACLE_INLINE PV<T> vlogarithm(PV<T> x) { // logarithm.hpp:26-61
+        using L = K<T>;
+        using V = PV<T>;
+        using B = PB<T>;
+        static constexpr auto series = Poly<T, L::logarithm>::reciprocals();
+        const auto small = x < L::tiny;
+        const V scaled = small ? x * L::lift : x;
+        const B pattern = vraw<T>(scaled);
+        V exponent = vcast<T>((pattern >> L::mantissa) | raw<T>(L::magnitude)) - L::exponent;
+        V mantissa = vcast<T>((pattern & L::fraction) | L::one);
+        const auto large = mantissa > L::root;
+        mantissa = large ? mantissa * T{0.5} : mantissa;
+        exponent = large ? exponent + T{1} : exponent;
+        exponent = small ? exponent - L::shift : exponent;
+        const V f = mantissa - T{1};
+        const V s = f / (f + T{2});
+        const V z = s * s;
+        const V tail = z * vhorner<T, L::logarithm, 0>(series, z);
+        const V half = (f * f) * T{0.5};
+        const V inner = vfma<T>(s, half + tail, exponent * L::remainder);
+        V result = exponent * L::natural - ((half - inner) - f);
+        constexpr T infinity = std::numeric_limits<T>::infinity();
+        result = x == infinity ? x : result;
+        result = x == T{0} ? V(-infinity) : result;
+        result = x < T{0} ? V(std::numeric_limits<T>::quiet_NaN()) : result;
+        return x != x ? x : result;
+    }
+
+    template <typename T, bool phase> ORACLE_INLINE PV<T> vtrigonometry(PV<T> x) { // trigonometry.hpp:29-74
+        using L = K<T>;
+        using V = PV<T>;
+        using B = PB<T>;
+        static constexpr auto odd = Poly<T, L::sine + 1>::template alternating<1>();
+        static constexpr auto even = Poly<T, L::cosine + 1>::template alternating<0>();
+        const V shifted = vfma<T>(x, V(L::quadrant), V(L::round));
+        const V quarters = shifted - L::round;
+        const V negative = -quarters;
+        V r = vfma<T>(negative, V(L::quarter[0]), x);
+        r = vfma<T>(negative, V(L::quarter[1]), r);
+        r = vfma<T>(negative, V(L::quarter[2]), r);
+        r = vfma<T>(negative, V(L::quarter[3]), r);
+        const V square = r * r;
+        const V sine = vfma<T>(r * square, vhorner<T, L::sine + 1, 1>(odd, square), r);
+        const V cosine = vfma<T>(square, vhorner<T, L::cosine + 1, 1>(even, square), V(T{1}));
+        B quadrant = vraw<T>(shifted);
+        if constexpr (phase) quadrant = quadrant ^ (quadrant << 1);
+        const B flip = (quadrant & Bits<T>{2}) << (L::width - 2);
+        const auto swap = (vraw<T>(shifted) & Bits<T>{1}) != Bits<T>{0};
+        V result = phase ? (swap ? sine : cosine) : (swap ? cosine : sine);
+        result = vcast<T>(vraw<T>(result) ^ flip);
+        const auto inside = (x <= L::reach) & (x >= -L::reach);
+        if (__builtin_reduce_and(inside) == 0) { // some lane beyond the reach, infinite or NaN
+            for (size_t l = 0; l < Lanes<T>::width; ++l)
+                if (!inside[l]) result[l] = trigonometry<T, phase>(x[l]);
+        }
+        return result;
+    }
+
+    template <typename T> ORACLE_INLINE PV<T> vlibrary(int code, PV<T> x) { // library.hpp:22-34
+        PV<T> out;
+        for (size_t l = 0; l < Lanes<T>::width; ++l) out[l] = library<T>(code, x[l]);
+        return out;
+    }
+
+    // One code over one full pack (operation.hpp:25-110 on the AVX-512 pack).
+    template <typename T> ORACLE_INLINE PV<T> vapply(int code, PV<T> a, PV<T> b, PV<T> c) {
+        using V = PV<T>;
+        constexpr Bits<T> sign = Bits<T>{1} << (sizeof(T) * 8 - 1);
+        const V one = T{1};
+        switch (code) {
+        case SCIHIP_COPY: return a;
+        case SCIHIP_NEGATE: return vcast<T>(vraw<T>(a) ^ sign);
+        case SCIHIP_ABSOLUTE: return vcast<T>(vraw<T>(a) & ~sign);
+        case SCIHIP_ROOT: return __builtin_elementwise_sqrt(a);
+        case SCIHIP_EXPONENTIAL: return vexponential<T>(a);
+        case SCIHIP_LOGARITHM: return vlogarithm<T>(a);
+        case SCIHIP_SIN: return vtrigonometry<T, false>(a);
+        case SCIHIP_COS: return vtrigonometry<T, true>(a);
+        case SCIHIP_TAN: return vtrigonometry<T, false>(a) / vtrigonometry<T, true>(a);
+        case SCIHIP_ASIN: case SCIHIP_ACOS: case SCIHIP_ATAN:
+        case SCIHIP_SINH: case SCIHIP_COSH: case SCIHIP_TANH:
+        case SCIHIP_ASINH: case SCIHIP_ACOSH: case SCIHIP_ATANH:
+            return vlibrary<T>(code, a);
+        case SCIHIP_CSC: return one / vtrigonometry<T, false>(a);
+        case SCIHIP_SEC: return one / vtrigonometry<T, true>(a);
+        case SCIHIP_COT: return one / (vtrigonometry<T, false>(a) / vtrigonometry<T, true>(a));
+        case SCIHIP_ACSC: return vlibrary<T>(SCIHIP_ASIN, one / a);
+        case SCIHIP_ASEC: return vlibrary<T>(SCIHIP_ACOS, one / a);
+        case SCIHIP_ACOT: return vlibrary<T>(SCIHIP_ATAN, one / a);
+        case SCIHIP_CSCH: return one / vlibrary<T>(SCIHIP_SINH, a);
+        case SCIHIP_SECH: return one / vlibrary<T>(SCIHIP_COSH, a);
+        case SCIHIP_COTH: return one / vlibrary<T>(SCIHIP_TANH, a);
+        case SCIHIP_ACSCH: return vlibrary<T>(SCIHIP_ASINH, one / a);
+        case SCIHIP_ASECH: return vlibrary<T>(SCIHIP_ACOSH, one / a);
+        case SCIHIP_ACOTH: return vlibrary<T>(SCIHIP_ATANH, one / a);
+        case SCIHIP_ADD: return a + b;
+        case SCIHIP_SUBTRACT: return a - b;
+        case SCIHIP_MULTIPLY: return a * b;
+        case SCIHIP_DIVIDE: return a / b;
+        case SCIHIP_MINIMUM: return ((a != a) | (b != b)) ? a + b : (b < a ? b : a);
+        case SCIHIP_MAXIMUM: return ((a != a) | (b != b)) ? a + b : (b > a ? b : a);
+        default: return vfma<T>(a, b, c);
+        }
+    }
+
+    // One code over n elements of a block: full packs, then the tail element by element. Each case
+    // is its own loop, so the code's pack body is straight-line inside it.
+    template <typename T, int C>
+    ORACLE_INLINE void vsweep(const T *a, const T *b, const T *c, T *t, size_t n) {
+        constexpr size_t W = Lanes<T>::width;
+        size_t e = 0;
+        for (; e + W <= n; e += W) {
+            const PV<T> x = vload(a + e);
+            const PV<T> y = b ? vload(b + e) : PV<T>(T{});
+            const PV<T> z = c ? vload(c + e) : PV<T>(T{});
+            vstore(t + e, vapply<T>(C, x, y, z));
+        }
+        for (; e < n; ++e) t[e] = apply<T>(C, a[e], b ? b[e] : T{}, c ? c[e] : T{});
+    }
+
+    template <typename T> ORACLE_INLINE void run_code_impl(int code, const T *a, const T *b, const T *c, T *t, size_t n) {
+        switch (code) {
+#define ORACLE_CASE(k) case k: vsweep<T, k>(a, b, c, t, n); return;
+            ORACLE_CASE(0) ORACLE_CASE(1) ORACLE_CASE(2) ORACLE_CASE(3) ORACLE_CASE(4) ORACLE_CASE(5)
+            ORACLE_CASE(6) ORACLE_CASE(7) ORACLE_CASE(8) ORACLE_CASE(9) ORACLE_CASE(10) ORACLE_CASE(11)
+            ORACLE_CASE(12) ORACLE_CASE(13) ORACLE_CASE(14) ORACLE_CASE(15) ORACLE_CASE(16) ORACLE_CASE(17)
+            ORACLE_CASE(18) ORACLE_CASE(19) ORACLE_CASE(20) ORACLE_CASE(21) ORACLE_CASE(22) ORACLE_CASE(23)
+            ORACLE_CASE(24) ORACLE_CASE(25) ORACLE_CASE(26) ORACLE_CASE(27) ORACLE_CASE(28) ORACLE_CASE(29)
+            ORACLE_CASE(30) ORACLE_CASE(31) ORACLE_CASE(32) ORACLE_CASE(33) ORACLE_CASE(34) ORACLE_CASE(35)
+#undef ORACLE_CASE
+        default: vsweep<T, SCIHIP_FUSE>(a, b, c, t, n); return;
+        }
+    }
+
+    // Compiled for AVX-512, for AVX2 + FMA and for baseline x86-64 (where the pack's fma becomes a
+    // call of fmaf / fma, still correctly rounded); the loader picks the one the host supports.
+    __attribute__((target_clones("avx512f", "arch=haswell", "default")))
+    void run_code_f32(int code, const float *a, const float *b, const float *c, float *t, size_t n) {
+        run_code_impl<float>(code, a, b, c, t, n);
+    }
+    __attribute__((target_clones("avx512f", "arch=haswell", "default")))
+    void run_code_f64(int code, const double *a, const double *b, const double *c, double *t, size_t n) {
+        run_code_impl<double>(code, a, b, c, t, n);
+    }
+    inline void run_code(int code, const float *a, const float *b, const float *c, float *t, size_t n) {
+        run_code_f32(code, a, b, c, t, n);
+    }
+    inline void run_code(int code, const double *a, const double *b, const double *c, double *t, size_t n) {
+        run_code_f64(code, a, b, c, t, n);
+    }
+
     // ---------------------------------------------------------------- folds (reductions.hpp:14-124)
 
-    // A pack of `width` lanes is modelled lane by lane; the vector bodies use the vector min / max.
-    template <typename T> T fold_sum(const T *src, size_t count, size_t width) {
-        std::vector<T> acc(4 * width, T{0});
-        size_t i = 0;
-        for (; i + 4 * width <= count; i += 4 * width)
-            for (size_t j = 0; j < 4; ++j)
-                for (size_t l = 0; l < width; ++l) acc[j * width + l] = acc[j * width + l] + src[i + j * width + l];
-        for (; i + width <= count; i += width)
-            for (size_t l = 0; l < width; ++l) acc[l] = acc[l] + src[i + l];
+    // A pack of W lanes is modelled lane by lane (W a compile-time width, so the lane loops are
+    // vector code); the vector bodies use the vector min / max.
+    // The folds at the full 64-byte width, four pack accumulators in registers; the same lane
+    // order as the array forms below (accumulator j, lane l takes element 4Wk + jW + l).
+    template <typename T> ORACLE_INLINE T fold_lanes(PV<T> a0, PV<T> a1, PV<T> a2, PV<T> a3) {
         T result{0};
-        for (size_t l = 0; l < width; ++l)
-            result += (acc[l] + acc[width + l]) + (acc[2 * width + l] + acc[3 * width + l]);
+        for (size_t l = 0; l < Lanes<T>::width; ++l) result += (a0[l] + a1[l]) + (a2[l] + a3[l]);
+        return result;
+    }
+
+    template <typename T> ORACLE_INLINE T fold_sum_packs(const T *src, size_t count) {
+        constexpr size_t W = Lanes<T>::width;
+        PV<T> a0 = T{0}, a1 = T{0}, a2 = T{0}, a3 = T{0};
+        size_t i = 0;
+        for (; i + 4 * W <= count; i += 4 * W) {
+            a0 = a0 + vload(src + i);
+            a1 = a1 + vload(src + i + W);
+            a2 = a2 + vload(src + i + 2 * W);
+            a3 = a3 + vload(src + i + 3 * W);
+        }
+        for (; i + W <= count; i += W) a0 = a0 + vload(src + i);
+        T result = fold_lanes<T>(a0, a1, a2, a3);
         for (; i < count; ++i) result += src[i];
         return result;
     }
 
-    template <typename T> T fold_dot(const T *a, const T *b, size_t count, size_t width) {
-        std::vector<T> acc(4 * width, T{0});
+    template <typename T> ORACLE_INLINE T fold_dot_packs(const T *a, const T *b, size_t count) {
+        constexpr size_t W = Lanes<T>::width;
+        PV<T> a0 = T{0}, a1 = T{0}, a2 = T{0}, a3 = T{0};
         size_t i = 0;
-        for (; i + 4 * width <= count; i += 4 * width)
-            for (size_t j = 0; j < 4; ++j)
-                for (size_t l = 0; l < width; ++l) {
-                    const size_t at = i + j * width + l;
-                    acc[j * width + l] = std::fma(a[at], b[at], acc[j * width + l]);
-                }
-        for (; i + width <= count; i += width)
-            for (size_t l = 0; l < width; ++l) acc[l] = std::fma(a[i + l], b[i + l], acc[l]);
-        T result{0};
-        for (size_t l = 0; l < width; ++l)
-            result += (acc[l] + acc[width + l]) + (acc[2 * width + l] + acc[3 * width + l]);
+        for (; i + 4 * W <= count; i += 4 * W) {
+            a0 = vfma<T>(vload(a + i), vload(b + i), a0);
+            a1 = vfma<T>(vload(a + i + W), vload(b + i + W), a1);
+            a2 = vfma<T>(vload(a + i + 2 * W), vload(b + i + 2 * W), a2);
+            a3 = vfma<T>(vload(a + i + 3 * W), vload(b + i + 3 * W), a3);
+        }
+        for (; i + W <= count; i += W) a0 = vfma<T>(vload(a + i), vload(b + i), a0);
+        T result = fold_lanes<T>(a0, a1, a2, a3);
         for (; i < count; ++i) result = std::fma(a[i], b[i], result);
         return result;
     }
 
-    template <typename T> T fold_extreme(const T *src, size_t count, size_t width, bool minimum) {
+    template <typename T, size_t W> ORACLE_INLINE T fold_sum(const T *src, size_t count) {
+        if constexpr (W == Lanes<T>::width) return fold_sum_packs<T>(src, count);
+        T acc[4 * W] = {};
+        size_t i = 0;
+        for (; i + 4 * W <= count; i += 4 * W)
+            for (size_t j = 0; j < 4 * W; ++j) acc[j] = acc[j] + src[i + j];
+        for (; i + W <= count; i += W)
+            for (size_t l = 0; l < W; ++l) acc[l] = acc[l] + src[i + l];
+        T result{0};
+        for (size_t l = 0; l < W; ++l) result += (acc[l] + acc[W + l]) + (acc[2 * W + l] + acc[3 * W + l]);
+        for (; i < count; ++i) result += src[i];
+        return result;
+    }
+
+    template <typename T, size_t W> ORACLE_INLINE T fold_dot(const T *a, const T *b, size_t count) {
+        if constexpr (W == Lanes<T>::width) return fold_dot_packs<T>(a, b, count);
+        T acc[4 * W] = {};
+        size_t i = 0;
+        for (; i + 4 * W <= count; i += 4 * W)
+            for (size_t j = 0; j < 4 * W; ++j) acc[j] = std::fma(a[i + j], b[i + j], acc[j]);
+        for (; i + W <= count; i += W)
+            for (size_t l = 0; l < W; ++l) acc[l] = std::fma(a[i + l], b[i + l], acc[l]);
+        T result{0};
+        for (size_t l = 0; l < W; ++l) result += (acc[l] + acc[W + l]) + (acc[2 * W + l] + acc[3 * W + l]);
+        for (; i < count; ++i) result = std::fma(a[i], b[i], result);
+        return result;
+    }
+
+    template <typename T, size_t W> ORACLE_INLINE T fold_extreme(const T *src, size_t count, bool minimum) {
         auto vec = [&](T l, T r) { return minimum ? lesser_vector(l, r) : greater_vector(l, r); };
         auto one = [&](T l, T r) { return minimum ? lesser(l, r) : greater(l, r); };
         // A one-lane pack is the Scalar pack: its "vector" operation is the scalar one.
-        auto body = [&](T l, T r) { return width == 1 ? one(l, r) : vec(l, r); };
+        auto body = [&](T l, T r) { return W == 1 ? one(l, r) : vec(l, r); };
         T result = src[0];
         size_t i = 0;
-        if (count >= 4 * width) {
-            std::vector<T> acc(src, src + 4 * width);
-            for (i = 4 * width; i + 4 * width <= count; i += 4 * width)
-                for (size_t j = 0; j < 4; ++j)
-                    for (size_t l = 0; l < width; ++l)
-                        acc[j * width + l] = body(acc[j * width + l], src[i + j * width + l]);
-            for (size_t l = 0; l < width; ++l) {
-                const T first = body(body(acc[l], acc[width + l]), body(acc[2 * width + l], acc[3 * width + l]));
+        if (count >= 4 * W) {
+            T acc[4 * W];
+            for (size_t j = 0; j < 4 * W; ++j) acc[j] = src[j];
+            for (i = 4 * W; i + 4 * W <= count; i += 4 * W)
+                for (size_t j = 0; j < 4 * W; ++j) acc[j] = body(acc[j], src[i + j]);
+            for (size_t l = 0; l < W; ++l) {
+                const T first = body(body(acc[l], acc[W + l]), body(acc[2 * W + l], acc[3 * W + l]));
                 result = one(result, first);
             }
         }
         for (; i < count; ++i) result = one(result, src[i]);
         return result;
+    }
+
+    // The fold of one block (reductions.hpp:14-124) at the pack width of the chosen table.
+    template <typename T, size_t W>
+    ORACLE_INLINE T fold_block(int fold, bool contract, const T *a, const T *b, size_t n) {
+        if (contract) return fold_dot<T, W>(a, b, n);
+        if (fold == SCIHIP_FOLD_SUM) return fold_sum<T, W>(a, n);
+        return fold_extreme<T, W>(a, n, fold == SCIHIP_FOLD_MINIMUM);
+    }
+    template <typename T>
+    ORACLE_INLINE T fold_block_impl(int fold, bool contract, const T *a, const T *b, size_t n, size_t width) {
+        switch (width) {
+        case 16: return fold_block<T, 16>(fold, contract, a, b, n);
+        case 8: return fold_block<T, 8>(fold, contract, a, b, n);
+        case 4: return fold_block<T, 4>(fold, contract, a, b, n);
+        default: return fold_block<T, 1>(fold, contract, a, b, n);
+        }
+    }
+    __attribute__((target_clones("avx512f", "arch=haswell", "default")))
+    float fold_block_f32(int fold, bool contract, const float *a, const float *b, size_t n, size_t width) {
+        return fold_block_impl<float>(fold, contract, a, b, n, width);
+    }
+    __attribute__((target_clones("avx512f", "arch=haswell", "default")))
+    double fold_block_f64(int fold, bool contract, const double *a, const double *b, size_t n, size_t width) {
+        return fold_block_impl<double>(fold, contract, a, b, n, width);
+    }
+    inline float fold_block(int fold, bool contract, const float *a, const float *b, size_t n, size_t width) {
+        return fold_block_f32(fold, contract, a, b, n, width);
+    }
+    inline double fold_block(int fold, bool contract, const double *a, const double *b, size_t n, size_t width) {
+        return fold_block_f64(fold, contract, a, b, n, width);
     }
 
     // ---------------------------------------------------------------- accumulator (accumulator.hpp:21-38)
@@ -407,19 +686,78 @@ namespace {
         }
     };
 
+    // A fork-join pool on threads that live for the process, as the reference's Pool
+    // (src/core/drivers/processor/pool.cpp:44-157): the caller takes parts too, helpers claim parts
+    // from a shared counter, and the call returns when every part is done. Calls are serialised.
+    class Pool {
+    public:
+        static Pool &global() {
+            static Pool pool;
+            return pool;
+        }
+
+        void run(size_t parts, size_t threads, const std::function<void(size_t)> &work) {
+            std::lock_guard<std::mutex> call(serial);
+            const size_t helpers = std::min(threads, parts) - 1;
+            std::unique_lock<std::mutex> lock(guard);
+            while (workers.size() < helpers) workers.emplace_back([this, k = workers.size()] { loop(k); });
+            job = &work;
+            total = parts;
+            next.store(0);
+            wanted = helpers;
+            busy = helpers;
+            ++generation;
+            wake.notify_all();
+            lock.unlock();
+            for (size_t p; (p = next.fetch_add(1)) < parts;) work(p);
+            lock.lock();
+            finished.wait(lock, [&] { return busy == 0; });
+            job = nullptr;
+        }
+
+        ~Pool() {
+            {
+                std::lock_guard<std::mutex> lock(guard);
+                stopping = true;
+            }
+            wake.notify_all();
+            for (auto &t : workers) t.join();
+        }
+
+    private:
+        void loop(size_t index) {
+            uint64_t seen = 0;
+            std::unique_lock<std::mutex> lock(guard);
+            for (;;) {
+                wake.wait(lock, [&] { return stopping || (generation != seen && index < wanted); });
+                if (stopping) return;
+                seen = generation;
+                const std::function<void(size_t)> *work = job;
+                const size_t parts = total;
+                lock.unlock();
+                for (size_t p; (p = next.fetch_add(1)) < parts;) (*work)(p);
+                lock.lock();
+                if (--busy == 0) finished.notify_all();
+            }
+        }
+
+        std::mutex serial, guard;
+        std::condition_variable wake, finished;
+        std::vector<std::thread> workers;
+        const std::function<void(size_t)> *job = nullptr;
+        std::atomic<size_t> next{0};
+        size_t total = 0, wanted = 0, busy = 0;
+        uint64_t generation = 0;
+        bool stopping = false;
+    };
+
     template <typename F> void fork(size_t parts, size_t threads, F &&work) {
         if (parts <= 1 || threads <= 1) {
             for (size_t p = 0; p < parts; ++p) work(p);
             return;
         }
-        std::atomic<size_t> next{0};
-        std::vector<std::thread> pool;
-        const size_t n = std::min(threads, parts);
-        for (size_t t = 0; t < n; ++t)
-            pool.emplace_back([&] {
-                for (size_t p; (p = next.fetch_add(1)) < parts;) work(p);
-            });
-        for (auto &t : pool) t.join();
+        const std::function<void(size_t)> function = std::ref(work);
+        Pool::global().run(parts, threads, function);
     }
 
     // Runs the first `steps` instructions over [first, last) block by block and hands each block's
@@ -442,9 +780,7 @@ namespace {
             for (uint8_t s = 0; s < steps; ++s) {
                 const scihip_instruction &in = chunk.instructions[s];
                 const T *a = values[in.sources[0]], *b = values[in.sources[1]], *c = values[in.sources[2]];
-                T *t = values[in.target];
-                for (size_t e = 0; e < n; ++e)
-                    t[e] = apply<T>(in.code, a ? a[e] : T{}, b ? b[e] : T{}, c ? c[e] : T{});
+                run_code(in.code, a, b, c, values[in.target], n);
             }
             visit(values, n);
         }
@@ -474,10 +810,10 @@ namespace {
             const auto [first, last] = plan.range(part);
             Accumulator acc(fold);
             sweep<T>(chunk, static_cast<T *>(nullptr), first, last, steps, [&](T *const *values, size_t n) {
-                T v;
-                if (contract) v = fold_dot<T>(values[product->sources[0]], values[product->sources[1]], n, width);
-                else if (fold == SCIHIP_FOLD_SUM) v = fold_sum<T>(values[chunk.result], n, width);
-                else v = fold_extreme<T>(values[chunk.result], n, width, fold == SCIHIP_FOLD_MINIMUM);
+                const T v = contract ? fold_block(fold, true, values[product->sources[0]],
+                                                  values[product->sources[1]], n, width)
+                                     : fold_block(fold, false, values[chunk.result], static_cast<const T *>(nullptr),
+                                                  n, width);
                 acc.add(static_cast<double>(v));
             });
             partials[part] = acc.total;
@@ -769,7 +1105,10 @@ void oracle_map_f32(int code, const float *a, const float *b, const float *c, fl
     const size_t parts = t > 1 ? std::min<size_t>(t * 8, (n + block - 1) / block) : 1;
     fork(parts, t, [&](size_t p) {
         const size_t first = n * p / parts, last = n * (p + 1) / parts;
-        for (size_t i = first; i < last; ++i) out[i] = apply<float>(code, a[i], b ? b[i] : 0.0f, c ? c[i] : 0.0f);
+        for (size_t start = first; start < last; start += block) {
+            const size_t m = std::min(block, last - start);
+            run_code(code, a + start, b ? b + start : nullptr, c ? c + start : nullptr, out + start, m);
+        }
     });
 }
 
