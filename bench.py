@@ -5,14 +5,24 @@ Headline (`value`): the fused chunk store exp(a)*0.5 + log(|b|+1)*c over N = 2^2
 written: 16 bytes per element) and as a fraction of the 8 TB/s HBM peak. A step is one pass of the
 chunk over the whole arrays, inputs resident in HBM.
 
-Beside it in the same line: the other half of the metric, the n = 16384 LU solve with 32
-right-hand sides at fp64 accuracy (plain fp64 LU against an fp32 factorisation refined in fp64),
-the differentiation workload (fused forward + backward of sum(sinh(a*b + c) * exp(-(d*d))) at
-N = 2^26 against its forward pass, and the product adjoint at 4096^2), launch-bound chains of small
-chunks issued call by call against one graph replay, the 16384 x 16384 device transpose and axis
-folds, and the CPU baseline.
+Beside it in the same line, every workload of the measurement plan:
+- the streaming ceilings of the store's traffic mix (the same 16 B per element with one fma per
+  element; a plain copy, 8 B per element);
+- the C2 reductions sum / mean / minimum / maximum of sin(a)*cos(c) + tanh(b) and dot(a, c);
+- the matrix products 8192^3 and 4096 x 16384 x 1024 in fp32, bf16 and bf16x3, with their errors
+  against a float64 product of 64 sampled rows;
+- the n = 16384 LU solves with 32 right-hand sides at fp64 accuracy, on the well-conditioned and the
+  graded system, plain fp64 LU against the mixed factorisations refined in fp64, each with its
+  backward error and its forward error against LAPACK's float64 solve;
+- the differentiation workload (fused forward + backward of sum(sinh(a*b + c) * exp(-(d*d))) at
+  N = 2^26 against its forward pass, and the product adjoint at 4096^2);
+- launch-bound chains of small chunks issued call by call against one graph replay, the 16384 x
+  16384 device transpose and axis folds, the fp64 store, small-array latency;
+- the CPU baseline: the oracle (the reference's CPU path restated: AVX-512 packs, 1,024-element
+  blocks, a fork-join pool on every host core) on the same workloads, median of 20 runs after 3
+  warm-ups as the reference's demo times (src/main.cpp:21-22, 43-51).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--extra] [--linear]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-lu] [--no-products]
 """
 
 from __future__ import annotations
@@ -73,65 +83,187 @@ def time_launches(device, events, launch, steps: int, warmup: int):
     return wall, events.elapsed_ms(start, stop) / steps
 
 
-def cpu_baseline(sample: int, threads: int) -> dict:
-    """The oracle (a restatement of the reference's CPU path) on a bounded sample of the workload."""
+def median_time(run, repeats: int = 20, warmups: int = 3) -> float:
+    """Seconds per run: the median of `repeats` timed runs after `warmups` untimed ones, as the
+    reference's demo times its operations (src/main.cpp:21-22, 43-51)."""
+    for _ in range(warmups):
+        run()
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hosts, threads: int) -> dict:
+    """The oracle (the reference's CPU path restated: 64-byte packs in the AVX-512 table's operation
+    order, 1,024-element blocks, part split and fork-join pool, bit for bit the reference's results)
+    on the host cores, on the workloads the GPU runs. `hosts` are the C2 inputs a, b (1% NaN), c
+    and the NaN-free b as host arrays of the full size."""
     import numpy as np
 
     import oracle
-    from libsci_amd.chunk import Code, Precision, record_postfix
+    from libsci_amd.chunk import Code, Fold, Precision, record_postfix
 
-    C2_STORE = [("in", 0), ("op", Code.exponential), ("const", 0.5), ("op", Code.multiply),
-                ("in", 1), ("op", Code.absolute), ("const", 1.0), ("op", Code.add), ("op", Code.logarithm),
-                ("in", 2), ("op", Code.multiply), ("op", Code.add)]
-
-    a = oracle.uniform(sample, 1, -4.0, 4.0)
-    b = oracle.uniform(sample, 2, -4.0, 4.0, nan_seed=3, nan_fraction=0.01)
-    c = oracle.uniform(sample, 4, -4.0, 4.0)
-    chunk = record_postfix(C2_STORE, Precision.single, "output", [a.ctypes.data, b.ctypes.data, c.ctypes.data])
-    out = np.empty(sample, dtype=np.float32)
     lib = oracle.lib()
-    lib.oracle_execute(ctypes.addressof(chunk.struct), out.ctypes.data, sample, threads)
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        lib.oracle_execute(ctypes.addressof(chunk.struct), out.ctypes.data, sample, threads)
-        times.append(time.perf_counter() - t0)
-    seconds = statistics.median(times)
-    return {"value": round(16 * sample / seconds / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"the same store over N=2^{sample.bit_length() - 1} float32 elements, median of 3 passes "
-                      f"after 1 warm-up, oracle/oracle.cpp (block-wise restatement of dispatch.cpp) on "
-                      f"{threads} host threads"}
+    a, b, c, clean = hosts
+    n = a.size
+    store = [("in", 0), ("op", Code.exponential), ("const", 0.5), ("op", Code.multiply),
+             ("in", 1), ("op", Code.absolute), ("const", 1.0), ("op", Code.add), ("op", Code.logarithm),
+             ("in", 2), ("op", Code.multiply), ("op", Code.add)]
+    chunk = record_postfix(store, Precision.single, "output", [a.ctypes.data, b.ctypes.data, c.ctypes.data])
+    out = np.empty(n, dtype=np.float32)
+    seconds = median_time(lambda: lib.oracle_execute(ctypes.addressof(chunk.struct), out.ctypes.data, n, threads))
+    result = {"value": round(16 * n / seconds / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+              "sample": f"the headline store itself, N=2^{n.bit_length() - 1} float32, median of 20 passes after "
+                        f"3 warm-ups, oracle/oracle.cpp on {threads} host threads",
+              "cpu_model": cpu_model()}
+    paths = {}
+    # C1: c = a*b + 2 stored, then sum(sin(c)), 1e6 elements (the reference's CPU-runnable workload)
+    m = 1_000_000
+    ramp = oracle.linspace(0.0, 1.0, m)
+    ones = np.ones(m, dtype=np.float32)
+    c1 = np.empty(m, dtype=np.float32)
+    c1_store = record_postfix([("in", 0), ("in", 1), ("op", Code.multiply), ("const", 2.0), ("op", Code.add)],
+                              Precision.single, "output", [ramp.ctypes.data, ones.ctypes.data])
+    c1_sum = record_postfix([("in", 0), ("op", Code.sin)], Precision.single, "reduce", [c1.ctypes.data])
+    store_s = median_time(lambda: lib.oracle_execute(ctypes.addressof(c1_store.struct), c1.ctypes.data, m, threads))
+    sum_s = median_time(lambda: oracle.reduce(c1_sum, Fold.sum, m, threads=threads))
+    paths["c1"] = {"store_us": round(store_s * 1e6, 1), "sum_sin_us": round(sum_s * 1e6, 1),
+                   "total": oracle.reduce(c1_sum, Fold.sum, m, threads=threads)}
+    # C2 reduction (glibc tanh per lane dominates): a 2^24-element sample; the dot at full size
+    k = min(n, 1 << 24)
+    reduction = record_postfix([("in", 0), ("op", Code.sin), ("in", 2), ("op", Code.cos), ("op", Code.multiply),
+                                ("in", 1), ("op", Code.tanh), ("op", Code.add)], Precision.single, "reduce",
+                               [a.ctypes.data, clean.ctypes.data, c.ctypes.data])
+    seconds = median_time(lambda: oracle.reduce(reduction, Fold.sum, k, threads=threads))
+    paths["c2_reduce_sum"] = {"GBps": round(12 * k / seconds / 1e9, 2), "sample": f"first 2^{k.bit_length() - 1} "
+                                                                                   f"elements"}
+    dot = record_postfix([("in", 0), ("in", 1), ("op", Code.multiply)], Precision.single, "reduce",
+                         [a.ctypes.data, c.ctypes.data])
+    seconds = median_time(lambda: oracle.reduce(dot, Fold.sum, n, threads=threads))
+    paths["c2_dot"] = {"GBps": round(8 * n / seconds / 1e9, 2)}
+    # C3: the register-blocked fp32 product (dispatch.cpp:171-224) at 1024^3
+    q = 1024
+    left = oracle.uniform(q * q, 31, -1.0, 1.0).reshape(q, q)
+    right = oracle.uniform(q * q, 32, -1.0, 1.0).reshape(q, q)
+    seconds = median_time(lambda: oracle.product(left, right, threads=threads), repeats=5, warmups=1)
+    paths["c3_product_fp32"] = {"GFLOPs": round(2 * q ** 3 / seconds / 1e9, 1), "sample": f"{q}^3, median of 5"}
+    # C4: the reference's unblocked LU with partial pivoting (decompose.hpp:29-72), fp64, n = 2048
+    r = 2048
+    matrix = oracle.uniform(r * r, 41, -1.0, 1.0, dtype=np.float64).reshape(r, r)
+    matrix[np.diag_indices(r)] += r / 4
+    seconds = median_time(lambda: oracle.factor(matrix, threads=threads), repeats=5, warmups=1)
+    paths["c4_factor_fp64"] = {"GFLOPs": round(2 * r ** 3 / 3 / seconds / 1e9, 1), "ms": round(seconds * 1e3, 1),
+                               "sample": f"n={r}, median of 5"}
+    result["paths"] = paths
+    return result
 
 
-def measure_linear(sci, _abi, device, events) -> dict:
-    """8192^3 products in every mode (TFLOP/s, error against float64 on 64 sampled rows) and the
-    n = 16384 solves in every mode and both matrix variants."""
+def measure_ceilings(sci, _abi, device, events, arrays, out, steps: int) -> dict:
+    """What the store's traffic costs without its arithmetic, on the same arrays: the same 16 B per
+    element with one fma per element (a*c + b), and a plain copy (8 B per element)."""
+    a, b, c = arrays
+    n = a.size
+    result = {}
+    for name, expression, traffic in (("fma_store", a * c + b, 16), ("copy", sci.copy(a), 8)):
+        chunk = sci.record(expression)
+        device.prepare(expression)
+        device.set_policy(2)
+
+        def run():
+            _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), out.pointer, n))
+
+        _, ms = time_launches(device, events, run, steps, 3)
+        device.set_policy(0)
+        result[f"{name}_GBps"] = round(traffic * n / (ms * 1e-3) / 1e9, 1)
+    return result
+
+
+def measure_reductions(sci, _abi, device, arrays, clean, steps: int) -> dict:
+    """C2 reductions over N = 2^28: sum / mean / minimum / maximum of sin(a)*cos(c) + tanh(b) (12 B
+    per element), with the 1%-NaN b of the workload (every result NaN) and the NaN-free b, and
+    dot(a, c) (8 B per element). Each call waits for its result: the time includes both kernels,
+    the 8-byte copy and the host wait."""
+    from libsci_amd.chunk import Fold
+
+    a, b, c = arrays
+    n = a.size
+    result = {}
+    for label, bb in (("nan", b), ("clean", clean)):
+        expression = sci.sin(a) * sci.cos(c) + sci.tanh(bb)
+        chunk = sci.record(expression, reduce=True)
+        device.prepare(expression, reduce=True)
+        device.set_policy(2)
+        total = ctypes.c_double()
+        entry = {}
+        for fold in Fold:
+            def call():
+                _abi.check(_abi.lib().scihip_reduce(device.handle, ctypes.byref(chunk.struct), int(fold), n,
+                                                    ctypes.byref(total)))
+
+            seconds = median_time(call, steps, 2)
+            entry[fold.name] = {"value": total.value, "GBps": round(12 * n / seconds / 1e9, 1)}
+        entry["mean"] = float(sci.mean(expression))
+        device.set_policy(0)
+        result[label] = entry
+    expression = a * c
+    chunk = sci.record(expression, reduce=True)
+    device.prepare(expression, reduce=True)
+    device.set_policy(2)
+    total = ctypes.c_double()
+
+    def dot():
+        _abi.check(_abi.lib().scihip_reduce(device.handle, ctypes.byref(chunk.struct), 0, n, ctypes.byref(total)))
+
+    seconds = median_time(dot, steps, 2)
+    device.set_policy(0)
+    result["dot"] = {"value": total.value, "GBps": round(8 * n / seconds / 1e9, 1)}
+    return result
+
+
+def measure_products(sci, _abi, device, events) -> dict:
+    """C3: 8192^3 and 4096 x 16384 times 16384 x 1024, entries seeded uniform[-1, 1], in fp32, bf16
+    (fp32 accumulation) and bf16x3, each as TFLOP/s of 2MNK with its max abs and relative Frobenius
+    error against a float64 product of 64 seeded rows."""
     import numpy as np
 
     out = {}
-    n = 8192
-    left = sci.uniform(n * n, device, 31, -1.0, 1.0).reshape(n, n)
-    right = sci.uniform(n * n, device, 32, -1.0, 1.0).reshape(n, n)
-    product = sci.Array(device, (n, n), "float32")
-    rows = np.arange(0, n, n // 64)
-    exact = left.host()[rows].astype(np.float64) @ right.host().astype(np.float64)
-    for name, mode in (("fp32", 0), ("bf16", 1), ("bf16x3", 2)):
-        def run():
-            _abi.check(_abi.lib().scihip_product(device.handle, mode, left.pointer, right.pointer, product.pointer,
-                                                 n, n, n))
-        _, ms = time_launches(device, events, run, 3, 1)
-        got = product.host()[rows].astype(np.float64)
-        out[f"product_8192_{name}"] = {"TFLOPs": round(2 * n ** 3 / (ms * 1e-3) / 1e12, 1), "ms": round(ms, 2),
-                                       "max_abs_error": float(np.abs(got - exact).max()),
-                                       "relative_frobenius": float(np.linalg.norm(got - exact) / np.linalg.norm(exact))}
-    del left, right, product
-    out.update(measure_solves(sci, _abi, device, 16384))
+    for m, k, n in ((8192, 8192, 8192), (4096, 16384, 1024)):
+        left = sci.uniform(m * k, device, 31, -1.0, 1.0).reshape(m, k)
+        right = sci.uniform(k * n, device, 32, -1.0, 1.0).reshape(k, n)
+        product = sci.Array(device, (m, n), "float32")
+        rows = np.sort(np.random.default_rng(m + n).choice(m, 64, replace=False))
+        exact = left.host()[rows].astype(np.float64) @ right.host().astype(np.float64)
+        for name, mode in (("fp32", 0), ("bf16", 1), ("bf16x3", 2)):
+            def run():
+                _abi.check(_abi.lib().scihip_product(device.handle, mode, left.pointer, right.pointer,
+                                                     product.pointer, m, n, k))
+            _, ms = time_launches(device, events, run, 5, 1)
+            got = product.host()[rows].astype(np.float64)
+            out[f"{m}x{k}x{n}_{name}"] = {
+                "TFLOPs": round(2 * m * n * k / (ms * 1e-3) / 1e12, 1), "ms": round(ms, 3),
+                "max_abs_error": float(np.abs(got - exact).max()),
+                "relative_frobenius": float(np.linalg.norm(got - exact) / np.linalg.norm(exact))}
+        del left, right, product
     return out
 
 
 def solve_system(sci, device, order, width, graded):
     """A = U[-1,1] + (order/4) I, rows scaled by 10^(-6 i / order) when graded; B = U[-1,1]. Generated
-    on the device (seeded), the diagonal shift and the row scaling applied on the host."""
+    on the device (seeded), the diagonal shift and the row scaling applied on the host. Returns the
+    device arrays and their host copies."""
     import numpy as np
 
     a_host = sci.uniform(order * order, device, 41, -1.0, 1.0, dtype=np.float64).host().reshape(order, order)
@@ -140,37 +272,24 @@ def solve_system(sci, device, order, width, graded):
         a_host *= (10.0 ** (-6.0 * np.arange(order) / order))[:, None]
     a = sci.Array.from_numpy(a_host, device)
     b = sci.uniform(order * width, device, 42, -1.0, 1.0, dtype=np.float64).reshape(order, width)
-    return a, b
+    return a, b, a_host, b.host()
 
 
-def measure_solves(sci, _abi, device, order, width=32, graded_cases=(False, True),
-                   modes=(("fp64", 0), ("fp32", 1), ("bf16x3", 2), ("bf16", 3))) -> dict:
-    """A X = B to backward error 1e-14: plain fp64 LU against mixed factorisations + fp64 refinement."""
-    out = {}
-    for graded in graded_cases:
-        a, b = solve_system(sci, device, order, width, graded)
-        x = sci.Array(device, (order, width), "float64")
-        label = "graded" if graded else "shifted"
-        for name, mode in modes:
-            stats = _abi.SolveStats()
-            _abi.check(_abi.lib().scihip_solve(device.handle, mode, a.pointer, b.pointer, x.pointer, order, width,
-                                               1e-14, 30, ctypes.byref(stats)))
-            key = f"solve_{order}_{label}_{name}"
-            out[key] = {"ms": round(stats.solve_ms, 1), "factor_ms": round(stats.factor_ms, 1),
-                        "refinements": stats.iterations, "backward_error": stats.backward_error,
-                        "converged": bool(stats.converged)}
-            print(json.dumps({key: out[key]}), file=sys.stderr, flush=True)
-    return out
+def measure_lu(sci, _abi, device, order=16384, width=32, graded=False, repeats=2) -> dict:
+    """A X = B to backward error 1e-14: plain fp64 LU against the mixed-precision factorisations (the
+    trailing update on fp32, bf16x3 or bf16 matrix cores, the panel in fp32) refined in fp64; best of
+    `repeats` calls each. Every solve states its backward error (the library's, and recomputed on the
+    host in float64) and its forward error against LAPACK's float64 LU solve of the same system.
+    `mixed` names the fastest mixed variant that converged, `mixed_speedup` compares it with the
+    plain solve."""
+    import numpy as np
+    import scipy.linalg
 
-
-def measure_lu_headline(sci, _abi, device, order=16384, width=32, repeats=2) -> dict:
-    """The metric's second half: plain fp64 LU solve against the mixed-precision factorisations (the
-    trailing update on fp32, bf16x3 or bf16 matrix cores, the panel in fp32) refined in fp64, all to
-    backward error 1e-14, on the well-conditioned system; best of `repeats` calls each. `mixed` names
-    the fastest mixed variant that converged, and `mixed_speedup` compares it with the plain solve."""
-    a, b = solve_system(sci, device, order, width, False)
+    a, b, a_host, b_host = solve_system(sci, device, order, width, graded)
     x = sci.Array(device, (order, width), "float64")
-    out = {"order": order, "right_hand_sides": width, "tolerance": 1e-14}
+    out = {"order": order, "right_hand_sides": width, "tolerance": 1e-14,
+           "system": "graded (rows scaled by 10^(-6i/n))" if graded else "shifted (U[-1,1] + n/4 I)"}
+    solutions = {}
     for name, mode in (("plain_fp64", 0), ("mixed_fp32", 1), ("mixed_bf16x3", 2), ("mixed_bf16", 3)):
         best = None
         for _ in range(repeats):
@@ -179,11 +298,21 @@ def measure_lu_headline(sci, _abi, device, order=16384, width=32, repeats=2) -> 
                                                1e-14, 30, ctypes.byref(stats)))
             if best is None or stats.solve_ms < best.solve_ms:
                 best = stats
+        solutions[name] = x.host()
         lu_flops = 2.0 * order ** 3 / 3.0
         out[name] = {"ms": round(best.solve_ms, 1), "factor_ms": round(best.factor_ms, 1),
                      "factor_TFLOPs": round(lu_flops / (best.factor_ms * 1e-3) / 1e12, 1),
                      "refinements": best.iterations, "backward_error": best.backward_error,
                      "converged": bool(best.converged)}
+    t0 = time.perf_counter()
+    reference = scipy.linalg.solve(a_host, b_host)
+    out["lapack_seconds"] = round(time.perf_counter() - t0, 2)
+    norm = np.abs(a_host).sum(axis=1).max()
+    for name, solution in solutions.items():
+        residual = np.abs(b_host - a_host @ solution).max(axis=0)
+        backward = residual / (norm * np.abs(solution).max(axis=0) + np.abs(b_host).max(axis=0))
+        out[name]["backward_error_host"] = float(backward.max())
+        out[name]["forward_error"] = float(np.abs(solution - reference).max() / np.abs(reference).max())
     mixed = [name for name in ("mixed_fp32", "mixed_bf16x3", "mixed_bf16") if out[name]["converged"]]
     if mixed:
         best = min(mixed, key=lambda name: out[name]["ms"])
@@ -371,43 +500,54 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--log2n", type=int, default=28)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    parser.add_argument("--extra", action="store_true", help="also time reductions and the interpreter")
+    parser.add_argument("--no-lu", action="store_true", help="skip the n = 16384 LU solves")
+    parser.add_argument("--no-products", action="store_true", help="skip the C3 products")
+    parser.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurements")
     parser.add_argument("--solve-only", type=int, default=0, metavar="ORDER",
-                        help="only run one fp64 and one fp32-refined solve of this order (for profiling)")
+                        help="only time solves of this order (for profiling), in the --modes given")
     parser.add_argument("--modes", default="fp64,fp32",
                         help="with --solve-only: the factorisations to time (fp64, fp32, bf16x3, bf16)")
+    parser.add_argument("--products-only", action="store_true", help="only the C3 products (for profiling)")
+    parser.add_argument("--reductions-only", action="store_true", help="only the C2 reductions (for profiling)")
     parser.add_argument("--store-only", action="store_true",
                         help="only the headline store, then a plain copy chunk of the same size (8 B per element) "
                              "that calibrates hardware byte counters; for profiling")
-    parser.add_argument("--no-lu", action="store_true", help="skip the n = 16384 LU solves")
-    parser.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurements")
-    parser.add_argument("--linear", action="store_true",
-                        help="also time the 8192^3 products and the n = 16384 solves in every mode")
     args = parser.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; sharding across GPUs is designed but not built",
                           "n_gpus": args.gpus}))
         return 2
 
+    import numpy as np
+
     import libsci_amd as sci
     from libsci_amd import _abi
-    from libsci_amd.chunk import Fold
 
-    if args.solve_only:
-        device = sci.Device(0)
-        codes = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16": 3}
-        modes = tuple((name, codes[name]) for name in args.modes.split(","))
-        print(json.dumps(measure_solves(sci, _abi, device, args.solve_only, graded_cases=(False,), modes=modes)),
-              flush=True)
-        device.close()
-        return 0
-
-    n = 1 << args.log2n
     device = sci.Device(0)
     stream = ctypes.c_void_p()
     _abi.check(_abi.lib().scihip_stream(device.handle, ctypes.byref(stream)))
     events = Events(stream.value)
 
+    if args.solve_only:
+        codes = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16": 3}
+        a, b, _, _ = solve_system(sci, device, args.solve_only, 32, False)
+        x = sci.Array(device, (args.solve_only, 32), "float64")
+        out = {}
+        for name in args.modes.split(","):
+            stats = _abi.SolveStats()
+            _abi.check(_abi.lib().scihip_solve(device.handle, codes[name], a.pointer, b.pointer, x.pointer,
+                                               args.solve_only, 32, 1e-14, 30, ctypes.byref(stats)))
+            out[name] = {"ms": round(stats.solve_ms, 1), "factor_ms": round(stats.factor_ms, 1),
+                         "refinements": stats.iterations, "backward_error": stats.backward_error}
+        print(json.dumps(out), flush=True)
+        device.close()
+        return 0
+    if args.products_only:
+        print(json.dumps(measure_products(sci, _abi, device, events)), flush=True)
+        device.close()
+        return 0
+
+    n = 1 << args.log2n
     a = sci.uniform(n, device, 1, -4.0, 4.0)
     b = sci.uniform(n, device, 2, -4.0, 4.0, nan_seed=3, nan_fraction=0.01)
     c = sci.uniform(n, device, 4, -4.0, 4.0)
@@ -420,57 +560,20 @@ def main() -> int:
     def store():
         _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), out.pointer, n))
 
+    if args.reductions_only:
+        clean = sci.uniform(n, device, 2, -4.0, 4.0)
+        print(json.dumps(measure_reductions(sci, _abi, device, (a, b, c), clean, args.steps)), flush=True)
+        device.close()
+        return 0
     bytes_per_step = 16 * n
     wall_ms, kernel_ms = time_launches(device, events, store, args.steps, args.warmup)
     achieved = bytes_per_step / (kernel_ms * 1e-3)
     value = bytes_per_step / (wall_ms * 1e-3) / 1e9
-    traffic = None
-    traffic_file = os.path.join(HERE, "profiles", "traffic_c2_store.txt")
-    if os.path.exists(traffic_file):
-        for line in open(traffic_file):
-            if line.startswith("traffic_bytes:"):
-                traffic = int(line.split(":")[1])
-
-    extra = {}
-    if args.extra:
-        device.set_policy(1)
-        w, k = time_launches(device, events, store, max(args.steps // 2, 3), 2)
-        extra["interpreter_store_GBps"] = round(bytes_per_step / (k * 1e-3) / 1e9, 1)
-        device.set_policy(0)
-        total = ctypes.c_double()
-
-        def reduction(expression):
-            # A recorded reduction called through the C ABI; each call waits for its result (the
-            # kernel, the 8-byte result copy and the host wait are all inside the time).
-            chunk = sci.record(expression, reduce=True)
-            device.prepare(expression, reduce=True)
-            return lambda: _abi.check(_abi.lib().scihip_reduce(device.handle, ctypes.byref(chunk.struct),
-                                                               int(Fold.sum), n, ctypes.byref(total)))
-
-        reduce_call = reduction(sci.sin(a) * sci.cos(c) + sci.tanh(b))
-        w, _ = time_launches(device, events, reduce_call, max(args.steps // 2, 5), 2)
-        extra["c2_reduce_sum_GBps"] = round(12 * n / (w * 1e-3) / 1e9, 1)
-        # the same traffic with one fma per element: the streaming ceiling of this kernel shape
-        fma_expr = a * c + b
-        fma_chunk = sci.record(fma_expr)
-        device.prepare(fma_expr)
-        device.set_policy(2)
-
-        def fma_store():
-            _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(fma_chunk.struct), out.pointer, n))
-
-        _, k = time_launches(device, events, fma_store, max(args.steps // 2, 3), 2)
-        extra["fma_store_GBps"] = round(bytes_per_step / (k * 1e-3) / 1e9, 1)
-        device.set_policy(0)
-        dot_call = reduction(a * c)
-        w, _ = time_launches(device, events, dot_call, max(args.steps // 2, 5), 2)
-        extra["c2_dot_GBps"] = round(8 * n / (w * 1e-3) / 1e9, 1)
-        extra["fp64"] = measure_wide_store(sci, _abi, device, events, n // 2, max(args.steps // 2, 3))
-        extra["small_arrays"] = measure_small_arrays(sci, _abi, device, events)
-
+    device.set_policy(0)
     if args.store_only:
         copy_chunk = sci.record(sci.copy(a))
         device.prepare(sci.copy(a))
+        device.set_policy(2)
 
         def copy():
             _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(copy_chunk.struct), out.pointer, n))
@@ -482,15 +585,44 @@ def main() -> int:
         device.close()
         return 0
 
-    device.set_policy(0)
-    lu = measure_lu_headline(sci, _abi, device) if not args.no_lu else None
-    adjoint = measure_adjoint(sci, _abi, device, events) if not args.no_adjoint else None
+    # HBM bytes per launch of the store from rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated on a plain
+    # copy of known size (the summary names the kernel and the command that measured it)
+    traffic = None
+    traffic_file = os.path.join(HERE, "profiles", "traffic_c2_store.txt")
+    if os.path.exists(traffic_file):
+        for line in open(traffic_file):
+            if line.startswith("traffic_bytes:"):
+                traffic = int(line.split(":")[1])
+
+    def progress(stage):
+        print(f"[bench] {stage}", file=sys.stderr, flush=True)
+
+    progress("ceilings")
+    ceilings = measure_ceilings(sci, _abi, device, events, (a, b, c), out, max(args.steps // 2, 5))
+    clean = sci.uniform(n, device, 2, -4.0, 4.0)
+    progress("reductions")
+    reductions = measure_reductions(sci, _abi, device, (a, b, c), clean, max(args.steps // 2, 5))
+    hosts = (a.host(), b.host(), c.host(), clean.host()) if not args.no_cpu else None
+    del out
+    products = None
+    if not args.no_products:
+        progress("products")
+        products = measure_products(sci, _abi, device, events)
+    lu = lu_graded = None
+    if not args.no_lu:
+        progress("lu (shifted)")
+        lu = measure_lu(sci, _abi, device)
+        progress("lu (graded)")
+        lu_graded = measure_lu(sci, _abi, device, graded=True, repeats=1)
+    adjoint = None
+    if not args.no_adjoint:
+        progress("adjoint")
+        adjoint = measure_adjoint(sci, _abi, device, events)
+    progress("graph, 2-D, fp64, small arrays")
     graph = measure_graph(sci, _abi, device, events)
     two_dimensional = measure_two_dimensional(sci, _abi, device, events)
-
-    linear = {}
-    if args.linear:
-        linear = measure_linear(sci, _abi, device, events)
+    wide = measure_wide_store(sci, _abi, device, events, n // 2, max(args.steps // 2, 3))
+    small = measure_small_arrays(sci, _abi, device, events)
 
     result = {
         "metric": "fused-chunk HBM GB/s (% peak); LU solve time at FP64 accuracy, mixed vs plain",
@@ -509,22 +641,26 @@ def main() -> int:
                                f"specialised kernel", "elements": n, "bytes_per_step": bytes_per_step},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic},
+        "ceilings": ceilings,
+        "reductions": reductions,
     }
+    if products:
+        result["products"] = products
     if lu:
         result["lu_solve"] = lu
+        result["lu_graded"] = lu_graded
     if adjoint:
         result["adjoint"] = adjoint
     result["graph"] = graph
     result["two_dimensional"] = two_dimensional
-    if extra:
-        result["extra"] = extra
-    if linear:
-        result["linear"] = linear
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(min(n, 1 << 26), threads)
-    print(json.dumps(result), flush=True)
+    result["fp64_store"] = wide
+    result["small_arrays"] = small
     device.close()
+    if hosts is not None:
+        progress("cpu baseline")
+        threads = len(os.sched_getaffinity(0))
+        result["cpu_baseline"] = cpu_baseline(hosts, threads)
+    print(json.dumps(result), flush=True)
     return 0
 
 
