@@ -185,7 +185,7 @@ def measure_ceilings(sci, _abi, device, events, arrays, out, steps: int) -> dict
         def run():
             _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), out.pointer, n))
 
-        _, ms = time_launches(device, events, run, steps, 3)
+        _, ms = time_launches(device, events, run, steps, 10)
         device.set_policy(0)
         result[f"{name}_GBps"] = round(traffic * n / (ms * 1e-3) / 1e9, 1)
     return result
@@ -367,7 +367,8 @@ def measure_wide_store(sci, _abi, device, events, count: int, steps: int) -> dic
     def store():
         _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), out.pointer, count))
 
-    _, ms = time_launches(device, events, store, steps, 2)
+    # fresh arrays: the first launches on them ran up to 20% slower, so warm up longer than 2 launches
+    _, ms = time_launches(device, events, store, steps, 10)
     device.set_policy(0)
     return {"workload": f"exp(a)*0.5 + log(|b|+1)*c, N={count} float64", "ms": round(ms, 4),
             "GBps": round(32 * count / (ms * 1e-3) / 1e9, 1)}

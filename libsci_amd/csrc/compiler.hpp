@@ -55,7 +55,9 @@ namespace scihip {
         static std::string generate_axis(const scihip_chunk &chunk, int axis, bool contract, uint32_t fold = 0);
 
         static constexpr size_t limit = 1024; // distinct chunks compiled at most (vulkan/compiler.cpp:27)
-        static constexpr int kUnroll = 4;     // vectors per thread per sweep of a specialised kernel
+        static constexpr int kUnroll = 4;     // vectors per thread per sweep of a specialised reduction
+        static constexpr int kStoreThreads = 64; // threads per workgroup of a specialised store (one vector each)
+        static constexpr uint64_t kStoreGrid = uint64_t{1} << 24; // at most this many store workgroups
 
     private:
         void loop();

@@ -293,8 +293,15 @@ namespace scihip {
                 kernel = device->compiler->loaded(key);
             }
         }
-        const unsigned blocks = kernel ? blocks_for(device, count, Traits<T>::width, Compiler::kUnroll, mode)
-                                       : blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width, 1, mode);
+        // A specialised store covers the arrays once with one vector per thread; the other kernels
+        // sweep them grid-stride on a grid sized to the device.
+        const bool one_pass = kernel && mode == 0;
+        const uint64_t vectors = (count + Traits<T>::width - 1) / Traits<T>::width;
+        const unsigned blocks =
+                one_pass ? unsigned(std::min<uint64_t>((vectors + Compiler::kStoreThreads - 1) / Compiler::kStoreThreads,
+                                                       Compiler::kStoreGrid))
+                : kernel ? blocks_for(device, count, Traits<T>::width, Compiler::kUnroll, mode)
+                         : blocks_for(device, count, registers == kLarge ? 1 : Traits<T>::width, 1, mode);
         if (mode == 1 && blocks > sink.capacity) {
             fail("the reduction needs " + std::to_string(blocks) + " partials; its buffer holds " +
                  std::to_string(sink.capacity));
@@ -304,8 +311,9 @@ namespace scihip {
         if (kernel) {
             void *parameters[] = {&arguments}; // copied at launch (and into a graph node under capture)
             ++device->specialised;
-            rc = status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, 256, 1, 1, 0, device->stream, parameters,
-                                              nullptr),
+            const unsigned threads = one_pass ? unsigned(Compiler::kStoreThreads) : 256u;
+            rc = status(hipModuleLaunchKernel(kernel->function, blocks, 1, 1, threads, 1, 1, 0, device->stream,
+                                              parameters, nullptr),
                         "specialised launch");
         } else {
             ++device->interpreted;
