@@ -267,7 +267,7 @@ class Literal:
 
 
 class Node:
-    """An operation over operands (include/array/node.hpp:250-331)."""
+    """An operation over operands (include/array/node.hpp:26-88)."""
 
     def __init__(self, code: Code, children):
         self.code = Code(code)
@@ -284,7 +284,7 @@ class Node:
         self.dtype = dtypes.pop()
 
     def record(self, chunk: Chunk) -> int:
-        # Operands first, left to right, then this operation (node.hpp:299-312).
+        # Operands first, left to right, then this operation (node.hpp:75-88).
         slots = [c.record(chunk) for c in self.children]
         return chunk.emit(self.code, *slots)
 

@@ -3,9 +3,9 @@
 //
 // exp, log, sin and cos follow libsci's own algorithms so that they give the CPU path's bits:
 // the Cody-Waite / Taylor exponential (include/core/drivers/processor/kernels/generic/
-// exponential.hpp:19-356), fdlibm's logarithm form (generic/logarithm.hpp:90-126) and the
-// four-part quadrant reduction with odd / even Taylor polynomials (generic/trigonometry.hpp:162-207),
-// with the constants of generic/constants.hpp:26-208 and the polynomials of
+// exponential.hpp:24-61), fdlibm's logarithm form (generic/logarithm.hpp:26-66) and the
+// four-part quadrant reduction with odd / even Taylor polynomials (generic/trigonometry.hpp:29-80),
+// with the constants of generic/constants.hpp:16-119 and the polynomials of
 // generic/polynomial.hpp:17-61. Every step that the CPU rounds separately is a separate operation
 // here (built with -ffp-contract=off); every fused step is __builtin_fma. Arguments beyond 2^20 in
 // sin / cos, and the codes the CPU sends to the C library (asin ... atanh, generic/operation.hpp:45-87),
@@ -142,7 +142,7 @@ namespace scihip {
         return r;
     }
 
-    // NaN in either gives NaN, as numpy's minimum / maximum (scalar.hpp:265-277).
+    // NaN in either gives NaN, as numpy's minimum / maximum (scalar.hpp:112-127).
     template <typename T> __device__ __forceinline__ T lesser(T l, T r) {
         return (unordered(l) || unordered(r)) ? l + r : (r < l ? r : l);
     }
@@ -150,7 +150,7 @@ namespace scihip {
         return (unordered(l) || unordered(r)) ? l + r : (r > l ? r : l);
     }
 
-    // e^x (exponential.hpp:319-344). The clamp to [lower, upper] is two compare-selects that let a NaN
+    // e^x (exponential.hpp:24-49). The clamp to [lower, upper] is two compare-selects that let a NaN
     // through unchanged, which gives the same values as the CPU's NaN-propagating minimum / maximum
     // (a NaN then flows through the arithmetic to the result).
     template <typename T> __device__ __forceinline__ T exponential(T x) {
@@ -184,7 +184,7 @@ namespace scihip {
         return fused(fused(-d, q, f), y, q);
     }
 
-    // ln x (logarithm.hpp:90-126). `ordinary`: x is a positive normal number or NaN, so the CPU's
+    // ln x (logarithm.hpp:26-66). `ordinary`: x is a positive normal number or NaN, so the CPU's
     // scaling of subnormals and its special cases (0, +inf, negative) do not apply, and a NaN is
     // returned as it came.
     template <bool ordinary, typename T> __device__ __forceinline__ T logarithm_of(T x) {
@@ -237,7 +237,7 @@ namespace scihip {
     template <> __device__ __forceinline__ float library_cos(float x) { return ::cosf(x); }
     template <> __device__ __forceinline__ double library_cos(double x) { return ::cos(x); }
 
-    // Sine (phase false) or cosine (phase true) (trigonometry.hpp:162-207).
+    // Sine (phase false) or cosine (phase true) (trigonometry.hpp:29-80).
     template <bool phase, typename T> __device__ __forceinline__ T trigonometry(T x) {
         typedef Limits<T> L;
         typedef typename L::U U;

@@ -9,12 +9,12 @@
 //    (include/core/bytecode/accumulator.hpp:21-38);
 //  - the element kernels of every code (include/core/drivers/processor/kernels/generic/
 //    operation.hpp:25-110) with libsci's own exp / log / sin / cos
-//    (generic/exponential.hpp:19-356, generic/logarithm.hpp:90-126, generic/trigonometry.hpp:162-207,
-//    constants in generic/constants.hpp:26-208, polynomials in generic/polynomial.hpp:17-61),
-//    glibc for the library codes (generic/library.hpp:135-147) and a true fma (generic/scalar.hpp:225-240
+//    (generic/exponential.hpp:24-61, generic/logarithm.hpp:26-66, generic/trigonometry.hpp:29-80,
+//    constants in generic/constants.hpp:16-119, polynomials in generic/polynomial.hpp:17-61),
+//    glibc for the library codes (generic/library.hpp:22-34) and a true fma (generic/scalar.hpp:69-89
 //    with __FMA__, or the AVX-512 pack);
 //  - the folds sum / dot / extreme with the lane order of a pack of `width` lanes
-//    (generic/reductions.hpp:14-124, pack semantics src/core/drivers/processor/kernels/avx512.cpp:143-209);
+//    (generic/reductions.hpp:14-124, pack semantics src/core/drivers/processor/kernels/avx512.cpp:17-83);
 //  - the register-blocked product (src/core/drivers/processor/dispatch.cpp:171-224);
 //  - LU with partial pivoting and the substitutions (include/linear/decompose.hpp:29-72,
 //    include/linear/solve.hpp:25-52);
@@ -47,8 +47,8 @@
 
 namespace {
 
-    constexpr size_t block = 1024;            // plan.hpp:326
-    constexpr size_t threshold = size_t{1} << 17; // plan.hpp:362
+    constexpr size_t block = 1024;            // plan.hpp:23
+    constexpr size_t threshold = size_t{1} << 17; // plan.hpp:59
 
     // ---------------------------------------------------------------- constants (constants.hpp)
 
@@ -147,7 +147,7 @@ namespace {
     template <typename T> inline Bits<T> raw(T v) { return std::bit_cast<Bits<T>>(v); }
     template <typename T> inline T cast(Bits<T> v) { return std::bit_cast<T>(v); }
 
-    // NaN-propagating minimum / maximum of the one-lane pack (scalar.hpp:265-277).
+    // NaN-propagating minimum / maximum of the one-lane pack (scalar.hpp:112-127).
     template <typename T> inline T lesser(T l, T r) {
         if (std::isnan(l) || std::isnan(r)) return l + r;
         return r < l ? r : l;
@@ -156,7 +156,7 @@ namespace {
         if (std::isnan(l) || std::isnan(r)) return l + r;
         return r > l ? r : l;
     }
-    // The vector forms: blend(isnan(l), min_ps(l, r), l) (avx512.cpp:169-175); min_ps gives r on ties and NaN.
+    // The vector forms: blend(isnan(l), min_ps(l, r), l) (avx512.cpp:43-49); min_ps gives r on ties and NaN.
     template <typename T> inline T lesser_vector(T l, T r) {
         if (std::isnan(l)) return l;
         return l < r ? l : r;
@@ -166,7 +166,7 @@ namespace {
         return l > r ? l : r;
     }
 
-    // ---------------------------------------------------------------- e^x (exponential.hpp:19-356)
+    // ---------------------------------------------------------------- e^x (exponential.hpp:24-61)
 
     template <typename T> T exponential(T x) {
         using L = K<T>;
@@ -186,7 +186,7 @@ namespace {
         return (significand * scale(half)) * scale(rest);
     }
 
-    // ---------------------------------------------------------------- ln x (logarithm.hpp:90-126)
+    // ---------------------------------------------------------------- ln x (logarithm.hpp:26-66)
 
     template <typename T> T logarithm(T x) {
         using L = K<T>;
@@ -217,7 +217,7 @@ namespace {
         return result;
     }
 
-    // ---------------------------------------------------------------- sin / cos (trigonometry.hpp:162-207)
+    // ---------------------------------------------------------------- sin / cos (trigonometry.hpp:29-80)
 
     template <typename T, bool phase> T trigonometry(T x) {
         using L = K<T>;
@@ -672,7 +672,7 @@ namespace {
                chunk.instructions[chunk.length - 1].target == chunk.result;
     }
 
-    struct Plan { // plan.hpp:331-353
+    struct Plan { // plan.hpp:28-50
         size_t count, parts = 1;
         Plan(const scihip_chunk &chunk, size_t n, size_t threads) : count(n) {
             const size_t blocks = (n + block - 1) / block;

@@ -29,8 +29,67 @@
 #include <string>
 #include <vector>
 
+#include "scihip.h"
+
 using namespace sci::core;
 using namespace sci::core::drivers::processor;
+
+// The C ABI's chunk is the reference's Chunk for every field a binding reads
+// (include/core/bytecode/chunk.hpp:178-196): the same types, sizes and offsets, field by field, and
+// the same numbering of codes, value-id tags, folds and precisions
+// (include/core/bytecode/code.hpp:18-61, instruction.hpp:11-35, fold.hpp:11, precision.hpp:12). A
+// binding therefore copies the public fields (or the whole public prefix) with memcpy and re-encodes
+// nothing; this file does not compile if that stops being true. The reference's Chunk has private
+// members after the public ones (chunk.hpp:226-230), so only the public prefix is compared;
+// offsetof on it is conditionally supported (mixed access control), and clang supports it.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winvalid-offsetof"
+#define SAME_FIELD(field)                                                                                            \
+    static_assert(offsetof(Chunk, field) == offsetof(scihip_chunk, field), "offset of " #field);                   \
+    static_assert(sizeof(Chunk::field) == sizeof(scihip_chunk::field), "size of " #field)
+SAME_FIELD(precision);
+SAME_FIELD(length);
+SAME_FIELD(arguments);
+SAME_FIELD(literals);
+SAME_FIELD(temporaries);
+SAME_FIELD(result);
+SAME_FIELD(instructions);
+SAME_FIELD(inputs);
+SAME_FIELD(constants);
+#undef SAME_FIELD
+static_assert(offsetof(Chunk, constants) + sizeof(Chunk::constants) == sizeof(scihip_chunk),
+              "scihip_chunk is exactly Chunk's public prefix");
+static_assert(offsetof(Instruction, code) == offsetof(scihip_instruction, code) &&
+                      offsetof(Instruction, target) == offsetof(scihip_instruction, target) &&
+                      offsetof(Instruction, sources) == offsetof(scihip_instruction, sources) &&
+                      sizeof(Instruction) == sizeof(scihip_instruction) &&
+                      alignof(Instruction) == alignof(scihip_instruction),
+              "Instruction layout");
+#pragma clang diagnostic pop
+static_assert(Chunk::capacity == SCIHIP_CAPACITY, "capacity");
+static_assert(slot::input == SCIHIP_SLOT_INPUT && slot::literal == SCIHIP_SLOT_LITERAL &&
+                      slot::temporary == SCIHIP_SLOT_TEMPORARY && slot::output == SCIHIP_SLOT_OUTPUT &&
+                      slot::none == SCIHIP_SLOT_NONE && slot::types == SCIHIP_SLOT_TYPES,
+              "value-id tags");
+static_assert(static_cast<int>(Fold::sum) == SCIHIP_FOLD_SUM && static_cast<int>(Fold::minimum) == SCIHIP_FOLD_MINIMUM &&
+                      static_cast<int>(Fold::maximum) == SCIHIP_FOLD_MAXIMUM,
+              "folds");
+static_assert(static_cast<int>(Precision::single) == SCIHIP_SINGLE && static_cast<int>(Precision::wide) == SCIHIP_WIDE,
+              "precisions");
+#define SAME_CODE(name, abi) static_assert(static_cast<int>(Code::name) == abi, "code " #name)
+SAME_CODE(copy, SCIHIP_COPY); SAME_CODE(negate, SCIHIP_NEGATE); SAME_CODE(absolute, SCIHIP_ABSOLUTE);
+SAME_CODE(root, SCIHIP_ROOT); SAME_CODE(exponential, SCIHIP_EXPONENTIAL); SAME_CODE(logarithm, SCIHIP_LOGARITHM);
+SAME_CODE(sin, SCIHIP_SIN); SAME_CODE(cos, SCIHIP_COS); SAME_CODE(tan, SCIHIP_TAN); SAME_CODE(asin, SCIHIP_ASIN);
+SAME_CODE(acos, SCIHIP_ACOS); SAME_CODE(atan, SCIHIP_ATAN); SAME_CODE(csc, SCIHIP_CSC); SAME_CODE(sec, SCIHIP_SEC);
+SAME_CODE(cot, SCIHIP_COT); SAME_CODE(acsc, SCIHIP_ACSC); SAME_CODE(asec, SCIHIP_ASEC); SAME_CODE(acot, SCIHIP_ACOT);
+SAME_CODE(sinh, SCIHIP_SINH); SAME_CODE(cosh, SCIHIP_COSH); SAME_CODE(tanh, SCIHIP_TANH);
+SAME_CODE(asinh, SCIHIP_ASINH); SAME_CODE(acosh, SCIHIP_ACOSH); SAME_CODE(atanh, SCIHIP_ATANH);
+SAME_CODE(csch, SCIHIP_CSCH); SAME_CODE(sech, SCIHIP_SECH); SAME_CODE(coth, SCIHIP_COTH);
+SAME_CODE(acsch, SCIHIP_ACSCH); SAME_CODE(asech, SCIHIP_ASECH); SAME_CODE(acoth, SCIHIP_ACOTH);
+SAME_CODE(add, SCIHIP_ADD); SAME_CODE(subtract, SCIHIP_SUBTRACT); SAME_CODE(multiply, SCIHIP_MULTIPLY);
+SAME_CODE(divide, SCIHIP_DIVIDE); SAME_CODE(minimum, SCIHIP_MINIMUM); SAME_CODE(maximum, SCIHIP_MAXIMUM);
+SAME_CODE(fuse, SCIHIP_FUSE); SAME_CODE(count, SCIHIP_CODES);
+#undef SAME_CODE
 
 namespace {
 

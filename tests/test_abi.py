@@ -47,7 +47,10 @@ def test_exports_match_header():
 
 def test_chunk_layout_matches_reference_chunk():
     # core::Chunk's public fields (include/core/bytecode/chunk.hpp:180-196): six bytes, 64 five-byte
-    # instructions, then 64 pointers and 64 doubles at 8-byte alignment.
+    # instructions, then 64 pointers and 64 doubles at 8-byte alignment. The C side is checked against
+    # the reference's own header at compile time: oracle/ref_harness.cpp static_asserts every public
+    # field's offset and size, and the numbering of codes, tags, folds and precisions; here the Python
+    # binding is checked against the C layout.
     assert ChunkStruct.instructions.offset == 6
     assert ChunkStruct.inputs.offset == 328
     assert ChunkStruct.constants.offset == 840
