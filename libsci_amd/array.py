@@ -19,6 +19,8 @@ import math
 
 import numpy as np
 
+from builtins import sum as builtins_sum
+
 from . import _abi
 from .chunk import Chunk, Code, Fold, Precision
 
@@ -357,17 +359,103 @@ def _device_of(expression) -> Device:
     return devices[0]
 
 
-def record(expression, reduce: bool = False) -> Chunk:
-    precision = _DTYPES[expression.leaf_dtype()]
-    chunk = Chunk(precision)
+def _record(expression, reduce: bool) -> Chunk:
+    chunk = Chunk(_DTYPES[expression.leaf_dtype()])
     value = expression.record(chunk)
     if reduce:
         chunk.reduce(value)
     else:
         chunk.output(value)
+    return chunk
+
+
+def record(expression, reduce: bool = False) -> Chunk:
+    """The one chunk of an expression; raises when it outgrows a chunk (see `stages` for the split)."""
+    chunk = _record(expression, reduce)
     if not chunk.valid():
         raise ValueError("the expression outgrew a chunk (64 instructions, inputs or constants)")
     return chunk
+
+
+class Staged:
+    """A leaf standing for the result of an earlier stage of a split expression (a device temporary
+    once the stages run). While the split is planned it records as an input of its own."""
+
+    def __init__(self, index: int, dtype, shape):
+        self.index = index
+        self.dtype = np.dtype(dtype)
+        self.shape = shape
+
+    def record(self, chunk: Chunk) -> int:
+        return chunk.input(0x10 + 16 * self.index)  # an address no allocation has; only identity matters
+
+    def devices(self):
+        return []
+
+    def leaf_dtype(self):
+        return self.dtype
+
+    def element_shape(self):
+        return self.shape
+
+
+def _operations(expression) -> int:
+    if not isinstance(expression, Node):
+        return 0
+    return 1 + builtins_sum(_operations(c) for c in expression.children)
+
+
+def stages(expression, reduce: bool = False):
+    """Splits an expression that outgrows one chunk (64 instructions, inputs, constants or
+    temporaries, include/core/bytecode/chunk.hpp:31) into chunks run one after another on the device,
+    each later one reading earlier results from device temporaries. The reference runs such an
+    expression element by element on the host instead (include/array/expression.hpp:178-184,
+    204-221). Returns (stages, final): stage k is a store whose Staged leaves name earlier stages;
+    `final` is stored or reduced. The rule, shared with include/sci/hip.hpp: a node that fits one
+    chunk stays whole; otherwise each child is made to fit on its own, then, while the node still
+    outgrows a chunk, its operation child with the most operations (the leftmost of equals) becomes a
+    stage of its own. A node over leaves always fits, so the split ends."""
+    planned = []
+
+    def fits(node, as_reduction):
+        return _record(node, as_reduction).valid()
+
+    def cut(node, as_reduction):
+        if not isinstance(node, Node) or fits(node, as_reduction):
+            return node
+        children = [cut(c, False) for c in node.children]
+        while True:
+            candidate = Node(node.code, children)
+            if fits(candidate, as_reduction):
+                return candidate
+            k = max((i for i, c in enumerate(children) if isinstance(c, Node)),
+                    key=lambda i: (_operations(children[i]), -i))
+            planned.append(children[k])
+            children[k] = Staged(len(planned) - 1, children[k].leaf_dtype(), children[k].element_shape())
+
+    final = cut(expression, reduce)
+    return planned, final
+
+
+def bind_stages(expression, results):
+    """The expression with each Staged leaf replaced by results[index]."""
+    if isinstance(expression, Staged):
+        return results[expression.index]
+    if isinstance(expression, Node):
+        return Node(expression.code, [bind_stages(c, results) for c in expression.children])
+    return expression
+
+
+def _runnable(expression, reduce: bool):
+    """The expression as one chunk can run it: unchanged when it fits, else its earlier stages are
+    queued on the device (asynchronously, each into a temporary) and the last stage is returned."""
+    if isinstance(expression, Node) and not _record(expression, reduce).valid():
+        planned, final = stages(expression, reduce)
+        results = []
+        for stage in planned:
+            results.append(evaluate(bind_stages(stage, results)))
+        return bind_stages(final, results)
+    return expression
 
 
 def assign(target: Array, expression) -> None:
@@ -379,9 +467,11 @@ def assign(target: Array, expression) -> None:
     device = _device_of(expression)
     if device is not target.device:
         raise ValueError("the target and the expression's arrays must live on the same device")
-    chunk = record(expression)
+    runnable = _runnable(expression, False)  # holds the stages' temporaries until the call is queued
+    chunk = record(runnable)
     _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), target.pointer, target.size),
                "scihip_execute")
+    del runnable
 
 
 def evaluate(expression) -> Array:
@@ -397,11 +487,13 @@ def evaluate(expression) -> Array:
 def reduce_raw(expression, fold: Fold) -> float:
     """The fold of an expression in double, before the cast to the element type."""
     device = _device_of(expression)
-    chunk = record(expression, reduce=True)
+    runnable = _runnable(expression, True)  # holds the stages' temporaries until the call is queued
+    chunk = record(runnable, reduce=True)
     count = expression.size
     result = ctypes.c_double()
     _abi.check(_abi.lib().scihip_reduce(device.handle, ctypes.byref(chunk.struct), int(fold), count,
                                         ctypes.byref(result)), "scihip_reduce")
+    del runnable
     return result.value
 
 
@@ -412,11 +504,13 @@ def reduce_axis(expression, fold: Fold, axis: int) -> Array:
     if shape is None or len(shape) != 2 or axis not in (0, 1):
         raise ValueError("axis reductions take a matrix expression and axis 0 or 1")
     device = _device_of(expression)
-    chunk = record(expression, reduce=True)
+    runnable = _runnable(expression, True)  # holds the stages' temporaries until the call is queued
+    chunk = record(runnable, reduce=True)
     rows, columns = shape
     out = Array(device, (rows if axis == 1 else columns,), expression.leaf_dtype())
     _abi.check(_abi.lib().scihip_reduce_axis(device.handle, ctypes.byref(chunk.struct), int(fold), rows, columns,
                                              axis, out.pointer), "scihip_reduce_axis")
+    del runnable
     return out
 
 
@@ -503,7 +597,7 @@ def gradient(expression, arrays, seed: Array | None = None, loss: bool = True):
     expression in double (None when `loss` is false); an array the expression does not use gets zeros.
     """
     device = _device_of(expression)
-    chunk = record(expression, reduce=True)
+    chunk = record(_runnable(expression, True), reduce=True)
     count = expression.size
     inputs = [chunk.struct.inputs[i] for i in range(chunk.struct.arguments)]
     outputs = {}
@@ -647,4 +741,4 @@ def uniform(count: int, device: Device, seed: int, lower: float, upper: float, d
     return out
 
 
-__all__ = [name for name in dir() if not name.startswith("_") and name not in ("ctypes", "math", "np")]
+__all__ = [name for name in dir() if not name.startswith("_") and name not in ("ctypes", "math", "np", "builtins_sum")]

@@ -8,9 +8,13 @@
 //                             sum(sin(c)) (total.f64), the gradient of sum(sinh(a*b + c) * exp(-(d*d)))
 //                             on 1000 elements of the inputs in inputs.f32 (gradients.f32, loss.f64),
 //                             a 64 x 48 x 40 product (product.f32) and the 3 x 3 solve KAT (solve.f64).
+//   workload split PROGRAM DIRECTORY COUNT INPUTS
+//                             runs an expression too large for one chunk (a postfix program) on GPU 0:
+//                             prints its split into chunks and writes its result and folds.
 #include "sci/hip.hpp"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -168,9 +172,67 @@ namespace {
         return 0;
     }
 
+    template <typename T> std::vector<T> read(const std::string &path, size_t count) {
+        std::vector<T> values(count);
+        FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f || std::fread(values.data(), sizeof(T), count, f) != count) {
+            std::fprintf(stderr, "cannot read %s\n", path.c_str());
+            std::exit(1);
+        }
+        std::fclose(f);
+        return values;
+    }
+
+    // An expression too large for one chunk: the postfix program in PROGRAM (lines "in K",
+    // "const V", "op CODE") over INPUTS arrays of COUNT floats read from DIRECTORY/inputK.f32. Prints
+    // the split of its store and of its reduction (one chunk per stage), then writes the stored result
+    // (split.f32) and its sum, minimum and maximum (folds.f64) into DIRECTORY.
+    int split_run(const std::string &path, const std::string &directory, size_t count, int inputs) {
+        Device device(0);
+        std::vector<Array<float>> arrays;
+        for (int k = 0; k < inputs; ++k)
+            arrays.push_back(Array<float>::from(device, read<float>(directory + "/input" + std::to_string(k) + ".f32",
+                                                                    count)));
+        std::vector<Expr<float>> stack;
+        FILE *f = std::fopen(path.c_str(), "r");
+        if (!f) return 1;
+        char kind[16];
+        double value;
+        while (std::fscanf(f, "%15s %lf", kind, &value) == 2) {
+            if (std::strcmp(kind, "in") == 0) stack.emplace_back(arrays.at(size_t(value)));
+            else if (std::strcmp(kind, "const") == 0) stack.emplace_back(float(value));
+            else {
+                const Code code = static_cast<Code>(int(value));
+                const size_t arity = code == Code::fuse ? 3 : int(code) >= int(Code::add) ? 2 : 1;
+                std::vector<Expr<float>> operands(stack.end() - long(arity), stack.end());
+                stack.erase(stack.end() - long(arity), stack.end());
+                stack.emplace_back(code, operands);
+            }
+        }
+        std::fclose(f);
+        const Expr<float> e = stack.at(0);
+        for (const bool reduce : {false, true}) {
+            const Split<float> parts = split(e, reduce);
+            for (const Expr<float> &stage : parts.stages) print(reduce ? "reduce_stage" : "store_stage", stage.chunk(false));
+            print(reduce ? "reduce_last" : "store_last", parts.last.chunk(reduce));
+        }
+        write(directory + "/split.f32", evaluate(e).host());
+        write(directory + "/folds.f64", std::vector<double>{reduce_raw(e, Fold::sum), reduce_raw(e, Fold::minimum),
+                                                            reduce_raw(e, Fold::maximum)});
+        return 0;
+    }
+
 } // namespace
 
 int main(int argc, char **argv) {
+    if (argc >= 6 && std::strcmp(argv[1], "split") == 0) {
+        try {
+            return split_run(argv[2], argv[3], std::strtoull(argv[4], nullptr, 10), std::atoi(argv[5]));
+        } catch (const Error &e) {
+            std::fprintf(stderr, "error: %s\n", e.what());
+            return 1;
+        }
+    }
     if (argc >= 2 && std::strcmp(argv[1], "record") == 0) return record();
     if (argc >= 3 && std::strcmp(argv[1], "run") == 0) {
         try {
@@ -180,6 +242,6 @@ int main(int argc, char **argv) {
             return 1;
         }
     }
-    std::fprintf(stderr, "usage: %s record | run DIRECTORY\n", argv[0]);
+    std::fprintf(stderr, "usage: %s record | run DIRECTORY | split PROGRAM DIRECTORY COUNT INPUTS\n", argv[0]);
     return 2;
 }

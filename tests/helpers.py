@@ -150,3 +150,142 @@ def c5_numpy(a, b, c, d):
     z = a * b + c
     e = np.exp(-(d * d))
     return np.sinh(z) * e, [np.cosh(z) * b * e, np.cosh(z) * a * e, np.cosh(z) * e, -2.0 * d * np.sinh(z) * e]
+
+
+# ---------------------------------------------------------------- split expressions
+
+class HostLeaf:
+    """A host array as an expression leaf: it records its own address, so that the oracle can run
+    chunks recorded from expression trees (libsci_amd.array.Node) over host data."""
+
+    def __init__(self, values: np.ndarray):
+        self.values = values
+
+    def record(self, chunk) -> int:
+        return chunk.input(self.values.ctypes.data)
+
+    def devices(self):
+        return []
+
+    def leaf_dtype(self):
+        return self.values.dtype
+
+    def element_shape(self):
+        return self.values.shape
+
+
+def deep_program(operations: int, inputs: int, seed: int, constants: bool = True):
+    """A seeded postfix program of `operations` operations over `inputs` arrays (and constants),
+    built bottom-up from a pool of values: each operation transforms a pool value or combines it with
+    a fresh input, a constant or another pool value; whatever remains is added together at the end.
+    Fresh inputs enter at every level, so the result keeps depending on every element's inputs (nested
+    sines alone would converge to a constant). Only codes the GPU computes bit for bit like the CPU
+    (IEEE arithmetic, min / max, sin, cos, exp, |x|, -x), arranged so that magnitudes stay near 1."""
+    from libsci_amd.chunk import Code
+
+    rng = np.random.default_rng(seed)
+    pool = [[("in", k)] for k in range(inputs)]
+    unary = [Code.sin, Code.cos, Code.negate, Code.absolute]
+    binary = [Code.add, Code.subtract, Code.multiply, Code.minimum, Code.maximum]
+    done = 0
+    while done < operations:
+        k = int(rng.integers(len(pool)))
+        r = rng.random()
+        if r < 0.25:
+            code = unary[int(rng.integers(len(unary)))]
+            if rng.random() < 0.15:  # exp of a bounded value: e^sin(x)
+                pool[k] = pool[k] + [("op", Code.sin), ("op", Code.exponential)]
+                done += 2
+            else:
+                pool[k] = pool[k] + [("op", code)]
+                done += 1
+            continue
+        code = binary[int(rng.integers(len(binary)))]
+        if r < 0.6:  # with a fresh input
+            pool[k] = pool[k] + [("in", int(rng.integers(inputs))), ("op", code)]
+        elif constants and r < 0.8:  # with a constant
+            pool[k] = pool[k] + [("const", float(np.float32(rng.uniform(-1.5, 1.5)))), ("op", code)]
+        elif len(pool) > 1:  # with another pool value
+            j = int(rng.integers(len(pool) - 1))
+            j += j >= k
+            merged = pool[k] + pool[j] + [("op", code)]
+            pool = [p for i, p in enumerate(pool) if i not in (k, j)] + [merged]
+            k = len(pool) - 1
+        else:
+            pool[k] = pool[k] + [("in", int(rng.integers(inputs))), ("op", code)]
+        done += 1
+        if code in (Code.add, Code.subtract, Code.multiply) and rng.random() < 0.5:
+            pool[k] = pool[k] + [("op", Code.sin)]  # back near 1
+            done += 1
+    program = pool[0]
+    for rest in pool[1:]:
+        program = program + rest + [("op", Code.add)]
+    return program
+
+
+def tree_of(program, leaves, dtype=np.float32):
+    """The expression tree (libsci_amd.array nodes) of a postfix program over the given leaves."""
+    from libsci_amd.array import Literal, Node
+    from libsci_amd.chunk import ARITY, Code
+
+    stack = []
+    for kind, value in program:
+        if kind == "in":
+            stack.append(leaves[value])
+        elif kind == "const":
+            stack.append(Literal(value, dtype))
+        else:
+            arity = ARITY[Code(value)]
+            operands = stack[-arity:]
+            del stack[-arity:]
+            stack.append(Node(Code(value), operands))
+    assert len(stack) == 1
+    return stack[0]
+
+
+def float64_of(program, arrays):
+    """The program evaluated in float64 by numpy: the exact-arithmetic reference of a split run."""
+    from libsci_amd.chunk import Code
+
+    f = {Code.sin: np.sin, Code.cos: np.cos, Code.negate: np.negative, Code.absolute: np.abs,
+         Code.exponential: np.exp, Code.add: np.add, Code.subtract: np.subtract, Code.multiply: np.multiply,
+         Code.minimum: np.minimum, Code.maximum: np.maximum}
+    stack = []
+    for kind, value in program:
+        if kind == "in":
+            stack.append(arrays[value].astype(np.float64))
+        elif kind == "const":
+            stack.append(np.float64(value))
+        else:
+            code = Code(value)
+            if code in (Code.sin, Code.cos, Code.negate, Code.absolute, Code.exponential):
+                stack.append(f[code](stack.pop()))
+            else:
+                b, a = stack.pop(), stack.pop()
+                stack.append(f[code](a, b))
+    return stack[0]
+
+
+def run_stages_on_oracle(expression, reduce_fold=None):
+    """Runs a split expression over host leaves the way the device runs it: every stage through the
+    oracle into a host temporary, then the last stage stored (reduce_fold None) or reduced."""
+    import oracle
+    from libsci_amd.array import _record, bind_stages, stages
+
+    planned, final = stages(expression, reduce_fold is not None)
+    results = []
+    for stage in planned:
+        bound = bind_stages(stage, results)
+        chunk = _record(bound, False)
+        assert chunk.valid()
+        results.append(HostLeaf(oracle.execute(chunk, int(np.prod(bound.element_shape())), threads=8,
+                                               dtype=bound.leaf_dtype())))
+    bound = bind_stages(final, results)
+    count = int(np.prod(bound.element_shape()))
+    if reduce_fold is None:
+        chunk = _record(bound, False)
+        assert chunk.valid()
+        return oracle.execute(chunk, count, threads=8, dtype=bound.leaf_dtype()), len(planned)
+    chunk = _record(bound, True)
+    assert chunk.valid()
+    return oracle.reduce(chunk, int(reduce_fold), count, threads=8), len(planned)
