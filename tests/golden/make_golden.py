@@ -10,7 +10,10 @@ oracle/Makefile) and records, through that library only:
                   from the AVX-512 table (src/core/drivers/processor/kernels/avx512.cpp), after checking
                   that the fallback table built with a true fma gives the same values;
 * reductions.json sum / minimum / maximum of chunks at several sizes, for 1 and 4 pool threads,
-                  through the reference's Plan, folds and Accumulator.
+                  through the reference's Plan, folds and Accumulator;
+* digests.json    (``--digests``, minutes) for exp, log, sin, cos, tan, csc, sec, cot and sqrt, the
+                  digest of the AVX-512 table's outputs over every float32 bit pattern, one per chunk
+                  of 2^24 patterns (tests/helpers.py digest_chunk gives the rule).
 
 Inputs are regenerated in the tests from the recorded seeds with the oracle's generator, whose
 values are fixed by a closed form (tests/test_oracle.py pins it), so only outputs are stored.
@@ -146,5 +149,34 @@ def main() -> None:
     print("wrote", len(chunks), "chunks,", len(arrays), "kernel outputs,", len(reductions), "reductions")
 
 
+def write_digests(threads: int = 8) -> None:
+    """digests.json: every chunk digest of every digested code, from the reference's AVX-512 table."""
+    import concurrent.futures
+
+    sys.path.insert(0, os.path.dirname(HERE))
+    from helpers import DIGEST_CHUNK, DIGEST_CODES, digest_chunk, digest_inputs
+
+    chunks = (1 << 32) // DIGEST_CHUNK
+
+    def one(job):
+        name, chunk = job
+        x = digest_inputs(chunk)
+        return name, chunk, digest_chunk(name, chunk, oracle.ref_kernel(int(Code[name]), x, isa=2))
+
+    table = {name: [None] * chunks for name in DIGEST_CODES}
+    with concurrent.futures.ThreadPoolExecutor(threads) as pool:
+        for name, chunk, value in pool.map(one, [(n, c) for n in DIGEST_CODES for c in range(chunks)]):
+            table[name][chunk] = f"{value:016x}"
+    out = {"rule": "tests/helpers.py digest_chunk", "chunk": DIGEST_CHUNK,
+           "source": "the reference's AVX-512 kernel table (src/core/drivers/processor/kernels/avx512.cpp) "
+                     "through oracle/_ref", "codes": table}
+    with open(os.path.join(HERE, "digests.json"), "w") as handle:
+        json.dump(out, handle, indent=0)
+
+
 if __name__ == "__main__":
-    main()
+    if "--digests" in sys.argv:
+        oracle.build(reference=True)
+        write_digests()
+    else:
+        main()

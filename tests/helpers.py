@@ -289,3 +289,40 @@ def run_stages_on_oracle(expression, reduce_fold=None):
     chunk = _record(bound, True)
     assert chunk.valid()
     return oracle.reduce(chunk, int(reduce_fold), count, threads=8), len(planned)
+
+
+# ---------------------------------------------------------------- exhaustive float32 digests
+
+DIGEST_CODES = ("exponential", "logarithm", "sin", "cos", "tan", "csc", "sec", "cot", "root")
+DIGEST_TRIG = ("sin", "cos", "tan", "csc", "sec", "cot")
+DIGEST_CHUNK = 1 << 24  # input patterns per chunk digest; 256 chunks cover all 2^32
+_C1, _C2 = 0xBF58476D1CE4E5B9, 0x94D049BB133111EB
+_REACH = 0x49800000  # |x| <= 2^20: the bit pattern of 1048576.0f
+
+
+def digest_chunk(code: str, chunk: int, outputs: np.ndarray) -> int:
+    """The digest of one chunk of a code's outputs over every float32 input pattern: the sum mod 2^64
+    of SplitMix64's finaliser of (pattern << 32 | output bits) over the chunk's 2^24 patterns, NaN
+    outputs counted as one pattern (0x7FC00000). For sin, cos and the codes built from them, inputs
+    beyond the Cody-Waite reach (|x| > 2^20, infinities, NaN) are left out: there the reference calls
+    glibc and the GPU the device library, which agree within 2 ulp, not bit for bit. A sum of mixed
+    terms is order-free, so a GPU can form it in any order, yet any changed output changes it."""
+    patterns = np.arange(chunk * DIGEST_CHUNK, (chunk + 1) * DIGEST_CHUNK, dtype=np.uint64)
+    bits = outputs.view(np.uint32).astype(np.uint64)
+    bits[np.isnan(outputs)] = 0x7FC00000
+    z = (patterns << np.uint64(32)) | bits
+    z ^= z >> np.uint64(30)
+    z *= np.uint64(_C1)
+    z ^= z >> np.uint64(27)
+    z *= np.uint64(_C2)
+    z ^= z >> np.uint64(31)
+    if code in DIGEST_TRIG:
+        z[(patterns & np.uint64(0x7FFFFFFF)) > np.uint64(_REACH)] = 0
+    low = int((z & np.uint64(0xFFFFFFFF)).sum(dtype=np.uint64))
+    high = int((z >> np.uint64(32)).sum(dtype=np.uint64))
+    return (low + (high << 32)) % (1 << 64)
+
+
+def digest_inputs(chunk: int) -> np.ndarray:
+    """The float32 inputs of one digest chunk: the bit patterns chunk * 2^24 ... in order."""
+    return np.arange(chunk * DIGEST_CHUNK, (chunk + 1) * DIGEST_CHUNK, dtype=np.uint64).astype(np.uint32).view(np.float32)

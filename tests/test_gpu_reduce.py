@@ -25,7 +25,7 @@ from libsci_amd.chunk import Code, Fold, Precision, record_postfix
 
 pytestmark = pytest.mark.gpu
 
-LIBRARY_ULPS = 8
+LIBRARY_ULPS = 4  # the measured maximum over every glibc-backed code (tests/test_gpu_chunks.py)
 
 C1_REDUCE = [("in", 0), ("op", Code.sin)]
 C2_REDUCE = [("in", 0), ("op", Code.sin), ("in", 2), ("op", Code.cos), ("op", Code.multiply),

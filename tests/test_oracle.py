@@ -304,3 +304,22 @@ def test_linspace_closed_form():
     i = np.arange(1_000_000, dtype=np.float64)
     assert (x == (0.0 + i * (1.0 / 999_999.0)).astype(np.float32)).all()
     assert x[0] == 0.0 and x[-1] == 1.0
+
+
+# ------------------------------------------------------------------------------------------ exhaustive digests
+
+DIGEST_SAMPLE = (0x00, 0x3F, 0xC0)  # tiny and subnormal values, [0.5, 2), [-8, -2)
+
+
+@pytest.mark.parametrize("name", ["exponential", "logarithm", "sin", "cos", "tan", "csc", "sec", "cot", "root"])
+def test_oracle_matches_exhaustive_digests(name):
+    """tests/golden/digests.json holds, per code, the digest of the reference's AVX-512 outputs over
+    every float32 pattern in 256 chunks; the oracle must reproduce the chunks sampled here. (The GPU
+    test checks all 256.)"""
+    from tests.helpers import digest_chunk, digest_inputs
+
+    fixture = golden_json("digests.json")
+    for chunk in DIGEST_SAMPLE:
+        x = digest_inputs(chunk)
+        got = digest_chunk(name, chunk, oracle.map_f32(Code[name], x, threads=8))
+        assert f"{got:016x}" == fixture["codes"][name][chunk], (name, hex(chunk))
