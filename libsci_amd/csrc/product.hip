@@ -234,6 +234,170 @@ namespace scihip {
             }
         }
 
+        // The large-tile bf16 product for long shared dimensions. A 256-thread workgroup computes a
+        // 256 x 256 tile of C, each wave 128 x 128 as 4 x 4 blocks of v_mfma_f32_32x32x16_bf16
+        // (256 accumulators per lane, one wave per SIMD). Operands stream global -> LDS with
+        // global_load_lds_dwordx4 (each lane's 16 bytes land at the wave's LDS base + 16 * lane), in
+        // k-tiles of 64 values (128 bytes per row), double-buffered: the next k-tile's loads are in
+        // flight while the current one is multiplied, and one barrier per k-tile both publishes the
+        // next tile and retires the current one. LDS rows are stored with their eight 16-byte chunks
+        // permuted, chunk c of row r at position c ^ ((r / 2) % 8): the 16 lanes that read one chunk
+        // of 16 consecutive rows in one LDS pass then cover all 64 banks once. The permutation costs
+        // nothing: each lane simply fetches the chunk that belongs at its LDS position.
+        // 32 x 32 x 16 layout (measured on gfx950): lane l supplies row (column) l % 32 of A (B) at k
+        // 8 (l / 32) ... + 7; accumulator v of lane l is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+        constexpr int kWide = 256;          // rows and columns of C per workgroup
+        constexpr int kWideRow = 128;       // bytes of a k-tile row (64 bf16)
+        constexpr int kWideStage = 2 * kWide * kWideRow; // A and B of one k-tile: 64 KiB
+
+        typedef float v16f __attribute__((ext_vector_type(16)));
+        typedef __attribute__((address_space(3))) void *lds_pointer;
+
+        __device__ __forceinline__ int wide_position(int row, int chunk) { return chunk ^ ((row >> 1) & 7); }
+
+        template <int Waves = 1>
+        __global__ __launch_bounds__(kThreads, Waves) void gemm_wide(const ProductArgs p) {
+            extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+            const int tiles_m = (int)((p.m + kWide - 1) / kWide);
+            const int tiles_n = (int)((p.n + kWide - 1) / kWide);
+            int tm, tn;
+            tile_of(tiles_m, tiles_n, tm, tn);
+            const int64_t row0 = (int64_t)tm * kWide, col0 = (int64_t)tn * kWide;
+            // the wave number is the same in every lane: say so, so that LDS bases (M0) are scalar
+            const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            const int wm = wave >> 1, wn = wave & 1;
+
+            // This lane's part of each direct load: wave `wave` fills the LDS blocks 8 wave ... 8 wave + 7
+            // of each operand (1 KiB each: 8 rows of 128 bytes); lane l of block b writes row 8 b + l / 8
+            // at position l % 8 and so fetches chunk (l % 8) ^ ((row / 2) % 8) of that row, which is
+            // (l % 8) ^ (4 (b % 2) + l / 16). The loads go through buffer descriptors based at the
+            // tile's first row and sized to the rows that exist: the per-lane offset (row and chunk) is
+            // range-checked, so rows past the matrix read zeros; the k offset rides in soffset.
+            const int64_t lda = p.lda * 2, ldb = p.ldb * 2;
+            // (every field is the same in all lanes; readfirstlane puts the descriptors in SGPRs, where
+            // the buffer instructions need them)
+            auto scalar = [](uint64_t x) {
+                return (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32 |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+            };
+            const int64_t a_bytes = (p.m - row0) * lda, b_bytes = (p.n - col0) * ldb;
+            const uint32_t a_records =
+                    __builtin_amdgcn_readfirstlane(a_bytes > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)a_bytes);
+            const uint32_t b_records =
+                    __builtin_amdgcn_readfirstlane(b_bytes > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)b_bytes);
+            const __amdgpu_buffer_rsrc_t a_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)scalar((uint64_t)(static_cast<const unsigned char *>(p.a_hi) + row0 * lda)), 0, a_records,
+                    0x00020000);
+            const __amdgpu_buffer_rsrc_t b_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)scalar((uint64_t)(static_cast<const unsigned char *>(p.b_hi) + col0 * ldb)), 0, b_records,
+                    0x00020000);
+            uint32_t a_offset[8], b_offset[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int row = (wave * 8 + i) * 8 + (lane >> 3);
+                const int chunk = wide_position(row, lane & 7);
+                a_offset[i] = (uint32_t)(row * lda + chunk * 16);
+                b_offset[i] = (uint32_t)(row * ldb + chunk * 16);
+            }
+            auto issue = [&](int64_t k_byte, int stage) {
+                unsigned char *base = lds + stage * kWideStage;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(a_rsrc, (lds_pointer)(base + (wave * 8 + i) * 1024), 16,
+                                                              a_offset[i], (int)k_byte, 0, 0);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            b_rsrc, (lds_pointer)(base + kWide * kWideRow + (wave * 8 + i) * 1024), 16, b_offset[i],
+                            (int)k_byte, 0, 0);
+                }
+            };
+
+            v16f acc[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+            // Fragment addresses within a stage: block i of A is rows 128 wm + 32 i + l % 32, block j of
+            // B rows 128 wn + 32 j + l % 32; k-step s (16 values) reads chunk 2 s + l / 32.
+            int a_row[4], b_row[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a_row[i] = wm * 128 + i * 32 + (lane & 31);
+                b_row[i] = wn * 128 + i * 32 + (lane & 31);
+            }
+            const int half = lane >> 5;
+
+            const int64_t tiles_all = p.kp / 64;
+            const int64_t k_begin = tiles_all * blockIdx.y / gridDim.y;
+            const int64_t tiles_k = tiles_all * (blockIdx.y + 1) / gridDim.y - k_begin;
+            if (tiles_k > 0) {
+                issue(k_begin * kWideRow, 0);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+            for (int64_t kt = 0; kt < tiles_k; ++kt) {
+                const int stage = (int)(kt & 1);
+                if (kt + 1 < tiles_k) issue((k_begin + kt + 1) * kWideRow, stage ^ 1);
+                const unsigned char *a_base = lds + stage * kWideStage, *b_base = a_base + kWide * kWideRow;
+                // Fragments in two register sets: step s + 1's LDS reads are issued before step s's
+                // sixteen MFMAs, so their latency hides under the matrix work.
+                v8bf a[2][4], b[2][4];
+                auto read = [&](int set, int s) {
+                    const int chunk = 2 * s + half;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        a[set][i] = *reinterpret_cast<const v8bf *>(a_base + a_row[i] * kWideRow +
+                                                                    wide_position(a_row[i], chunk) * 16);
+                        b[set][i] = *reinterpret_cast<const v8bf *>(b_base + b_row[i] * kWideRow +
+                                                                    wide_position(b_row[i], chunk) * 16);
+                    }
+                };
+                read(0, 0);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    if (s < 3) read((s + 1) & 1, s + 1);
+                    // keep the compiler from sinking those reads next to their MFMAs
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s & 1][i], b[s & 1][j], acc[i][j],
+                                                                                 0, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+
+            float *const c_base = static_cast<float *>(p.c) + (int64_t)blockIdx.y * p.slice_stride;
+            const float alpha = (float)p.alpha, beta = (float)p.beta;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int64_t col = col0 + wn * 128 + j * 32 + (lane & 31);
+                    float old[16];
+                    if (beta != 0.0f) {
+#pragma unroll
+                        for (int v = 0; v < 16; ++v) {
+                            const int64_t row = row0 + wm * 128 + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
+                            old[v] = (row < p.m && col < p.n) ? c_base[row * p.ldc + col] : 0.0f;
+                        }
+                    }
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int64_t row = row0 + wm * 128 + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
+                        if (row >= p.m || col >= p.n) continue;
+                        const float x = acc[i][j][v];
+                        c_base[row * p.ldc + col] = beta == 0.0f ? (alpha == 1.0f ? x : alpha * x)
+                                                                  : alpha * x + beta * old[v];
+                    }
+                }
+        }
+
         // 64 x 64 tiles through LDS, so that both the plain and the transposed copy read and write
         // global memory along rows.
         template <typename S, typename D, bool split>
@@ -277,7 +441,24 @@ namespace scihip {
     } // namespace
 
     namespace {
+        // The large-tile kernel takes bf16 products whose output spans at least a 256 x 256 tile and
+        // whose shared dimension has at least 8 k-tiles of 64; shorter ones (such as the LU's rank-128
+        // updates, bound by reading and writing C) keep the 128 x 128 kernel.
+        bool wide_fits(int mode, const ProductArgs &args) {
+            // enough 256 x 256 tiles to give every compute unit one (fewer run faster as 128 x 128 tiles),
+            // and 32-bit buffer offsets: a tile's 256 rows of the operands span less than 2 GiB
+            const int64_t tiles = ((args.m + kWide - 1) / kWide) * ((args.n + kWide - 1) / kWide);
+            return mode == kBfloat16 && tiles >= 256 && args.kp % 64 == 0 && args.kp >= 512 &&
+                   int64_t(kWide) * std::max(args.lda, args.ldb) * 2 < (int64_t(1) << 31);
+        }
+
         hipError_t launch_gemm(int mode, const ProductArgs &args, int slices, hipStream_t stream, bool alongside = false) {
+            if (wide_fits(mode, args)) {
+                const int64_t wide_tiles = ((args.m + kWide - 1) / kWide) * ((args.n + kWide - 1) / kWide);
+                hipLaunchKernelGGL(gemm_wide<1>, dim3((unsigned)wide_tiles, (unsigned)slices), dim3(kThreads),
+                                   2 * kWideStage, stream, args);
+                return hipGetLastError();
+            }
             const int64_t tiles = ((args.m + kTile - 1) / kTile) * ((args.n + kTile - 1) / kTile);
             if (tiles == 0) return hipSuccess;
             const dim3 grid((unsigned)tiles, (unsigned)slices), block(kThreads);
