@@ -234,168 +234,300 @@ namespace scihip {
             }
         }
 
-        // The large-tile bf16 product for long shared dimensions. A 256-thread workgroup computes a
-        // 256 x 256 tile of C, each wave 128 x 128 as 4 x 4 blocks of v_mfma_f32_32x32x16_bf16
-        // (256 accumulators per lane, one wave per SIMD). Operands stream global -> LDS with
-        // global_load_lds_dwordx4 (each lane's 16 bytes land at the wave's LDS base + 16 * lane), in
-        // k-tiles of 64 values (128 bytes per row), double-buffered: the next k-tile's loads are in
-        // flight while the current one is multiplied, and one barrier per k-tile both publishes the
-        // next tile and retires the current one. LDS rows are stored with their eight 16-byte chunks
-        // permuted, chunk c of row r at position c ^ ((r / 2) % 8): the 16 lanes that read one chunk
-        // of 16 consecutive rows in one LDS pass then cover all 64 banks once. The permutation costs
-        // nothing: each lane simply fetches the chunk that belongs at its LDS position.
-        // 32 x 32 x 16 layout (measured on gfx950): lane l supplies row (column) l % 32 of A (B) at k
-        // 8 (l / 32) ... + 7; accumulator v of lane l is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
-        constexpr int kWide = 256;          // rows and columns of C per workgroup
-        constexpr int kWideRow = 128;       // bytes of a k-tile row (64 bf16)
-        constexpr int kWideStage = 2 * kWide * kWideRow; // A and B of one k-tile: 64 KiB
+        // The large-tile bf16 product for long shared dimensions. Its operands are packed in slabs
+        // (product.hpp): each 32-value k-stage of 256 rows is one contiguous 16 KiB block, rows 64 bytes
+        // apart with their four 16-byte chunks permuted (chunk c of row r at position c ^ ((r / 4) % 4)).
+        // A persistent grid, one 256-thread workgroup per compute unit, walks the 256 x 256 tiles of C;
+        // each wave computes 128 x 128 of a tile as 4 x 4 blocks of v_mfma_f32_32x32x16_bf16 (256
+        // accumulators per lane, one wave per SIMD).
+        // - Stages stream global -> LDS with buffer_load_dwordx4 ... lds, lane l's 16 bytes landing at
+        //   the wave's LDS base + 16 l, so each load copies 1 KiB of a slab verbatim. A ring of
+        //   kSlabStages slots (160 KiB, all of the LDS) keeps the loads kSlabStages - 1 stages ahead.
+        // - Each stage is two k-steps of 16. A wave's fragment reads for one k-step go out while the
+        //   other k-step's sixteen MFMAs run, and the one barrier per stage sits between the two
+        //   halves, so neither the LDS reads nor the barrier leave the matrix cores idle for long.
+        // - The permutation makes the 16 lanes of one LDS pass (one chunk of 16 consecutive rows) cover
+        //   all 64 banks once.
+        // - The MFMA takes the B fragment as its first operand, so it computes blocks of C transposed:
+        //   accumulator v of lane l is C[32 i + l % 32][32 j + 8 (v / 4) + 4 (l / 32) + v % 4], four
+        //   consecutive columns per lane and group of four, stored as one 16-byte vector.
+        // 32 x 32 x 16 layout (measured on gfx950): lane l supplies row l % 32 of its first operand and
+        // column l % 32 of its second, at k 8 (l / 32) ... + 7; accumulator v of lane l is row
+        // 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32 of the product.
+        constexpr int kWide = kSlabRows;   // rows and columns of C per tile
+        constexpr int kStageBytes = kSlabRows * kSlabK * 2;    // one operand's stage: 16 KiB
+        constexpr int kSlotBytes = 2 * kStageBytes;              // A and B
+        constexpr int kSlabStages = 5;                          // 160 KiB of LDS
+        constexpr int kStageLoads = kStageBytes / 1024 / 4;      // 1 KiB loads per wave per operand
 
         typedef float v16f __attribute__((ext_vector_type(16)));
         typedef __attribute__((address_space(3))) void *lds_pointer;
 
-        __device__ __forceinline__ int wide_position(int row, int chunk) { return chunk ^ ((row >> 1) & 7); }
+        __device__ __forceinline__ int slab_position(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
 
-        template <int Waves = 1>
-        __global__ __launch_bounds__(kThreads, Waves) void gemm_wide(const ProductArgs p) {
+        // Waits until at most `pending` stages of this wave's loads are in flight (the newest ones:
+        // loads complete in order for the count).
+        __device__ __forceinline__ void wait_stages(int pending) {
+            static_assert(2 * kStageLoads == 8, "the counts below assume eight loads per stage");
+            switch (pending) {
+            case 0: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+            case 1: asm volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+            case 2: asm volatile("s_waitcnt vmcnt(16)" ::: "memory"); break;
+            default: asm volatile("s_waitcnt vmcnt(24)" ::: "memory"); break;
+            }
+        }
+
+        // A workgroup barrier that waits for nothing in flight: the compiler's own barrier would first
+        // wait for every global load outstanding, the very loads the ring keeps running ahead. Each
+        // wave has waited (wait_stages) for its share of the stage the others will read, and the LDS
+        // reads it issued before the barrier have returned their data to the MFMAs that consumed them;
+        // the clobber keeps the compiler from moving LDS accesses across it.
+        __device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+        template <int Stages = kSlabStages>
+        __global__ __launch_bounds__(kThreads, 1) void gemm_slabs(const ProductArgs p) {
+            static_assert(Stages >= 3 && Stages <= 5, "wait_stages counts up to three pending stages");
             extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
             const int tiles_m = (int)((p.m + kWide - 1) / kWide);
             const int tiles_n = (int)((p.n + kWide - 1) / kWide);
-            int tm, tn;
-            tile_of(tiles_m, tiles_n, tm, tn);
-            const int64_t row0 = (int64_t)tm * kWide, col0 = (int64_t)tn * kWide;
+            const int tiles = tiles_m * tiles_n;
             // the wave number is the same in every lane: say so, so that LDS bases (M0) are scalar
             const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-            const int wm = wave >> 1, wn = wave & 1;
+            const int wm = wave >> 1, wn = wave & 1, half = lane >> 5;
+            const int64_t stages = p.kp / kSlabK;
+            const uint32_t slab = (uint32_t)(stages * kStageBytes); // bytes of one 256-row slab
 
-            // This lane's part of each direct load: wave `wave` fills the LDS blocks 8 wave ... 8 wave + 7
-            // of each operand (1 KiB each: 8 rows of 128 bytes); lane l of block b writes row 8 b + l / 8
-            // at position l % 8 and so fetches chunk (l % 8) ^ ((row / 2) % 8) of that row, which is
-            // (l % 8) ^ (4 (b % 2) + l / 16). The loads go through buffer descriptors based at the
-            // tile's first row and sized to the rows that exist: the per-lane offset (row and chunk) is
-            // range-checked, so rows past the matrix read zeros; the k offset rides in soffset.
-            const int64_t lda = p.lda * 2, ldb = p.ldb * 2;
+            // This lane's part of each load: wave w copies the 1 KiB pieces 4 w ... 4 w + 3 of each
+            // operand's stage, lane l the 16 bytes at 16 l of a piece. The stage offset rides in soffset.
+            uint32_t lane_offset[kStageLoads];
+#pragma unroll
+            for (int i = 0; i < kStageLoads; ++i) lane_offset[i] = (uint32_t)((wave * kStageLoads + i) * 1024 + lane * 16);
+            // Fragment addresses within a slot: block i of A is rows 128 wm + 32 i + l % 32, block j of
+            // B rows 128 wn + 32 j + l % 32; k-step s reads chunk 2 s + l / 32 of its row.
+            int a_at[2][4], b_at[2][4];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int a_row = wm * 128 + i * 32 + (lane & 31), b_row = wn * 128 + i * 32 + (lane & 31);
+                    a_at[s][i] = a_row * 64 + slab_position(a_row, 2 * s + half) * 16;
+                    b_at[s][i] = kStageBytes + b_row * 64 + slab_position(b_row, 2 * s + half) * 16;
+                }
             // (every field is the same in all lanes; readfirstlane puts the descriptors in SGPRs, where
             // the buffer instructions need them)
             auto scalar = [](uint64_t x) {
                 return (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32 |
                        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
             };
-            const int64_t a_bytes = (p.m - row0) * lda, b_bytes = (p.n - col0) * ldb;
-            const uint32_t a_records =
-                    __builtin_amdgcn_readfirstlane(a_bytes > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)a_bytes);
-            const uint32_t b_records =
-                    __builtin_amdgcn_readfirstlane(b_bytes > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)b_bytes);
-            const __amdgpu_buffer_rsrc_t a_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-                    (void *)scalar((uint64_t)(static_cast<const unsigned char *>(p.a_hi) + row0 * lda)), 0, a_records,
-                    0x00020000);
-            const __amdgpu_buffer_rsrc_t b_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-                    (void *)scalar((uint64_t)(static_cast<const unsigned char *>(p.b_hi) + col0 * ldb)), 0, b_records,
-                    0x00020000);
-            uint32_t a_offset[8], b_offset[8];
+            auto descriptor = [&](const void *base, int index) {
+                return __builtin_amdgcn_make_buffer_rsrc(
+                        (void *)scalar((uint64_t)(static_cast<const unsigned char *>(base) + (uint64_t)index * slab)),
+                        0, slab, 0x00020000);
+            };
+            auto issue = [&](__amdgpu_buffer_rsrc_t a_rsrc, __amdgpu_buffer_rsrc_t b_rsrc, int64_t stage, int slot) {
+                unsigned char *base = lds + slot * kSlotBytes;
+                const int k_byte = (int)(stage * kStageBytes);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int row = (wave * 8 + i) * 8 + (lane >> 3);
-                const int chunk = wide_position(row, lane & 7);
-                a_offset[i] = (uint32_t)(row * lda + chunk * 16);
-                b_offset[i] = (uint32_t)(row * ldb + chunk * 16);
-            }
-            auto issue = [&](int64_t k_byte, int stage) {
-                unsigned char *base = lds + stage * kWideStage;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(a_rsrc, (lds_pointer)(base + (wave * 8 + i) * 1024), 16,
-                                                              a_offset[i], (int)k_byte, 0, 0);
+                for (int i = 0; i < kStageLoads; ++i) {
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(a_rsrc, (lds_pointer)(base + (wave * kStageLoads + i) * 1024),
+                                                              16, lane_offset[i], k_byte, 0, 0);
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                            b_rsrc, (lds_pointer)(base + kWide * kWideRow + (wave * 8 + i) * 1024), 16, b_offset[i],
-                            (int)k_byte, 0, 0);
+                            b_rsrc, (lds_pointer)(base + kStageBytes + (wave * kStageLoads + i) * 1024), 16,
+                            lane_offset[i], k_byte, 0, 0);
                 }
             };
+            auto place = [&](int t, int &tm, int &tn) {
+                // XCD-contiguous numbering, then groups of kGroup tile rows walked column by column
+                int id = t;
+                if (tiles % kXcds == 0) id = (id % kXcds) * (tiles / kXcds) + id / kXcds;
+                const int width = kGroup * tiles_n;
+                const int first = (id / width) * kGroup;
+                const int rows = min(tiles_m - first, kGroup);
+                const int within = id % width;
+                tm = first + within % rows;
+                tn = within / rows;
+            };
+            auto prologue = [&](int t) {
+                int tm, tn;
+                place(t, tm, tn);
+                const __amdgpu_buffer_rsrc_t a_rsrc = descriptor(p.a_hi, tm), b_rsrc = descriptor(p.b_hi, tn);
+#pragma unroll
+                for (int s = 0; s < Stages - 1; ++s)
+                    if (s < stages) issue(a_rsrc, b_rsrc, s, s);
+            };
 
-            v16f acc[4][4];
+            const bool vector_store = p.beta == 0.0 && (p.ldc & 3) == 0 && ((uintptr_t)p.c & 15) == 0;
+            const float alpha = (float)p.alpha, beta = (float)p.beta;
+            if ((int)blockIdx.x < tiles) prologue(blockIdx.x);
+            for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+                int tm, tn;
+                place(t, tm, tn);
+                const __amdgpu_buffer_rsrc_t a_rsrc = descriptor(p.a_hi, tm), b_rsrc = descriptor(p.b_hi, tn);
+                v16f acc[4][4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                    for (int j = 0; j < 4; ++j)
 #pragma unroll
-                    for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+                        for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-            // Fragment addresses within a stage: block i of A is rows 128 wm + 32 i + l % 32, block j of
-            // B rows 128 wn + 32 j + l % 32; k-step s (16 values) reads chunk 2 s + l / 32.
-            int a_row[4], b_row[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a_row[i] = wm * 128 + i * 32 + (lane & 31);
-                b_row[i] = wn * 128 + i * 32 + (lane & 31);
-            }
-            const int half = lane >> 5;
-
-            const int64_t tiles_all = p.kp / 64;
-            const int64_t k_begin = tiles_all * blockIdx.y / gridDim.y;
-            const int64_t tiles_k = tiles_all * (blockIdx.y + 1) / gridDim.y - k_begin;
-            if (tiles_k > 0) {
-                issue(k_begin * kWideRow, 0);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-            }
-            for (int64_t kt = 0; kt < tiles_k; ++kt) {
-                const int stage = (int)(kt & 1);
-                if (kt + 1 < tiles_k) issue((k_begin + kt + 1) * kWideRow, stage ^ 1);
-                const unsigned char *a_base = lds + stage * kWideStage, *b_base = a_base + kWide * kWideRow;
-                // Fragments in two register sets: step s + 1's LDS reads are issued before step s's
-                // sixteen MFMAs, so their latency hides under the matrix work.
-                v8bf a[2][4], b[2][4];
-                auto read = [&](int set, int s) {
-                    const int chunk = 2 * s + half;
+                v8bf x[2][4], y[2][4]; // fragments of k-step 0 (x) and 1 (y) of the current stage: [A, B][block]
+                auto read = [&](v8bf (&f)[2][4], int slot, int step) {
+                    const unsigned char *base = lds + slot * kSlotBytes;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        a[set][i] = *reinterpret_cast<const v8bf *>(a_base + a_row[i] * kWideRow +
-                                                                    wide_position(a_row[i], chunk) * 16);
-                        b[set][i] = *reinterpret_cast<const v8bf *>(b_base + b_row[i] * kWideRow +
-                                                                    wide_position(b_row[i], chunk) * 16);
+                        f[0][i] = *reinterpret_cast<const v8bf *>(base + a_at[step][i]);
+                        f[1][i] = *reinterpret_cast<const v8bf *>(base + b_at[step][i]);
                     }
                 };
-                read(0, 0);
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    if (s < 3) read((s + 1) & 1, s + 1);
-                    // keep the compiler from sinking those reads next to their MFMAs
-                    __builtin_amdgcn_sched_barrier(0);
+                auto multiply = [&](const v8bf (&f)[2][4]) {
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s & 1][i], b[s & 1][j], acc[i][j],
-                                                                                 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][j], f[0][i], acc[i][j], 0, 0, 0);
+                };
+
+                // this tile's first stages were issued by the prologue (with the previous tile's stores
+                // after them in the count, so wait for everything)
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                lds_barrier();
+                read(x, 0, 0);
+                int slot = 0;
+                // One stage: k-step 1's reads go out, k-step 0's MFMAs run; then (unless this is the last
+                // stage) the next stage is awaited and published, the ring slot two behind refilled and
+                // the next stage's k-step 0 read while k-step 1's MFMAs run. `more` is a constant at
+                // every call, so the LDS reads are never conditional (which would make the compiler
+                // wait for all of them before the MFMAs).
+                auto advance = [&](int64_t kt, bool more, int pending) {
+                    read(y, slot, 1);
                     __builtin_amdgcn_sched_barrier(0);
+                    multiply(x);
+                    __builtin_amdgcn_sched_barrier(0);
+                    const int next = slot + 1 == Stages ? 0 : slot + 1;
+                    if (more) {
+                        wait_stages(pending);
+                        lds_barrier();
+                        // every wave is past stage kt - 1: its slot takes stage kt + Stages - 1
+                        if (kt + Stages - 1 < stages)
+                            issue(a_rsrc, b_rsrc, kt + Stages - 1, slot == 0 ? Stages - 1 : slot - 1);
+                        read(x, next, 0);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                    multiply(y);
+                    __builtin_amdgcn_sched_barrier(0);
+                    slot = next;
+                };
+                // stage kt + 1 has landed once only the stages after it, up to kt + Stages - 2, are pending
+                int64_t kt = 0;
+                for (; kt < stages - 1 && kt < stages - Stages + 2; ++kt) advance(kt, true, Stages - 3);
+                for (; kt < stages - 1; ++kt) advance(kt, true, (int)(stages - 2 - kt));
+                advance(stages - 1, false, 0);
+                // every wave has read its last fragments: the next tile's first stages may land
+                lds_barrier();
+                if (t + (int)gridDim.x < tiles) prologue(t + gridDim.x);
+
+                // Epilogue: block (i, j) of this wave covers rows row0 + 32 i + l % 32 and columns
+                // col0 + 32 j + 8 g + 4 (l / 32) + e for accumulator v = 4 g + e.
+                const int64_t row0 = (int64_t)tm * kWide + wm * 128, col0 = (int64_t)tn * kWide + wn * 128;
+                float *const c_base = static_cast<float *>(p.c);
+                const bool interior = row0 + 128 <= p.m && col0 + 128 <= p.n;
+                if (vector_store && interior) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        float *row = c_base + (row0 + i * 32 + (lane & 31)) * p.ldc + col0 + 4 * half;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+#pragma unroll
+                            for (int g = 0; g < 4; ++g) {
+                                const v4f value = {acc[i][j][4 * g], acc[i][j][4 * g + 1], acc[i][j][4 * g + 2],
+                                                   acc[i][j][4 * g + 3]};
+                                *reinterpret_cast<v4f *>(row + j * 32 + g * 8) = alpha == 1.0f ? value : alpha * value;
+                            }
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int64_t row = row0 + i * 32 + (lane & 31);
+                        if (row >= p.m) continue;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+#pragma unroll
+                            for (int v = 0; v < 16; ++v) {
+                                const int64_t col = col0 + j * 32 + (v >> 2) * 8 + 4 * half + (v & 3);
+                                if (col >= p.n) continue;
+                                float *target = c_base + row * p.ldc + col;
+                                const float value = acc[i][j][v];
+                                *target = beta == 0.0f ? (alpha == 1.0f ? value : alpha * value)
+                                                       : alpha * value + beta * *target;
+                            }
+                    }
                 }
-                __builtin_amdgcn_s_waitcnt(0);
+            }
+        }
+
+        // Packs one 32-value k-stage of one 256-row slab per workgroup. Thread t writes the 16-byte
+        // positions t, t + 256, t + 512 and t + 768 of the stage (row q / 4, position q % 4, holding
+        // chunk (q % 4) ^ ((row / 4) % 4), k 8 chunk ... + 7), so each wave writes 1 KiB in order.
+        // Plain sources: each thread reads its chunk's eight values in place (two 16-byte loads when
+        // the rows allow). Transposed sources: the workgroup first copies the 32 x 256 block, read
+        // along the source's rows, into LDS.
+        template <typename S> constexpr int transpose_rows() { return kSlabK; }
+        template <typename S> struct Quad;
+        template <> struct Quad<float> { typedef float4 type; };
+        template <> struct Quad<double> { typedef double4 type; };
+
+        template <typename S>
+        __global__ __launch_bounds__(256) void pack_slab(const S *source, int64_t rows, int64_t depth, int64_t ld,
+                                                         bool transpose, uint4 *out, int64_t kp) {
+            typedef typename Quad<S>::type S4;
+            constexpr int kStride = kSlabRows + 4; // LDS row stride, 16-byte aligned rows
+            __shared__ __attribute__((aligned(16))) S tile[transpose_rows<S>()][kStride];
+            const int64_t stage = blockIdx.x, row0 = (int64_t)blockIdx.y * kSlabRows, k0 = stage * kSlabK;
+            const int t = threadIdx.x;
+            // four consecutive elements of a source row at once when rows keep 16-byte alignment
+            const bool quads = (ld & 3) == 0 && ((uintptr_t)source & (sizeof(S4) - 1)) == 0;
+            if (transpose) { // element (i, k) = source[k * ld + i]
+#pragma unroll
+                for (int pass = 0; pass < kSlabK / 4; ++pass) {
+                    const int kk = pass * 4 + (t >> 6), c = (t & 63) * 4;
+                    const int64_t k = k0 + kk, i = row0 + c;
+                    S4 value;
+                    if (quads && k < depth && i + 3 < rows) {
+                        value = *reinterpret_cast<const S4 *>(source + k * ld + i);
+                    } else {
+                        S v[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) v[e] = (k < depth && i + e < rows) ? source[k * ld + i + e] : S(0);
+                        value = S4{v[0], v[1], v[2], v[3]};
+                    }
+                    *reinterpret_cast<S4 *>(&tile[kk][c]) = value;
+                }
                 __syncthreads();
             }
-
-            float *const c_base = static_cast<float *>(p.c) + (int64_t)blockIdx.y * p.slice_stride;
-            const float alpha = (float)p.alpha, beta = (float)p.beta;
+            uint4 *const stage_out = out + ((int64_t)blockIdx.y * (kp / kSlabK) + stage) * (kSlabRows * kSlabK / 8);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int q4 = 0; q4 < 4; ++q4) {
+                const int q = t + q4 * 256, r = q >> 2, chunk = slab_position(r, q & 3);
+                const int64_t i = row0 + r, k = k0 + chunk * 8;
+                S v[8];
+                if (transpose) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int64_t col = col0 + wn * 128 + j * 32 + (lane & 31);
-                    float old[16];
-                    if (beta != 0.0f) {
+                    for (int e = 0; e < 8; ++e) v[e] = tile[chunk * 8 + e][r];
+                } else if (quads && i < rows && k + 8 <= depth) {
+                    const S4 lo = *reinterpret_cast<const S4 *>(source + i * ld + k);
+                    const S4 hi = *reinterpret_cast<const S4 *>(source + i * ld + k + 4);
+                    v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
+                    v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
+                } else {
 #pragma unroll
-                        for (int v = 0; v < 16; ++v) {
-                            const int64_t row = row0 + wm * 128 + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
-                            old[v] = (row < p.m && col < p.n) ? c_base[row * p.ldc + col] : 0.0f;
-                        }
-                    }
-#pragma unroll
-                    for (int v = 0; v < 16; ++v) {
-                        const int64_t row = row0 + wm * 128 + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
-                        if (row >= p.m || col >= p.n) continue;
-                        const float x = acc[i][j][v];
-                        c_base[row * p.ldc + col] = beta == 0.0f ? (alpha == 1.0f ? x : alpha * x)
-                                                                  : alpha * x + beta * old[v];
-                    }
+                    for (int e = 0; e < 8; ++e) v[e] = (i < rows && k + e < depth) ? source[i * ld + k + e] : S(0);
                 }
+                __bf16 h[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) h[e] = (__bf16)(float)v[e];
+                stage_out[q] = __builtin_bit_cast(uint4, h);
+            }
         }
 
         // 64 x 64 tiles through LDS, so that both the plain and the transposed copy read and write
@@ -441,22 +573,28 @@ namespace scihip {
     } // namespace
 
     namespace {
-        // The large-tile kernel takes bf16 products whose output spans at least a 256 x 256 tile and
-        // whose shared dimension has at least 8 k-tiles of 64; shorter ones (such as the LU's rank-128
-        // updates, bound by reading and writing C) keep the 128 x 128 kernel.
-        bool wide_fits(int mode, const ProductArgs &args) {
-            // enough 256 x 256 tiles to give every compute unit one (fewer run faster as 128 x 128 tiles),
-            // and 32-bit buffer offsets: a tile's 256 rows of the operands span less than 2 GiB
-            const int64_t tiles = ((args.m + kWide - 1) / kWide) * ((args.n + kWide - 1) / kWide);
-            return mode == kBfloat16 && tiles >= 256 && args.kp % 64 == 0 && args.kp >= 512 &&
-                   int64_t(kWide) * std::max(args.lda, args.ldb) * 2 < (int64_t(1) << 31);
+        // Compute units of the current device (the persistent kernel's grid), cached per device.
+        int compute_units() {
+            static int cached[64] = {};
+            int device = 0;
+            if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return 256;
+            if (cached[device] == 0) {
+                int count = 0;
+                if (hipDeviceGetAttribute(&count, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || count <= 0)
+                    count = 256;
+                cached[device] = count;
+            }
+            return cached[device];
         }
 
         hipError_t launch_gemm(int mode, const ProductArgs &args, int slices, hipStream_t stream, bool alongside = false) {
-            if (wide_fits(mode, args)) {
-                const int64_t wide_tiles = ((args.m + kWide - 1) / kWide) * ((args.n + kWide - 1) / kWide);
-                hipLaunchKernelGGL(gemm_wide<1>, dim3((unsigned)wide_tiles, (unsigned)slices), dim3(kThreads),
-                                   2 * kWideStage, stream, args);
+            if (args.slabs) {
+                if (mode != kBfloat16 || slices != 1 || args.kp % kSlabK != 0) return hipErrorInvalidValue;
+                const int64_t tiles = ((args.m + kWide - 1) / kWide) * ((args.n + kWide - 1) / kWide);
+                if (tiles == 0) return hipSuccess;
+                const int grid = (int)std::min<int64_t>(tiles, compute_units());
+                hipLaunchKernelGGL(gemm_slabs<kSlabStages>, dim3(grid), dim3(kThreads), kSlabStages * kSlotBytes,
+                                   stream, args);
                 return hipGetLastError();
             }
             const int64_t tiles = ((args.m + kTile - 1) / kTile) * ((args.n + kTile - 1) / kTile);
@@ -521,6 +659,28 @@ namespace scihip {
         ProductArgs whole = args;
         whole.slice_stride = 0;
         return launch_gemm(mode, whole, 1, stream, true);
+    }
+
+    bool slab_product(int mode, int64_t m, int64_t n, int64_t kp) {
+        // enough 256 x 256 tiles to give every compute unit one (fewer run faster as 128 x 128 tiles),
+        // at least 16 stages, and a slab's stage offsets within 32 bits
+        const int64_t tiles = ((m + kWide - 1) / kWide) * ((n + kWide - 1) / kWide);
+        return mode == kBfloat16 && tiles >= 256 && kp % kSlabK == 0 && kp >= 16 * kSlabK &&
+               kp * int64_t(kSlabRows) * 2 < (int64_t(1) << 31);
+    }
+
+    hipError_t pack_slabs(const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,
+                           void *out, int64_t kp, hipStream_t stream) {
+        if (rows == 0 || kp == 0) return hipSuccess;
+        if (kp % kSlabK != 0) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(kp / kSlabK), (unsigned)((rows + kSlabRows - 1) / kSlabRows)), block(256);
+        if (wide)
+            hipLaunchKernelGGL(pack_slab<double>, grid, block, 0, stream, static_cast<const double *>(source), rows,
+                               depth, ld, transpose, static_cast<uint4 *>(out), kp);
+        else
+            hipLaunchKernelGGL(pack_slab<float>, grid, block, 0, stream, static_cast<const float *>(source), rows,
+                               depth, ld, transpose, static_cast<uint4 *>(out), kp);
+        return hipGetLastError();
     }
 
     hipError_t pack(int mode, const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,

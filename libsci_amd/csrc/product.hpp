@@ -27,7 +27,28 @@ namespace scihip {
         int64_t lda, ldb, ldc;
         double alpha, beta;
         int64_t slice_stride; // split-K launches: elements between consecutive slices' outputs
+        int slabs = 0;       // kBfloat16 only: a_hi and b_hi are packed in slabs (pack_slabs), lda/ldb unused
     };
+
+    // Slab layout of the large-tile bf16 product: an operand of R rows over Kp is ceil(R / 256)
+    // slabs of 256 rows (rows past R zero), each Kp / 32 stages of 16 KiB, one stage holding the 32
+    // k values of all 256 rows: row r at byte 64 r, its 16-byte chunk c (k 8 c ... 8 c + 7) at
+    // position c ^ ((r / 4) % 4). One stage is one contiguous block, copied to LDS as it lies.
+    constexpr int kSlabRows = 256;
+    constexpr int kSlabK = 32;
+    inline size_t slab_bytes(int64_t rows, int64_t kp) {
+        return size_t((rows + kSlabRows - 1) / kSlabRows) * kSlabRows * size_t(kp) * 2;
+    }
+
+    // Whether a product of an m x n output over kp (a multiple of k_tile(mode)) takes the large-tile
+    // kernel: bf16, at least one 256 x 256 tile per compute unit and at least 16 stages.
+    bool slab_product(int mode, int64_t m, int64_t n, int64_t kp);
+
+    // Packs a row-major float (wide: double) source into slabs, as pack() does into rows: element
+    // (i, k) is src[i][k] (transpose false) or src[k][i] (transpose true), rounded to float and then
+    // to bf16, zero for k >= depth or i >= rows.
+    hipError_t pack_slabs(const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,
+                           void *out, int64_t kp, hipStream_t stream);
 
     // Bytes of one k-tile row in LDS and the k extent it holds, per mode.
     constexpr int k_tile(int mode) { return mode == kFloat32 ? 32 : mode == kFloat64 ? 16 : mode == kSplit ? 32 : 64; }

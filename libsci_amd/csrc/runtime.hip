@@ -381,8 +381,31 @@ namespace scihip {
         const bool split = mode == kSplit, wide = mode == kFloat64;
         void *a_hi = nullptr, *a_lo = nullptr, *b_hi = nullptr, *b_lo = nullptr;
         int rc = SCIHIP_OK;
-        const size_t a_bytes = std::max<size_t>(size_t(rows * kp) * eb, 16);
-        const size_t b_bytes = std::max<size_t>(size_t(columns * kp) * eb, 16);
+        if (slab_product(mode, rows, columns, kp)) {
+            // the large-tile kernel: both operands packed in slabs
+            rc = device->allocator->allocate(slab_bytes(rows, kp), &a_hi);
+            if (!rc) rc = device->allocator->allocate(slab_bytes(columns, kp), &b_hi);
+            if (!rc)
+                rc = status(pack_slabs(left, false, rows, depth, left_ld, left_transposed, a_hi, kp, device->stream),
+                            "pack");
+            if (!rc)
+                rc = status(pack_slabs(right, false, columns, depth, right_ld, right_transposed, b_hi, kp,
+                                        device->stream),
+                            "pack");
+            if (!rc) {
+                ProductArgs args{a_hi, nullptr, b_hi, nullptr, output, rows, columns, kp, kp, kp, columns, 1.0, 0.0};
+                args.slabs = 1;
+                rc = status(product(mode, args, device->stream), "product");
+            }
+            for (void *block : {a_hi, b_hi})
+                if (block) device->allocator->dispose(block);
+            return rc;
+        }
+        // Packed rows a multiple of 1 KiB apart put the same k of every row on the same few L2
+        // channels; 128 bytes more per row spread them (the padding is never read).
+        const int64_t ld = kp + ((kp * int64_t(eb)) % 1024 == 0 ? 128 / int64_t(eb) : 0);
+        const size_t a_bytes = std::max<size_t>(size_t(rows * ld) * eb, 16);
+        const size_t b_bytes = std::max<size_t>(size_t(columns * ld) * eb, 16);
         if (!rc) rc = device->allocator->allocate(a_bytes, &a_hi);
         if (!rc) rc = device->allocator->allocate(b_bytes, &b_hi);
         if (!rc && split) rc = device->allocator->allocate(a_bytes, &a_lo);
@@ -391,14 +414,15 @@ namespace scihip {
         void *partials = nullptr;
         if (!rc && slices > 1) rc = device->allocator->allocate(size_t(slices) * rows * columns * (wide ? 8 : 4), &partials);
         if (!rc)
-            rc = status(pack(mode, left, wide, rows, depth, left_ld, left_transposed, a_hi, a_lo, kp, device->stream),
+            rc = status(pack(mode, left, wide, rows, depth, left_ld, left_transposed, a_hi, a_lo, kp, device->stream,
+                             ld),
                         "pack");
         if (!rc)
             rc = status(pack(mode, right, wide, columns, depth, right_ld, right_transposed, b_hi, b_lo, kp,
-                             device->stream),
+                             device->stream, ld),
                         "pack");
         if (!rc) {
-            ProductArgs args{a_hi, a_lo, b_hi, b_lo, output, rows, columns, kp, kp, kp, columns, 1.0, 0.0};
+            ProductArgs args{a_hi, a_lo, b_hi, b_lo, output, rows, columns, kp, ld, ld, columns, 1.0, 0.0};
             rc = status(product_split(mode, args, slices, partials, device->stream), "product");
         }
         for (void *block : {a_hi, a_lo, b_hi, b_lo, partials})

@@ -118,3 +118,28 @@ def test_metric_sizes(device, shape):
               flush=True)
         assert (error <= 2 * k * UNIT[mode] * magnitude).all()
         assert relative < frobenius[mode]
+
+
+@pytest.mark.parametrize("shape", [(4100, 1000, 4200), (4352, 576, 4097)], ids=["4100x1000x4200", "4352x576x4097"])
+def test_large_tiles_ragged(device, shape):
+    """bf16 products with at least 256 output tiles of 256 x 256 take the large-tile kernel; these
+    have partial last tiles in both directions and a shared dimension padded up to a multiple of 64
+    (1000 -> 1024) or not (576). Sampled rows, the last rows among them, against float64."""
+    m, k, n = shape
+    print(f"[test] bf16 large tiles at {shape}", flush=True)
+    left = sci.uniform(m * k, device, 41, -1.0, 1.0).reshape(m, k)
+    right = sci.uniform(k * n, device, 42, -1.0, 1.0).reshape(k, n)
+    output = sci.Array(device, (m, n), np.float32)
+    left_h = oracle.uniform(m * k, 41, -1.0, 1.0).reshape(m, k)
+    right_h = oracle.uniform(k * n, 42, -1.0, 1.0).reshape(k, n)
+    chosen = np.random.default_rng(m + n).choice(m, 40, replace=False)
+    rows = np.unique(np.concatenate([chosen, [0, 255, 256, m - 257, m - 256, m - 5, m - 1]]))
+    assert np.array_equal(left.host()[rows], left_h[rows])
+    right64 = right_h.astype(np.float64)
+    want = left_h[rows].astype(np.float64) @ right64
+    magnitude = np.abs(left_h[rows]).astype(np.float64) @ np.abs(right64)
+    _abi.check(_abi.lib().scihip_product(device.handle, MODES["bf16"], left.pointer, right.pointer,
+                                         output.pointer, m, n, k), "scihip_product")
+    got = output.host()[rows].astype(np.float64)
+    assert (np.abs(got - want) <= 2 * k * UNIT["bf16"] * magnitude).all()
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-2
