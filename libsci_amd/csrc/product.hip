@@ -285,7 +285,7 @@ namespace scihip {
         __device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
         template <int Stages = kSlabStages>
-        __global__ __launch_bounds__(kThreads, 1) void gemm_slabs(const ProductArgs p) {
+        __global__ __launch_bounds__(kThreads, 1) void gemm_slabs(const ProductArgs p, int slices) {
             static_assert(Stages >= 3 && Stages <= 5, "wait_stages counts up to three pending stages");
             extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
             const int tiles_m = (int)((p.m + kWide - 1) / kWide);
@@ -294,8 +294,15 @@ namespace scihip {
             // the wave number is the same in every lane: say so, so that LDS bases (M0) are scalar
             const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
             const int wm = wave >> 1, wn = wave & 1, half = lane >> 5;
-            const int64_t stages = p.kp / kSlabK;
-            const uint32_t slab = (uint32_t)(stages * kStageBytes); // bytes of one 256-row slab
+            const int64_t stages_all = p.kp / kSlabK;
+            const uint32_t slab = (uint32_t)(stages_all * kStageBytes); // bytes of one 256-row slab
+            // work item w: tile w % tiles over the k-stages of slice w / tiles, [first, first + count)
+            const int items = tiles * slices;
+            auto range = [&](int w, int64_t &first, int64_t &count) {
+                const int slice = w / tiles;
+                first = stages_all * slice / slices;
+                count = stages_all * (slice + 1) / slices - first;
+            };
 
             // This lane's part of each load: wave w copies the 1 KiB pieces 4 w ... 4 w + 3 of each
             // operand's stage, lane l the 16 bytes at 16 l of a piece. The stage offset rides in soffset.
@@ -347,21 +354,25 @@ namespace scihip {
                 tm = first + within % rows;
                 tn = within / rows;
             };
-            auto prologue = [&](int t) {
+            auto prologue = [&](int w) {
                 int tm, tn;
-                place(t, tm, tn);
+                int64_t first, count;
+                place(w % tiles, tm, tn);
+                range(w, first, count);
                 const __amdgpu_buffer_rsrc_t a_rsrc = descriptor(p.a_hi, tm), b_rsrc = descriptor(p.b_hi, tn);
 #pragma unroll
                 for (int s = 0; s < Stages - 1; ++s)
-                    if (s < stages) issue(a_rsrc, b_rsrc, s, s);
+                    if (s < count) issue(a_rsrc, b_rsrc, first + s, s);
             };
 
             const bool vector_store = p.beta == 0.0 && (p.ldc & 3) == 0 && ((uintptr_t)p.c & 15) == 0;
             const float alpha = (float)p.alpha, beta = (float)p.beta;
-            if ((int)blockIdx.x < tiles) prologue(blockIdx.x);
-            for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+            if ((int)blockIdx.x < items) prologue(blockIdx.x);
+            for (int w = blockIdx.x; w < items; w += gridDim.x) {
                 int tm, tn;
-                place(t, tm, tn);
+                int64_t first, stages;
+                place(w % tiles, tm, tn);
+                range(w, first, stages);
                 const __amdgpu_buffer_rsrc_t a_rsrc = descriptor(p.a_hi, tm), b_rsrc = descriptor(p.b_hi, tn);
                 v16f acc[4][4];
 #pragma unroll
@@ -410,7 +421,7 @@ namespace scihip {
                         lds_barrier();
                         // every wave is past stage kt - 1: its slot takes stage kt + Stages - 1
                         if (kt + Stages - 1 < stages)
-                            issue(a_rsrc, b_rsrc, kt + Stages - 1, slot == 0 ? Stages - 1 : slot - 1);
+                            issue(a_rsrc, b_rsrc, first + kt + Stages - 1, slot == 0 ? Stages - 1 : slot - 1);
                         read(x, next, 0);
                         __builtin_amdgcn_sched_barrier(0);
                     }
@@ -425,12 +436,12 @@ namespace scihip {
                 advance(stages - 1, false, 0);
                 // every wave has read its last fragments: the next tile's first stages may land
                 lds_barrier();
-                if (t + (int)gridDim.x < tiles) prologue(t + gridDim.x);
+                if (w + (int)gridDim.x < items) prologue(w + gridDim.x);
 
                 // Epilogue: block (i, j) of this wave covers rows row0 + 32 i + l % 32 and columns
                 // col0 + 32 j + 8 g + 4 (l / 32) + e for accumulator v = 4 g + e.
                 const int64_t row0 = (int64_t)tm * kWide + wm * 128, col0 = (int64_t)tn * kWide + wn * 128;
-                float *const c_base = static_cast<float *>(p.c);
+                float *const c_base = static_cast<float *>(p.c) + (int64_t)(w / tiles) * p.slice_stride;
                 const bool interior = row0 + 128 <= p.m && col0 + 128 <= p.n;
                 if (vector_store && interior) {
 #pragma unroll
@@ -589,12 +600,13 @@ namespace scihip {
 
         hipError_t launch_gemm(int mode, const ProductArgs &args, int slices, hipStream_t stream, bool alongside = false) {
             if (args.slabs) {
-                if (mode != kBfloat16 || slices != 1 || args.kp % kSlabK != 0) return hipErrorInvalidValue;
-                const int64_t tiles = ((args.m + kWide - 1) / kWide) * ((args.n + kWide - 1) / kWide);
-                if (tiles == 0) return hipSuccess;
-                const int grid = (int)std::min<int64_t>(tiles, compute_units());
+                if (mode != kBfloat16 || slices < 1 || args.kp % kSlabK != 0 || args.kp / kSlabK < slices)
+                    return hipErrorInvalidValue;
+                const int64_t items = ((args.m + kWide - 1) / kWide) * ((args.n + kWide - 1) / kWide) * slices;
+                if (items == 0) return hipSuccess;
+                const int grid = (int)std::min<int64_t>(items, compute_units());
                 hipLaunchKernelGGL(gemm_slabs<kSlabStages>, dim3(grid), dim3(kThreads), kSlabStages * kSlotBytes,
-                                   stream, args);
+                                   stream, args, slices);
                 return hipGetLastError();
             }
             const int64_t tiles = ((args.m + kTile - 1) / kTile) * ((args.n + kTile - 1) / kTile);
@@ -661,12 +673,17 @@ namespace scihip {
         return launch_gemm(mode, whole, 1, stream, true);
     }
 
-    bool slab_product(int mode, int64_t m, int64_t n, int64_t kp) {
-        // enough 256 x 256 tiles to give every compute unit one (fewer run faster as 128 x 128 tiles),
-        // at least 16 stages, and a slab's stage offsets within 32 bits
-        const int64_t tiles = ((m + kWide - 1) / kWide) * ((n + kWide - 1) / kWide);
-        return mode == kBfloat16 && tiles >= 256 && kp % kSlabK == 0 && kp >= 16 * kSlabK &&
-               kp * int64_t(kSlabRows) * 2 < (int64_t(1) << 31);
+    int slab_slices(int mode, int64_t m, int64_t n, int64_t kp) {
+        const int64_t tiles = ((m + kWide - 1) / kWide) * ((n + kWide - 1) / kWide), stages = kp / kSlabK;
+        // a slab's stage offsets within 32 bits
+        if (mode != kBfloat16 || tiles == 0 || kp % kSlabK != 0 || kp * int64_t(kSlabRows) * 2 >= (int64_t(1) << 31))
+            return 0;
+        // enough 256 x 256 tiles to give every compute unit one, over at least 16 stages
+        if (tiles >= 256) return stages >= 16 ? 1 : 0;
+        // fewer tiles: split the stages so that tiles x slices fills the chip, each slice at least 32
+        // stages (k >= 1024), at most 16 slices; below half the chip the 128 x 128 kernel does better
+        const int64_t slices = std::min<int64_t>({(256 + tiles - 1) / tiles, stages / 32, 16});
+        return slices >= 2 && tiles * slices >= 128 ? (int)slices : 0;
     }
 
     hipError_t pack_slabs(const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,

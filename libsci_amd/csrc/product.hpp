@@ -41,8 +41,10 @@ namespace scihip {
     }
 
     // Whether a product of an m x n output over kp (a multiple of k_tile(mode)) takes the large-tile
-    // kernel: bf16, at least one 256 x 256 tile per compute unit and at least 16 stages.
-    bool slab_product(int mode, int64_t m, int64_t n, int64_t kp);
+    // kernel, and over how many slices of its shared dimension (0: it does not): bf16 only; 1 with at
+    // least one 256 x 256 tile per compute unit; fewer tiles split the stages (product_split) so that
+    // tiles x slices fills the chip.
+    int slab_slices(int mode, int64_t m, int64_t n, int64_t kp);
 
     // Packs a row-major float (wide: double) source into slabs, as pack() does into rows: element
     // (i, k) is src[i][k] (transpose false) or src[k][i] (transpose true), rounded to float and then

@@ -381,23 +381,25 @@ namespace scihip {
         const bool split = mode == kSplit, wide = mode == kFloat64;
         void *a_hi = nullptr, *a_lo = nullptr, *b_hi = nullptr, *b_lo = nullptr;
         int rc = SCIHIP_OK;
-        if (slab_product(mode, rows, columns, kp)) {
-            // the large-tile kernel: both operands packed in slabs
+        if (const int slab = slab_slices(mode, rows, columns, kp)) {
+            // the large-tile kernel: both operands packed in slabs, the shared dimension maybe sliced
+            void *partials = nullptr;
             rc = device->allocator->allocate(slab_bytes(rows, kp), &a_hi);
             if (!rc) rc = device->allocator->allocate(slab_bytes(columns, kp), &b_hi);
+            if (!rc && slab > 1) rc = device->allocator->allocate(size_t(slab) * rows * columns * 4, &partials);
             if (!rc)
                 rc = status(pack_slabs(left, false, rows, depth, left_ld, left_transposed, a_hi, kp, device->stream),
                             "pack");
             if (!rc)
                 rc = status(pack_slabs(right, false, columns, depth, right_ld, right_transposed, b_hi, kp,
-                                        device->stream),
+                                       device->stream),
                             "pack");
             if (!rc) {
                 ProductArgs args{a_hi, nullptr, b_hi, nullptr, output, rows, columns, kp, kp, kp, columns, 1.0, 0.0};
                 args.slabs = 1;
-                rc = status(product(mode, args, device->stream), "product");
+                rc = status(product_split(mode, args, slab, partials, device->stream), "product");
             }
-            for (void *block : {a_hi, b_hi})
+            for (void *block : {a_hi, b_hi, partials})
                 if (block) device->allocator->dispose(block);
             return rc;
         }

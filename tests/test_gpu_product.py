@@ -120,11 +120,14 @@ def test_metric_sizes(device, shape):
         assert relative < frobenius[mode]
 
 
-@pytest.mark.parametrize("shape", [(4100, 1000, 4200), (4352, 576, 4097)], ids=["4100x1000x4200", "4352x576x4097"])
+@pytest.mark.parametrize("shape", [(4100, 1000, 4200), (4352, 576, 4097), (2000, 8200, 2050)],
+                         ids=["4100x1000x4200", "4352x576x4097", "2000x8200x2050"])
 def test_large_tiles_ragged(device, shape):
     """bf16 products with at least 256 output tiles of 256 x 256 take the large-tile kernel; these
     have partial last tiles in both directions and a shared dimension padded up to a multiple of 64
-    (1000 -> 1024) or not (576). Sampled rows, the last rows among them, against float64."""
+    (1000 -> 1024, 8200 -> 8256) or not (576). 2000 x 2050 has 72 tiles, so its 258 stages are split
+    four ways (uneven slices) and the partial products folded. Sampled rows, the last rows among
+    them, against float64."""
     m, k, n = shape
     print(f"[test] bf16 large tiles at {shape}", flush=True)
     left = sci.uniform(m * k, device, 41, -1.0, 1.0).reshape(m, k)
