@@ -241,8 +241,8 @@ namespace scihip {
         // each wave computes 128 x 128 of a tile as 4 x 4 blocks of v_mfma_f32_32x32x16_bf16 (256
         // accumulators per lane, one wave per SIMD).
         // - Stages stream global -> LDS with buffer_load_dwordx4 ... lds, lane l's 16 bytes landing at
-        //   the wave's LDS base + 16 l, so each load copies 1 KiB of a slab verbatim. A ring of
-        //   kSlabStages slots (160 KiB, all of the LDS) keeps the loads kSlabStages - 1 stages ahead.
+        //   the wave's LDS base + 16 l, so each load copies 1 KiB of a slab verbatim. A ring of five
+        //   slots (160 KiB, all of the LDS) keeps the loads four stages ahead.
         // - Each stage is two k-steps of 16. A wave's fragment reads for one k-step go out while the
         //   other k-step's sixteen MFMAs run, and the one barrier per stage sits between the two
         //   halves, so neither the LDS reads nor the barrier leave the matrix cores idle for long.
@@ -256,16 +256,21 @@ namespace scihip {
         // 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32 of the product.
         constexpr int kWide = kSlabRows;   // rows and columns of C per tile
         constexpr int kStageBytes = kSlabRows * kSlabK * 2;    // one operand's stage: 16 KiB
-        constexpr int kSlotBytes = 2 * kStageBytes;              // A and B
-        constexpr int kSlabStages = 5;                          // 160 KiB of LDS
         constexpr int kStageLoads = kStageBytes / 1024 / 4;      // 1 KiB loads per wave per operand
+        // Per mode: operands per stage (A, B; bf16x3 adds A_lo, B_lo) and ring slots in 160 KiB of LDS.
+        template <int Mode> struct SlabRing {
+            static constexpr int operands = Mode == kSplit ? 4 : 2;
+            static constexpr int slot = operands * kStageBytes;
+            static constexpr int slots = Mode == kSplit ? 2 : 5;
+            static constexpr int lds = slot * slots;
+        };
 
         typedef float v16f __attribute__((ext_vector_type(16)));
         typedef __attribute__((address_space(3))) void *lds_pointer;
 
         __device__ __forceinline__ int slab_position(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
 
-        // Waits until at most `pending` stages of this wave's loads are in flight (the newest ones:
+        // Waits until at most `pending` bf16 stages of this wave's loads are in flight (the newest ones:
         // loads complete in order for the count).
         __device__ __forceinline__ void wait_stages(int pending) {
             static_assert(2 * kStageLoads == 8, "the counts below assume eight loads per stage");
@@ -284,9 +289,11 @@ namespace scihip {
         // the clobber keeps the compiler from moving LDS accesses across it.
         __device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
-        template <int Stages = kSlabStages>
+        template <int Mode>
         __global__ __launch_bounds__(kThreads, 1) void gemm_slabs(const ProductArgs p, int slices) {
-            static_assert(Stages >= 3 && Stages <= 5, "wait_stages counts up to three pending stages");
+            typedef SlabRing<Mode> Ring;
+            constexpr int Stages = Ring::slots, kSlotBytes = Ring::slot;
+            static_assert(Mode == kSplit || (Stages >= 3 && Stages <= 5), "wait_stages counts up to three pending stages");
             extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
             const int tiles_m = (int)((p.m + kWide - 1) / kWide);
             const int tiles_n = (int)((p.n + kWide - 1) / kWide);
@@ -331,16 +338,24 @@ namespace scihip {
                         (void *)scalar((uint64_t)(static_cast<const unsigned char *>(base) + (uint64_t)index * slab)),
                         0, slab, 0x00020000);
             };
-            auto issue = [&](__amdgpu_buffer_rsrc_t a_rsrc, __amdgpu_buffer_rsrc_t b_rsrc, int64_t stage, int slot) {
+            // The operands' descriptors in slot order: A, B, then (bf16x3) A_lo, B_lo. (Descriptors are
+            // sizeless values: no arrays or structs of them.)
+            auto issue = [&](__amdgpu_buffer_rsrc_t a, __amdgpu_buffer_rsrc_t b, __amdgpu_buffer_rsrc_t a_lo,
+                             __amdgpu_buffer_rsrc_t b_lo, int64_t stage, int slot) {
                 unsigned char *base = lds + slot * kSlotBytes;
                 const int k_byte = (int)(stage * kStageBytes);
 #pragma unroll
                 for (int i = 0; i < kStageLoads; ++i) {
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(a_rsrc, (lds_pointer)(base + (wave * kStageLoads + i) * 1024),
-                                                              16, lane_offset[i], k_byte, 0, 0);
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                            b_rsrc, (lds_pointer)(base + kStageBytes + (wave * kStageLoads + i) * 1024), 16,
-                            lane_offset[i], k_byte, 0, 0);
+                    const int piece = (wave * kStageLoads + i) * 1024;
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(a, (lds_pointer)(base + piece), 16, lane_offset[i], k_byte, 0, 0);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(b, (lds_pointer)(base + kStageBytes + piece), 16, lane_offset[i],
+                                                              k_byte, 0, 0);
+                    if constexpr (Ring::operands == 4) {
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(a_lo, (lds_pointer)(base + 2 * kStageBytes + piece), 16,
+                                                                  lane_offset[i], k_byte, 0, 0);
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(b_lo, (lds_pointer)(base + 3 * kStageBytes + piece), 16,
+                                                                  lane_offset[i], k_byte, 0, 0);
+                    }
                 }
             };
             auto place = [&](int t, int &tm, int &tn) {
@@ -359,10 +374,12 @@ namespace scihip {
                 int64_t first, count;
                 place(w % tiles, tm, tn);
                 range(w, first, count);
-                const __amdgpu_buffer_rsrc_t a_rsrc = descriptor(p.a_hi, tm), b_rsrc = descriptor(p.b_hi, tn);
+                const __amdgpu_buffer_rsrc_t a = descriptor(p.a_hi, tm), b = descriptor(p.b_hi, tn);
+                const __amdgpu_buffer_rsrc_t a_lo = Ring::operands == 4 ? descriptor(p.a_lo, tm) : a;
+                const __amdgpu_buffer_rsrc_t b_lo = Ring::operands == 4 ? descriptor(p.b_lo, tn) : b;
 #pragma unroll
                 for (int s = 0; s < Stages - 1; ++s)
-                    if (s < count) issue(a_rsrc, b_rsrc, first + s, s);
+                    if (s < count) issue(a, b, a_lo, b_lo, first + s, s);
             };
 
             const bool vector_store = p.beta == 0.0 && (p.ldc & 3) == 0 && ((uintptr_t)p.c & 15) == 0;
@@ -373,7 +390,9 @@ namespace scihip {
                 int64_t first, stages;
                 place(w % tiles, tm, tn);
                 range(w, first, stages);
-                const __amdgpu_buffer_rsrc_t a_rsrc = descriptor(p.a_hi, tm), b_rsrc = descriptor(p.b_hi, tn);
+                const __amdgpu_buffer_rsrc_t a = descriptor(p.a_hi, tm), b = descriptor(p.b_hi, tn);
+                const __amdgpu_buffer_rsrc_t a_lo = Ring::operands == 4 ? descriptor(p.a_lo, tm) : a;
+                const __amdgpu_buffer_rsrc_t b_lo = Ring::operands == 4 ? descriptor(p.b_lo, tn) : b;
                 v16f acc[4][4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
@@ -382,58 +401,101 @@ namespace scihip {
 #pragma unroll
                         for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-                v8bf x[2][4], y[2][4]; // fragments of k-step 0 (x) and 1 (y) of the current stage: [A, B][block]
-                auto read = [&](v8bf (&f)[2][4], int slot, int step) {
-                    const unsigned char *base = lds + slot * kSlotBytes;
+                if constexpr (Mode == kBfloat16) {
+                    v8bf x[2][4], y[2][4]; // fragments of k-step 0 (x) and 1 (y) of the current stage: [A, B][block]
+                    auto read = [&](v8bf (&f)[2][4], int slot, int step) {
+                        const unsigned char *base = lds + slot * kSlotBytes;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        f[0][i] = *reinterpret_cast<const v8bf *>(base + a_at[step][i]);
-                        f[1][i] = *reinterpret_cast<const v8bf *>(base + b_at[step][i]);
-                    }
-                };
-                auto multiply = [&](const v8bf (&f)[2][4]) {
+                        for (int i = 0; i < 4; ++i) {
+                            f[0][i] = *reinterpret_cast<const v8bf *>(base + a_at[step][i]);
+                            f[1][i] = *reinterpret_cast<const v8bf *>(base + b_at[step][i]);
+                        }
+                    };
+                    auto multiply = [&](const v8bf (&f)[2][4]) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
+                        for (int i = 0; i < 4; ++i)
 #pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][j], f[0][i], acc[i][j], 0, 0, 0);
-                };
+                            for (int j = 0; j < 4; ++j)
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][j], f[0][i], acc[i][j], 0, 0, 0);
+                    };
 
-                // this tile's first stages were issued by the prologue (with the previous tile's stores
-                // after them in the count, so wait for everything)
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                lds_barrier();
-                read(x, 0, 0);
-                int slot = 0;
-                // One stage: k-step 1's reads go out, k-step 0's MFMAs run; then (unless this is the last
-                // stage) the next stage is awaited and published, the ring slot two behind refilled and
-                // the next stage's k-step 0 read while k-step 1's MFMAs run. `more` is a constant at
-                // every call, so the LDS reads are never conditional (which would make the compiler
-                // wait for all of them before the MFMAs).
-                auto advance = [&](int64_t kt, bool more, int pending) {
-                    read(y, slot, 1);
-                    __builtin_amdgcn_sched_barrier(0);
-                    multiply(x);
-                    __builtin_amdgcn_sched_barrier(0);
-                    const int next = slot + 1 == Stages ? 0 : slot + 1;
-                    if (more) {
-                        wait_stages(pending);
+                    // this tile's first stages were issued by the prologue (with the previous tile's stores
+                    // after them in the count, so wait for everything)
+                    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                    lds_barrier();
+                    read(x, 0, 0);
+                    int slot = 0;
+                    // One stage: k-step 1's reads go out, k-step 0's MFMAs run; then (unless this is the last
+                    // stage) the next stage is awaited and published, the ring slot two behind refilled and
+                    // the next stage's k-step 0 read while k-step 1's MFMAs run. `more` is a constant at
+                    // every call, so the LDS reads are never conditional (which would make the compiler
+                    // wait for all of them before the MFMAs).
+                    auto advance = [&](int64_t kt, bool more, int pending) {
+                        read(y, slot, 1);
+                        __builtin_amdgcn_sched_barrier(0);
+                        multiply(x);
+                        __builtin_amdgcn_sched_barrier(0);
+                        const int next = slot + 1 == Stages ? 0 : slot + 1;
+                        if (more) {
+                            wait_stages(pending);
+                            lds_barrier();
+                            // every wave is past stage kt - 1: its slot takes stage kt + Stages - 1
+                            if (kt + Stages - 1 < stages)
+                                issue(a, b, a_lo, b_lo, first + kt + Stages - 1, slot == 0 ? Stages - 1 : slot - 1);
+                            read(x, next, 0);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                        multiply(y);
+                        __builtin_amdgcn_sched_barrier(0);
+                        slot = next;
+                    };
+                    // stage kt + 1 has landed once only the stages after it, up to kt + Stages - 2, are pending
+                    int64_t kt = 0;
+                    for (; kt < stages - 1 && kt < stages - Stages + 2; ++kt) advance(kt, true, Stages - 3);
+                    for (; kt < stages - 1; ++kt) advance(kt, true, (int)(stages - 2 - kt));
+                    advance(stages - 1, false, 0);
+                } else {
+                    // bf16x3: two slots, each stage awaited whole before it is read; a stage is 96 MFMAs
+                    // per wave (three per block and k-step), so one stage of lookahead covers the loads.
+                    v8bf x[4][4], y[4][4]; // k-steps 0 and 1: [A, B, A_lo, B_lo][block]
+                    auto read = [&](v8bf (&f)[4][4], int slot, int step) {
+                        const unsigned char *base = lds + slot * kSlotBytes;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            f[0][i] = *reinterpret_cast<const v8bf *>(base + a_at[step][i]);
+                            f[1][i] = *reinterpret_cast<const v8bf *>(base + b_at[step][i]);
+                            f[2][i] = *reinterpret_cast<const v8bf *>(base + 2 * kStageBytes + a_at[step][i]);
+                            f[3][i] = *reinterpret_cast<const v8bf *>(base + 2 * kStageBytes + b_at[step][i]);
+                        }
+                    };
+                    // the small cross terms first, then the leading term
+                    auto multiply = [&](const v8bf (&f)[4][4]) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][j], f[2][i], acc[i][j], 0, 0, 0);
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3][j], f[0][i], acc[i][j], 0, 0, 0);
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][j], f[0][i], acc[i][j], 0, 0, 0);
+                            }
+                    };
+                    for (int64_t kt = 0; kt < stages; ++kt) {
+                        const int slot = (int)(kt & 1);
+                        // stage kt: issued by the prologue or one stage ago (for the first, with the previous
+                        // tile's stores after it in the count)
+                        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                         lds_barrier();
-                        // every wave is past stage kt - 1: its slot takes stage kt + Stages - 1
-                        if (kt + Stages - 1 < stages)
-                            issue(a_rsrc, b_rsrc, first + kt + Stages - 1, slot == 0 ? Stages - 1 : slot - 1);
-                        read(x, next, 0);
+                        // every wave is past stage kt - 1: the other slot takes stage kt + 1
+                        if (kt + 1 < stages) issue(a, b, a_lo, b_lo, first + kt + 1, slot ^ 1);
+                        read(x, slot, 0);
+                        read(y, slot, 1);
+                        __builtin_amdgcn_sched_barrier(0);
+                        multiply(x);
+                        __builtin_amdgcn_sched_barrier(0);
+                        multiply(y);
                         __builtin_amdgcn_sched_barrier(0);
                     }
-                    multiply(y);
-                    __builtin_amdgcn_sched_barrier(0);
-                    slot = next;
-                };
-                // stage kt + 1 has landed once only the stages after it, up to kt + Stages - 2, are pending
-                int64_t kt = 0;
-                for (; kt < stages - 1 && kt < stages - Stages + 2; ++kt) advance(kt, true, Stages - 3);
-                for (; kt < stages - 1; ++kt) advance(kt, true, (int)(stages - 2 - kt));
-                advance(stages - 1, false, 0);
+                }
                 // every wave has read its last fragments: the next tile's first stages may land
                 lds_barrier();
                 if (w + (int)gridDim.x < items) prologue(w + gridDim.x);
@@ -488,9 +550,9 @@ namespace scihip {
         template <> struct Quad<float> { typedef float4 type; };
         template <> struct Quad<double> { typedef double4 type; };
 
-        template <typename S>
+        template <typename S, bool split>
         __global__ __launch_bounds__(256) void pack_slab(const S *source, int64_t rows, int64_t depth, int64_t ld,
-                                                         bool transpose, uint4 *out, int64_t kp) {
+                                                         bool transpose, uint4 *out, uint4 *out_lo, int64_t kp) {
             typedef typename Quad<S>::type S4;
             constexpr int kStride = kSlabRows + 4; // LDS row stride, 16-byte aligned rows
             __shared__ __attribute__((aligned(16))) S tile[transpose_rows<S>()][kStride];
@@ -534,10 +596,17 @@ namespace scihip {
 #pragma unroll
                     for (int e = 0; e < 8; ++e) v[e] = (i < rows && k + e < depth) ? source[i * ld + k + e] : S(0);
                 }
-                __bf16 h[8];
+                // as pack_element: the value rounded to float, then to bf16; the low part the rounded
+                // remainder, zero where the value is not finite
+                __bf16 h[8], l[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) h[e] = (__bf16)(float)v[e];
+                for (int e = 0; e < 8; ++e) {
+                    const float f = (float)v[e];
+                    h[e] = (__bf16)f;
+                    if constexpr (split) l[e] = (__bf16)(__builtin_isfinite(f) ? f - (float)h[e] : 0.0f);
+                }
                 stage_out[q] = __builtin_bit_cast(uint4, h);
+                if constexpr (split) (out_lo + (stage_out - out))[q] = __builtin_bit_cast(uint4, l);
             }
         }
 
@@ -600,13 +669,18 @@ namespace scihip {
 
         hipError_t launch_gemm(int mode, const ProductArgs &args, int slices, hipStream_t stream, bool alongside = false) {
             if (args.slabs) {
-                if (mode != kBfloat16 || slices < 1 || args.kp % kSlabK != 0 || args.kp / kSlabK < slices)
+                if ((mode != kBfloat16 && mode != kSplit) || slices < 1 || args.kp % kSlabK != 0 ||
+                    args.kp / kSlabK < slices)
                     return hipErrorInvalidValue;
                 const int64_t items = ((args.m + kWide - 1) / kWide) * ((args.n + kWide - 1) / kWide) * slices;
                 if (items == 0) return hipSuccess;
                 const int grid = (int)std::min<int64_t>(items, compute_units());
-                hipLaunchKernelGGL(gemm_slabs<kSlabStages>, dim3(grid), dim3(kThreads), kSlabStages * kSlotBytes,
-                                   stream, args, slices);
+                if (mode == kSplit)
+                    hipLaunchKernelGGL(gemm_slabs<kSplit>, dim3(grid), dim3(kThreads), SlabRing<kSplit>::lds, stream,
+                                       args, slices);
+                else
+                    hipLaunchKernelGGL(gemm_slabs<kBfloat16>, dim3(grid), dim3(kThreads), SlabRing<kBfloat16>::lds,
+                                       stream, args, slices);
                 return hipGetLastError();
             }
             const int64_t tiles = ((args.m + kTile - 1) / kTile) * ((args.n + kTile - 1) / kTile);
@@ -676,7 +750,8 @@ namespace scihip {
     int slab_slices(int mode, int64_t m, int64_t n, int64_t kp) {
         const int64_t tiles = ((m + kWide - 1) / kWide) * ((n + kWide - 1) / kWide), stages = kp / kSlabK;
         // a slab's stage offsets within 32 bits
-        if (mode != kBfloat16 || tiles == 0 || kp % kSlabK != 0 || kp * int64_t(kSlabRows) * 2 >= (int64_t(1) << 31))
+        if ((mode != kBfloat16 && mode != kSplit) || tiles == 0 || kp % kSlabK != 0 ||
+            kp * int64_t(kSlabRows) * 2 >= (int64_t(1) << 31))
             return 0;
         // enough 256 x 256 tiles to give every compute unit one, over at least 16 stages
         if (tiles >= 256) return stages >= 16 ? 1 : 0;
@@ -687,16 +762,20 @@ namespace scihip {
     }
 
     hipError_t pack_slabs(const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,
-                           void *out, int64_t kp, hipStream_t stream) {
+                          void *out, void *out_lo, int64_t kp, hipStream_t stream) {
         if (rows == 0 || kp == 0) return hipSuccess;
         if (kp % kSlabK != 0) return hipErrorInvalidValue;
         const dim3 grid((unsigned)(kp / kSlabK), (unsigned)((rows + kSlabRows - 1) / kSlabRows)), block(256);
-        if (wide)
-            hipLaunchKernelGGL(pack_slab<double>, grid, block, 0, stream, static_cast<const double *>(source), rows,
-                               depth, ld, transpose, static_cast<uint4 *>(out), kp);
-        else
-            hipLaunchKernelGGL(pack_slab<float>, grid, block, 0, stream, static_cast<const float *>(source), rows,
-                               depth, ld, transpose, static_cast<uint4 *>(out), kp);
+        uint4 *hi = static_cast<uint4 *>(out), *lo = static_cast<uint4 *>(out_lo);
+        if (wide) {
+            const double *x = static_cast<const double *>(source);
+            if (lo) hipLaunchKernelGGL((pack_slab<double, true>), grid, block, 0, stream, x, rows, depth, ld, transpose, hi, lo, kp);
+            else hipLaunchKernelGGL((pack_slab<double, false>), grid, block, 0, stream, x, rows, depth, ld, transpose, hi, lo, kp);
+        } else {
+            const float *x = static_cast<const float *>(source);
+            if (lo) hipLaunchKernelGGL((pack_slab<float, true>), grid, block, 0, stream, x, rows, depth, ld, transpose, hi, lo, kp);
+            else hipLaunchKernelGGL((pack_slab<float, false>), grid, block, 0, stream, x, rows, depth, ld, transpose, hi, lo, kp);
+        }
         return hipGetLastError();
     }
 

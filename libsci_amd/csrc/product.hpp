@@ -27,7 +27,7 @@ namespace scihip {
         int64_t lda, ldb, ldc;
         double alpha, beta;
         int64_t slice_stride; // split-K launches: elements between consecutive slices' outputs
-        int slabs = 0;       // kBfloat16 only: a_hi and b_hi are packed in slabs (pack_slabs), lda/ldb unused
+        int slabs = 0;       // kBfloat16 and kSplit: the operands are packed in slabs (pack_slabs), lda/ldb unused
     };
 
     // Slab layout of the large-tile bf16 product: an operand of R rows over Kp is ceil(R / 256)
@@ -41,16 +41,17 @@ namespace scihip {
     }
 
     // Whether a product of an m x n output over kp (a multiple of k_tile(mode)) takes the large-tile
-    // kernel, and over how many slices of its shared dimension (0: it does not): bf16 only; 1 with at
+    // kernel, and over how many slices of its shared dimension (0: it does not): bf16 and bf16x3; 1 with at
     // least one 256 x 256 tile per compute unit; fewer tiles split the stages (product_split) so that
     // tiles x slices fills the chip.
     int slab_slices(int mode, int64_t m, int64_t n, int64_t kp);
 
     // Packs a row-major float (wide: double) source into slabs, as pack() does into rows: element
     // (i, k) is src[i][k] (transpose false) or src[k][i] (transpose true), rounded to float and then
-    // to bf16, zero for k >= depth or i >= rows.
+    // to bf16, zero for k >= depth or i >= rows; with out_lo (bf16x3) also the low parts, as
+    // pack_element makes them.
     hipError_t pack_slabs(const void *source, bool wide, int64_t rows, int64_t depth, int64_t ld, bool transpose,
-                           void *out, int64_t kp, hipStream_t stream);
+                          void *out, void *out_lo, int64_t kp, hipStream_t stream);
 
     // Bytes of one k-tile row in LDS and the k extent it holds, per mode.
     constexpr int k_tile(int mode) { return mode == kFloat32 ? 32 : mode == kFloat64 ? 16 : mode == kSplit ? 32 : 64; }

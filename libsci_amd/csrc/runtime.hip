@@ -382,24 +382,27 @@ namespace scihip {
         void *a_hi = nullptr, *a_lo = nullptr, *b_hi = nullptr, *b_lo = nullptr;
         int rc = SCIHIP_OK;
         if (const int slab = slab_slices(mode, rows, columns, kp)) {
-            // the large-tile kernel: both operands packed in slabs, the shared dimension maybe sliced
+            // the large-tile kernel: operands packed in slabs, the shared dimension maybe sliced
             void *partials = nullptr;
             rc = device->allocator->allocate(slab_bytes(rows, kp), &a_hi);
             if (!rc) rc = device->allocator->allocate(slab_bytes(columns, kp), &b_hi);
+            if (!rc && split) rc = device->allocator->allocate(slab_bytes(rows, kp), &a_lo);
+            if (!rc && split) rc = device->allocator->allocate(slab_bytes(columns, kp), &b_lo);
             if (!rc && slab > 1) rc = device->allocator->allocate(size_t(slab) * rows * columns * 4, &partials);
             if (!rc)
-                rc = status(pack_slabs(left, false, rows, depth, left_ld, left_transposed, a_hi, kp, device->stream),
+                rc = status(pack_slabs(left, false, rows, depth, left_ld, left_transposed, a_hi, a_lo, kp,
+                                       device->stream),
                             "pack");
             if (!rc)
-                rc = status(pack_slabs(right, false, columns, depth, right_ld, right_transposed, b_hi, kp,
+                rc = status(pack_slabs(right, false, columns, depth, right_ld, right_transposed, b_hi, b_lo, kp,
                                        device->stream),
                             "pack");
             if (!rc) {
-                ProductArgs args{a_hi, nullptr, b_hi, nullptr, output, rows, columns, kp, kp, kp, columns, 1.0, 0.0};
+                ProductArgs args{a_hi, a_lo, b_hi, b_lo, output, rows, columns, kp, kp, kp, columns, 1.0, 0.0};
                 args.slabs = 1;
                 rc = status(product_split(mode, args, slab, partials, device->stream), "product");
             }
-            for (void *block : {a_hi, b_hi, partials})
+            for (void *block : {a_hi, a_lo, b_hi, b_lo, partials})
                 if (block) device->allocator->dispose(block);
             return rc;
         }
