@@ -404,38 +404,41 @@ def measure_small_arrays(sci, _abi, device, events, steps: int = 50) -> dict:
     return out
 
 
-def measure_graph(sci, _abi, device, events, count=1_000_000, chain=32, steps=20) -> dict:
-    """The launch-bound case: `chain` dependent chunk stores over 1e6 float32 elements (the CPU-runnable
-    workload's size: c = a*b + 2, then alternately sin / cos of the previous result times b), issued
-    call by call against one replay of the same calls captured in a graph."""
-    a = sci.linspace(0.0, 1.0, count, device)
-    b = sci.ones(count, device)
-    buffers = [sci.Array(device, (count,), "float32") for _ in range(2)]
-    stages = [(a * b + 2.0, buffers[0])]
-    for k in range(1, chain):
-        source, target = buffers[(k - 1) % 2], buffers[k % 2]
-        stages.append(((sci.sin(source) if k % 2 else sci.cos(source)) * b, target))
-    chunks = [(sci.record(e), t) for e, t in stages]
-    for e, _ in stages:
-        device.prepare(e)
-    device.set_policy(2)
+def measure_graph(sci, _abi, device, events, counts=(1_000, 10_000, 1_000_000), chain=32, steps=20) -> dict:
+    """The launch-bound case: `chain` dependent chunk stores (c = a*b + 2, then alternately sin / cos
+    of the previous result times b) over 1e3 and 1e4 float32 elements, where launching dominates, and
+    over 1e6 (the CPU-runnable workload's size), issued call by call against one replay of the same
+    calls captured in a graph."""
+    out = {"workload": f"{chain} dependent chunk stores per replay, float32"}
+    for count in counts:
+        a = sci.linspace(0.0, 1.0, count, device)
+        b = sci.ones(count, device)
+        buffers = [sci.Array(device, (count,), "float32") for _ in range(2)]
+        stages = [(a * b + 2.0, buffers[0])]
+        for k in range(1, chain):
+            source, target = buffers[(k - 1) % 2], buffers[k % 2]
+            stages.append(((sci.sin(source) if k % 2 else sci.cos(source)) * b, target))
+        chunks = [(sci.record(e), t) for e, t in stages]
+        for e, _ in stages:
+            device.prepare(e)
+        device.set_policy(2)
 
-    def eager():
-        for chunk, target in chunks:
-            _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), target.pointer, count))
+        def eager():
+            for chunk, target in chunks:
+                _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), target.pointer, count))
 
-    device.begin_capture()
-    eager()
-    graph = device.end_capture()
-    eager_wall, eager_ms = time_launches(device, events, eager, steps, 3)
-    graph_wall, graph_ms = time_launches(device, events, graph.replay, steps, 3)
-    graph.release()
-    device.set_policy(0)
-    return {"workload": f"{chain} dependent chunk stores over {count} float32 elements",
-            "eager_us_per_chunk": round(eager_wall * 1e3 / chain, 2),
-            "graph_us_per_chunk": round(graph_wall * 1e3 / chain, 2),
-            "eager_device_us_per_chunk": round(eager_ms * 1e3 / chain, 2),
-            "graph_device_us_per_chunk": round(graph_ms * 1e3 / chain, 2)}
+        device.begin_capture()
+        eager()
+        graph = device.end_capture()
+        eager_wall, eager_ms = time_launches(device, events, eager, steps, 3)
+        graph_wall, graph_ms = time_launches(device, events, graph.replay, steps, 3)
+        graph.release()
+        device.set_policy(0)
+        out[str(count)] = {"eager_us_per_chunk": round(eager_wall * 1e3 / chain, 2),
+                           "graph_us_per_chunk": round(graph_wall * 1e3 / chain, 2),
+                           "eager_device_us_per_chunk": round(eager_ms * 1e3 / chain, 2),
+                           "graph_device_us_per_chunk": round(graph_ms * 1e3 / chain, 2)}
+    return out
 
 
 def measure_adjoint(sci, _abi, device, events, log2n=26, steps=10) -> dict:
