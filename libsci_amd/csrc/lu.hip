@@ -278,10 +278,14 @@ namespace scihip {
                                                                 unsigned long long *lines,
                                                                 unsigned long long *published,
                                                                 unsigned long long *diagonal, unsigned launch,
-                                                                unsigned *error) {
+                                                                unsigned *error, int spread) {
             constexpr int stride = kPanel + 1;
             constexpr int S = kSlots<T>;
-            const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+            // With spread > 1 the grid is spread times the panel's workgroups and only every spread-th
+            // block works: blocks are dealt to the XCDs round-robin, so with spread = 8 the panel's
+            // workgroups all run on one XCD and exchange their candidates through one L2.
+            if (spread > 1 && blockIdx.x % spread != 0) return;
+            const int G = gridDim.x / spread, b = blockIdx.x / spread, t = threadIdx.x;
             const int lane = t & 63, wave = t >> 6;
             const int k_own = t & (kPanel - 1), part = t / kPanel; // a panel column and which half of the threads
             const int64_t first = j0 + (int64_t)b * kPanelRows;
@@ -1416,8 +1420,10 @@ namespace scihip {
         unsigned long long *diagonal = published + (size_t)kParities * max_groups * kPanel * kSlots<T>;
 
         StageTimer timer(stream);
-        // Panel workgroups the device holds at once.
-        int resident = 0;
+        // Panel workgroups the device holds at once, and how many panel workgroups are confined to one
+        // XCD (SCIHIP_LU_CONFINE=g: panels of at most g workgroups, capped at what one XCD holds).
+        int resident = 0, confine = 0, confine_limit = 0;
+        if (const char *text = std::getenv("SCIHIP_LU_CONFINE")) confine_limit = std::max(0, std::atoi(text));
         {
             int device = 0, units = 0, per_unit = 0;
             hipError_t e = hipGetDevice(&device);
@@ -1427,6 +1433,7 @@ namespace scihip {
                                                                  kPanelThreads, 0);
             if (e != hipSuccess) return (int)e;
             resident = units * per_unit;
+            confine = std::min(units / 8 * per_unit, confine_limit);
         }
         // Clears candidate line set `set` on stream `s` (each launch expects its lines zero). Panels
         // run on the main stream and alternate between the sets; the set of the panel after next is
@@ -1448,15 +1455,19 @@ namespace scihip {
                 ws.launches = 1;
             }
             unsigned launch = ws.launches;
+            int spread = 1;
             void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &lines, &published, &diagonal, &launch,
-                            &ws.error};
+                            &ws.error, &spread};
             // The workgroups wait for each other, so all of them must be resident at once. When the grid
             // fits the device, a plain launch is enough: workgroups queued behind the concurrent update
             // start as soon as its workgroups finish, which they do without waiting on the panel (and
             // every wait is bounded). Larger grids take the cooperative launch, which checks residency.
-            if (groups <= resident)
+            if (groups <= confine) // one XCD holds them all (every XCD has units / 8 CUs)
+                hipLaunchKernelGGL((panel<T>), dim3(groups * 8), dim3(kPanelThreads), 0, s, a, n, j0, w, pivots,
+                                   lines, published, diagonal, launch, ws.error, 8);
+            else if (groups <= resident)
                 hipLaunchKernelGGL((panel<T>), dim3(groups), dim3(kPanelThreads), 0, s, a, n, j0, w, pivots, lines,
-                                   published, diagonal, launch, ws.error);
+                                   published, diagonal, launch, ws.error, 1);
             else
                 e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
                                                dim3(kPanelThreads), args, 0, s);
