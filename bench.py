@@ -191,6 +191,30 @@ def measure_ceilings(sci, _abi, device, events, arrays, out, steps: int) -> dict
     return result
 
 
+def measure_interpreter(sci, _abi, device, events, arrays, out, steps: int) -> dict:
+    """The interpreter (policy 1: no specialised kernel) on the headline store (16 B per element) and on
+    the C2 sum of sin(a)*cos(c) + tanh(b) (12 B per element): what a chunk runs at before its
+    specialised kernel is compiled."""
+    a, b, c = arrays
+    n = a.size
+    store_chunk = sci.record(sci.exponential(a) * 0.5 + sci.logarithm(sci.absolute(b) + 1.0) * c)
+    sum_chunk = sci.record(sci.sin(a) * sci.cos(c) + sci.tanh(b), reduce=True)
+    device.set_policy(1)
+    total = ctypes.c_double()
+
+    def store():
+        _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(store_chunk.struct), out.pointer, n))
+
+    def fold():
+        _abi.check(_abi.lib().scihip_reduce(device.handle, ctypes.byref(sum_chunk.struct), 0, n, ctypes.byref(total)))
+
+    _, store_ms = time_launches(device, events, store, steps, 2)
+    sum_seconds = median_time(fold, steps, 2)
+    device.set_policy(0)
+    return {"store_GBps": round(16 * n / (store_ms * 1e-3) / 1e9, 1), "store_ms": round(store_ms, 4),
+            "sum_call_GBps": round(12 * n / sum_seconds / 1e9, 1)}
+
+
 def measure_reductions(sci, _abi, device, arrays, clean, steps: int) -> dict:
     """C2 reductions over N = 2^28: sum / mean / minimum / maximum of sin(a)*cos(c) + tanh(b) (12 B
     per element), with the 1%-NaN b of the workload (every result NaN) and the NaN-free b, and
@@ -513,6 +537,8 @@ def main() -> int:
                         help="with --solve-only: the factorisations to time (fp64, fp32, bf16x3, bf16)")
     parser.add_argument("--products-only", action="store_true", help="only the C3 products (for profiling)")
     parser.add_argument("--reductions-only", action="store_true", help="only the C2 reductions (for profiling)")
+    parser.add_argument("--interpreter-only", action="store_true",
+                        help="only the interpreter's store and sum (for profiling)")
     parser.add_argument("--store-only", action="store_true",
                         help="only the headline store, then a plain copy chunk of the same size (8 B per element) "
                              "that calibrates hardware byte counters; for profiling")
@@ -564,6 +590,10 @@ def main() -> int:
     def store():
         _abi.check(_abi.lib().scihip_execute(device.handle, ctypes.byref(chunk.struct), out.pointer, n))
 
+    if args.interpreter_only:
+        print(json.dumps(measure_interpreter(sci, _abi, device, events, (a, b, c), out, args.steps)), flush=True)
+        device.close()
+        return 0
     if args.reductions_only:
         clean = sci.uniform(n, device, 2, -4.0, 4.0)
         print(json.dumps(measure_reductions(sci, _abi, device, (a, b, c), clean, args.steps)), flush=True)
@@ -603,6 +633,8 @@ def main() -> int:
 
     progress("ceilings")
     ceilings = measure_ceilings(sci, _abi, device, events, (a, b, c), out, max(args.steps // 2, 5))
+    progress("interpreter")
+    interpreter = measure_interpreter(sci, _abi, device, events, (a, b, c), out, max(args.steps // 2, 5))
     clean = sci.uniform(n, device, 2, -4.0, 4.0)
     progress("reductions")
     reductions = measure_reductions(sci, _abi, device, (a, b, c), clean, max(args.steps // 2, 5))
@@ -646,6 +678,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic},
         "ceilings": ceilings,
+        "interpreter": interpreter,
         "reductions": reductions,
     }
     if products:

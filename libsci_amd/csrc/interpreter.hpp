@@ -44,45 +44,6 @@ namespace scihip {
         T v[W];
     };
 
-    // One code over W lanes; `C` is a compile-time code, so apply() folds to that code's body.
-    template <uint32_t C, typename T, int W>
-    __device__ __forceinline__ Lanes<T, W> lanes(const Lanes<T, W> &a, const Lanes<T, W> &b, const Lanes<T, W> &c) {
-        Lanes<T, W> o;
-#pragma unroll
-        for (int k = 0; k < W; ++k) o.v[k] = apply<T>(C, a.v[k], b.v[k], c.v[k]);
-        return o;
-    }
-
-#define SCIHIP_CASE(C)                                                                                                 \
-    case C:                                                                                                            \
-        return lanes<C, T, W>(a, b, c);
-
-    // The runtime dispatch: a uniform switch, each case a straight-line body over W lanes.
-    template <typename T, int W>
-    __device__ __forceinline__ Lanes<T, W> dispatch(uint32_t code, const Lanes<T, W> &a, const Lanes<T, W> &b,
-                                                    const Lanes<T, W> &c) {
-        switch (code) {
-            SCIHIP_CASE(0) SCIHIP_CASE(1) SCIHIP_CASE(2) SCIHIP_CASE(3) SCIHIP_CASE(4) SCIHIP_CASE(5)
-            SCIHIP_CASE(6) SCIHIP_CASE(7) SCIHIP_CASE(8) SCIHIP_CASE(9) SCIHIP_CASE(10) SCIHIP_CASE(11)
-            SCIHIP_CASE(12) SCIHIP_CASE(13) SCIHIP_CASE(14) SCIHIP_CASE(15) SCIHIP_CASE(16) SCIHIP_CASE(17)
-            SCIHIP_CASE(18) SCIHIP_CASE(19) SCIHIP_CASE(20) SCIHIP_CASE(21) SCIHIP_CASE(22) SCIHIP_CASE(23)
-            SCIHIP_CASE(24) SCIHIP_CASE(25) SCIHIP_CASE(26) SCIHIP_CASE(27) SCIHIP_CASE(28) SCIHIP_CASE(29)
-            SCIHIP_CASE(30) SCIHIP_CASE(31) SCIHIP_CASE(32) SCIHIP_CASE(33) SCIHIP_CASE(34) SCIHIP_CASE(35)
-            SCIHIP_CASE(37) SCIHIP_CASE(38) SCIHIP_CASE(39)
-        default:
-            return lanes<kFuse, T, W>(a, b, c);
-        }
-    }
-#undef SCIHIP_CASE
-
-    template <typename T, int W> struct Vector;
-    template <> struct Vector<float, 4> {
-        typedef float type __attribute__((ext_vector_type(4)));
-    };
-    template <> struct Vector<double, 2> {
-        typedef double type __attribute__((ext_vector_type(2)));
-    };
-
     // The register file: one array of R values per lane, each its own local array, so that the
     // compiler keeps it in VGPRs and reads it with wave-uniform indices (s_set_gpr_idx + v_mov)
     // instead of scratch memory. R * sizeof(T) must stay within 32 dwords per lane for that.
@@ -111,6 +72,46 @@ namespace scihip {
     T bank0[R], bank1[W > 1 ? R : 1], bank2[W > 2 ? R : 1], bank3[W > 3 ? R : 1];                                     \
     const File<T, W, R> regs{bank0, *reinterpret_cast<T(*)[R]>(bank1), *reinterpret_cast<T(*)[R]>(bank2),             \
                              *reinterpret_cast<T(*)[R]>(bank3)}
+
+    // One code over W lanes; `C` is a compile-time code, so apply() folds to that code's body.
+    template <uint32_t C, typename T, int W>
+    __device__ __forceinline__ Lanes<T, W> lanes(const Lanes<T, W> &a, const Lanes<T, W> &b, const Lanes<T, W> &c) {
+        Lanes<T, W> o;
+#pragma unroll
+        for (int k = 0; k < W; ++k) o.v[k] = apply<T>(C, a.v[k], b.v[k], c.v[k]);
+        return o;
+    }
+
+#define SCIHIP_CASE(C)                                                                                                 \
+    case C:                                                                                                            \
+        return lanes<C, T, W>(regs.get(first), regs.get(second), regs.get(third));
+
+    // The runtime dispatch: a uniform switch, each case a straight-line body over W lanes that reads
+    // only the registers its code uses (the reads of unused operands fold away with the code).
+    template <typename T, int W, int R>
+    __device__ __forceinline__ Lanes<T, W> dispatch(uint32_t code, const File<T, W, R> &regs, uint32_t first,
+                                                    uint32_t second, uint32_t third) {
+        switch (code) {
+            SCIHIP_CASE(0) SCIHIP_CASE(1) SCIHIP_CASE(2) SCIHIP_CASE(3) SCIHIP_CASE(4) SCIHIP_CASE(5)
+            SCIHIP_CASE(6) SCIHIP_CASE(7) SCIHIP_CASE(8) SCIHIP_CASE(9) SCIHIP_CASE(10) SCIHIP_CASE(11)
+            SCIHIP_CASE(12) SCIHIP_CASE(13) SCIHIP_CASE(14) SCIHIP_CASE(15) SCIHIP_CASE(16) SCIHIP_CASE(17)
+            SCIHIP_CASE(18) SCIHIP_CASE(19) SCIHIP_CASE(20) SCIHIP_CASE(21) SCIHIP_CASE(22) SCIHIP_CASE(23)
+            SCIHIP_CASE(24) SCIHIP_CASE(25) SCIHIP_CASE(26) SCIHIP_CASE(27) SCIHIP_CASE(28) SCIHIP_CASE(29)
+            SCIHIP_CASE(30) SCIHIP_CASE(31) SCIHIP_CASE(32) SCIHIP_CASE(33) SCIHIP_CASE(34) SCIHIP_CASE(35)
+            SCIHIP_CASE(37) SCIHIP_CASE(38) SCIHIP_CASE(39)
+        default:
+            return lanes<kFuse, T, W>(regs.get(first), regs.get(second), regs.get(third));
+        }
+    }
+#undef SCIHIP_CASE
+
+    template <typename T, int W> struct Vector;
+    template <> struct Vector<float, 4> {
+        typedef float type __attribute__((ext_vector_type(4)));
+    };
+    template <> struct Vector<double, 2> {
+        typedef double type __attribute__((ext_vector_type(2)));
+    };
 
     // W elements at `at` of input `i` of a program (Program or Adjoint: both have inputs, count and
     // aligned); the specialised kernels' load.
@@ -168,10 +169,7 @@ namespace scihip {
             const uint32_t word = p.word[s];
             const uint32_t code = word & 0xFFu;
             const uint32_t target = (word >> 8) & 0xFFu;
-            const Lanes<T, W> a = regs.get((word >> 16) & 0xFFu);
-            const Lanes<T, W> b = regs.get(word >> 24);
-            const Lanes<T, W> c = regs.get(p.third[s]);
-            regs.set(target, dispatch<T, W>(code, a, b, c));
+            regs.set(target, dispatch<T, W, R>(code, regs, (word >> 16) & 0xFFu, word >> 24, p.third[s]));
         }
     }
 

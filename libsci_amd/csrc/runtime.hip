@@ -228,7 +228,10 @@ namespace scihip {
             fail("chunk needs more registers than the interpreter has");
             return SCIHIP_EINVAL;
         }
-        registers = needed <= kSmall ? kSmall : needed <= kMedium ? kMedium : kLarge;
+        // Programs of up to 16 registers take the 16-register file: its wave-uniform indexing compiles to
+        // one indexed move per bank (s_set_gpr_idx), where the 8-register file's compiles to chains of
+        // selects.
+        registers = needed <= kMedium ? kMedium : kLarge;
         return SCIHIP_OK;
     }
 
