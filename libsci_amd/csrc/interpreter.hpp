@@ -165,11 +165,14 @@ namespace scihip {
     // Runs the first `steps` instructions of a program (Program or Adjoint) on the register file.
     template <typename T, int W, int R, typename P>
     __device__ __forceinline__ void run(const P &p, const File<T, W, R> &regs, uint32_t steps) {
+        // The third sources are read four to a dword: a byte load would be a vector load, whose value
+        // the compiler cannot prove uniform (every register read through it would become a loop over
+        // the lanes' indices), where a dword of the program is a scalar load.
+        const uint32_t *thirds = reinterpret_cast<const uint32_t *>(p.third);
         for (uint32_t s = 0; s < steps; ++s) {
             const uint32_t word = p.word[s];
-            const uint32_t code = word & 0xFFu;
-            const uint32_t target = (word >> 8) & 0xFFu;
-            regs.set(target, dispatch<T, W, R>(code, regs, (word >> 16) & 0xFFu, word >> 24, p.third[s]));
+            const uint32_t third = (thirds[s >> 2] >> (8 * (s & 3))) & 0xFFu;
+            regs.set((word >> 8) & 0xFFu, dispatch<T, W, R>(word & 0xFFu, regs, (word >> 16) & 0xFFu, word >> 24, third));
         }
     }
 
