@@ -537,6 +537,7 @@ def main() -> int:
                         help="with --solve-only: the factorisations to time (fp64, fp32, bf16x3, bf16)")
     parser.add_argument("--products-only", action="store_true", help="only the C3 products (for profiling)")
     parser.add_argument("--reductions-only", action="store_true", help="only the C2 reductions (for profiling)")
+    parser.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurements (for profiling)")
     parser.add_argument("--interpreter-only", action="store_true",
                         help="only the interpreter's store and sum (for profiling)")
     parser.add_argument("--store-only", action="store_true",
@@ -574,6 +575,10 @@ def main() -> int:
         return 0
     if args.products_only:
         print(json.dumps(measure_products(sci, _abi, device, events)), flush=True)
+        device.close()
+        return 0
+    if args.adjoint_only:
+        print(json.dumps(measure_adjoint(sci, _abi, device, events)), flush=True)
         device.close()
         return 0
 

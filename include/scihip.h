@@ -153,7 +153,7 @@ int scihip_reduce(scihip_device *device, const scihip_chunk *chunk, uint32_t fol
  * host: `result` is device memory of this device (8 bytes) and receives the value asynchronously.
  * The per-workgroup partials go to a static scratch array of 4096 partials (64 KiB) on each GPU, held
  * until the process ends; a launch that would need more partials is refused with SCIHIP_EINVAL
- * (MI355X launches at most 4 per compute unit, 1024). The scratch keeps the last call's contents and
+ * (MI355X launches at most 16 per compute unit, 4096). The scratch keeps the last call's contents and
  * nothing in it needs to start at zero. Every call on a GPU shares that GPU's scratch: calls on one
  * device are ordered on its stream, but calls through two devices opened on one GPU race. */
 int scihip_reduce_static(scihip_device *device, const scihip_chunk *chunk, uint32_t fold, size_t count,

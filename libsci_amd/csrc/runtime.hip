@@ -237,12 +237,14 @@ namespace scihip {
 
     // Workgroups for `count` elements at `width` lanes and `unroll` vectors per thread per sweep,
     // each sweeping the array grid-stride. Stores: up to 64 per CU (16384 on MI355X; the C2 store
-    // measured 1.5% faster than with 32 per CU). Reductions: 4 per CU, one partial each for the
-    // fold kernel (a read-only dot at 2^28 measured faster with 4 per CU than with 8).
+    // measured 1.5% faster than with 32 per CU). Reductions: 16 per CU (4096 on MI355X, four waves
+    // per SIMD), one partial each for the fold kernel: with 4 per CU a wave's transcendentals did not
+    // overlap its loads (C2 sum 5.45 -> 6.01 TB/s, adjoint forward 3.18 -> 3.63 TB/s; the dot, all
+    // loads, is the same at either).
     unsigned blocks_for(scihip_device *device, size_t count, int width, int unroll, int mode) {
         const size_t steps = (count + width - 1) / width;
         const size_t wanted = (steps + 256 * size_t(unroll) - 1) / (256 * size_t(unroll));
-        const size_t fill = size_t(device->units) * (mode == 0 ? 64 : 4);
+        const size_t fill = size_t(device->units) * (mode == 0 ? 64 : 16);
         return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>({wanted, fill, kMaxBlocks})));
     }
 
