@@ -538,6 +538,7 @@ def main() -> int:
     parser.add_argument("--products-only", action="store_true", help="only the C3 products (for profiling)")
     parser.add_argument("--reductions-only", action="store_true", help="only the C2 reductions (for profiling)")
     parser.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurements (for profiling)")
+    parser.add_argument("--small-only", action="store_true", help="only the small-array latencies")
     parser.add_argument("--interpreter-only", action="store_true",
                         help="only the interpreter's store and sum (for profiling)")
     parser.add_argument("--store-only", action="store_true",
@@ -575,6 +576,10 @@ def main() -> int:
         return 0
     if args.products_only:
         print(json.dumps(measure_products(sci, _abi, device, events)), flush=True)
+        device.close()
+        return 0
+    if args.small_only:
+        print(json.dumps(measure_small_arrays(sci, _abi, device, events)), flush=True)
         device.close()
         return 0
     if args.adjoint_only:

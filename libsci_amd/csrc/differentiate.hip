@@ -602,13 +602,12 @@ namespace scihip {
                 blocks = blocks_for(device, count, width, 1, mode);
                 rc = status(interpret_adjoint<T>(launch, registers, blocks, device->stream), "adjoint launch");
             }
+            // the loss goes straight into the pinned host word (no copy launch)
             if (!rc && plan.loss)
-                rc = status(fold_partials(0, launch.partials, blocks, device->result, device->stream), "loss fold");
+                rc = status(fold_partials(0, launch.partials, blocks, loss ? device->staging : device->result,
+                                          device->stream),
+                            "loss fold");
             if (rc || !loss) return rc;
-            if ((rc = status(hipMemcpyAsync(device->staging, device->result, sizeof(double), hipMemcpyDeviceToHost,
-                                            device->stream),
-                             "loss copy")))
-                return rc;
             if ((rc = status(hipStreamSynchronize(device->stream), "adjoint"))) return rc;
             *loss = *device->staging;
             return SCIHIP_OK;

@@ -272,10 +272,23 @@ namespace scihip {
 
     // Where a reduction leaves its partials and its result, and whether the call may compile.
     struct Sink {
-        double *result;      // device memory for the folded result
+        double *result;      // where the fold kernel writes the folded result
         unsigned capacity;   // partials arguments.partials holds
         bool compile;        // false: only a kernel already loaded runs (loading a module allocates)
+        Partial *single = nullptr; // when set, a one-workgroup reduction leaves its partial here and
+                                   // runs no fold kernel (*folded tells the caller)
+        bool *folded = nullptr;
     };
+
+    // What the fold kernel makes of a single partial (fold_partials_block, then the NaN flag): the
+    // fold's identity combined with the value, so a sum of -0 gives +0 as it does there.
+    double fold_single(uint32_t fold, const Partial &partial) {
+        if (partial.poisoned) return std::numeric_limits<double>::quiet_NaN();
+        const double v = partial.value;
+        if (fold == 0) return 0.0 + v;
+        const double inf = std::numeric_limits<double>::infinity();
+        return fold == 1 ? (v < inf ? v : inf) : (v > -inf ? v : -inf);
+    }
 
     // Runs a chunk: its specialised kernel when ready (or forced), else the interpreter. A reduction
     // folds its partials into sink.result.
@@ -313,6 +326,9 @@ namespace scihip {
             return SCIHIP_EINVAL;
         }
         int rc = SCIHIP_OK;
+        const bool single = mode == 1 && blocks == 1 && sink.single;
+        if (single) arguments.partials = sink.single;
+        if (sink.folded) *sink.folded = !single;
         if (kernel) {
             void *parameters[] = {&arguments}; // copied at launch (and into a graph node under capture)
             ++device->specialised;
@@ -324,7 +340,7 @@ namespace scihip {
             ++device->interpreted;
             rc = status(interpret(arguments, registers, mode, blocks, device->stream), "interpreter launch");
         }
-        if (rc || mode != 1) return rc;
+        if (rc || mode != 1 || single) return rc;
         return status(fold_partials(arguments.p.fold, arguments.partials, blocks, sink.result, device->stream),
                       "reduction fold");
     }
@@ -352,15 +368,16 @@ namespace scihip {
         int registers = 0;
         if (const int rc = encode<T>(device, chunk, nullptr, count, fold, true, arguments, registers)) return rc;
         const bool contract = contracted(chunk, fold);
-        if (const int rc = launch<T>(device, chunk, arguments, registers, 1, contract,
-                                     Sink{device->result, kMaxBlocks, true}))
-            return rc;
-        if (const int rc = status(hipMemcpyAsync(device->staging, device->result, sizeof(double),
-                                                 hipMemcpyDeviceToHost, device->stream),
-                                  "result copy"))
-            return rc;
+        // The fold kernel writes the value straight into the pinned host word (no copy launch); a
+        // reduction small enough for one workgroup leaves its partial in pinned host memory instead,
+        // and the host folds it (no fold launch either).
+        bool folded = true;
+        Sink sink{device->staging, kMaxBlocks, true};
+        sink.single = device->staging_partial;
+        sink.folded = &folded;
+        if (const int rc = launch<T>(device, chunk, arguments, registers, 1, contract, sink)) return rc;
         if (const int rc = status(hipStreamSynchronize(device->stream), "reduce")) return rc;
-        *result = *device->staging;
+        *result = folded ? *device->staging : fold_single(arguments.p.fold, *device->staging_partial);
         return SCIHIP_OK;
     }
 
@@ -682,6 +699,9 @@ int scihip_open(uint32_t ordinal, scihip_device **out) {
     if (int rc = status(hipMalloc(reinterpret_cast<void **>(&device->result), 256), "hipMalloc")) return rc;
     if (int rc = status(hipHostMalloc(reinterpret_cast<void **>(&device->staging), sizeof(double)), "hipHostMalloc"))
         return rc;
+    if (int rc = status(hipHostMalloc(reinterpret_cast<void **>(&device->staging_partial), sizeof(Partial)),
+                        "hipHostMalloc"))
+        return rc;
     // This GPU's copy of the static scratch (hipSetDevice above made it current).
     void *symbol = nullptr;
     if (int rc = status(hipGetSymbolAddress(&symbol, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return rc;
@@ -714,6 +734,7 @@ int scihip_close(scihip_device *device) {
         (void)hipFree(device->partials);
         (void)hipFree(device->result);
         (void)hipHostFree(device->staging);
+        (void)hipHostFree(device->staging_partial);
         (void)hipStreamDestroy(device->stream);
     }
     delete device;

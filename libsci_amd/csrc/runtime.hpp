@@ -91,7 +91,8 @@ struct scihip_device {
     void *partials = nullptr;   // reduction partials, one per workgroup
     scihip::Partial *static_partials = nullptr; // this GPU's static scratch (scihip_reduce_static)
     double *result = nullptr;   // reduction result on the device
-    double *staging = nullptr;  // pinned host copy of it
+    double *staging = nullptr;  // pinned host word the fold kernel writes a synchronous result into
+    scihip::Partial *staging_partial = nullptr; // pinned host partial of a one-workgroup reduction
     int policy = 0;             // 0 automatic, 1 interpreter only, 2 specialised only
     scihip::Workspace lu;       // LU barrier counter, error flag, solve flags, panel publications
     unsigned epoch = 0;         // launches of the solve kernels so far (their ready-flag values)

@@ -124,6 +124,34 @@ def test_nan_wins_and_empty(device):
     assert math.isnan(gpu_reduce(device, [("in", 0)], [db], Fold.sum))
 
 
+@pytest.mark.parametrize("policy", [1, 2], ids=["interpreter", "specialised"])
+def test_one_workgroup_reductions(device, policy):
+    """Reductions small enough for one workgroup leave their partial in pinned host memory and the
+    host folds it (no fold kernel): the same bits as the oracle, signed zeros and NaN included, and as
+    the static variant, whose fold runs on the device."""
+    cases = [[-0.0], [-0.0, -0.0], [0.0, -0.0], [np.nan], [np.inf], [-np.inf], [5.0, -3.0, 0.25],
+             [np.nan, 1.0, np.nan], [-np.inf, np.inf]]
+    result_device = sci.Array(device, (1,), np.float64)
+    for values in cases:
+        a = np.array(values, dtype=np.float32)
+        dev = sci.Array.from_numpy(a, device)
+        chunk = record_postfix([("in", 0)], Precision.single, "reduce", [a.ctypes.data])
+        for fold in Fold:
+            got = gpu_reduce(device, [("in", 0)], [dev], fold, policy)
+            want = oracle.reduce(chunk, fold, a.size, threads=1)
+            assert np.array_equal(np.float64(got), np.float64(want), equal_nan=True), (values, fold, got, want)
+            assert math.copysign(1.0, got) == math.copysign(1.0, want) or math.isnan(want), (values, fold, got, want)
+            device.set_policy(policy)
+            static = record_postfix([("in", 0)], Precision.single, "reduce", [dev.pointer])
+            _abi.check(_abi.lib().scihip_reduce_static(device.handle, ctypes.byref(static.struct), int(fold), a.size,
+                                                       ctypes.c_void_p(result_device.pointer)), "scihip_reduce_static")
+            device.synchronize()
+            device.set_policy(0)
+            on_device = float(result_device.host()[0])
+            assert np.array_equal(np.float64(got), np.float64(on_device), equal_nan=True), (values, fold, got, on_device)
+            assert math.isnan(got) or math.copysign(1.0, got) == math.copysign(1.0, on_device)
+
+
 def test_front_end_reductions(device):
     """tests/operations/test_reduce.cpp:20-27 on the device."""
     count = 1_000_000
