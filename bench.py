@@ -539,6 +539,7 @@ def main() -> int:
     parser.add_argument("--reductions-only", action="store_true", help="only the C2 reductions (for profiling)")
     parser.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurements (for profiling)")
     parser.add_argument("--small-only", action="store_true", help="only the small-array latencies")
+    parser.add_argument("--matrix-only", action="store_true", help="only the 2-D transpose and axis folds")
     parser.add_argument("--interpreter-only", action="store_true",
                         help="only the interpreter's store and sum (for profiling)")
     parser.add_argument("--store-only", action="store_true",
@@ -576,6 +577,10 @@ def main() -> int:
         return 0
     if args.products_only:
         print(json.dumps(measure_products(sci, _abi, device, events)), flush=True)
+        device.close()
+        return 0
+    if args.matrix_only:
+        print(json.dumps(measure_two_dimensional(sci, _abi, device, events)), flush=True)
         device.close()
         return 0
     if args.small_only:
