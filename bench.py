@@ -281,6 +281,21 @@ def measure_products(sci, _abi, device, events) -> dict:
                 "max_abs_error": float(np.abs(got - exact).max()),
                 "relative_frobenius": float(np.linalg.norm(got - exact) / np.linalg.norm(exact))}
         del left, right, product
+    # The same 8192^3 bf16 product on matrices of ones: the exact result (8192 everywhere) and the
+    # same work, but operands whose bits barely toggle, which the matrix cores' power limit shows.
+    m = k = n = 8192
+    ones = sci.full((m, k), 1.0, device)
+    product = sci.Array(device, (m, n), "float32")
+
+    def run_ones():
+        _abi.check(_abi.lib().scihip_product(device.handle, 1, ones.pointer, ones.pointer, product.pointer, m, n, k))
+
+    _, ms = time_launches(device, events, run_ones, 5, 1)
+    got = product.host()
+    out["8192x8192x8192_bf16_ones"] = {"TFLOPs": round(2 * m * n * k / (ms * 1e-3) / 1e12, 1), "ms": round(ms, 3),
+                                       "max_abs_error": float(np.abs(got.astype(np.float64) - k).max()),
+                                       "relative_frobenius": 0.0 if np.all(got == k) else float("nan")}
+    del ones, product
     return out
 
 
