@@ -293,6 +293,9 @@ namespace scihip {
         __global__ __launch_bounds__(kThreads, 1) void gemm_slabs(const ProductArgs p, int slices) {
             typedef SlabRing<Mode> Ring;
             constexpr int Stages = Ring::slots, kSlotBytes = Ring::slot;
+            // stages a tile's prologue issues: all but one slot for bf16 (each stage refills the slot the
+            // one before it used), both slots for bf16x3 (each stage refills its own slot)
+            constexpr int Ahead = Mode == kSplit ? Stages : Stages - 1;
             static_assert(Mode == kSplit || (Stages >= 3 && Stages <= 5), "wait_stages counts up to three pending stages");
             extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
             const int tiles_m = (int)((p.m + kWide - 1) / kWide);
@@ -378,7 +381,7 @@ namespace scihip {
                 const __amdgpu_buffer_rsrc_t a_lo = Ring::operands == 4 ? descriptor(p.a_lo, tm) : a;
                 const __amdgpu_buffer_rsrc_t b_lo = Ring::operands == 4 ? descriptor(p.b_lo, tn) : b;
 #pragma unroll
-                for (int s = 0; s < Stages - 1; ++s)
+                for (int s = 0; s < Ahead; ++s)
                     if (s < count) issue(a, b, a_lo, b_lo, first + s, s);
             };
 
@@ -403,98 +406,137 @@ namespace scihip {
 
                 if constexpr (Mode == kBfloat16) {
                     v8bf x[2][4], y[2][4]; // fragments of k-step 0 (x) and 1 (y) of the current stage: [A, B][block]
-                    auto read = [&](v8bf (&f)[2][4], int slot, int step) {
+                    // Fragment q of a k-step in the order the MFMAs first need them: A0, B0, B1, B2, B3,
+                    // A1, A2, A3 (MFMA q runs block (q / 4, q % 4)).
+                    auto read_one = [&](v8bf (&f)[2][4], int slot, int step, int q) {
                         const unsigned char *base = lds + slot * kSlotBytes;
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            f[0][i] = *reinterpret_cast<const v8bf *>(base + a_at[step][i]);
-                            f[1][i] = *reinterpret_cast<const v8bf *>(base + b_at[step][i]);
-                        }
+                        const int operand = (q == 0 || q >= 5) ? 0 : 1;
+                        const int block = q == 0 ? 0 : (q >= 5 ? q - 4 : q - 1);
+                        f[operand][block] = *reinterpret_cast<const v8bf *>(
+                                base + (operand == 0 ? a_at[step][block] : b_at[step][block]));
                     };
-                    auto multiply = [&](const v8bf (&f)[2][4]) {
-#pragma unroll
-                        for (int i = 0; i < 4; ++i)
-#pragma unroll
-                            for (int j = 0; j < 4; ++j)
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][j], f[0][i], acc[i][j], 0, 0, 0);
+                    // Load q of a stage: operand q % 2 (A, B), this wave's 1 KiB piece q / 2.
+                    auto load_one = [&](int64_t stage, int slot, int q) {
+                        unsigned char *base = lds + slot * kSlotBytes + (q & 1) * kStageBytes;
+                        const int piece = (wave * kStageLoads + (q >> 1)) * 1024;
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(q & 1 ? b : a, (lds_pointer)(base + piece), 16,
+                                                                  lane_offset[q >> 1], (int)(stage * kStageBytes), 0, 0);
+                    };
+                    auto mfma_one = [&](const v8bf (&f)[2][4], int q) {
+                        acc[q >> 2][q & 3] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][q & 3], f[0][q >> 2],
+                                                                                     acc[q >> 2][q & 3], 0, 0, 0);
                     };
 
                     // this tile's first stages were issued by the prologue (with the previous tile's stores
                     // after them in the count, so wait for everything)
                     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                     lds_barrier();
-                    read(x, 0, 0);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) read_one(x, 0, 0, q);
                     int slot = 0;
-                    // One stage: k-step 1's reads go out, k-step 0's MFMAs run; then (unless this is the last
-                    // stage) the next stage is awaited and published, the ring slot two behind refilled and
-                    // the next stage's k-step 0 read while k-step 1's MFMAs run. `more` is a constant at
-                    // every call, so the LDS reads are never conditional (which would make the compiler
-                    // wait for all of them before the MFMAs).
-                    auto advance = [&](int64_t kt, bool more, int pending) {
-                        read(y, slot, 1);
-                        __builtin_amdgcn_sched_barrier(0);
-                        multiply(x);
-                        __builtin_amdgcn_sched_barrier(0);
+                    // One stage, two halves of sixteen MFMAs. With one wave per SIMD every other instruction
+                    // takes an issue slot of its own, so the LDS reads and the loads are spread between the
+                    // MFMAs (one after every other MFMA), where each issues while an MFMA runs:
+                    // - first half: k-step 0's MFMAs, with k-step 1's eight fragment reads;
+                    // - then (unless this is the last stage) the wait for stage kt + 1 and the barrier;
+                    // - second half: k-step 1's MFMAs, with the eight loads refilling the ring slot of stage
+                    //   kt - 1 (which every wave is past) with stage kt + Stages - 1, and the next stage's
+                    //   k-step 0 reads.
+                    // `more` and `load` are constants at every call, so nothing in a half is conditional.
+                    auto advance = [&](int64_t kt, bool more, bool load, int pending) {
                         const int next = slot + 1 == Stages ? 0 : slot + 1;
+                        const int refill = slot == 0 ? Stages - 1 : slot - 1;
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) {
+                            mfma_one(x, q);
+                            if ((q & 1) == 0) read_one(y, slot, 1, q >> 1);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
                         if (more) {
                             wait_stages(pending);
                             lds_barrier();
-                            // every wave is past stage kt - 1: its slot takes stage kt + Stages - 1
-                            if (kt + Stages - 1 < stages)
-                                issue(a, b, a_lo, b_lo, first + kt + Stages - 1, slot == 0 ? Stages - 1 : slot - 1);
-                            read(x, next, 0);
+                        }
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) {
+                            mfma_one(y, q);
+                            if (load && (q & 1) == 0) load_one(first + kt + Stages - 1, refill, q >> 1);
+                            if (more && (q & 1) == 1) read_one(x, next, 0, q >> 1);
                             __builtin_amdgcn_sched_barrier(0);
                         }
-                        multiply(y);
-                        __builtin_amdgcn_sched_barrier(0);
                         slot = next;
                     };
                     // stage kt + 1 has landed once only the stages after it, up to kt + Stages - 2, are pending
                     int64_t kt = 0;
-                    for (; kt < stages - 1 && kt < stages - Stages + 2; ++kt) advance(kt, true, Stages - 3);
-                    for (; kt < stages - 1; ++kt) advance(kt, true, (int)(stages - 2 - kt));
-                    advance(stages - 1, false, 0);
+                    for (; kt < stages - Stages + 1; ++kt) advance(kt, true, true, Stages - 3);
+                    for (; kt < stages - 1 && kt < stages - Stages + 2; ++kt) advance(kt, true, false, Stages - 3);
+                    for (; kt < stages - 1; ++kt) advance(kt, true, false, (int)(stages - 2 - kt));
+                    advance(stages - 1, false, false, 0);
                 } else {
-                    // bf16x3: two slots, each stage awaited whole before it is read; a stage is 96 MFMAs
-                    // per wave (three per block and k-step), so one stage of lookahead covers the loads.
+                    // bf16x3: two slots; a stage is 96 MFMAs per wave (three per block and k-step), laid
+                    // out as the bf16 stage is: k-step 0's MFMAs with k-step 1's reads, the barrier, then
+                    // k-step 1's MFMAs with the loads of stage kt + 2 into this stage's slot and the next
+                    // stage's k-step 0 reads (one read or load after every third MFMA). The loads have a
+                    // stage of 96 MFMAs to land.
                     v8bf x[4][4], y[4][4]; // k-steps 0 and 1: [A, B, A_lo, B_lo][block]
-                    auto read = [&](v8bf (&f)[4][4], int slot, int step) {
-                        const unsigned char *base = lds + slot * kSlotBytes;
+                    // fragment q in the order the MFMAs first need them: block (0, 0)'s B, A_lo, B_lo, A;
+                    // then B and B_lo of blocks (0, 1) ... (0, 3); then A_lo and A of rows 1 ... 3
+                    auto read_one = [&](v8bf (&f)[4][4], int slot, int step, int q) {
+                        constexpr int kOperand[16] = {1, 2, 3, 0, 1, 3, 1, 3, 1, 3, 2, 0, 2, 0, 2, 0};
+                        constexpr int kBlock[16] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 1, 1, 2, 2, 3, 3};
+                        const int operand = kOperand[q], block = kBlock[q];
+                        const unsigned char *base = lds + slot * kSlotBytes + (operand >= 2 ? 2 * kStageBytes : 0);
+                        f[operand][block] = *reinterpret_cast<const v8bf *>(
+                                base + ((operand & 1) == 0 ? a_at[step][block] : b_at[step][block]));
+                    };
+                    // load q: operand q % 4 (A, B, A_lo, B_lo), this wave's 1 KiB piece q / 4
+                    auto load_one = [&](int64_t stage, int slot, int q) {
+                        const int operand = q & 3;
+                        unsigned char *base = lds + slot * kSlotBytes + operand * kStageBytes;
+                        const int piece = (wave * kStageLoads + (q >> 2)) * 1024;
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(operand == 0 ? a : operand == 1 ? b : operand == 2 ? a_lo : b_lo,
+                                                                  (lds_pointer)(base + piece), 16, lane_offset[q >> 2],
+                                                                  (int)(stage * kStageBytes), 0, 0);
+                    };
+                    // MFMA q: block (q / 12, (q / 3) % 4), term q % 3: the small cross terms first, then
+                    // the leading term
+                    auto mfma_one = [&](const v8bf (&f)[4][4], int q) {
+                        const int i = q / 12, j = (q / 3) % 4, term = q % 3;
+                        const v8bf &left = term == 1 ? f[3][j] : f[1][j];
+                        const v8bf &right = term == 0 ? f[2][i] : f[0][i];
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(left, right, acc[i][j], 0, 0, 0);
+                    };
+                    // this tile's first two stages were issued by the prologue (with the previous tile's
+                    // stores after them in the count, so wait for everything)
+                    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                    lds_barrier();
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            f[0][i] = *reinterpret_cast<const v8bf *>(base + a_at[step][i]);
-                            f[1][i] = *reinterpret_cast<const v8bf *>(base + b_at[step][i]);
-                            f[2][i] = *reinterpret_cast<const v8bf *>(base + 2 * kStageBytes + a_at[step][i]);
-                            f[3][i] = *reinterpret_cast<const v8bf *>(base + 2 * kStageBytes + b_at[step][i]);
+                    for (int q = 0; q < 16; ++q) read_one(x, 0, 0, q);
+                    auto advance = [&](int64_t kt, bool more, bool load) {
+                        const int slot = (int)(kt & 1);
+#pragma unroll
+                        for (int q = 0; q < 48; ++q) {
+                            mfma_one(x, q);
+                            if (q % 3 == 0) read_one(y, slot, 1, q / 3);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                        if (more) {
+                            // stage kt + 1 has landed for every wave, and every wave's reads of this slot
+                            // have returned (its loads come next)
+                            asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+                            lds_barrier();
+                        }
+#pragma unroll
+                        for (int q = 0; q < 48; ++q) {
+                            mfma_one(y, q);
+                            if (load && q % 3 == 0) load_one(first + kt + 2, slot, q / 3);
+                            if (more && q % 3 == 1) read_one(x, slot ^ 1, 0, q / 3);
+                            __builtin_amdgcn_sched_barrier(0);
                         }
                     };
-                    // the small cross terms first, then the leading term
-                    auto multiply = [&](const v8bf (&f)[4][4]) {
-#pragma unroll
-                        for (int i = 0; i < 4; ++i)
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][j], f[2][i], acc[i][j], 0, 0, 0);
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3][j], f[0][i], acc[i][j], 0, 0, 0);
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1][j], f[0][i], acc[i][j], 0, 0, 0);
-                            }
-                    };
-                    for (int64_t kt = 0; kt < stages; ++kt) {
-                        const int slot = (int)(kt & 1);
-                        // stage kt: issued by the prologue or one stage ago (for the first, with the previous
-                        // tile's stores after it in the count)
-                        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                        lds_barrier();
-                        // every wave is past stage kt - 1: the other slot takes stage kt + 1
-                        if (kt + 1 < stages) issue(a, b, a_lo, b_lo, first + kt + 1, slot ^ 1);
-                        read(x, slot, 0);
-                        read(y, slot, 1);
-                        __builtin_amdgcn_sched_barrier(0);
-                        multiply(x);
-                        __builtin_amdgcn_sched_barrier(0);
-                        multiply(y);
-                        __builtin_amdgcn_sched_barrier(0);
-                    }
+                    int64_t kt = 0;
+                    for (; kt < stages - 2; ++kt) advance(kt, true, true);
+                    for (; kt < stages - 1; ++kt) advance(kt, true, false);
+                    advance(stages - 1, false, false);
                 }
                 // every wave has read its last fragments: the next tile's first stages may land
                 lds_barrier();
