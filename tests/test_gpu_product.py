@@ -120,16 +120,18 @@ def test_metric_sizes(device, shape):
         assert relative < frobenius[mode]
 
 
-@pytest.mark.parametrize("shape", [(4100, 1000, 4200), (4352, 576, 4097), (2000, 8200, 2050)],
-                         ids=["4100x1000x4200", "4352x576x4097", "2000x8200x2050"])
-def test_large_tiles_ragged(device, shape):
-    """bf16 products with at least 256 output tiles of 256 x 256 take the large-tile kernel; these
-    have partial last tiles in both directions and a shared dimension padded up to a multiple of 64
-    (1000 -> 1024, 8200 -> 8256) or not (576). 2000 x 2050 has 72 tiles, so its 258 stages are split
-    four ways (uneven slices) and the partial products folded. Sampled rows, the last rows among
-    them, against float64."""
+@pytest.mark.parametrize("shape", [(4100, 1000, 4200), (4100, 1003, 4200), (4352, 576, 4097), (2000, 8200, 2050)],
+                         ids=["4100x1000x4200", "4100x1003x4200", "4352x576x4097", "2000x8200x2050"])
+@pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
+def test_large_tiles_ragged(device, shape, mode):
+    """bf16 and bf16x3 products with at least 256 output tiles of 256 x 256 take the large-tile
+    kernel; these have partial last tiles in both directions and a shared dimension padded up to a
+    multiple of 64 (1000 and 1003 -> 1024, 8200 -> 8256) or not (576); 1003 also ends inside a
+    16-byte chunk of the packed rows. 2000 x 2050 has 72 tiles, so its 258 stages are split four
+    ways (uneven slices) and the partial products folded. Sampled rows, the last rows among them,
+    against float64."""
     m, k, n = shape
-    print(f"[test] bf16 large tiles at {shape}", flush=True)
+    print(f"[test] {mode} large tiles at {shape}", flush=True)
     left = sci.uniform(m * k, device, 41, -1.0, 1.0).reshape(m, k)
     right = sci.uniform(k * n, device, 42, -1.0, 1.0).reshape(k, n)
     output = sci.Array(device, (m, n), np.float32)
@@ -141,8 +143,8 @@ def test_large_tiles_ragged(device, shape):
     right64 = right_h.astype(np.float64)
     want = left_h[rows].astype(np.float64) @ right64
     magnitude = np.abs(left_h[rows]).astype(np.float64) @ np.abs(right64)
-    _abi.check(_abi.lib().scihip_product(device.handle, MODES["bf16"], left.pointer, right.pointer,
+    _abi.check(_abi.lib().scihip_product(device.handle, MODES[mode], left.pointer, right.pointer,
                                          output.pointer, m, n, k), "scihip_product")
     got = output.host()[rows].astype(np.float64)
-    assert (np.abs(got - want) <= 2 * k * UNIT["bf16"] * magnitude).all()
-    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-2
+    assert (np.abs(got - want) <= 2 * k * UNIT[mode] * magnitude).all()
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < {"bf16": 1e-2, "bf16x3": 1e-4}[mode]
