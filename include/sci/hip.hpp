@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <initializer_list>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -237,11 +238,19 @@ namespace sci::hip {
     };
 
     // A row-major device array. Copies share the allocation (like views); it is disposed with the last.
+    //
+    // Element access (a[i], a.at(i, j)) reads and writes a host copy of the elements, since HBM is not
+    // mapped into the host as the reference's device memory is (include/array/array.hpp:186-217,
+    // include/array/storage.hpp:100-102). The copy is downloaded at the first access after device work
+    // (which the download waits for), shared by the array's copies, and handed back at the next device
+    // use: data(), the pointer every device operation takes, uploads it first if an element was written
+    // and drops it, since the operation may write the array.
     template <typename T> class Array {
     public:
         static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>, "float or double elements");
 
-        Array(Device &device, std::vector<size_t> shape) : owner(&device), dims(std::move(shape)) {
+        Array(Device &device, std::vector<size_t> shape)
+            : owner(&device), dims(std::move(shape)), mirror(std::make_shared<Mirror>()) {
             size_t n = 1;
             for (size_t d : dims) n *= d;
             count = n;
@@ -264,6 +273,24 @@ namespace sci::hip {
             return values;
         }
 
+        // a[i]: an element by its flat index, through the host copy (written back at the next device use).
+        T &operator[](size_t index) {
+            T &value = values().at(index);
+            mirror->dirty = true;
+            return value;
+        }
+        const T &operator[](size_t index) const { return values().at(index); }
+
+        // a.at(i, j, ...): an element by its index along each dimension (row-major).
+        template <typename... Indices> T &at(Indices... indices) {
+            T &value = values().at(offset({static_cast<size_t>(indices)...}));
+            mirror->dirty = true;
+            return value;
+        }
+        template <typename... Indices> const T &at(Indices... indices) const {
+            return values().at(offset({static_cast<size_t>(indices)...}));
+        }
+
         // Writes the array to a .npy file (its shape and elements; see sci/npy.hpp).
         void save(const std::string &path) const { npy::write(path, host(), dims); }
 
@@ -275,16 +302,52 @@ namespace sci::hip {
             return from(device, values, shape);
         }
 
-        T *data() const noexcept { return static_cast<T *>(storage.get()); }
+        // The device pointer, for device work: elements written through the host copy go up first, and
+        // the copy is dropped (the work may write the array).
+        T *data() const {
+            if (mirror->valid) {
+                if (mirror->dirty) owner->upload(storage.get(), mirror->values.data(), count * sizeof(T));
+                mirror->values = std::vector<T>();
+                mirror->valid = mirror->dirty = false;
+            }
+            return static_cast<T *>(storage.get());
+        }
         size_t size() const noexcept { return count; }
         const std::vector<size_t> &shape() const noexcept { return dims; }
         Device &device() const noexcept { return *owner; }
 
     private:
+        struct Mirror {
+            std::vector<T> values;
+            bool valid = false; // values hold the elements as of the last device use
+            bool dirty = false; // an element was written since
+        };
+
+        std::vector<T> &values() const {
+            if (!mirror->valid) {
+                mirror->values.resize(count);
+                owner->download(mirror->values.data(), storage.get(), count * sizeof(T));
+                mirror->valid = true;
+                mirror->dirty = false;
+            }
+            return mirror->values;
+        }
+
+        size_t offset(std::initializer_list<size_t> indices) const {
+            if (indices.size() != dims.size()) throw Error(SCIHIP_EINVAL, "one index per dimension");
+            size_t flat = 0, k = 0;
+            for (size_t index : indices) {
+                if (index >= dims[k]) throw Error(SCIHIP_EINVAL, "index out of range");
+                flat = flat * dims[k++] + index;
+            }
+            return flat;
+        }
+
         Device *owner;
         std::vector<size_t> dims;
         size_t count = 0;
         std::shared_ptr<void> storage;
+        std::shared_ptr<Mirror> mirror;
     };
 
     // ---------------------------------------------------------------------------------- expressions

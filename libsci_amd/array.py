@@ -114,12 +114,38 @@ class Graph:
 # ------------------------------------------------------------------------------------------ arrays
 
 class _Storage:
-    """A device allocation, returned to the device when the last array using it goes."""
+    """A device allocation, returned to the device when the last array using it goes.
+
+    Element access (``a[i]``, ``a[i, j] = v``) goes through a host copy of the allocation, since HBM
+    is not mapped into the host as the reference's device memory is (include/array/array.hpp:186-217,
+    include/array/storage.hpp:100-102). The copy is downloaded at the first access after device work
+    (the download waits for that work), shared by the arrays viewing the allocation, and handed back
+    at the next device use: ``Array.pointer``, which every device operation takes, uploads it first if
+    an element was written and drops it, since the operation may write the allocation."""
 
     def __init__(self, device: Device, nbytes: int):
         self.device = device
         self.pointer = device.allocate(nbytes)
         self.nbytes = nbytes
+        self.mirror = None   # the host copy (bytes), or None
+        self.dirty = False   # an element was written through the copy since it was taken
+
+    def host_bytes(self) -> np.ndarray:
+        if self.mirror is None:
+            mirror = np.empty(self.nbytes, dtype=np.uint8)
+            if self.nbytes:
+                _abi.check(_abi.lib().scihip_download(self.device.handle, mirror.ctypes.data, self.pointer,
+                                                      self.nbytes), "scihip_download")
+            self.mirror, self.dirty = mirror, False
+        return self.mirror
+
+    def release(self) -> None:
+        """Before device work: elements written on the host go up, and the host copy is dropped."""
+        if self.mirror is not None:
+            if self.dirty and self.nbytes:
+                _abi.check(_abi.lib().scihip_upload(self.device.handle, self.pointer, self.mirror.ctypes.data,
+                                                    self.nbytes), "scihip_upload")
+            self.mirror, self.dirty = None, False
 
     def __del__(self):
         try:
@@ -145,6 +171,8 @@ class Array:
     # -- identity
     @property
     def pointer(self) -> int:
+        """The device address, for device work (hands the host copy of the elements back first)."""
+        self.storage.release()
         return self.storage.pointer + self.offset
 
     @property
@@ -187,8 +215,18 @@ class Array:
             return out
         return Array.from_numpy(self.host(), device)
 
+    # -- element access through the allocation's host copy (include/array/array.hpp:186-217)
+    def _elements(self) -> np.ndarray:
+        raw = self.storage.host_bytes()
+        return raw[self.offset:self.offset + self.size * self.dtype.itemsize].view(self.dtype).reshape(self.shape)
+
     def __getitem__(self, index):
-        return self.host()[index]
+        value = self._elements()[index]
+        return value.copy() if isinstance(value, np.ndarray) else value
+
+    def __setitem__(self, index, value) -> None:
+        self._elements()[index] = value
+        self.storage.dirty = True
 
     def reshape(self, *shape) -> "Array":
         shape = shape[0] if len(shape) == 1 and isinstance(shape[0], (tuple, list)) else shape

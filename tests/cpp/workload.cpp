@@ -168,6 +168,25 @@ namespace {
         const scihip_solve_stats stats = device.solve(Factoring::float32, A.data(), B.data(), X.data(), 3, 1);
         std::printf("solve: %d refinements, backward error %.3g\n", stats.iterations, stats.backward_error);
         write(directory + "/solve.f64", X.host());
+
+        // Element access through the host copy (include/array/array.hpp:186-217): reads after device
+        // work see its result, writes reach the next device use, copies share the elements.
+        Array<float> e = Array<float>::from(device, std::vector<float>{1, 2, 3, 4, 5, 6}, {2, 3});
+        std::vector<float> seen;
+        seen.push_back(e.at(1, 2));       // 6
+        Array<float> alias = e;           // shares the allocation and its host copy
+        e[0] = 10.0f;
+        alias.at(1, 1) = 20.0f;
+        seen.push_back(alias[0]);         // 10
+        Array<float> twice = evaluate(e * 2.0f);
+        for (size_t i = 0; i < twice.size(); ++i) seen.push_back(twice[i]); // 20 4 6 8 40 12
+        seen.push_back(twice.at(1, 1));   // 40
+        assign(twice, Expr<float>(e) + 1.0f); // a device write into an array whose copy was read
+        seen.push_back(twice[0]);         // 11
+        seen.push_back(twice.at(1, 1));   // 21
+        const std::vector<float> host = e.host();
+        seen.insert(seen.end(), host.begin(), host.end()); // 10 2 3 4 20 6
+        write(directory + "/elements.f32", seen);
         device.synchronize();
         return 0;
     }

@@ -102,5 +102,8 @@ def test_cpp_workload_on_the_gpu(tmp_path):
     assert "reloaded shape 64 x 48" in result.stdout  # Array::load
     assert (np.fromfile(tmp_path / "reloaded.f32", dtype=np.float32).reshape(64, 48) == product).all()
 
+    # element access through the host copy: reads after device work, writes before it, shared by copies
+    elements = np.fromfile(tmp_path / "elements.f32", dtype=np.float32)
+    assert elements.tolist() == [6, 10, 20, 4, 6, 8, 40, 12, 40, 11, 21, 10, 2, 3, 4, 20, 6]
     x = np.fromfile(tmp_path / "solve.f64", dtype=np.float64)
     assert np.abs(x - [1.0, -2.0, 3.0]).max() < 1e-12
