@@ -338,9 +338,13 @@ namespace scihip {
                     }
                 return row;
             };
-            // Publishes the candidate `row` for column `col` (threads of part 0), and the diagonal row if
-            // this workgroup owns it (part 1); `pending`: the rows' columns beyond col still lack the
-            // update of column col - 1, whose multipliers are in `multipliers`.
+            // Publishes the candidate `row` for column `col`, and the diagonal row if this workgroup owns
+            // it; `pending`: the rows' columns beyond col still lack the update of column col - 1, whose
+            // multipliers are in `multipliers`. Every publication store comes from the waves of part 1,
+            // and every load of another workgroup's publication from those of part 0: a wave's wait for
+            // a load also waits for the stores it issued before (one counter counts both), and an
+            // agent-scope store is acknowledged only once it reaches memory, so the waves that poll
+            // never store.
             auto publish = [&](int col, int64_t row, bool pending) {
                 const int64_t dd = j0 + col;
                 const unsigned tag = tag_of(col);
@@ -353,10 +357,10 @@ namespace scihip {
                         put_tagged(line + field<T>(fields + 2), pending ? multipliers[i] : T(0), tag);
                     }
                 };
-                if (part == 0 && row != LLONG_MAX) send((int)(row - first), published + row_index(col, b), kValue);
-                if (part == 1 && dd >= first && dd < first + mine)
-                    send((int)(dd - first), diagonal + diagonal_index(col), kDiagonal);
-                if (t == 0) {
+                if (part != 1) return;
+                if (row != LLONG_MAX) send((int)(row - first), published + row_index(col, b), kValue);
+                if (dd >= first && dd < first + mine) send((int)(dd - first), diagonal + diagonal_index(col), kDiagonal);
+                if (t == kPanelRows) {
                     const T best = row != LLONG_MAX ? T(__builtin_fabs(rows[(row - first) * stride + col])) : T(-1);
                     __hip_atomic_store(line, candidate_word<T>(best, row, j0, (unsigned)col + 1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -374,25 +378,25 @@ namespace scihip {
                 T *const u = pivot_rows[c & 1];
                 const T *const u_before = pivot_rows[(c + 1) & 1];
 
-                // A: gather. Thread g polls the lines of workgroups g, g + 256, ... (kPoll at once) with their
-                // fields until each carries this column, and keeps the largest word with its fields; the
-                // workgroup takes the largest word, so all workgroups pick the same pivot. Equal float keys
-                // of different doubles are settled with the exact values.
+                // A: gather. Thread g < kPanelRows polls the lines of workgroups g, g + 128, ... (kPoll at
+                // once) with their fields until each carries this column, and keeps the largest word with
+                // its fields; the workgroup takes the largest word, so all workgroups pick the same pivot.
+                // Equal float keys of different doubles are settled with the exact values.
                 unsigned long long mine_best = 0; // this thread's largest word, and how many share its key
                 int sharing = 0;
                 T value = T(0), next = T(0), factor = T(0); // the fields of that word's line
-                for (int g0 = t; g0 < G; g0 += kPoll * kPanelThreads) {
+                for (int g0 = t; part == 0 && g0 < G; g0 += kPoll * kPanelRows) {
                     unsigned waiting = 0;
 #pragma unroll
                     for (int q = 0; q < kPoll; ++q)
-                        if (g0 + q * kPanelThreads < G) waiting |= 1u << q;
+                        if (g0 + q * kPanelRows < G) waiting |= 1u << q;
                     unsigned spins = 0;
                     while (waiting) {
                         unsigned long long slot[kPoll][2 * kAllPairs<T>];
 #pragma unroll
                         for (int q = 0; q < kPoll; ++q) {
                             if (!(waiting >> q & 1u)) continue;
-                            const int g = g0 + q * kPanelThreads;
+                            const int g = g0 + q * kPanelRows;
                             const unsigned base = (unsigned)(line_index(c, g) * 8);
 #pragma unroll
                             for (int e = 0; e < kMainPairs<T>; ++e) load_pair(line_buffer, base + 16 * e, slot[q] + 2 * e);
@@ -404,7 +408,7 @@ namespace scihip {
 #pragma unroll
                         for (int q = 0; q < kPoll; ++q) {
                             if (!(waiting >> q & 1u)) continue;
-                            const int g = g0 + q * kPanelThreads;
+                            const int g = g0 + q * kPanelRows;
                             const unsigned long long *s = slot[q];
                             const unsigned long long x = s[0];
                             bool ready = (x & 0xFFu) == epoch;
@@ -535,7 +539,7 @@ namespace scihip {
                     rows[t * stride + c] = diagonal_value;
                     if (more) rows[t * stride + c + 1] = e1;
                 }
-                if (b == 0 && t == 0) pivots[d] = p;
+                if (b == 0 && t == kPanelRows) pivots[d] = p;
 
                 // C: multipliers and column c + 1 (look-ahead), each thread on its row.
                 const bool eliminate = pivot != T(0); // a zero pivot skips the column (decompose.hpp:44-45)
@@ -582,7 +586,7 @@ namespace scihip {
                     const int64_t row = search(c + 1); // its barrier also orders B and C before publication
                     if (receiver && (row == p || d + 1 == p)) {
                         // the swapped-in row is needed whole now
-                        if (part == 1) receive(diagonal_buffer, diagonal, diagonal_index(c), diagonal_factor, rows + (p - first) * stride);
+                        if (part == 0) receive(diagonal_buffer, diagonal, diagonal_index(c), diagonal_factor, rows + (p - first) * stride);
                         received = true;
                         settle();
                     }
@@ -591,8 +595,8 @@ namespace scihip {
                     settle();
                 }
 
-                // E: the rest of the pivot row (part 0) and, at workgroup owner_p, of the diagonal row
-                // (part 1); then row d at workgroup owner_d.
+                // E: the rest of the pivot row and, at workgroup owner_p, of the diagonal row (both by part
+                // 0); then row d at workgroup owner_d.
                 if (part == 0) {
                     if (at_diagonal) receive(diagonal_buffer, diagonal, diagonal_index(c), pivot_factor, u);
                     else receive(row_buffer, published, row_index(c, owner_p), pivot_factor, u);
@@ -600,8 +604,8 @@ namespace scihip {
                         u[c] = pivot;
                         if (more) u[c + 1] = u1;
                     }
-                } else if (receiver && !received) {
-                    receive(diagonal_buffer, diagonal, diagonal_index(c), diagonal_factor, rows + (p - first) * stride);
+                    if (receiver && !received)
+                        receive(diagonal_buffer, diagonal, diagonal_index(c), diagonal_factor, rows + (p - first) * stride);
                 }
                 settle();
                 if (swapped && b == owner_d && part == 0 && k_own < w) rows[(d - first) * stride + k_own] = u[k_own];
