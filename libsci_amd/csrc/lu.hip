@@ -651,6 +651,762 @@ namespace scihip {
                 if (k_own < w) a[(first + i) * n + j0 + k_own] = rows[i * stride + k_own];
         }
 
+        // The panel in strips: columns are factored kStrip at a time. Per column only the strip's
+        // entries of each row change; the candidate line carries the candidate's whole strip row, so
+        // one exchange between workgroups per column gives every workgroup the pivot row it needs. The
+        // rows' entries outside the strip stay as they were at the strip's start (not updated, not
+        // swapped) until the strip ends; then the rows that moved are exchanged once and every row
+        // takes the strip's updates. Every entry still gets the reference's eliminations one at a
+        // time, in column order, each as eliminated(), so the result equals panel()'s bit for bit.
+        template <typename T> constexpr int kStrip = sizeof(T) == 8 ? 8 : 16;       // columns per strip
+        template <typename T> constexpr int kStripRows = sizeof(T) == 8 ? 128 : 256; // rows per workgroup
+        constexpr int kStripThreads = 256;
+        constexpr int kRestSlots = 2;     // rest buffers, used in turn by consecutive strips
+        constexpr int kStripLineMax = 32; // 64-bit slots per line (256 bytes)
+        constexpr int kLineLanes = 16;    // lanes that load one line, a 16-byte pair each, in one instruction
+        // Lines a polling lane loads at once (G + 1 lines take (G + 1) / (8 kPollLoads<T>) passes, one round
+        // trip each): one pass up to an order of 18176 (float) or 17280 (double). The unrolled poll is
+        // kept this short so that the per-column code stays in the instruction cache.
+        template <typename T> constexpr int kPollLoads = sizeof(T) == 8 ? 17 : 9;
+        // From this many workgroups on, the gather takes two steps: every workgroup polls only the words of
+        // the lines (16 bytes of each), then loads the winner's line and the diagonal line. Every workgroup
+        // reading every whole line made the polls' traffic the bound (a first poll of 4.7 K cycles with 128
+        // workgroups of double rows against 1.2 K with one).
+        constexpr int kTwoStepGroups = 48;
+
+        // Slots of a line for column c of a strip [s0, s1). Each workgroup's line holds the candidate word
+        // (candidate_word; also the arrival flag), the candidate's strip row (entries s0..s1-1 as they
+        // stand: multipliers left of c, then the values; zeros past a short strip or without a candidate)
+        // and the multiplier of the update its entries right of c still lack (0 if none). One more line,
+        // after the workgroups', holds the same for the diagonal row; its word carries only the column.
+        // Values take kSlots<T> tagged slots each, a double's two from a 16-byte boundary.
+        template <typename T> struct StripLine {
+            static constexpr int S = kSlots<T>, B = kStrip<T>;
+            static constexpr int values = S == 2 ? 2 : 1;
+            static constexpr int factor = values + B * S;
+            static constexpr int end = factor + S;
+            static constexpr int pairs = (end + 1) / 2; // 16-byte pairs a reader loads
+            static_assert(pairs <= kLineLanes && end <= kStripLineMax, "strip line");
+        };
+
+        // The largest 64-bit value across the wave, in every lane (the wave's lanes all active): within
+        // each row of 16 lanes by DPP, then the four rows' results read with v_readlane (cross-row
+        // shuffles go through the LDS unit, each a round trip).
+        __device__ __forceinline__ unsigned long long wave_max_rows(unsigned long long x) {
+            auto step = [&](unsigned long long o) { x = o > x ? o : x; };
+            step((unsigned long long)dpp<0xB1>((unsigned)(x >> 32)) << 32 | dpp<0xB1>((unsigned)x));
+            step((unsigned long long)dpp<0x4E>((unsigned)(x >> 32)) << 32 | dpp<0x4E>((unsigned)x));
+            step((unsigned long long)dpp<0x141>((unsigned)(x >> 32)) << 32 | dpp<0x141>((unsigned)x));
+            step((unsigned long long)dpp<0x140>((unsigned)(x >> 32)) << 32 | dpp<0x140>((unsigned)x));
+            unsigned long long best = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const unsigned long long v =
+                        (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(x >> 32), 16 * r) << 32 |
+                        (unsigned)__builtin_amdgcn_readlane((int)x, 16 * r);
+                best = v > best ? v : best;
+            }
+            return best;
+        }
+        // The sum across the wave, in every lane (the wave's lanes all active), the same way.
+        __device__ __forceinline__ int wave_sum_rows(int x) {
+            x += (int)dpp<0xB1>((unsigned)x);
+            x += (int)dpp<0x4E>((unsigned)x);
+            x += (int)dpp<0x141>((unsigned)x);
+            x += (int)dpp<0x140>((unsigned)x);
+            return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) + __builtin_amdgcn_readlane(x, 32) +
+                   __builtin_amdgcn_readlane(x, 48);
+        }
+
+        // A 32-bit OR over each row of 16 lanes (DPP rotations within the row).
+        __device__ __forceinline__ unsigned row_or(unsigned x) {
+            x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x121, 0xF, 0xF, false); // row_ror:1
+            x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x122, 0xF, 0xF, false); // row_ror:2
+            x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x124, 0xF, 0xF, false); // row_ror:4
+            x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false); // row_ror:8
+            return x;
+        }
+
+        // Factors columns [j0, j0 + w) of rows [j0, n), as panel() does. Workgroup b owns rows
+        // [j0 + b R, ...) and keeps their panel part in LDS; thread t < R owns row t of them. Waves 0-1
+        // make every load of another workgroup's publication and waves 2-3 every publication store (a
+        // wave's wait for a load also waits for the stores it issued before).
+        //
+        // Per column c of strip [s0, s1), as in panel(): gather every line, pick the pivot p (the first
+        // maximum; NaN never wins; a NaN diagonal stays), swap the strip entries of rows p and d = j0 + c
+        // (the new ones come in the lines), divide the multipliers and update column c + 1, publish the
+        // candidate for c + 1 (its entries right of c + 1 still lack column c's update, which the
+        // reader applies), then update the strip's other columns.
+        //
+        // At the strip's end every workgroup knows the strip's swaps. The rows that moved (the pivot rows,
+        // now U rows at d_q, and the rows that arrive at the pivots' places) are published by their owners
+        // into the rest buffer with their entries outside the strip (left of it: multipliers of the
+        // panel's earlier strips; right of it: values), one slot per row: slot q < B for the U row of
+        // strip column q, slot B + q for the row arriving at p_q. Every workgroup reads the U rows and
+        // applies the strip's part of L11 to their entries right of the strip (U = L11^-1 rows, in the
+        // reference's order); each reads the rows arriving at its own places; then every row below the
+        // strip takes the strip's eliminations, column by column.
+        //
+        // The lines are polled kLineLanes lanes to a line, each lane loading one 16-byte pair, so that one
+        // load instruction covers whole lines: a lane issuing several loads of one line waits for them one
+        // after another (a round trip of 4.4 us for a 19-slot line polled that way, 1.0 us with one load
+        // per lane, measured on MI355X).
+        //
+        // Buffers: lines[kParities][G + 1][kStripLineMax] (cleared before each launch), rests[kRestSlots]
+        // [2 B][kPanel] values in tagged slots, told apart by the launch and strip in their tags.
+        template <typename T>
+        __global__ __launch_bounds__(kStripThreads) void panel_strips(T *a, int64_t n, int64_t j0, int w, int64_t *pivots,
+                                                                      unsigned long long *lines,
+                                                                      unsigned long long *rests, unsigned launch,
+                                                                      unsigned *error) {
+            using Line = StripLine<T>;
+            constexpr int S = kSlots<T>, B = kStrip<T>, R = kStripRows<T>;
+            constexpr int stride = kPanel + 1;
+            constexpr int row_waves = R / 64;             // waves whose threads own a row
+            constexpr int shares = kStripThreads / R;     // threads per row in the bulk updates
+            constexpr int pollers = 128;                  // waves 0-1
+            const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+            const int lane = t & 63, wave = t >> 6;
+            const int tp = t - pollers; // >= 0: a thread of waves 2-3, which publish
+            const int64_t first = j0 + (int64_t)b * R;
+            const int mine = (int)min((int64_t)R, n - first);
+            __shared__ T rows[R * stride];
+            __shared__ T strip_u[B][B];          // the pivot row of strip column q: its strip entries, current
+            __shared__ T strip_ut[B][B];         // the same transposed (L11 by columns)
+            __shared__ T u_rest[kPanel][B];      // strip end: the U rows' entries outside the strip (column-major)
+            __shared__ T incoming[B][kPanel];    // strip end: rows arriving at this workgroup's places
+            __shared__ int64_t strip_pivot[B];
+            __shared__ int slot_source[2 * B];   // strip end: the row (offset from j0) slot j carries, or -1
+            __shared__ int slot_target[2 * B];   // and the place it goes to
+            __shared__ T multipliers[R];
+            __shared__ unsigned long long wave_word[kWaves];
+            __shared__ int wave_sharing[kWaves];
+            __shared__ T wave_fields[kWaves][B + 1];
+            __shared__ T diagonal_fields[B + 1];
+            __shared__ T tie_fields[B + 1];
+            __shared__ T search_value[kWaves];
+            __shared__ unsigned long long search_rank[kWaves];
+            __shared__ int64_t search_row[kWaves];
+
+            const __amdgpu_buffer_rsrc_t line_buffer = buffer_over(lines);
+            const __amdgpu_buffer_rsrc_t rest_buffer = buffer_over(rests);
+
+            for (int e = t; e < mine * kPanel; e += kStripThreads) {
+                const int i = e / kPanel, k = e % kPanel;
+                if (k < w) rows[i * stride + k] = a[(first + i) * n + j0 + k];
+            }
+            if (t < R) multipliers[t] = T(0);
+            settle();
+
+            auto tag_of = [&](int col) { return launch << 8 | (unsigned)(col + 1); };
+            // slot index of line g (g = G: the diagonal line) for column col
+            auto line_at = [&](int col, int g) {
+                return ((int64_t)(col & (kParities - 1)) * (G + 1) + g) * kStripLineMax;
+            };
+
+            // This workgroup's candidate row for column `col` (the same in every thread; LLONG_MAX if none).
+            auto search = [&](int col) -> int64_t {
+                const int64_t dd = j0 + col;
+                const unsigned long long rank =
+                        t < mine && first + t >= dd ? rank_of(rows[t * stride + col]) : 0ull; // t < mine <= R
+                const unsigned long long top = wave_max_rows(rank);
+                const unsigned long long hits = __ballot(rank == top && top != 0);
+                if (lane == 0) {
+                    search_rank[wave] = top;
+                    search_row[wave] = hits ? first + wave * 64 + (__ffsll((long long)hits) - 1) : LLONG_MAX;
+                }
+                settle();
+                unsigned long long best = search_rank[0];
+                int64_t row = search_row[0];
+#pragma unroll
+                for (int q = 1; q < row_waves; ++q)
+                    if (search_rank[q] > best) {
+                        best = search_rank[q];
+                        row = search_row[q];
+                    }
+                return row;
+            };
+            // Publishes the candidate `row` for column `col` of strip [s0, s1), and the diagonal row if this
+            // workgroup owns it; `pending`: their entries right of col still lack the previous column's
+            // update, whose multipliers are in `multipliers`. Lane u of wave 2 writes slot u's share of this
+            // workgroup's line (u < B: entry s0 + u; B: the factor; 63: the word), wave 3 the diagonal line.
+            // With `update` (column col - 1 eliminated, strip row `uq` of strip_u holding its pivot row),
+            // the lanes that read the rows' entries right of col for the line also give them that column's
+            // elimination, which the other threads then skip for these rows (F below): no barrier is needed
+            // between the reads for the line and the update.
+            auto publish = [&](int col, int s0, int s1, int64_t row, bool pending, bool update = false, int uq = 0) {
+                if (tp < 0) return;
+                const int64_t dd = j0 + col;
+                const bool diagonal = tp >= 64;
+                if (diagonal && !(dd >= first && dd < first + mine)) return;
+                const int u = tp & 63;
+                // When the candidate is the diagonal row, wave 2 writes both lines' values (one wave reads the
+                // row and then updates it) and wave 3 only the diagonal line's word.
+                const bool both = !diagonal && row == dd;
+                if (diagonal && row == dd && u != 63) return;
+                const unsigned tag = tag_of(col);
+                unsigned long long *line = lines + line_at(col, diagonal ? G : b);
+                unsigned long long *diagonal_line = lines + line_at(col, G);
+                const int64_t r = diagonal ? dd : row;
+                if (u == 63) {
+                    unsigned long long word = ((unsigned)col + 1) & 0xFFu;
+                    if (!diagonal) {
+                        const T best = row != LLONG_MAX ? T(__builtin_fabs(rows[(row - first) * stride + col])) : T(-1);
+                        word = candidate_word<T>(best, row, j0, (unsigned)col + 1);
+                    }
+                    __hip_atomic_store(line, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else if (u < B) {
+                    const bool present = r != LLONG_MAX && u < s1 - s0;
+                    const T v = present ? rows[(r - first) * stride + s0 + u] : T(0);
+                    put_tagged(line + Line::values + u * S, v, tag);
+                    if (both) put_tagged(diagonal_line + Line::values + u * S, v, tag);
+                    if (update && present && s0 + u > col)
+                        rows[(r - first) * stride + s0 + u] = eliminated(v, multipliers[r - first], strip_u[uq][u]);
+                } else if (u == B) {
+                    const T f = r != LLONG_MAX && pending ? multipliers[r - first] : T(0);
+                    put_tagged(line + Line::factor, f, tag);
+                    if (both) put_tagged(diagonal_line + Line::factor, f, tag);
+                }
+            };
+
+            publish(0, 0, min(B, w), search(0), false);
+            for (int s0 = 0; s0 < w; s0 += B) {
+                const int s1 = min(s0 + B, w), width = s1 - s0;
+                bool previous = false;           // whether the previous column of this strip eliminated
+                unsigned eliminated_columns = 0; // the strip's columns whose pivot was not zero
+                for (int c = s0; c < s1; ++c) {
+                    const int q = c - s0;
+                    const int64_t d = j0 + c;
+                    const unsigned tag = tag_of(c);
+                    const unsigned epoch = ((unsigned)c + 1) & 0xFFu;
+                    const int owner_d = (int)((d - j0) / R);
+                    const bool more = c + 1 < s1;
+
+                    // A: gather. Below kTwoStepGroups workgroups, lane t of waves 0-1 loads pair t % 16 of lines
+                    // t / 16 + 8 i (the diagonal line, index G, among them) until each carries this column; each
+                    // group of 16 lanes keeps the largest word of its lines with its pair of that line, and the
+                    // waves then take the largest. From kTwoStepGroups on, the lanes poll the words only, and the
+                    // winner's and the diagonal line follow (step 2, after the pivot is known).
+                    unsigned long long best_word = 0;
+                    unsigned best_pair[4] = {0u, 0u, 0u, 0u};
+                    unsigned diagonal_pair[4] = {0u, 0u, 0u, 0u}; // this lane's pair of the diagonal line
+                    int sharing = 0; // lines of the group whose float key equals its best one's
+                    const int pair = t % kLineLanes, group = t / kLineLanes;
+                    constexpr int groups = pollers / kLineLanes;
+                    // A pair's values into an array of B + 1 (the strip entries, then the factor).
+                    auto unpack = [&](const unsigned *x, T *fields) {
+                        if constexpr (S == 2) {
+                            const int k = pair - Line::values / 2; // the value this pair holds
+                            const T v = __builtin_bit_cast(T, (unsigned long long)x[1] << 32 | x[3]);
+                            if (k >= 0 && k < B) fields[k] = v;
+                            else if (2 * pair == Line::factor) fields[B] = v;
+                        } else {
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                const int slot = 2 * pair + h;
+                                const T v = __builtin_bit_cast(T, x[2 * h + 1]);
+                                if (slot >= Line::values && slot < Line::factor) fields[slot - Line::values] = v;
+                                else if (slot == Line::factor) fields[B] = v;
+                            }
+                        }
+                    };
+                    const bool two_step = G >= kTwoStepGroups;
+                    if (two_step) {
+                        // Step 1: lane t loads the first pair (the word) of lines t, t + 128, ...
+                        for (int base = 0; t < pollers && base < G; base += pollers) {
+                            const int g = min(base + t, G - 1);
+                            bool waiting = base + t < G;
+                            unsigned long long x = 0;
+                            unsigned spins = 0;
+                            for (;;) {
+                                const auto v = __builtin_amdgcn_raw_buffer_load_b128(
+                                        line_buffer, (unsigned)(line_at(c, g) * 8), 0, kCoherent);
+                                x = (unsigned long long)v[1] << 32 | v[0];
+                                if ((x & 0xFFu) == epoch) waiting = false;
+                                if (__ballot(waiting) == 0) break;
+                                if (++spins > kSpinLimit) {
+                                    __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                    break;
+                                }
+                                if ((spins & 63u) == 0 && relaxed(error)) break;
+                                __builtin_amdgcn_s_sleep(1);
+                            }
+                            if (base + t >= G) continue;
+                            if (sharing == 0 || (x >> 32) > (best_word >> 32)) {
+                                best_word = x;
+                                sharing = 1;
+                            } else if ((x >> 32) == (best_word >> 32)) {
+                                if ((x >> 8) > (best_word >> 8)) best_word = x;
+                                ++sharing;
+                            }
+                        }
+                    } else {
+                        // Every branch around a load is uniform across the wave (a load under a branch taken by some
+                        // lanes only was compiled to wait for the loads before it): a wave issues `count` loads per
+                        // poll, lanes past the last line or the last pair loading the diagonal line or their line's
+                        // last pair again.
+                        const int wave_group0 = (t / 64) * (64 / kLineLanes); // the wave's first group
+                        for (int pass0 = 0; t < pollers && wave_group0 + pass0 <= G; pass0 += groups * kPollLoads<T>) {
+                            const int count = min(kPollLoads<T>, (G - wave_group0 - pass0) / groups + 1);
+                            unsigned pending = 0;
+    #pragma unroll
+                            for (int i = 0; i < kPollLoads<T>; ++i)
+                                if (i < count && group + pass0 + groups * i <= G) pending |= 1u << i;
+                            const int loaded_pair = min(pair, Line::pairs - 1);
+                            unsigned got[kPollLoads<T>][4];
+                            unsigned spins = 0;
+                            for (;;) {
+    #pragma unroll
+                                for (int i = 0; i < kPollLoads<T>; ++i) {
+                                    if (i >= count) continue;
+                                    const int g = min(group + pass0 + groups * i, G);
+                                    const auto v = __builtin_amdgcn_raw_buffer_load_b128(
+                                            line_buffer, (unsigned)((line_at(c, g) + 2 * loaded_pair) * 8), 0, kCoherent);
+                                    got[i][0] = v[0];
+                                    got[i][1] = v[1];
+                                    got[i][2] = v[2];
+                                    got[i][3] = v[3];
+                                }
+                                unsigned stale = 0; // lines whose pair in this lane is not current yet
+    #pragma unroll
+                                for (int i = 0; i < kPollLoads<T>; ++i) {
+                                    if (i >= count) continue;
+                                    bool ok;
+                                    if (pair == 0) ok = (got[i][0] & 0xFFu) == epoch && (S == 2 || got[i][2] == tag);
+                                    else ok = got[i][0] == tag && (2 * pair + 1 >= Line::end || got[i][2] == tag);
+                                    if (!ok && pair < Line::pairs) stale |= 1u << i;
+                                }
+                                pending &= row_or(stale); // a line is done when every lane of its group is
+                                if (__ballot(pending != 0) == 0) break;
+                                if (++spins > kSpinLimit) {
+                                    __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                    break;
+                                }
+                                if ((spins & 63u) == 0 && relaxed(error)) break;
+                                __builtin_amdgcn_s_sleep(1);
+                            }
+    #pragma unroll
+                            for (int i = 0; i < kPollLoads<T>; ++i) {
+                                if (i >= count) continue;
+                                const int g = group + pass0 + groups * i;
+                                if (g == G) {
+    #pragma unroll
+                                    for (int h = 0; h < 4; ++h) diagonal_pair[h] = got[i][h];
+                                }
+                                // the line's word, from the group's first lane
+                                const unsigned lo = (unsigned)__shfl((int)got[i][0], t & ~(kLineLanes - 1), 64);
+                                const unsigned hi = (unsigned)__shfl((int)got[i][1], t & ~(kLineLanes - 1), 64);
+                                const unsigned long long x = (unsigned long long)hi << 32 | lo;
+                                if (g >= G) continue;
+                                bool take_it = false;
+                                if (sharing == 0 || (x >> 32) > (best_word >> 32)) {
+                                    take_it = true;
+                                    sharing = 1;
+                                } else if ((x >> 32) == (best_word >> 32)) {
+                                    take_it = (x >> 8) > (best_word >> 8);
+                                    ++sharing;
+                                }
+                                if (take_it) {
+                                    best_word = x;
+    #pragma unroll
+                                    for (int h = 0; h < 4; ++h) best_pair[h] = got[i][h];
+                                }
+                            }
+                        }
+                    // the group that polled the diagonal line (line G of every pass is polled by group G % 8)
+                    if (t < pollers && group == G % groups && pair < Line::pairs) unpack(diagonal_pair, diagonal_fields);
+                    }
+                    {
+                        const bool leader = t < pollers && (two_step || pair == 0);
+                        const unsigned long long top = wave_max_rows(sharing ? best_word >> 8 : 0); // key << 24 | ~offset
+                        const bool holder = t < pollers && sharing > 0 && (best_word >> 8) == top && (top & 0xFFFFFFu) != 0;
+                        // lines sharing the float key: only double keys can tie (they are rounded)
+                        const int carrying = sizeof(T) == 8 ? wave_sum_rows(leader && sharing > 0 && (best_word >> 32) == (top >> 24)
+                                                                                    ? sharing : 0)
+                                                            : 0;
+                        if (!two_step && holder && pair < Line::pairs) unpack(best_pair, wave_fields[wave]);
+                        if (lane == 0) {
+                            wave_word[wave] = top;
+                            wave_sharing[wave] = carrying;
+                        }
+                    }
+                    settle();
+                    unsigned long long top = wave_word[0];
+                    int from = 0;
+#pragma unroll
+                    for (int v = 1; v < kWaves; ++v)
+                        if (wave_word[v] > top) {
+                            top = wave_word[v];
+                            from = v;
+                        }
+                    int64_t p = row_of(top << 8, j0);
+                    const T *pf = wave_fields[from];
+                    if constexpr (sizeof(T) == 8) {
+                        int carrying = 0;
+#pragma unroll
+                        for (int v = 0; v < kWaves; ++v)
+                            if ((wave_word[v] >> 24) == (top >> 24)) carrying += wave_sharing[v];
+                        if (carrying > 1) {
+                            // Two or more candidates share the float key: compare the exact values. The lines
+                            // of this column are not rewritten before this workgroup publishes again.
+                            T v = T(-1);
+                            p = LLONG_MAX;
+                            for (int g = t; g < G; g += kStripThreads) {
+                                const unsigned long long *line = lines + line_at(c, g);
+                                const unsigned long long x =
+                                        __hip_atomic_load(line, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                if ((x >> 32) == (top >> 24) && (x & 0xFFu) == epoch && has_candidate(x))
+                                    better(v, p, T(__builtin_fabs(take<T>(line + Line::values + q * S, tag, error))),
+                                           row_of(x, j0));
+                            }
+                            wave_best(v, p);
+                            if (lane == 0) {
+                                search_value[wave] = v;
+                                search_row[wave] = p;
+                            }
+                            settle();
+                            v = search_value[0];
+                            p = search_row[0];
+#pragma unroll
+                            for (int u = 1; u < kWaves; ++u) better(v, p, search_value[u], search_row[u]);
+                            if (!two_step && p >= j0 && p < n) {
+                                const unsigned long long *line = lines + line_at(c, (int)((p - j0) / R));
+                                if (t < width) tie_fields[t] = take<T>(line + Line::values + t * S, tag, error);
+                                else if (t == B) tie_fields[B] = take<T>(line + Line::factor, tag, error);
+                                settle();
+                                pf = tie_fields;
+                            }
+                        }
+                    }
+                    if (two_step) {
+                        // Step 2: the winner's line (lanes 0-15) and the diagonal line (lanes 16-31).
+                        const bool found = p >= j0 && p < n;
+                        if (t < 2 * kLineLanes) {
+                            const bool winner = t < kLineLanes;
+                            const int g = winner && found ? (int)((p - j0) / R) : G;
+                            const int loaded_pair = min(pair, Line::pairs - 1);
+                            bool waiting = pair < Line::pairs;
+                            unsigned got[4] = {0u, 0u, 0u, 0u};
+                            unsigned spins = 0;
+                            for (;;) {
+                                const auto v = __builtin_amdgcn_raw_buffer_load_b128(
+                                        line_buffer, (unsigned)((line_at(c, g) + 2 * loaded_pair) * 8), 0, kCoherent);
+                                got[0] = v[0];
+                                got[1] = v[1];
+                                got[2] = v[2];
+                                got[3] = v[3];
+                                bool ok;
+                                if (pair == 0) ok = (got[0] & 0xFFu) == epoch && (S == 2 || got[2] == tag);
+                                else ok = got[0] == tag && (2 * pair + 1 >= Line::end || got[2] == tag);
+                                if (ok) waiting = false;
+                                if (__ballot(waiting) == 0) break;
+                                if (++spins > kSpinLimit) {
+                                    __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                    break;
+                                }
+                                if ((spins & 63u) == 0 && relaxed(error)) break;
+                                __builtin_amdgcn_s_sleep(1);
+                            }
+                            if (pair < Line::pairs) unpack(got, winner ? wave_fields[0] : diagonal_fields);
+                        }
+                        settle();
+                        pf = wave_fields[0];
+                    }
+                    // A NaN diagonal stays the pivot: the reference starts from the diagonal and takes a row
+                    // only if |a| is larger (decompose.hpp:35-37), which nothing is than NaN.
+                    if (diagonal_fields[q] != diagonal_fields[q]) p = d;
+                    if (p < d || p >= n) p = d; // only after a failed wait: keep every index in range
+                    const bool swapped = p != d;
+                    if (!swapped) pf = diagonal_fields;
+                    // Strip entry k of a published row as it stands now: entries right of c still lack the
+                    // previous column's update (its multiplier is the row's field B).
+                    auto now = [&](const T *f, int k) -> T {
+                        return previous && k > q ? eliminated(f[k], f[B], strip_u[q - 1][k]) : f[k];
+                    };
+                    const T pivot = pf[q];
+                    const bool eliminate = pivot != T(0); // a zero pivot skips the column (decompose.hpp:44-45)
+                    const int owner_p = (int)((p - j0) / R);
+
+                    // B: swap the strip entries of rows p and d; record the pivot row and the step.
+                    if (swapped && b == owner_p && t == (int)(p - first))
+#pragma unroll
+                        for (int k = 0; k < B; ++k)
+                            if (k < width) rows[t * stride + s0 + k] = now(diagonal_fields, k);
+                    if (swapped && b == owner_d && t == (int)(d - first))
+#pragma unroll
+                        for (int k = 0; k < B; ++k)
+                            if (k < width) rows[t * stride + s0 + k] = now(pf, k);
+                    if (t < width) {
+                        const T v = now(pf, t);
+                        strip_u[q][t] = v;
+                        strip_ut[t][q] = v;
+                    }
+                    if (t == 0) {
+                        strip_pivot[q] = p;
+                        if (b == 0) pivots[d] = p;
+                    }
+                    if (eliminate) eliminated_columns |= 1u << q;
+
+                    // C: multipliers and column c + 1 (look-ahead), each thread on its row.
+                    if (eliminate && t < R) {
+                        const T u1 = more ? now(pf, q + 1) : T(0);
+                        T l = T(0);
+                        if (t < mine && first + t > d) {
+                            l = rows[t * stride + c] / pivot;
+                            rows[t * stride + c] = l;
+                            if (more) rows[t * stride + c + 1] = eliminated(rows[t * stride + c + 1], l, u1);
+                        }
+                        multipliers[t] = l;
+                    }
+
+                    // D: the candidate for column c + 1 goes out before the rest of this column's work; the
+                    // lanes that publish a row also give it F (below). The next column's first barrier, after
+                    // its gather, orders F before anything reads the rows again.
+                    int64_t candidate = LLONG_MAX;
+                    if (more) {
+                        candidate = search(c + 1); // its barrier also orders B and C before publication
+                        publish(c + 1, s0, s1, candidate, eliminate, eliminate, q);
+                    } else {
+                        settle();
+                    }
+
+                    // F: the strip's other columns (but in the rows just published); the threads of a row
+                    // split them.
+                    if (eliminate && more) {
+                        const int i = t % R, part = t / R;
+                        if (i < mine && first + i > d && first + i != candidate && first + i != d + 1) {
+                            const T l = multipliers[i];
+                            T *row = rows + i * stride;
+                            const int span = s1 - (c + 2);
+                            const int end = c + 2 + span * (part + 1) / shares;
+                            for (int k = c + 2 + span * part / shares; k < end; ++k)
+                                row[k] = eliminated(row[k], l, strip_u[q][k - s0]);
+                        }
+                    }
+                    previous = eliminate;
+                }
+                // A panel of one strip is done; after the last of several, the rows that moved still take
+                // their entries left of the strip.
+                if (s0 == 0 && s1 == w) break;
+
+                // The strip's end. Slots: which row each carries and where it goes (wave 0).
+                const int nq = width;
+                if (t < 2 * B) {
+                    int64_t sp[B];
+#pragma unroll
+                    for (int r = 0; r < B; ++r) sp[r] = strip_pivot[r]; // entries r >= nq are not used
+                    const int qq = t % B;
+                    const bool u_slot = t < B;
+                    int source = -1, target = -1;
+                    if (qq < nq) {
+                        const int64_t place = u_slot ? j0 + s0 + qq : strip_pivot[qq];
+                        bool wanted = u_slot || place >= j0 + s1;
+#pragma unroll
+                        for (int r = 0; r < B; ++r)
+                            if (!u_slot && r < qq) wanted = wanted && sp[r] != place; // the first of equal places
+                        int64_t x = place; // the row that ends at `place`: the strip's swaps traced backwards
+#pragma unroll
+                        for (int r = B - 1; r >= 0; --r) {
+                            if (r >= nq) continue;
+                            const int64_t dr = j0 + s0 + r;
+                            if (x == dr) x = sp[r];
+                            else if (x == sp[r]) x = dr;
+                        }
+                        if (wanted && (u_slot || x != place)) {
+                            source = (int)(x - j0);
+                            target = (int)(place - j0);
+                        }
+                    }
+                    slot_source[t] = source;
+                    slot_target[t] = target;
+                }
+                settle();
+                const int span = w - s1;
+                const unsigned rest_tag = tag_of(s1 - 1);
+                unsigned long long *rest = rests + (size_t)((s0 / B) % kRestSlots) * 2 * B * kPanel * S;
+                // The rows in the slots move whole: their entries left of the strip (the multipliers of the
+                // panel's earlier strips) and right of it. Entry index e < outside counts those columns.
+                const int outside = w - width;
+                auto column_of = [&](int e) { return e < s0 ? e : e - s0 + s1; };
+                // Owners publish them (waves 2-3).
+                if (tp >= 0 && tp < outside) {
+                    const int k = column_of(tp);
+#pragma unroll
+                    for (int j = 0; j < 2 * B; ++j) {
+                        const int64_t source = j0 + slot_source[j];
+                        if (slot_source[j] >= 0 && source >= first && source < first + mine)
+                            put_tagged(rest + ((size_t)j * kPanel + k) * S, rows[(source - first) * stride + k], rest_tag);
+                    }
+                }
+                // Every workgroup reads the U rows, and the rows arriving at its places (waves 0-1). A 16-byte
+                // load takes two float entries (k even: s0 and s1 are multiples of the strip width) or one
+                // double.
+                if (t < pollers) {
+                    constexpr int per_load = 2 / S;
+                    const int left_loads = s0 / per_load;
+                    const int loads = left_loads + (span + per_load - 1) / per_load;
+                    // lanes past the last entry load the first one again (every load is unconditional per lane)
+                    const int lane_load = t < loads ? t : 0;
+                    const int k = lane_load < left_loads ? lane_load * per_load : s1 + (lane_load - left_loads) * per_load;
+                    const int k_end = lane_load < left_loads ? s0 : w; // entries k + 1 at or past this are not published
+                    const unsigned rest_byte = (unsigned)((rest - rests) * 8);
+                    // Two batches of B slots: the U rows (every workgroup), then the arriving rows (the workgroups
+                    // they arrive at). Which slots a workgroup reads is the same in all its lanes.
+#pragma unroll 1
+                    for (int batch = 0; batch < 2; ++batch) {
+                        unsigned needed = 0;
+#pragma unroll
+                        for (int jj = 0; jj < B; ++jj) {
+                            const int j = batch * B + jj;
+                            const int64_t target = j0 + slot_target[j];
+                            if (slot_source[j] >= 0 && (batch == 0 || (target >= first && target < first + mine)))
+                                needed |= 1u << jj;
+                        }
+                        if (!needed) continue;
+                        unsigned waiting = t < loads ? needed : 0u;
+                        unsigned long long got[B][2];
+                        unsigned spins = 0;
+                        for (;;) {
+#pragma unroll
+                            for (int jj = 0; jj < B; ++jj)
+                                if (needed >> jj & 1u)
+                                    load_pair(rest_buffer,
+                                              rest_byte + (unsigned)(((size_t)(batch * B + jj) * kPanel + k) * S * 8), got[jj]);
+#pragma unroll
+                            for (int jj = 0; jj < B; ++jj) {
+                                if (!(waiting >> jj & 1u)) continue;
+                                bool ok;
+                                if constexpr (S == 2) ok = current<T>(got[jj], rest_tag);
+                                else ok = (unsigned)got[jj][0] == rest_tag && (k + 1 >= k_end || (unsigned)got[jj][1] == rest_tag);
+                                if (!ok) continue;
+                                waiting &= ~(1u << jj);
+                                T *target = batch == 0 ? nullptr : incoming[jj];
+                                if (batch == 0) u_rest[k][jj] = value_of<T>(got[jj]);
+                                else target[k] = value_of<T>(got[jj]);
+                                if constexpr (S == 1) {
+                                    if (k + 1 < k_end) {
+                                        if (batch == 0) u_rest[k + 1][jj] = value_of<T>(got[jj] + 1);
+                                        else target[k + 1] = value_of<T>(got[jj] + 1);
+                                    }
+                                }
+                            }
+                            if (__ballot(waiting != 0) == 0) break;
+                            if (++spins > kSpinLimit) {
+                                __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                break;
+                            }
+                            if ((spins & 63u) == 0 && relaxed(error)) break;
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                    }
+                }
+                settle();
+                // U rows right of the strip: row q of the strip subtracts l[q][r] u[r] for r = 0, 1, ..., q - 1
+                // in that order, skipping columns whose pivot was zero; a thread per column. The rows are
+                // taken together, r by r, so each step's subtractions are independent. (Entries right of the
+                // strip exist only after a strip of full width: nq = B here.)
+                const bool all_eliminated = eliminated_columns == (B == 32 ? ~0u : (1u << B) - 1u);
+                if (t < span) {
+                    const int k = s1 + t;
+                    T u[B];
+#pragma unroll
+                    for (int qq = 0; qq < B; ++qq) u[qq] = u_rest[k][qq];
+                    if (all_eliminated) {
+#pragma unroll
+                        for (int r = 0; r < B - 1; ++r)
+#pragma unroll
+                            for (int qq = r + 1; qq < B; ++qq) u[qq] = eliminated(u[qq], strip_ut[r][qq], u[r]);
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < B - 1; ++r) {
+                            if (!(eliminated_columns >> r & 1u)) continue;
+#pragma unroll
+                            for (int qq = r + 1; qq < B; ++qq) u[qq] = eliminated(u[qq], strip_ut[r][qq], u[r]);
+                        }
+                    }
+#pragma unroll
+                    for (int qq = 0; qq < B; ++qq) u_rest[k][qq] = u[qq];
+                }
+                settle();
+                // Every row from the strip's first place down: a U row takes its entries outside the strip
+                // (left: as they were; right: solved), an arriving row its entries left of the strip, and
+                // every row below the strip the strip's eliminations right of it (an arriving row's entries).
+                {
+                    const int i = t % R, part = t / R;
+                    const int64_t x = first + i;
+                    if (i < mine && x >= j0 + s0) {
+                        T *row = rows + i * stride;
+                        const int end = s1 + span * (part + 1) / shares;
+                        const int begin = s1 + span * part / shares;
+                        const int left_end = s0 * (part + 1) / shares, left_begin = s0 * part / shares;
+                        if (x < j0 + s1) {
+                            const int qq = (int)(x - (j0 + s0));
+                            for (int k = left_begin; k < left_end; ++k) row[k] = u_rest[k][qq];
+                            for (int k = begin; k < end; ++k) row[k] = u_rest[k][qq];
+                        } else {
+                            int arriving = -1;
+#pragma unroll
+                            for (int j = B; j < 2 * B; ++j)
+                                if (slot_source[j] >= 0 && j0 + slot_target[j] == x) arriving = j - B;
+                            if (arriving >= 0)
+                                for (int k = left_begin; k < left_end; ++k) row[k] = incoming[arriving][k];
+                            const T *from = arriving >= 0 ? incoming[arriving] : row;
+                            if (begin < end) { // right of the strip: a full strip (nq = B)
+                                T l[B];
+#pragma unroll
+                                for (int r = 0; r < B; ++r) l[r] = row[s0 + r];
+                                if (all_eliminated) {
+                                    // four columns at a time: four independent chains of subtractions; u_rest's
+                                    // entries read four at a time
+                                    int k = begin;
+                                    for (; k + 4 <= end; k += 4) {
+                                        T v[4];
+#pragma unroll
+                                        for (int j = 0; j < 4; ++j) v[j] = from[k + j];
+#pragma unroll
+                                        for (int r0 = 0; r0 < B; r0 += 4) {
+                                            T u[4][4];
+#pragma unroll
+                                            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                                                for (int h = 0; h < 4; ++h) u[j][h] = u_rest[k + j][r0 + h];
+#pragma unroll
+                                            for (int h = 0; h < 4; ++h)
+#pragma unroll
+                                                for (int j = 0; j < 4; ++j) v[j] = eliminated(v[j], l[r0 + h], u[j][h]);
+                                        }
+#pragma unroll
+                                        for (int j = 0; j < 4; ++j) row[k + j] = v[j];
+                                    }
+                                    for (; k < end; ++k) {
+                                        T v = from[k];
+#pragma unroll
+                                        for (int r = 0; r < B; ++r) v = eliminated(v, l[r], u_rest[k][r]);
+                                        row[k] = v;
+                                    }
+                                } else {
+                                    for (int k = begin; k < end; ++k) {
+                                        T v = from[k];
+#pragma unroll
+                                        for (int r = 0; r < B; ++r)
+                                            if (eliminated_columns >> r & 1u) v = eliminated(v, l[r], u_rest[k][r]);
+                                        row[k] = v;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                settle();
+                if (s1 < w) publish(s1, s1, min(s1 + B, w), search(s1), false);
+            }
+
+            for (int e = t; e < mine * kPanel; e += kStripThreads) {
+                const int i = e / kPanel, k = e % kPanel;
+                if (k < w) a[(first + i) * n + j0 + k] = rows[i * stride + k];
+            }
+        }
+
         // U12 = L11^-1 A12 in place (rows [j0, j0 + kPanel) of columns [j0 + kPanel, n)) by forward
         // substitution with the unit lower L11. Row r subtracts l[r][s] times row s for s = 0, 1, ...,
         // r - 1 in that order, which is the order in which the reference's elimination reaches A12
@@ -1339,14 +2095,16 @@ namespace scihip {
     size_t factor_scratch(int64_t order, int mode) {
         (void)mode;
         // L21 and U12 packed (up to 8 bytes, split pairs), then two sets of the panel's candidate lines
-        // (kParities buffers, one 128-byte line per workgroup).
-        return 4 * (size_t)order * kPanel * 8 + 2 * kParities * ((size_t)order / kPanelRows + 2) * 128 + 1024;
+        // (kParities buffers, one line of up to kStripLineMax slots per workgroup of at least 128 rows).
+        return 4 * (size_t)order * kPanel * 8 + 2 * kParities * ((size_t)order / kPanelRows + 2) * kStripLineMax * 8 +
+               1024;
     }
 
     size_t panel_publish_bytes(int64_t order, size_t element) {
         const size_t groups = (size_t)((order + kPanelRows - 1) / kPanelRows);
         const size_t slots = element / 4; // tagged 64-bit slots per value
-        return kParities * (groups + 1) * kPanel * slots * 8;
+        // panel(): candidate rows and the diagonal row; panel_strips(): the rest buffers after them
+        return kParities * (groups + 1) * kPanel * slots * 8 + (size_t)kRestSlots * 2 * 16 * kPanel * slots * 8;
     }
 
     namespace {
@@ -1422,6 +2180,12 @@ namespace scihip {
         }
         unsigned long long *published = static_cast<unsigned long long *>(ws.publish);
         unsigned long long *diagonal = published + (size_t)kParities * max_groups * kPanel * kSlots<T>;
+        unsigned long long *rests = published + (size_t)kParities * (max_groups + 1) * kPanel * kSlots<T>;
+        // SCIHIP_LU_PANEL=strips: the strip panel kernel (panel_strips) instead of the row-wise one (panel).
+        // Both give the same bits; the row-wise one is faster at n = 16384 (fp32 panels 455 against 709 us,
+        // fp64 555 against 735 us on MI355X), so it stays the default.
+        const char *panel_text = std::getenv("SCIHIP_LU_PANEL");
+        const bool strips = panel_text && std::strcmp(panel_text, "strips") == 0;
 
         StageTimer timer(stream);
         // Panel workgroups the device holds at once, and how many panel workgroups are confined to one
@@ -1433,8 +2197,10 @@ namespace scihip {
             hipError_t e = hipGetDevice(&device);
             if (e == hipSuccess) e = hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device);
             if (e == hipSuccess)
-                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, reinterpret_cast<const void *>(&panel<T>),
-                                                                 kPanelThreads, 0);
+                e = strips ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                     &per_unit, reinterpret_cast<const void *>(&panel_strips<T>), kStripThreads, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                     &per_unit, reinterpret_cast<const void *>(&panel<T>), kPanelThreads, 0);
             if (e != hipSuccess) return (int)e;
             resident = units * per_unit;
             confine = std::min(units / 8 * per_unit, confine_limit);
@@ -1442,7 +2208,7 @@ namespace scihip {
         // Clears candidate line set `set` on stream `s` (each launch expects its lines zero). Panels
         // run on the main stream and alternate between the sets; the set of the panel after next is
         // cleared on the side stream meanwhile.
-        const size_t line_bytes = (size_t)kParities * max_groups * kLine * sizeof(unsigned long long);
+        const size_t line_bytes = (size_t)kParities * (max_groups + 1) * kStripLineMax * sizeof(unsigned long long);
         unsigned long long *line_sets[2] = {lines, lines + line_bytes / sizeof(unsigned long long)};
         auto clear_lines = [&](hipStream_t s, int set) { return hipMemsetAsync(line_sets[set], 0, line_bytes, s); };
         // Factors the panel at column j0 on stream `s` with candidate line set `set` (already clear).
@@ -1459,6 +2225,19 @@ namespace scihip {
                 ws.launches = 1;
             }
             unsigned launch = ws.launches;
+            if (strips) {
+                const int strip_groups = (int)((n - j0 + kStripRows<T> - 1) / kStripRows<T>);
+                if (strip_groups <= resident) {
+                    hipLaunchKernelGGL((panel_strips<T>), dim3(strip_groups), dim3(kStripThreads), 0, s, a, n, j0, w,
+                                       pivots, lines, rests, launch, ws.error);
+                } else {
+                    void *strip_args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &lines, &rests, &launch, &ws.error};
+                    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel_strips<T>), dim3(strip_groups),
+                                                   dim3(kStripThreads), strip_args, 0, s);
+                }
+                if (e == hipSuccess) e = hipGetLastError();
+                return e;
+            }
             int spread = 1;
             void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &lines, &published, &diagonal, &launch,
                             &ws.error, &spread};

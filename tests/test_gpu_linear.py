@@ -296,3 +296,53 @@ def test_metric_size_solves(device, graded):
         assert stats.converged == 1, (mode, stats.backward_error)
         assert backward <= 1e-14 * 1.01, (mode, backward)
         assert forward < (1e-10 if graded else 1e-13), (mode, forward)
+
+
+def panel_case(n, case):
+    """Matrices for the panel comparison: seeded uniform; planted pivot ties across workgroups; a NaN
+    that spreads; zero columns (steps whose pivot is zero skip their elimination); a shuffled
+    diagonally dominant matrix (a swap across workgroups at every step); the same unshuffled (no swaps,
+    the diagonal row its workgroup's candidate at every step)."""
+    a = oracle.uniform(n * n, n + 3, -1.0, 1.0, dtype=np.float64).reshape(n, n)
+    if case == "ties":
+        a *= 0.4
+        a[10, 0], a[150, 0] = 1.0, 1.0 + 2.0 ** -40
+        a[20, 1], a[140, 1] = -2.0, 2.0
+        a[5, 2], a[170, 2] = 0.75, -0.75
+    elif case == "nan":
+        a += 0.5 * np.eye(n)
+        a[200, 3] = np.nan
+    elif case == "zero":
+        a[:, 3] = 0.0
+        a[:, 130] = 0.0
+    elif case == "permuted":
+        a[np.diag_indices(n)] += n / 4
+        a = a[np.random.default_rng(7).permutation(n)]
+    elif case == "shifted":
+        # the diagonal row is its workgroup's candidate at every step (one row published in both lines)
+        a[np.diag_indices(n)] += n / 4
+    return a
+
+
+@pytest.mark.parametrize("n,case", [(7, "plain"), (100, "plain"), (129, "plain"), (200, "ties"), (300, "nan"),
+                                    (300, "zero"), (1000, "plain"), (2048, "plain"), (4096, "permuted"),
+                                    (8192, "plain"), (12800, "permuted"), (4096, "shifted"), (12800, "shifted")])
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_panel_kernels_agree(device, mode, n, case, monkeypatch):
+    """The strip panel kernel (SCIHIP_LU_PANEL=strips) and the row-wise one (the default) give the same
+    pivots and the same factors bit for bit: both apply every elimination to every entry one at a
+    time, in column order, with one rounding each; only where the entries travel between workgroups
+    differs. The larger orders run the strip kernel's two-step gather (48 or more workgroups)."""
+    print(f"[test] panel kernels n={n} {case} {mode}", flush=True)
+    matrix = panel_case(n, case).astype(np.float64 if mode == "fp64" else np.float32)
+    results = {}
+    for kind in ("rows", "strips"):
+        monkeypatch.setenv("SCIHIP_LU_PANEL", kind)
+        results[kind] = gpu_factor(device, matrix, mode)
+    (lu_rows, pivots_rows), (lu_strips, pivots_strips) = results["rows"], results["strips"]
+    assert (pivots_rows == pivots_strips).all(), int(np.flatnonzero(pivots_rows != pivots_strips)[0])
+    bits = np.uint64 if mode == "fp64" else np.uint32
+    same = (lu_rows.view(bits) == lu_strips.view(bits)) | (np.isnan(lu_rows) & np.isnan(lu_strips))
+    assert same.all(), np.argwhere(~same)[:5]
+    if case == "zero":
+        assert lu_strips[4:, 3].tolist() == [0.0] * (n - 4)
