@@ -116,6 +116,35 @@ namespace scihip {
             return x;
         }
 
+        // The largest 64-bit value across the wave, in every lane (the wave's lanes all active): within
+        // each row of 16 lanes by DPP, then the four rows' results read with v_readlane (cross-row
+        // shuffles go through the LDS unit, each a round trip).
+        __device__ __forceinline__ unsigned long long wave_max_rows(unsigned long long x) {
+            auto step = [&](unsigned long long o) { x = o > x ? o : x; };
+            step((unsigned long long)dpp<0xB1>((unsigned)(x >> 32)) << 32 | dpp<0xB1>((unsigned)x));
+            step((unsigned long long)dpp<0x4E>((unsigned)(x >> 32)) << 32 | dpp<0x4E>((unsigned)x));
+            step((unsigned long long)dpp<0x141>((unsigned)(x >> 32)) << 32 | dpp<0x141>((unsigned)x));
+            step((unsigned long long)dpp<0x140>((unsigned)(x >> 32)) << 32 | dpp<0x140>((unsigned)x));
+            unsigned long long best = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const unsigned long long v =
+                        (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(x >> 32), 16 * r) << 32 |
+                        (unsigned)__builtin_amdgcn_readlane((int)x, 16 * r);
+                best = v > best ? v : best;
+            }
+            return best;
+        }
+        // The sum across the wave, in every lane (the wave's lanes all active), the same way.
+        __device__ __forceinline__ int wave_sum_rows(int x) {
+            x += (int)dpp<0xB1>((unsigned)x);
+            x += (int)dpp<0x4E>((unsigned)x);
+            x += (int)dpp<0x141>((unsigned)x);
+            x += (int)dpp<0x140>((unsigned)x);
+            return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) + __builtin_amdgcn_readlane(x, 32) +
+                   __builtin_amdgcn_readlane(x, 48);
+        }
+
         // A workgroup's candidate for one column as one 64-bit word, which is also its arrival flag:
         // bits 63-32 the float bits of |a| (monotonic in |a| for a non-negative value; exact for float
         // matrices, rounded for double ones), bits 31-8 the complement of the row's offset from the
@@ -321,7 +350,7 @@ namespace scihip {
                 const int64_t dd = j0 + col;
                 const unsigned long long rank =
                         t < mine && first + t >= dd ? rank_of(rows[t * stride + col]) : 0ull;
-                const unsigned long long top = wave_max(rank);
+                const unsigned long long top = wave_max_rows(rank);
                 const unsigned long long hits = __ballot(rank == top && top != 0);
                 if (lane == 0) {
                     search_rank[wave] = top;
@@ -454,9 +483,12 @@ namespace scihip {
                 // The workgroup's largest word: each wave's, with its fields and how many lines of the
                 // wave carry its key, then the largest of the waves'.
                 {
-                    const unsigned long long top = wave_max(sharing ? mine_best >> 8 : 0); // key << 24 | ~offset
+                    const unsigned long long top = wave_max_rows(sharing ? mine_best >> 8 : 0); // key << 24 | ~offset
                     const bool holder = sharing > 0 && (mine_best >> 8) == top;
-                    const int carrying = wave_sum(sharing > 0 && (mine_best >> 32) == (top >> 24) ? sharing : 0);
+                    // lines sharing the float key: only double keys can tie (they are rounded)
+                    const int carrying = sizeof(T) == 8 ? wave_sum_rows(sharing > 0 && (mine_best >> 32) == (top >> 24)
+                                                                                ? sharing : 0)
+                                                        : 0;
                     if (holder) {
                         wave_fields[wave][0] = value;
                         wave_fields[wave][1] = next;
@@ -659,7 +691,7 @@ namespace scihip {
         // takes the strip's updates. Every entry still gets the reference's eliminations one at a
         // time, in column order, each as eliminated(), so the result equals panel()'s bit for bit.
         template <typename T> constexpr int kStrip = sizeof(T) == 8 ? 8 : 16;       // columns per strip
-        template <typename T> constexpr int kStripRows = sizeof(T) == 8 ? 128 : 256; // rows per workgroup
+        template <typename T> constexpr int kStripRows = 128; // rows per workgroup (a thread of waves 2-3 each)
         constexpr int kStripThreads = 256;
         constexpr int kRestSlots = 2;     // rest buffers, used in turn by consecutive strips
         constexpr int kStripLineMax = 32; // 64-bit slots per line (256 bytes)
@@ -689,35 +721,6 @@ namespace scihip {
             static_assert(pairs <= kLineLanes && end <= kStripLineMax, "strip line");
         };
 
-        // The largest 64-bit value across the wave, in every lane (the wave's lanes all active): within
-        // each row of 16 lanes by DPP, then the four rows' results read with v_readlane (cross-row
-        // shuffles go through the LDS unit, each a round trip).
-        __device__ __forceinline__ unsigned long long wave_max_rows(unsigned long long x) {
-            auto step = [&](unsigned long long o) { x = o > x ? o : x; };
-            step((unsigned long long)dpp<0xB1>((unsigned)(x >> 32)) << 32 | dpp<0xB1>((unsigned)x));
-            step((unsigned long long)dpp<0x4E>((unsigned)(x >> 32)) << 32 | dpp<0x4E>((unsigned)x));
-            step((unsigned long long)dpp<0x141>((unsigned)(x >> 32)) << 32 | dpp<0x141>((unsigned)x));
-            step((unsigned long long)dpp<0x140>((unsigned)(x >> 32)) << 32 | dpp<0x140>((unsigned)x));
-            unsigned long long best = 0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const unsigned long long v =
-                        (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(x >> 32), 16 * r) << 32 |
-                        (unsigned)__builtin_amdgcn_readlane((int)x, 16 * r);
-                best = v > best ? v : best;
-            }
-            return best;
-        }
-        // The sum across the wave, in every lane (the wave's lanes all active), the same way.
-        __device__ __forceinline__ int wave_sum_rows(int x) {
-            x += (int)dpp<0xB1>((unsigned)x);
-            x += (int)dpp<0x4E>((unsigned)x);
-            x += (int)dpp<0x141>((unsigned)x);
-            x += (int)dpp<0x140>((unsigned)x);
-            return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) + __builtin_amdgcn_readlane(x, 32) +
-                   __builtin_amdgcn_readlane(x, 48);
-        }
-
         // A 32-bit OR over each row of 16 lanes (DPP rotations within the row).
         __device__ __forceinline__ unsigned row_or(unsigned x) {
             x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x121, 0xF, 0xF, false); // row_ror:1
@@ -728,15 +731,18 @@ namespace scihip {
         }
 
         // Factors columns [j0, j0 + w) of rows [j0, n), as panel() does. Workgroup b owns rows
-        // [j0 + b R, ...) and keeps their panel part in LDS; thread t < R owns row t of them. Waves 0-1
-        // make every load of another workgroup's publication and waves 2-3 every publication store (a
-        // wave's wait for a load also waits for the stores it issued before).
+        // [j0 + b R, ...) and keeps their panel part in LDS. Waves 0-1 make every load of another
+        // workgroup's publication; thread 128 + r of waves 2-3 owns row r and holds the row's entries in the
+        // current strip in registers (a wave's wait for a load also waits for the stores it issued before,
+        // so the waves that poll never store).
         //
         // Per column c of strip [s0, s1), as in panel(): gather every line, pick the pivot p (the first
         // maximum; NaN never wins; a NaN diagonal stays), swap the strip entries of rows p and d = j0 + c
         // (the new ones come in the lines), divide the multipliers and update column c + 1, publish the
-        // candidate for c + 1 (its entries right of c + 1 still lack column c's update, which the
-        // reader applies), then update the strip's other columns.
+        // candidate for c + 1 (its entries right of c + 1 still lack column c's update, which the reader
+        // applies), then update the strip's other columns. Each row's thread does all of this on its own
+        // registers, publishing its row before updating it, so no barrier separates the publication from
+        // the update.
         //
         // At the strip's end every workgroup knows the strip's swaps. The rows that moved (the pivot rows,
         // now U rows at d_q, and the rows that arrive at the pivots' places) are published by their owners
@@ -750,7 +756,7 @@ namespace scihip {
         // The lines are polled kLineLanes lanes to a line, each lane loading one 16-byte pair, so that one
         // load instruction covers whole lines: a lane issuing several loads of one line waits for them one
         // after another (a round trip of 4.4 us for a 19-slot line polled that way, 1.0 us with one load
-        // per lane, measured on MI355X).
+        // per lane, measured on MI355X). For the same reason a line is stored one pair per lane.
         //
         // Buffers: lines[kParities][G + 1][kStripLineMax] (cleared before each launch), rests[kRestSlots]
         // [2 B][kPanel] values in tagged slots, told apart by the launch and strip in their tags.
@@ -762,14 +768,16 @@ namespace scihip {
             using Line = StripLine<T>;
             constexpr int S = kSlots<T>, B = kStrip<T>, R = kStripRows<T>;
             constexpr int stride = kPanel + 1;
-            constexpr int row_waves = R / 64;             // waves whose threads own a row
-            constexpr int shares = kStripThreads / R;     // threads per row in the bulk updates
-            constexpr int pollers = 128;                  // waves 0-1
+            constexpr int pollers = 128;              // waves 0-1
+            constexpr int shares = kStripThreads / R; // threads per row in the strip-end update
+            static_assert(R == kStripThreads - pollers, "a row thread per row");
             const int G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
             const int lane = t & 63, wave = t >> 6;
-            const int tp = t - pollers; // >= 0: a thread of waves 2-3, which publish
+            const int tp = t - pollers; // >= 0: a thread of waves 2-3, which own the rows
             const int64_t first = j0 + (int64_t)b * R;
             const int mine = (int)min((int64_t)R, n - first);
+            const bool owner = tp >= 0 && tp < mine; // this thread holds row first + tp
+            const int64_t my_row = first + tp;
             __shared__ T rows[R * stride];
             __shared__ T strip_u[B][B];          // the pivot row of strip column q: its strip entries, current
             __shared__ T strip_ut[B][B];         // the same transposed (L11 by columns)
@@ -778,7 +786,6 @@ namespace scihip {
             __shared__ int64_t strip_pivot[B];
             __shared__ int slot_source[2 * B];   // strip end: the row (offset from j0) slot j carries, or -1
             __shared__ int slot_target[2 * B];   // and the place it goes to
-            __shared__ T multipliers[R];
             __shared__ unsigned long long wave_word[kWaves];
             __shared__ int wave_sharing[kWaves];
             __shared__ T wave_fields[kWaves][B + 1];
@@ -787,15 +794,15 @@ namespace scihip {
             __shared__ T search_value[kWaves];
             __shared__ unsigned long long search_rank[kWaves];
             __shared__ int64_t search_row[kWaves];
+            __shared__ unsigned long long staged[2][2 * kLineLanes]; // a line's slots on their way out (per wave)
 
             const __amdgpu_buffer_rsrc_t line_buffer = buffer_over(lines);
             const __amdgpu_buffer_rsrc_t rest_buffer = buffer_over(rests);
 
             for (int e = t; e < mine * kPanel; e += kStripThreads) {
-                const int i = e / kPanel, k = e % kPanel;
-                if (k < w) rows[i * stride + k] = a[(first + i) * n + j0 + k];
+                const int r = e / kPanel, k = e % kPanel;
+                if (k < w) rows[r * stride + k] = a[(first + r) * n + j0 + k];
             }
-            if (t < R) multipliers[t] = T(0);
             settle();
 
             auto tag_of = [&](int col) { return launch << 8 | (unsigned)(col + 1); };
@@ -804,72 +811,175 @@ namespace scihip {
                 return ((int64_t)(col & (kParities - 1)) * (G + 1) + g) * kStripLineMax;
             };
 
-            // This workgroup's candidate row for column `col` (the same in every thread; LLONG_MAX if none).
-            auto search = [&](int col) -> int64_t {
+            T sv[B];    // this thread's row, entries s0 .. s0 + B - 1 of the current strip
+            T l = T(0); // and its multiplier of the current column
+            auto load_strip = [&](int s0, int width) {
+#pragma unroll
+                for (int k = 0; k < B; ++k) sv[k] = owner && k < width ? rows[tp * stride + s0 + k] : T(0);
+            };
+            auto store_strip = [&](int s0, int width) {
+#pragma unroll
+                for (int k = 0; k < B; ++k)
+                    if (owner && k < width) rows[tp * stride + s0 + k] = sv[k];
+            };
+
+            // This workgroup's candidate row for column `col`, strip entry `k` (the same in every thread;
+            // LLONG_MAX if none): the largest rank, on equal ranks the lowest row.
+            auto search = [&](int col, int k) -> int64_t {
                 const int64_t dd = j0 + col;
-                const unsigned long long rank =
-                        t < mine && first + t >= dd ? rank_of(rows[t * stride + col]) : 0ull; // t < mine <= R
+                const unsigned long long rank = owner && my_row >= dd ? rank_of(sv[k]) : 0ull;
                 const unsigned long long top = wave_max_rows(rank);
                 const unsigned long long hits = __ballot(rank == top && top != 0);
                 if (lane == 0) {
                     search_rank[wave] = top;
-                    search_row[wave] = hits ? first + wave * 64 + (__ffsll((long long)hits) - 1) : LLONG_MAX;
+                    search_row[wave] = hits ? first + (wave - 2) * 64 + (__ffsll((long long)hits) - 1) : LLONG_MAX;
                 }
                 settle();
-                unsigned long long best = search_rank[0];
-                int64_t row = search_row[0];
-#pragma unroll
-                for (int q = 1; q < row_waves; ++q)
-                    if (search_rank[q] > best) {
-                        best = search_rank[q];
-                        row = search_row[q];
-                    }
+                unsigned long long best = search_rank[2];
+                int64_t row = search_row[2];
+                if (search_rank[3] > best) row = search_row[3];
                 return row;
             };
-            // Publishes the candidate `row` for column `col` of strip [s0, s1), and the diagonal row if this
-            // workgroup owns it; `pending`: their entries right of col still lack the previous column's
-            // update, whose multipliers are in `multipliers`. Lane u of wave 2 writes slot u's share of this
-            // workgroup's line (u < B: entry s0 + u; B: the factor; 63: the word), wave 3 the diagonal line.
-            // With `update` (column col - 1 eliminated, strip row `uq` of strip_u holding its pivot row),
-            // the lanes that read the rows' entries right of col for the line also give them that column's
-            // elimination, which the other threads then skip for these rows (F below): no barrier is needed
-            // between the reads for the line and the update.
-            auto publish = [&](int col, int s0, int s1, int64_t row, bool pending, bool update = false, int uq = 0) {
+            // Publishes the line of column `col` (strip [s0, s0 + width)): this workgroup's candidate `row` and
+            // the diagonal row if this workgroup owns it (into line G), each from the registers of the thread
+            // that holds it (without a candidate, row thread 0 writes an empty line). `pending`: the rows'
+            // entries right of col still lack the previous column's update, with multiplier `l`. The thread
+            // stages the line's slots in LDS and the lanes of its wave store one 16-byte pair each, the whole
+            // 256-byte line. (Reading the holder's registers with v_readlane instead of staging measured
+            // slower: bf16 solve at n = 16384 108.7 against 103.1 ms.)
+            auto publish = [&](int col, int s0, int width, int64_t row, bool pending) {
                 if (tp < 0) return;
                 const int64_t dd = j0 + col;
-                const bool diagonal = tp >= 64;
-                if (diagonal && !(dd >= first && dd < first + mine)) return;
-                const int u = tp & 63;
-                // When the candidate is the diagonal row, wave 2 writes both lines' values (one wave reads the
-                // row and then updates it) and wave 3 only the diagonal line's word.
-                const bool both = !diagonal && row == dd;
-                if (diagonal && row == dd && u != 63) return;
                 const unsigned tag = tag_of(col);
-                unsigned long long *line = lines + line_at(col, diagonal ? G : b);
-                unsigned long long *diagonal_line = lines + line_at(col, G);
-                const int64_t r = diagonal ? dd : row;
-                if (u == 63) {
-                    unsigned long long word = ((unsigned)col + 1) & 0xFFu;
-                    if (!diagonal) {
-                        const T best = row != LLONG_MAX ? T(__builtin_fabs(rows[(row - first) * stride + col])) : T(-1);
-                        word = candidate_word<T>(best, row, j0, (unsigned)col + 1);
+                const int which_wave = wave - 2;
+                const int line_wave = row == LLONG_MAX ? 0 : (int)((row - first) / 64);
+                const bool diagonal_here = dd >= first && dd < first + mine;
+                const int diagonal_wave = diagonal_here ? (int)((dd - first) / 64) : -1;
+                auto stage = [&](bool empty, bool diagonal) {
+                    unsigned long long slots[2 * kLineLanes]; // the whole line, padding zero
+#pragma unroll
+                    for (int e = 0; e < 2 * kLineLanes; ++e) slots[e] = 0;
+                    const int q = col - s0;
+                    T current = T(0);
+#pragma unroll
+                    for (int k = 0; k < B; ++k) {
+                        const T v = !empty && k < width ? sv[k] : T(0);
+                        if (k == q) current = v;
+                        if constexpr (S == 2) {
+                            const unsigned long long bits = __builtin_bit_cast(unsigned long long, v);
+                            slots[Line::values + 2 * k] = (bits & 0xFFFFFFFF00000000ull) | tag;
+                            slots[Line::values + 2 * k + 1] = bits << 32 | tag;
+                        } else {
+                            slots[Line::values + k] = (unsigned long long)__builtin_bit_cast(unsigned, v) << 32 | tag;
+                        }
                     }
-                    __hip_atomic_store(line, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                } else if (u < B) {
-                    const bool present = r != LLONG_MAX && u < s1 - s0;
-                    const T v = present ? rows[(r - first) * stride + s0 + u] : T(0);
-                    put_tagged(line + Line::values + u * S, v, tag);
-                    if (both) put_tagged(diagonal_line + Line::values + u * S, v, tag);
-                    if (update && present && s0 + u > col)
-                        rows[(r - first) * stride + s0 + u] = eliminated(v, multipliers[r - first], strip_u[uq][u]);
-                } else if (u == B) {
-                    const T f = r != LLONG_MAX && pending ? multipliers[r - first] : T(0);
-                    put_tagged(line + Line::factor, f, tag);
-                    if (both) put_tagged(diagonal_line + Line::factor, f, tag);
+                    const T f = !empty && pending ? l : T(0);
+                    if constexpr (S == 2) {
+                        const unsigned long long bits = __builtin_bit_cast(unsigned long long, f);
+                        slots[Line::factor] = (bits & 0xFFFFFFFF00000000ull) | tag;
+                        slots[Line::factor + 1] = bits << 32 | tag;
+                    } else {
+                        slots[Line::factor] = (unsigned long long)__builtin_bit_cast(unsigned, f) << 32 | tag;
+                    }
+                    slots[0] = diagonal ? (unsigned long long)(((unsigned)col + 1) & 0xFFu)
+                                        : candidate_word<T>(empty ? T(-1) : T(__builtin_fabs(current)),
+                                                            empty ? LLONG_MAX : my_row, j0, (unsigned)col + 1);
+#pragma unroll
+                    for (int e = 0; e < 2 * kLineLanes; ++e) staged[which_wave][e] = slots[e];
+                };
+                // one wave stages and stores a line at a time (both lines may come from one wave)
+#pragma unroll
+                for (int kind = 0; kind < 2; ++kind) {
+                    const bool diagonal = kind == 1;
+                    if (diagonal ? diagonal_wave != which_wave : line_wave != which_wave) continue;
+                    const bool writer = diagonal ? owner && my_row == dd
+                                                 : (row == LLONG_MAX ? tp == 0 : owner && my_row == row);
+                    if (writer) stage(!diagonal && row == LLONG_MAX, diagonal);
+                    // the staging thread's LDS writes before the wave's reads (one wave: in order)
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+                    if (lane < kLineLanes) {
+                        const unsigned long long x0 = staged[which_wave][2 * lane], x1 = staged[which_wave][2 * lane + 1];
+                        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+                        const v4u v = {(unsigned)x0, (unsigned)(x0 >> 32), (unsigned)x1, (unsigned)(x1 >> 32)};
+                        __builtin_amdgcn_raw_buffer_store_b128(v, line_buffer,
+                                                               (unsigned)((line_at(col, diagonal ? G : b) + 2 * lane) * 8), 0,
+                                                               kCoherent);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
                 }
             };
 
-            publish(0, 0, min(B, w), search(0), false);
+            // Which rows arrive at place x in the last strip's swaps (the slot, or -1).
+            auto arriving_at = [&](int64_t x) -> int {
+                int arriving = -1;
+#pragma unroll
+                for (int j = B; j < 2 * B; ++j)
+                    if (slot_source[j] >= 0 && j0 + slot_target[j] == x) arriving = j - B;
+                return arriving;
+            };
+            // Strip [ps0, ps1)'s end, right of the strip, columns [k0, k1), in row r of this workgroup: a U row
+            // takes its solved entries, a row below the strip (an arriving row's entries) the strip's
+            // eliminations (mask: the strip's columns whose pivot was not zero; the strip is full, B columns).
+            auto update_right = [&](int r, int ps0, int ps1, unsigned mask, int k0, int k1) {
+                if (k0 >= k1) return;
+                const int64_t x = first + r;
+                T *row = rows + r * stride;
+                if (x < j0 + ps1) {
+                    const int qq = (int)(x - (j0 + ps0));
+                    for (int k = k0; k < k1; ++k) row[k] = u_rest[k][qq];
+                    return;
+                }
+                const int arriving = arriving_at(x);
+                const T *from = arriving >= 0 ? incoming[arriving] : row;
+                T lr[B];
+#pragma unroll
+                for (int h = 0; h < B; ++h) lr[h] = row[ps0 + h];
+                if (mask == (B == 32 ? ~0u : (1u << B) - 1u)) {
+                    // four columns at a time: four independent chains of subtractions; u_rest's entries read
+                    // four at a time
+                    int k = k0;
+                    for (; k + 4 <= k1; k += 4) {
+                        T v[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) v[j] = from[k + j];
+#pragma unroll
+                        for (int r0 = 0; r0 < B; r0 += 4) {
+                            T u[4][4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                                for (int h = 0; h < 4; ++h) u[j][h] = u_rest[k + j][r0 + h];
+#pragma unroll
+                            for (int h = 0; h < 4; ++h)
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) v[j] = eliminated(v[j], lr[r0 + h], u[j][h]);
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) row[k + j] = v[j];
+                    }
+                    for (; k < k1; ++k) {
+                        T v = from[k];
+#pragma unroll
+                        for (int h = 0; h < B; ++h) v = eliminated(v, lr[h], u_rest[k][h]);
+                        row[k] = v;
+                    }
+                } else {
+                    for (int k = k0; k < k1; ++k) {
+                        T v = from[k];
+#pragma unroll
+                        for (int h = 0; h < B; ++h)
+                            if (mask >> h & 1u) v = eliminated(v, lr[h], u_rest[k][h]);
+                        row[k] = v;
+                    }
+                }
+            };
+
+            load_strip(0, min(B, w));
+            publish(0, 0, min(B, w), search(0, 0), false);
             for (int s0 = 0; s0 < w; s0 += B) {
                 const int s1 = min(s0 + B, w), width = s1 - s0;
                 bool previous = false;           // whether the previous column of this strip eliminated
@@ -917,19 +1027,39 @@ namespace scihip {
                             const int g = min(base + t, G - 1);
                             bool waiting = base + t < G;
                             unsigned long long x = 0;
+                            // Three polls in flight, issued one after another: the wait ends about a third
+                            // of a round trip after the word lands instead of up to a whole one.
+                            const unsigned byte = (unsigned)(line_at(c, g) * 8);
+                            auto issue = [&]() {
+                                return __builtin_amdgcn_raw_buffer_load_b128(line_buffer, byte, 0, kCoherent);
+                            };
+                            auto done = [&](decltype(issue()) v) { // true once every lane of the wave has its word
+                                const unsigned long long y = (unsigned long long)v[1] << 32 | v[0];
+                                if (waiting && (y & 0xFFu) == epoch) {
+                                    waiting = false;
+                                    x = y;
+                                }
+                                return __ballot(waiting) == 0;
+                            };
+                            auto v0 = issue();
+                            __builtin_amdgcn_s_sleep(2);
+                            auto v1 = issue();
+                            __builtin_amdgcn_s_sleep(2);
+                            auto v2 = issue();
                             unsigned spins = 0;
                             for (;;) {
-                                const auto v = __builtin_amdgcn_raw_buffer_load_b128(
-                                        line_buffer, (unsigned)(line_at(c, g) * 8), 0, kCoherent);
-                                x = (unsigned long long)v[1] << 32 | v[0];
-                                if ((x & 0xFFu) == epoch) waiting = false;
-                                if (__ballot(waiting) == 0) break;
-                                if (++spins > kSpinLimit) {
+                                if (done(v0)) break;
+                                v0 = issue();
+                                if (done(v1)) break;
+                                v1 = issue();
+                                if (done(v2)) break;
+                                v2 = issue();
+                                spins += 3;
+                                if (spins > kSpinLimit) {
                                     __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                                     break;
                                 }
-                                if ((spins & 63u) == 0 && relaxed(error)) break;
-                                __builtin_amdgcn_s_sleep(1);
+                                if ((spins & 63u) < 3 && relaxed(error)) break;
                             }
                             if (base + t >= G) continue;
                             if (sharing == 0 || (x >> 32) > (best_word >> 32)) {
@@ -955,6 +1085,46 @@ namespace scihip {
                             const int loaded_pair = min(pair, Line::pairs - 1);
                             unsigned got[kPollLoads<T>][4];
                             unsigned spins = 0;
+                            if (count == 1) {
+                                // one line per lane: three polls in flight, as in the two-step gather
+                                const int g = min(group + pass0, G);
+                                const unsigned byte = (unsigned)((line_at(c, g) + 2 * loaded_pair) * 8);
+                                auto issue = [&]() {
+                                    return __builtin_amdgcn_raw_buffer_load_b128(line_buffer, byte, 0, kCoherent);
+                                };
+                                auto done = [&](decltype(issue()) v) {
+                                    bool ok;
+                                    if (pair == 0) ok = (v[0] & 0xFFu) == epoch && (S == 2 || v[2] == tag);
+                                    else ok = v[0] == tag && (2 * pair + 1 >= Line::end || v[2] == tag);
+                                    if (pending & 1u) {
+                                        got[0][0] = v[0];
+                                        got[0][1] = v[1];
+                                        got[0][2] = v[2];
+                                        got[0][3] = v[3];
+                                    }
+                                    pending &= row_or(!ok && pair < Line::pairs ? 1u : 0u);
+                                    return __ballot(pending != 0) == 0;
+                                };
+                                auto v0 = issue();
+                                __builtin_amdgcn_s_sleep(2);
+                                auto v1 = issue();
+                                __builtin_amdgcn_s_sleep(2);
+                                auto v2 = issue();
+                                for (;;) {
+                                    if (done(v0)) break;
+                                    v0 = issue();
+                                    if (done(v1)) break;
+                                    v1 = issue();
+                                    if (done(v2)) break;
+                                    v2 = issue();
+                                    spins += 3;
+                                    if (spins > kSpinLimit) {
+                                        __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                        break;
+                                    }
+                                    if ((spins & 63u) < 3 && relaxed(error)) break;
+                                }
+                            } else
                             for (;;) {
     #pragma unroll
                                 for (int i = 0; i < kPollLoads<T>; ++i) {
@@ -1128,14 +1298,14 @@ namespace scihip {
                     const int owner_p = (int)((p - j0) / R);
 
                     // B: swap the strip entries of rows p and d; record the pivot row and the step.
-                    if (swapped && b == owner_p && t == (int)(p - first))
+                    if (swapped && b == owner_p && my_row == p)
 #pragma unroll
                         for (int k = 0; k < B; ++k)
-                            if (k < width) rows[t * stride + s0 + k] = now(diagonal_fields, k);
-                    if (swapped && b == owner_d && t == (int)(d - first))
+                            if (k < width) sv[k] = now(diagonal_fields, k);
+                    if (swapped && b == owner_d && my_row == d)
 #pragma unroll
                         for (int k = 0; k < B; ++k)
-                            if (k < width) rows[t * stride + s0 + k] = now(pf, k);
+                            if (k < width) sv[k] = now(pf, k);
                     if (t < width) {
                         const T v = now(pf, t);
                         strip_u[q][t] = v;
@@ -1147,44 +1317,31 @@ namespace scihip {
                     }
                     if (eliminate) eliminated_columns |= 1u << q;
 
-                    // C: multipliers and column c + 1 (look-ahead), each thread on its row.
-                    if (eliminate && t < R) {
+                    // C: multipliers and column c + 1 (look-ahead), each row's thread on its row.
+                    const bool below = owner && my_row > d;
+                    l = T(0);
+                    if (eliminate && below) {
                         const T u1 = more ? now(pf, q + 1) : T(0);
-                        T l = T(0);
-                        if (t < mine && first + t > d) {
-                            l = rows[t * stride + c] / pivot;
-                            rows[t * stride + c] = l;
-                            if (more) rows[t * stride + c + 1] = eliminated(rows[t * stride + c + 1], l, u1);
-                        }
-                        multipliers[t] = l;
+                        l = sv[q] / pivot;
+                        sv[q] = l;
+                        if (more) sv[q + 1] = eliminated(sv[q + 1], l, u1);
                     }
 
-                    // D: the candidate for column c + 1 goes out before the rest of this column's work; the
-                    // lanes that publish a row also give it F (below). The next column's first barrier, after
-                    // its gather, orders F before anything reads the rows again.
-                    int64_t candidate = LLONG_MAX;
+                    // D: the candidate for column c + 1 goes out, then F: the strip's other columns. The next
+                    // column's first barrier, after its gather, orders F before anything reads the rows again.
                     if (more) {
-                        candidate = search(c + 1); // its barrier also orders B and C before publication
-                        publish(c + 1, s0, s1, candidate, eliminate, eliminate, q);
-                    } else {
-                        settle();
-                    }
-
-                    // F: the strip's other columns (but in the rows just published); the threads of a row
-                    // split them.
-                    if (eliminate && more) {
-                        const int i = t % R, part = t / R;
-                        if (i < mine && first + i > d && first + i != candidate && first + i != d + 1) {
-                            const T l = multipliers[i];
-                            T *row = rows + i * stride;
-                            const int span = s1 - (c + 2);
-                            const int end = c + 2 + span * (part + 1) / shares;
-                            for (int k = c + 2 + span * part / shares; k < end; ++k)
-                                row[k] = eliminated(row[k], l, strip_u[q][k - s0]);
+                        const int64_t candidate = search(c + 1, q + 1); // its barrier also orders B
+                        publish(c + 1, s0, width, candidate, eliminate);
+                        if (eliminate && below) {
+#pragma unroll
+                            for (int k = 0; k < B; ++k)
+                                if (k >= q + 2 && k < width) sv[k] = eliminated(sv[k], l, strip_u[q][k]);
                         }
                     }
                     previous = eliminate;
                 }
+                store_strip(s0, width);
+                settle();
                 // A panel of one strip is done; after the last of several, the rows that moved still take
                 // their entries left of the strip.
                 if (s0 == 0 && s1 == w) break;
@@ -1342,68 +1499,24 @@ namespace scihip {
                         if (x < j0 + s1) {
                             const int qq = (int)(x - (j0 + s0));
                             for (int k = left_begin; k < left_end; ++k) row[k] = u_rest[k][qq];
-                            for (int k = begin; k < end; ++k) row[k] = u_rest[k][qq];
                         } else {
-                            int arriving = -1;
-#pragma unroll
-                            for (int j = B; j < 2 * B; ++j)
-                                if (slot_source[j] >= 0 && j0 + slot_target[j] == x) arriving = j - B;
+                            const int arriving = arriving_at(x);
                             if (arriving >= 0)
                                 for (int k = left_begin; k < left_end; ++k) row[k] = incoming[arriving][k];
-                            const T *from = arriving >= 0 ? incoming[arriving] : row;
-                            if (begin < end) { // right of the strip: a full strip (nq = B)
-                                T l[B];
-#pragma unroll
-                                for (int r = 0; r < B; ++r) l[r] = row[s0 + r];
-                                if (all_eliminated) {
-                                    // four columns at a time: four independent chains of subtractions; u_rest's
-                                    // entries read four at a time
-                                    int k = begin;
-                                    for (; k + 4 <= end; k += 4) {
-                                        T v[4];
-#pragma unroll
-                                        for (int j = 0; j < 4; ++j) v[j] = from[k + j];
-#pragma unroll
-                                        for (int r0 = 0; r0 < B; r0 += 4) {
-                                            T u[4][4];
-#pragma unroll
-                                            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                                                for (int h = 0; h < 4; ++h) u[j][h] = u_rest[k + j][r0 + h];
-#pragma unroll
-                                            for (int h = 0; h < 4; ++h)
-#pragma unroll
-                                                for (int j = 0; j < 4; ++j) v[j] = eliminated(v[j], l[r0 + h], u[j][h]);
-                                        }
-#pragma unroll
-                                        for (int j = 0; j < 4; ++j) row[k + j] = v[j];
-                                    }
-                                    for (; k < end; ++k) {
-                                        T v = from[k];
-#pragma unroll
-                                        for (int r = 0; r < B; ++r) v = eliminated(v, l[r], u_rest[k][r]);
-                                        row[k] = v;
-                                    }
-                                } else {
-                                    for (int k = begin; k < end; ++k) {
-                                        T v = from[k];
-#pragma unroll
-                                        for (int r = 0; r < B; ++r)
-                                            if (eliminated_columns >> r & 1u) v = eliminated(v, l[r], u_rest[k][r]);
-                                        row[k] = v;
-                                    }
-                                }
-                            }
                         }
+                        update_right(i, s0, s1, eliminated_columns, begin, end);
                     }
                 }
                 settle();
-                if (s1 < w) publish(s1, s1, min(s1 + B, w), search(s1), false);
+                if (s1 < w) {
+                    load_strip(s1, min(B, w - s1));
+                    publish(s1, s1, min(B, w - s1), search(s1, 0), false);
+                }
             }
 
             for (int e = t; e < mine * kPanel; e += kStripThreads) {
-                const int i = e / kPanel, k = e % kPanel;
-                if (k < w) a[(first + i) * n + j0 + k] = rows[i * stride + k];
+                const int r = e / kPanel, k = e % kPanel;
+                if (k < w) a[(first + r) * n + j0 + k] = rows[r * stride + k];
             }
         }
 
