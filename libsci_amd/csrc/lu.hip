@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace scihip {
 
@@ -250,11 +251,37 @@ namespace scihip {
         // Fields of a candidate line after slot 0 (the candidate word), kSlots<T> slots each: the
         // candidate's value in the column, its value in the next column as stored when published,
         // and the multiplier of the update the next column still lacks then (0 if none); the same
-        // three for the diagonal row, in the line of the workgroup that owns it.
+        // three for the diagonal row, in the line of the workgroup that owns it. A double's fields
+        // start at slot 2, so that the diagonal's three begin on a 16-byte pair of their own.
         enum : int { kValue, kNext, kFactor, kDiagonal, kDiagonalNext, kDiagonalFactor };
-        template <typename T> __device__ __forceinline__ constexpr int field(int f) { return 1 + f * kSlots<T>; }
+        template <typename T> __device__ __forceinline__ constexpr int field(int f) {
+            return kSlots<T> == 2 ? 2 + 2 * f : 1 + f;
+        }
         template <typename T> constexpr int kMainPairs = (field<T>(kDiagonal) + 1) / 2; // word + candidate fields
         template <typename T> constexpr int kAllPairs = (field<T>(kDiagonalFactor + 1) + 1) / 2;
+
+        // What slot s of a line holds: 0 nothing, 1 the candidate word, 2 a tagged half of a field, 3 a
+        // tagged half of a next-column field (written only when the column has a next one).
+        template <typename T> __device__ __forceinline__ constexpr int slot_kind(int s) {
+            if (s == 0) return 1;
+            for (int f = kValue; f <= kDiagonalFactor; ++f)
+                if (s >= field<T>(f) && s < field<T>(f) + kSlots<T>) return f == kNext || f == kDiagonalNext ? 3 : 2;
+            return 0;
+        }
+        // Whether the 16-byte pair x (slots s0 and s0 + 1 of a line) belongs to this column: the word's
+        // epoch, the fields' tags.
+        template <typename T>
+        __device__ __forceinline__ bool pair_current(const unsigned long long *x, int s0, unsigned tag, unsigned epoch,
+                                                     bool more) {
+            bool ok = true;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int kind = slot_kind<T>(s0 + h);
+                if (kind == 1) ok = ok && (x[h] & 0xFFu) == epoch;
+                else if (kind == 2 || (kind == 3 && more)) ok = ok && (unsigned)x[h] == tag;
+            }
+            return ok;
+        }
 
         // Workgroup barrier for LDS traffic only: the publications (global stores) in flight are not
         // waited for, nothing in the panel needs them ordered.
@@ -290,7 +317,8 @@ namespace scihip {
         // trip gives the pivot p, the pivot value and the pivot row's next entry, and the diagonal
         // row's two entries: it swaps and eliminates column c, updates column c + 1 and publishes its
         // candidate for c + 1 before it reads the rest of the pivot row and of the diagonal row, whose
-        // round trip overlaps that of the next column's lines.
+        // round trip overlaps that of the next column's lines. The rest of column c's rank-one update
+        // is done by the waves that do not poll (part 1) while the others gather column c + 1.
         //
         // Publications are made as the rows stand in LDS, whose columns beyond c + 1 still lack column
         // c's update then; the reader applies it (eliminated() with the published multiplier and the
@@ -388,12 +416,51 @@ namespace scihip {
                 };
                 if (part != 1) return;
                 if (row != LLONG_MAX) send((int)(row - first), published + row_index(col, b), kValue);
+                else if (t == kPanelRows) { // no candidate: the fields still go out (zeros), every slot tagged
+                    put_tagged(line + field<T>(kValue), T(0), tag);
+                    if (col + 1 < w) put_tagged(line + field<T>(kNext), T(0), tag);
+                    put_tagged(line + field<T>(kFactor), T(0), tag);
+                }
                 if (dd >= first && dd < first + mine) send((int)(dd - first), diagonal + diagonal_index(col), kDiagonal);
                 if (t == kPanelRows) {
                     const T best = row != LLONG_MAX ? T(__builtin_fabs(rows[(row - first) * stride + col])) : T(-1);
                     __hip_atomic_store(line, candidate_word<T>(best, row, j0, (unsigned)col + 1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                 }
+            };
+
+            // The rest of column col's rank-one update (columns col + 2 onwards; the diagonal row dd, the
+            // multipliers in `multipliers`, the pivot row in pivot_rows[col & 1]), a thread of part 1 per
+            // row. A batch loads all its entries before it stores any, so it costs one LDS round trip;
+            // the last batch of a row is cut to a quarter, half or all of kUpdateBatch entries and its
+            // indices clamped to the row's end, where the repeated entries store the same value again.
+            // The batches depend only on col, so every branch here is uniform across the wave.
+            constexpr int kUpdateBatch = sizeof(T) == 8 ? 16 : 32;
+            auto update_rest = [&](int col, int64_t dd) {
+                const int i = t % kPanelRows;
+                if (i >= mine || first + i <= dd) return;
+                const T l = multipliers[i];
+                const T *const v = pivot_rows[col & 1];
+                T *const row = rows + i * stride;
+                const int end = w;
+                int k = col + 2;
+                auto batch = [&](auto size) {
+                    constexpr int B = decltype(size)::value;
+                    T x[B], y[B];
+#pragma unroll
+                    for (int j = 0; j < B; ++j) {
+                        const int at = min(k + j, end - 1);
+                        x[j] = row[at];
+                        y[j] = v[at];
+                    }
+#pragma unroll
+                    for (int j = 0; j < B; ++j) row[min(k + j, end - 1)] = eliminated(x[j], l, y[j]);
+                    k += B;
+                };
+                while (end - k > kUpdateBatch) batch(std::integral_constant<int, kUpdateBatch>());
+                if (end - k > kUpdateBatch / 2) batch(std::integral_constant<int, kUpdateBatch>());
+                else if (end - k > kUpdateBatch / 4) batch(std::integral_constant<int, kUpdateBatch / 2>());
+                else if (end - k > 0) batch(std::integral_constant<int, kUpdateBatch / 4>());
             };
 
             publish(0, search(0), false);
@@ -407,69 +474,61 @@ namespace scihip {
                 T *const u = pivot_rows[c & 1];
                 const T *const u_before = pivot_rows[(c + 1) & 1];
 
-                // A: gather. Thread g < kPanelRows polls the lines of workgroups g, g + 128, ... (kPoll at
-                // once) with their fields until each carries this column, and keeps the largest word with
-                // its fields; the workgroup takes the largest word, so all workgroups pick the same pivot.
-                // Equal float keys of different doubles are settled with the exact values.
+                // F of column c - 1: part 1 applies the rest of its rank-one update while part 0 polls
+                // for this column (nothing before this column's multipliers, after the barrier that ends
+                // the gather, reads those entries).
+                if (part == 1 && previous) update_rest(c - 1, d - 1);
+
+                // A: gather. Thread g < kPanelRows takes the lines of workgroups g, g + 128, ... (kPoll per
+                // pass) with their fields once each carries this column, and keeps the largest word with its
+                // fields; the workgroup takes the largest word, so all workgroups pick the same pivot. Equal
+                // float keys of different doubles are settled with the exact values.
+                //
+                // One lane's loads of one line are served one after another, so the P pairs of a line are
+                // loaded by P lanes: lane t polls pair p of line t ^ p (p < P, a group of P adjacent lanes
+                // covering P lines), until each of its pairs carries this column, then takes pair p of its
+                // own line t from lane t ^ p (DPP within the group). Lanes 64 .. 64 + D - 1 also poll the D
+                // pairs of the diagonal row's fields in workgroup owner_d's line. A poll is then one round
+                // trip, with the same loads as one lane per line.
                 unsigned long long mine_best = 0; // this thread's largest word, and how many share its key
                 int sharing = 0;
                 T value = T(0), next = T(0), factor = T(0); // the fields of that word's line
-                for (int g0 = t; part == 0 && g0 < G; g0 += kPoll * kPanelRows) {
-                    unsigned waiting = 0;
+                for (int g0 = 0; part == 0 && g0 < G; g0 += kPoll * kPanelRows) {
+                    constexpr int P = kMainPairs<T>, D = kAllPairs<T> - kMainPairs<T>;
+                    static_assert(P <= 4 && (P & (P - 1)) == 0 && kPoll * P + 1 <= 32, "gather layout");
+                    const int diagonal_pair = g0 == 0 ? t - 64 : -1; // this lane's pair of the diagonal fields
+                    unsigned long long got[kPoll][P][2] = {}, got_diagonal[2] = {};
+                    unsigned waiting = 0; // bit q * P + p: pair p of line g0 + (t ^ p) + q * kPanelRows; bit kPoll * P: the diagonal pair
 #pragma unroll
                     for (int q = 0; q < kPoll; ++q)
-                        if (g0 + q * kPanelRows < G) waiting |= 1u << q;
+#pragma unroll
+                        for (int p = 0; p < P; ++p)
+                            if (g0 + (t ^ p) + q * kPanelRows < G) waiting |= 1u << (q * P + p);
+                    if (diagonal_pair >= 0 && diagonal_pair < D) waiting |= 1u << (kPoll * P);
                     unsigned spins = 0;
                     while (waiting) {
-                        unsigned long long slot[kPoll][2 * kAllPairs<T>];
 #pragma unroll
-                        for (int q = 0; q < kPoll; ++q) {
-                            if (!(waiting >> q & 1u)) continue;
-                            const int g = g0 + q * kPanelRows;
-                            const unsigned base = (unsigned)(line_index(c, g) * 8);
+                        for (int q = 0; q < kPoll; ++q)
 #pragma unroll
-                            for (int e = 0; e < kMainPairs<T>; ++e) load_pair(line_buffer, base + 16 * e, slot[q] + 2 * e);
-                            if (g == owner_d)
+                            for (int p = 0; p < P; ++p)
+                                if (waiting >> (q * P + p) & 1u)
+                                    load_pair(line_buffer, (unsigned)(line_index(c, g0 + (t ^ p) + q * kPanelRows) * 8) + 16 * p,
+                                              got[q][p]);
+                        if (waiting >> (kPoll * P) & 1u)
+                            load_pair(line_buffer, (unsigned)(line_index(c, owner_d) * 8) + 16 * (P + diagonal_pair),
+                                      got_diagonal);
 #pragma unroll
-                                for (int e = kMainPairs<T>; e < kAllPairs<T>; ++e)
-                                    load_pair(line_buffer, base + 16 * e, slot[q] + 2 * e);
-                        }
+                        for (int q = 0; q < kPoll; ++q)
 #pragma unroll
-                        for (int q = 0; q < kPoll; ++q) {
-                            if (!(waiting >> q & 1u)) continue;
-                            const int g = g0 + q * kPanelRows;
-                            const unsigned long long *s = slot[q];
-                            const unsigned long long x = s[0];
-                            bool ready = (x & 0xFFu) == epoch;
-                            if (ready && has_candidate(x))
-                                ready = current<T>(s + field<T>(kValue), tag) &&
-                                        (!more || current<T>(s + field<T>(kNext), tag)) &&
-                                        current<T>(s + field<T>(kFactor), tag);
-                            if (ready && g == owner_d)
-                                ready = current<T>(s + field<T>(kDiagonal), tag) &&
-                                        (!more || current<T>(s + field<T>(kDiagonalNext), tag)) &&
-                                        current<T>(s + field<T>(kDiagonalFactor), tag);
-                            if (!ready) continue;
-                            waiting &= ~(1u << q);
-                            if (g == owner_d) {
-                                diagonal_fields[0] = value_of<T>(s + field<T>(kDiagonal));
-                                diagonal_fields[1] = value_of<T>(s + field<T>(kDiagonalNext));
-                                diagonal_fields[2] = value_of<T>(s + field<T>(kDiagonalFactor));
-                            }
-                            bool take_it = false;
-                            if (sharing == 0 || (x >> 32) > (mine_best >> 32)) {
-                                take_it = true;
-                                sharing = 1;
-                            } else if ((x >> 32) == (mine_best >> 32)) {
-                                take_it = (x >> 8) > (mine_best >> 8);
-                                ++sharing;
-                            }
-                            if (take_it) {
-                                mine_best = x;
-                                value = value_of<T>(s + field<T>(kValue));
-                                next = value_of<T>(s + field<T>(kNext));
-                                factor = value_of<T>(s + field<T>(kFactor));
-                            }
+                            for (int p = 0; p < P; ++p)
+                                if ((waiting >> (q * P + p) & 1u) && pair_current<T>(got[q][p], 2 * p, tag, epoch, more))
+                                    waiting &= ~(1u << (q * P + p));
+                        if (waiting >> (kPoll * P) & 1u) {
+                            bool ok = false;
+#pragma unroll
+                            for (int e = 0; e < D; ++e)
+                                if (diagonal_pair == e) ok = pair_current<T>(got_diagonal, 2 * (P + e), tag, epoch, more);
+                            if (ok) waiting &= ~(1u << (kPoll * P));
                         }
                         if (!waiting) break;
                         if (++spins > kSpinLimit) {
@@ -478,6 +537,49 @@ namespace scihip {
                         }
                         if ((spins & 63u) == 0 && relaxed(error)) break;
                         __builtin_amdgcn_s_sleep(1);
+                    }
+                    // The diagonal row's fields, each from the pair that holds it.
+                    if (diagonal_pair >= 0 && diagonal_pair < D) {
+#pragma unroll
+                        for (int e = 0; e < D; ++e)
+#pragma unroll
+                            for (int f = 0; f < 3; ++f) {
+                                constexpr int kFirst = 2 * kMainPairs<T>;
+                                const int at = field<T>(kDiagonal + f) - kFirst - 2 * e; // slot within pair e
+                                if (diagonal_pair == e && at >= 0 && at < 2) diagonal_fields[f] = value_of<T>(got_diagonal + at);
+                            }
+                    }
+                    // Every lane of the wave is here: pair p of line g0 + t + q * kPanelRows from lane t ^ p.
+#pragma unroll
+                    for (int q = 0; q < kPoll; ++q) {
+                        unsigned long long s[2 * P];
+#pragma unroll
+                        for (int p = 0; p < P; ++p)
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                const unsigned long long x = got[q][p][h];
+                                unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
+                                if (p == 1) lo = dpp<0xB1>(lo), hi = dpp<0xB1>(hi); // quad_perm [1, 0, 3, 2]
+                                if (p == 2) lo = dpp<0x4E>(lo), hi = dpp<0x4E>(hi); // quad_perm [2, 3, 0, 1]
+                                if (p == 3) lo = dpp<0x1B>(lo), hi = dpp<0x1B>(hi); // quad_perm [3, 2, 1, 0]
+                                s[2 * p + h] = (unsigned long long)hi << 32 | lo;
+                            }
+                        if (g0 + t + q * kPanelRows >= G) continue;
+                        const unsigned long long x = s[0];
+                        bool take_it = false;
+                        if (sharing == 0 || (x >> 32) > (mine_best >> 32)) {
+                            take_it = true;
+                            sharing = 1;
+                        } else if ((x >> 32) == (mine_best >> 32)) {
+                            take_it = (x >> 8) > (mine_best >> 8);
+                            ++sharing;
+                        }
+                        if (take_it) {
+                            mine_best = x;
+                            value = value_of<T>(s + field<T>(kValue));
+                            next = value_of<T>(s + field<T>(kNext));
+                            factor = value_of<T>(s + field<T>(kFactor));
+                        }
                     }
                 }
                 // The workgroup's largest word: each wave's, with its fields and how many lines of the
@@ -641,43 +743,10 @@ namespace scihip {
                 }
                 settle();
                 if (swapped && b == owner_d && part == 0 && k_own < w) rows[(d - first) * stride + k_own] = u[k_own];
-
-                // F: the rest of the rank-one update: the two threads of a row split its columns c + 2
-                // onwards, 16 per batch.
-                if (eliminate) {
-                    const int i = t % kPanelRows;
-                    if (i < mine && first + i > d) {
-                        const T l = multipliers[i];
-                        T *row = rows + i * stride;
-                        const int span = w - (c + 2);
-                        int k = c + 2 + span * part / kParts;
-                        const int end = c + 2 + span * (part + 1) / kParts;
-                        for (; k + 16 <= end; k += 16) {
-                            T x[16], y[16];
-#pragma unroll
-                            for (int j = 0; j < 16; ++j) {
-                                x[j] = row[k + j];
-                                y[j] = u[k + j];
-                            }
-#pragma unroll
-                            for (int j = 0; j < 16; ++j) row[k + j] = eliminated(x[j], l, y[j]);
-                        }
-                        for (; k + 4 <= end; k += 4) {
-                            T x[4], y[4];
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                x[j] = row[k + j];
-                                y[j] = u[k + j];
-                            }
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) row[k + j] = eliminated(x[j], l, y[j]);
-                        }
-                        for (; k < end; ++k) row[k] = eliminated(row[k], l, u[k]);
-                    }
-                }
-                settle();
+                // F (the rest of the rank-one update) follows at the top of the next column.
                 previous = eliminate;
             }
+            settle(); // the last column's row d
 
             for (int i = part; i < mine; i += kParts)
                 if (k_own < w) a[(first + i) * n + j0 + k_own] = rows[i * stride + k_own];
