@@ -335,7 +335,8 @@ namespace scihip {
                                                                 unsigned long long *lines,
                                                                 unsigned long long *published,
                                                                 unsigned long long *diagonal, unsigned launch,
-                                                                unsigned *error, int spread) {
+                                                                unsigned *error, int spread, __bf16 *pack_hi,
+                                                                __bf16 *pack_lo) {
             constexpr int stride = kPanel + 1;
             constexpr int S = kSlots<T>;
             // With spread > 1 the grid is spread times the panel's workgroups and only every spread-th
@@ -750,6 +751,18 @@ namespace scihip {
 
             for (int i = part; i < mine; i += kParts)
                 if (k_own < w) a[(first + i) * n + j0 + k_own] = rows[i * stride + k_own];
+            // pack_hi (float factors with a bf16 or bf16x3 update): L21 (the rows below the panel's first
+            // kPanel) also in the product's packed operand layout, as pack() would make it after the row
+            // swaps, which leave the panel's columns alone: row r - j0 - kPanel, kPanel entries apart.
+            if constexpr (sizeof(T) == 4) {
+                if (pack_hi && w == kPanel)
+                    for (int i = part; i < mine; i += kParts) {
+                        const int64_t r = first + i - (j0 + kPanel);
+                        if (r < 0) continue;
+                        if (pack_lo) pack_element<__bf16, true>(rows[i * stride + k_own], pack_hi, pack_lo, r * kPanel + k_own);
+                        else pack_element<__bf16, false>(rows[i * stride + k_own], pack_hi, pack_lo, r * kPanel + k_own);
+                    }
+            }
         }
 
         // The panel in strips: columns are factored kStrip at a time. Per column only the strip's
@@ -1759,11 +1772,13 @@ namespace scihip {
             return hipGetLastError();
         }
 
-        // Applies the panel's swaps to every column outside it, in step order.
+        // Applies the panel's swaps, in step order, to the columns outside it numbered [from, to) (column
+        // c < j0 is number c, column c >= j0 + w number c - w).
         template <typename T>
-        __global__ __launch_bounds__(256) void swap_rows(T *a, int64_t n, int64_t j0, int w, const int64_t *pivots) {
-            const int64_t raw = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-            if (raw >= n - w) return;
+        __global__ __launch_bounds__(256) void swap_rows(T *a, int64_t n, int64_t j0, int w, const int64_t *pivots,
+                                                         int64_t from, int64_t to) {
+            const int64_t raw = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            if (raw >= to) return;
             const int64_t col = raw < j0 ? raw : raw + w;
             for (int k = 0; k < w; ++k) {
                 const int64_t d = j0 + k, p = pivots[d];
@@ -2278,7 +2293,8 @@ namespace scihip {
         (void)mode;
         // L21 and U12 packed (up to 8 bytes, split pairs), then two sets of the panel's candidate lines
         // (kParities buffers, one line of up to kStripLineMax slots per workgroup of at least 128 rows).
-        return 4 * (size_t)order * kPanel * 8 + 2 * kParities * ((size_t)order / kPanelRows + 2) * kStripLineMax * 8 +
+        // A second set of L21 buffers after them (alternate steps use alternate sets).
+        return 6 * (size_t)order * kPanel * 8 + 2 * kParities * ((size_t)order / kPanelRows + 2) * kStripLineMax * 8 +
                1024;
     }
 
@@ -2339,6 +2355,8 @@ namespace scihip {
     // panels. At n = 16384 (threshold 0 / 4096 / 8192 / 12288) the fp32 solve measured 104.2 / 103.8 /
     // 103.3 / 104.4 ms and the bf16 one 94.0 / 92.6 / 92.6 / 92.8 ms (mean of two runs on one box).
     constexpr int64_t kFloatPairColumns = 8192;
+    // A float pair's update splits while this many columns or more remain beyond the next panel's.
+    constexpr int64_t kFloatSplitColumns = 10240;
 
     template <typename T>
     int factor(T *a, int64_t n, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
@@ -2352,7 +2370,11 @@ namespace scihip {
         const size_t kp = kPanel;
         unsigned char *base = static_cast<unsigned char *>(scratch);
         const size_t operand = (size_t)n * kp * 8; // bytes of one packed n x kPanel operand, at most
-        void *la_hi = base, *la_lo = base + operand;
+        // Packed L21, two sets used by alternate steps: a panel packs its own L21 (float factors with a
+        // bf16 or bf16x3 update) while the update beside it still reads the previous step's.
+        const size_t line_region = 2 * kParities * ((size_t)n / kPanelRows + 2) * kStripLineMax * 8;
+        void *const la_his[2] = {base, base + 4 * operand + line_region};
+        void *const la_los[2] = {base + operand, base + 5 * operand + line_region};
         void *ub_hi = base + 2 * operand, *ub_lo = base + 3 * operand;
         unsigned long long *lines = reinterpret_cast<unsigned long long *>(base + 4 * operand);
         const int max_groups = (int)((n + kPanelRows - 1) / kPanelRows);
@@ -2393,9 +2415,26 @@ namespace scihip {
         const size_t line_bytes = (size_t)kParities * (max_groups + 1) * kStripLineMax * sizeof(unsigned long long);
         unsigned long long *line_sets[2] = {lines, lines + line_bytes / sizeof(unsigned long long)};
         auto clear_lines = [&](hipStream_t s, int set) { return hipMemsetAsync(line_sets[set], 0, line_bytes, s); };
-        // Factors the panel at column j0 on stream `s` with candidate line set `set` (already clear).
-        auto launch_panel = [&](int64_t j0, hipStream_t s, int set) -> hipError_t {
+        // The row swaps of the panel at j0 (width w) applied to the columns outside it numbered [from, to)
+        // (see swap_rows), on stream `s`.
+        auto swap_columns = [&](int64_t j0, int w, int64_t from, int64_t to, hipStream_t s) -> hipError_t {
+            if (to <= from) return hipSuccess;
+            hipLaunchKernelGGL(swap_rows<T>, dim3((unsigned)((to - from + 255) / 256)), dim3(256), 0, s, a, n, j0, w,
+                               (const int64_t *)pivots, from, to);
+            return hipGetLastError();
+        };
+        // Whether the panel at column j0 also stores its L21 packed (into L21 set (j0 / kPanel) & 1): for
+        // a float factorisation with a bf16 or bf16x3 update, unless its step pairs with the previous one
+        // (a pair packs both steps' L21 after its row swaps) or nothing follows the panel.
+        auto packs_l21 = [&](int64_t j0, bool pairs) {
+            return sizeof(T) == 4 && (mode == kBfloat16 || mode == kSplit) && !strips && !pairs && n - j0 > kPanel;
+        };
+        // Factors the panel at column j0 on stream `s` with candidate line set `set` (already clear);
+        // `pack`: also store its L21 packed (packs_l21).
+        auto launch_panel = [&](int64_t j0, hipStream_t s, int set, bool pack) -> hipError_t {
             unsigned long long *lines = line_sets[set];
+            __bf16 *pack_hi = pack ? static_cast<__bf16 *>(la_his[(j0 / kPanel) & 1]) : nullptr;
+            __bf16 *pack_lo = pack && mode == kSplit ? static_cast<__bf16 *>(la_los[(j0 / kPanel) & 1]) : nullptr;
             int w = (int)std::min<int64_t>(kPanel, n - j0);
             const int groups = (int)((n - j0 + kPanelRows - 1) / kPanelRows);
             hipError_t e = hipSuccess;
@@ -2422,17 +2461,17 @@ namespace scihip {
             }
             int spread = 1;
             void *args[] = {&a, &n, (void *)&j0, (void *)&w, &pivots, &lines, &published, &diagonal, &launch,
-                            &ws.error, &spread};
+                            &ws.error, &spread, &pack_hi, &pack_lo};
             // The workgroups wait for each other, so all of them must be resident at once. When the grid
             // fits the device, a plain launch is enough: workgroups queued behind the concurrent update
             // start as soon as its workgroups finish, which they do without waiting on the panel (and
             // every wait is bounded). Larger grids take the cooperative launch, which checks residency.
             if (groups <= confine) // one XCD holds them all (every XCD has units / 8 CUs)
                 hipLaunchKernelGGL((panel<T>), dim3(groups * 8), dim3(kPanelThreads), 0, s, a, n, j0, w, pivots,
-                                   lines, published, diagonal, launch, ws.error, 8);
+                                   lines, published, diagonal, launch, ws.error, 8, pack_hi, pack_lo);
             else if (groups <= resident)
                 hipLaunchKernelGGL((panel<T>), dim3(groups), dim3(kPanelThreads), 0, s, a, n, j0, w, pivots, lines,
-                                   published, diagonal, launch, ws.error, 1);
+                                   published, diagonal, launch, ws.error, 1, pack_hi, pack_lo);
             else
                 e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&panel<T>), dim3(groups),
                                                dim3(kPanelThreads), args, 0, s);
@@ -2444,6 +2483,10 @@ namespace scihip {
         // SCIHIP_LU_PAIR=c (for tests at small orders): every step pairs while c or more columns remain,
         // and an fp64 step leaves at least half its columns to the next one.
         int64_t pair_columns = kFloatPairColumns;
+        // SCIHIP_LU_SPLIT=c: a float pair's update splits (step_float) while c or more columns remain beyond
+        // the next panel's (0: never).
+        int64_t split_columns = kFloatSplitColumns;
+        if (const char *text = std::getenv("SCIHIP_LU_SPLIT")) split_columns = std::strtoll(text, nullptr, 10);
         const char *pair_text = std::getenv("SCIHIP_LU_PAIR");
         const bool forced = pair_text != nullptr;
         if (forced) pair_columns = std::strtoll(pair_text, nullptr, 10);
@@ -2501,7 +2544,7 @@ namespace scihip {
             // stream, whose end the next step waits for (panel_done marks it).
             const int set = (int)((j0 + w) / kPanel) & 1;
             if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
-            if (e == hipSuccess) e = launch_panel(j0 + w, stream, set);
+            if (e == hipSuccess) e = launch_panel(j0 + w, stream, set, false);
             if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
             if (e == hipSuccess) e = clear_lines(ws.side, set ^ 1); // for the panel after next
             if (e != hipSuccess) return e;
@@ -2554,6 +2597,7 @@ namespace scihip {
             T *c = a + (j0 + w) * n + (j0 + w);
             const bool in_place = mode == kFloat32;
             const bool split = mode == kSplit;
+            void *const la_hi = la_his[(j0 / kPanel) & 1], *const la_lo = la_los[(j0 / kPanel) & 1];
             // Element `index` of a packed buffer (nullptr stays nullptr).
             auto at = [&](void *base, int64_t index) -> void * {
                 return base ? static_cast<void *>(static_cast<unsigned char *>(base) + (size_t)index * eb) : nullptr;
@@ -2573,7 +2617,7 @@ namespace scihip {
                     e = product(mode, slab, stream);
                 }
                 if (e != hipSuccess) return e;
-            } else if (!in_place) {
+            } else if (!in_place && !packs_l21(j0, false)) { // (else the panel packed its L21)
                 e = pack(mode, a + (j0 + w) * n + j0, wide, rest, w, n, false, la_hi, la_lo, kp, stream);
                 if (e != hipSuccess) return e;
             }
@@ -2606,10 +2650,15 @@ namespace scihip {
             // The next panel follows on the main stream at once; the rest of the step runs beside it on
             // the side stream, whose end the next step waits for (panel_done marks it).
             const int set = (int)((j0 + w) / kPanel) & 1;
+            const bool next_pairs = !pair && rest - next >= kPanel && rest >= pair_columns; // the next step pairs
             if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
-            if (e == hipSuccess) e = launch_panel(j0 + w, stream, set);
+            if (e == hipSuccess) e = launch_panel(j0 + w, stream, set, packs_l21(j0 + w, next_pairs));
             if (e == hipSuccess) e = hipStreamWaitEvent(ws.side, ws.columns_ready, 0);
             if (e == hipSuccess) e = clear_lines(ws.side, set ^ 1); // for the panel after next
+            if (!pair) { // the columns the main stream did not swap (the loop below swapped the next panel's)
+                if (e == hipSuccess) e = swap_columns(j0, w, 0, j0, ws.side);
+                if (e == hipSuccess) e = swap_columns(j0, w, j0 + next, n - w, ws.side);
+            }
             if (e != hipSuccess) return e;
             if (rest > next) { // U12's other columns, and their packed copy
                 const int64_t others = rest - next;
@@ -2619,14 +2668,22 @@ namespace scihip {
                 if (e == hipSuccess) e = pack_u12(next, others, ws.side);
                 if (e != hipSuccess) return e;
             }
-            carry = !pair && rest - next >= kPanel && rest >= pair_columns;
-            if (rest > next && !carry) e = update(next, rest - next, ws.side); // beside the panel
+            carry = next_pairs;
+            // A pair with at least split_columns columns beyond the next panel's leaves the update of the
+            // last half of them out of what the next step waits for: that step pairs with nothing, swaps
+            // only its next panel's columns on the chain and does the rest of its work on the side stream,
+            // behind this update, which then runs beside its panel too.
+            int64_t later = 0;
+            if (pair && split_columns > 0 && rest - next >= split_columns)
+                later = std::max<int64_t>(0, std::min((rest - next) / 2 / kPanel * kPanel, rest - next - kPanel));
+            if (rest > next && !carry) e = update(next, rest - next - later, ws.side); // beside the panel
             if (e == hipSuccess) e = hipEventRecord(ws.panel_done, ws.side);
+            if (e == hipSuccess && later > 0) e = update(rest - later, later, ws.side);
             return e;
         };
 
         hipError_t e = hipMemsetAsync(lines, 0, 2 * line_bytes, stream); // both sets
-        if (e == hipSuccess) e = launch_panel(0, stream, 0);
+        if (e == hipSuccess) e = launch_panel(0, stream, 0, packs_l21(0, false));
         if (e != hipSuccess) return (int)e;
         for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
             const int w = (int)std::min<int64_t>(kPanel, n - j0);
@@ -2635,11 +2692,14 @@ namespace scihip {
                 if (e != hipSuccess) return (int)e;
             }
             timer.stage(0);
-            if (n > w)
-                hipLaunchKernelGGL(swap_rows<T>, dim3((unsigned)((n - w + 255) / 256)), dim3(256), 0, stream, a, n, j0,
-                                   w, (const int64_t *)pivots);
-            timer.stage(1);
             const int64_t rest = n - j0 - w;
+            // A float step that does not pair with the previous one needs only the next panel's columns
+            // swapped on the chain of panels; its side stream swaps the others (step_float).
+            const bool narrow_swap = sizeof(T) == 4 && !carry && rest > 0;
+            e = narrow_swap ? swap_columns(j0, w, j0, j0 + std::min<int64_t>(kPanel, rest), stream)
+                            : swap_columns(j0, w, 0, n - w, stream);
+            if (e != hipSuccess) return (int)e;
+            timer.stage(1);
             if (rest > 0) {
                 if constexpr (sizeof(T) == 8) e = step_pair(j0, w, rest);
                 else e = step_float(j0, w, rest);

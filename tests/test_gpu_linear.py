@@ -112,6 +112,30 @@ def test_float_factor_reconstructs(device, mode, n, pair, monkeypatch):
     assert error < 4 * n * unit * np.abs(a).max(), error
 
 
+@pytest.mark.parametrize("n,pivoting", [(1000, False), (1024, True), (2048, True)])
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "bf16"])
+def test_split_pair_updates_agree(device, mode, n, pivoting, monkeypatch):
+    """Every step paired (SCIHIP_LU_PAIR=0), each pair's update split so that the next step does not
+    wait for its last columns (SCIHIP_LU_SPLIT=256) or not split (SCIHIP_LU_SPLIT=0): the same
+    products over the same operands in another order of launches, so the factors agree bit for bit,
+    on a shifted matrix and on one that pivots across workgroups."""
+    monkeypatch.setenv("SCIHIP_LU_PAIR", "0")
+    print(f"[test] split pair updates n={n} {mode} pivoting={pivoting}", flush=True)
+    a = oracle.uniform(n * n, 2, -1.0, 1.0, dtype=np.float64).reshape(n, n)
+    if not pivoting:
+        a += (n / 4) * np.eye(n)
+    a = a.astype(np.float32)
+    results = {}
+    for split in ("0", "256"):
+        monkeypatch.setenv("SCIHIP_LU_SPLIT", split)
+        results[split] = gpu_factor(device, a, mode)
+    (lu0, rows0), (lu1, rows1) = results["0"], results["256"]
+    assert (rows0 == rows1).all()
+    assert np.array_equal(lu0.view(np.uint32), lu1.view(np.uint32))
+    if not pivoting:
+        assert (rows0 == np.arange(n)).all()
+
+
 @pytest.mark.parametrize("n,width", [(300, 5), (512, 5), (512, 33)])
 def test_substitute_matches_oracle(device, n, width):
     # 300: the sync-free substitution; multiples of 128: the blocked one (inverted diagonal blocks)
