@@ -1687,12 +1687,17 @@ namespace scihip {
                                                                                int64_t n, int64_t columns, D *hi, D *lo,
                                                                                int64_t stride) {
             constexpr int R = kU12Rows;
-            __shared__ T lower[kPanel * kPanel];
+            // float: L11 transposed (entry (r, c) at c * kLowerStride + r, rows 16-byte aligned), so that one
+            // 16-byte LDS read gives four entries of a column; double: row-major.
+            constexpr int kLowerStride = sizeof(T) == 4 ? kPanel + 4 : kPanel;
+            __shared__ __attribute__((aligned(16))) T lower[kPanel * kLowerStride];
             __shared__ T solved[R][kPanel];
             const int t = (int)threadIdx.x;
             for (int e = t; e < kPanel * kPanel; e += kU12NarrowThreads) {
                 const int r = e / kPanel, c = e % kPanel;
-                lower[e] = c < r ? l[(int64_t)r * n + c] : T(0);
+                const T v = c < r ? l[(int64_t)r * n + c] : T(0);
+                if constexpr (sizeof(T) == 4) lower[c * kLowerStride + r] = v;
+                else lower[e] = v;
             }
             const int k = t % kPanel, q = t / kPanel; // q is the same in a whole wave
             const int64_t index = (int64_t)blockIdx.x * kPanel + k; // the column
@@ -1704,11 +1709,27 @@ namespace scihip {
             __syncthreads();
             for (int s = 0; s < kU12Blocks; ++s) {
                 if (q == s) {
-                    const T *block = lower + (s * R) * kPanel + s * R;
+                    if constexpr (sizeof(T) == 4) {
+                        // by columns: y[i] still takes j = 0, 1, ..., i - 1 in order
+                        typedef T Four __attribute__((ext_vector_type(4)));
 #pragma unroll
-                    for (int i = 1; i < R; ++i)
+                        for (int j = 0; j + 1 < R; ++j) {
+                            const Four *column_j = reinterpret_cast<const Four *>(lower + (s * R + j) * kLowerStride + s * R);
 #pragma unroll
-                        for (int j = 0; j < i; ++j) y[i] = eliminated(y[i], block[i * kPanel + j], y[j]);
+                            for (int h = (j + 1) / 4; h < R / 4; ++h) {
+                                const Four m = column_j[h];
+#pragma unroll
+                                for (int e = 0; e < 4; ++e)
+                                    if (4 * h + e > j) y[4 * h + e] = eliminated(y[4 * h + e], m[e], y[j]);
+                            }
+                        }
+                    } else {
+                        const T *block = lower + (s * R) * kPanel + s * R;
+#pragma unroll
+                        for (int i = 1; i < R; ++i)
+#pragma unroll
+                            for (int j = 0; j < i; ++j) y[i] = eliminated(y[i], block[i * kPanel + j], y[j]);
+                    }
 #pragma unroll
                     for (int i = 0; i < R; ++i) solved[i][k] = y[i];
                     if (live) {
@@ -1722,8 +1743,9 @@ namespace scihip {
                 __syncthreads();
                 if (q > s) {
                     // fp64 reads four entries of a row of L at once (at n = 16384, with the packed copy
-                    // stored too: 53 -> 38 us per step); float reads one (the four-entry form measured
-                    // 24 -> 33 us per step). Each y[i] takes j = 0, 1, ... in order either way.
+                    // stored too: 53 -> 38 us per step); float four entries of a column of the transposed
+                    // copy (four of a row measured 24 -> 33 us per step). Each y[i] takes j = 0, 1, ... in
+                    // order either way.
                     const T *block = lower + (q * R) * kPanel + s * R;
                     if constexpr (sizeof(T) == 8) {
                         typedef T Four __attribute__((ext_vector_type(4)));
@@ -1740,11 +1762,17 @@ namespace scihip {
                             }
                         }
                     } else {
+                        typedef T Four __attribute__((ext_vector_type(4)));
 #pragma unroll
                         for (int j = 0; j < R; ++j) {
                             const T x = solved[j][k];
+                            const Four *column_j = reinterpret_cast<const Four *>(lower + (s * R + j) * kLowerStride + q * R);
 #pragma unroll
-                            for (int i = 0; i < R; ++i) y[i] = eliminated(y[i], block[i * kPanel + j], x);
+                            for (int h = 0; h < R / 4; ++h) {
+                                const Four m = column_j[h];
+#pragma unroll
+                                for (int e = 0; e < 4; ++e) y[4 * h + e] = eliminated(y[4 * h + e], m[e], x);
+                            }
                         }
                     }
                 }
