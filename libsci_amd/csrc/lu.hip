@@ -1895,56 +1895,56 @@ namespace scihip {
         // true) in LDS, one workgroup per block, and stores it transposed: out[b][k][r] = inv(block
         // b)[r][k]. Lower: M L = I gives, for i > j, M[i][j] = -(L[i][j] + sum_{j<k<i} M[i][k] L[k][j]),
         // columns from the last; upper: N U = I gives N[j][j] = 1 / U[j][j] and, for i < j,
-        // N[i][j] = -(sum_{i<=k<j} N[i][k] U[k][j]) / U[j][j], columns from the first. Column j is
-        // overwritten only after every thread has read it.
+        // N[i][j] = -(sum_{i<=k<j} N[i][k] U[k][j]) / U[j][j], columns from the first. Thread i makes
+        // row i, which needs only its own earlier entries and the block, so the block and the inverse
+        // sit in separate LDS triangles (packed, so that both fit for doubles) and no thread waits for
+        // another between columns.
         template <typename T, bool upper>
-        __global__ __launch_bounds__(256) void invert_diagonal(const T *f, int64_t n, int size, T *out) {
-            constexpr int stride = kBlock + 1;
-            __shared__ T m[kBlock * stride];
-            const int64_t r0 = (int64_t)blockIdx.x * kBlock;
+        __global__ __launch_bounds__(kBlock) void invert_diagonal(const T *f, int64_t n, int size, T *out, int first) {
+            // packed triangle: lower strictly below the diagonal, upper with it
+            constexpr int kCount = upper ? kBlock * (kBlock + 1) / 2 : kBlock * (kBlock - 1) / 2;
+            __shared__ T a_s[kCount]; // the block
+            __shared__ T m_s[kCount]; // the inverse
+            auto at = [](int i, int k) -> int {
+                return upper ? i * kBlock - i * (i - 1) / 2 + (k - i) : i * (i - 1) / 2 + k;
+            };
+            const int64_t r0 = (int64_t)(first + (int)blockIdx.x) * kBlock;
             const int t = threadIdx.x;
             // A block of `size` < kBlock rows is completed by the identity, whose inverse it keeps.
-            for (int idx = t; idx < kBlock * kBlock; idx += blockDim.x) {
+            for (int idx = t; idx < kBlock * kBlock; idx += kBlock) {
                 const int i = idx / kBlock, k = idx % kBlock;
-                m[i * stride + k] = (i < size && k < size) ? f[(r0 + i) * n + r0 + k] : (i == k ? T(1) : T(0));
+                if (upper ? k >= i : k < i)
+                    a_s[at(i, k)] = (i < size && k < size) ? f[(r0 + i) * n + r0 + k] : (i == k ? T(1) : T(0));
             }
             __syncthreads();
-            const int i = t; // threads below kBlock each carry one row of the inverse
-            for (int step = 0; step < kBlock; ++step) {
-                const int j = upper ? step : kBlock - 1 - step;
-                T v = T(0);
-                const bool active = i < kBlock && (upper ? i <= j : i > j);
-                if (active) {
-                    // The dot product over k in [from, to) as four interleaved partial sums, so that four
-                    // independent fma chains (and their LDS reads) are in flight instead of one.
-                    auto dot = [&](int from, int to) {
-                        T part[4] = {T(0), T(0), T(0), T(0)};
-                        int k = from;
-                        for (; k + 4 <= to; k += 4)
+            const int i = t;
+            // The dot product of row i of the inverse and column j of the block over k in [from, to), as
+            // four interleaved partial sums, so that four independent fma chains are in flight.
+            auto dot = [&](int j, int from, int to) {
+                T part[4] = {T(0), T(0), T(0), T(0)};
+                int k = from;
+                for (; k + 4 <= to; k += 4)
 #pragma unroll
-                            for (int q = 0; q < 4; ++q)
-                                part[q] = __builtin_fma(m[i * stride + k + q], m[(k + q) * stride + j], part[q]);
-                        for (; k < to; ++k) part[0] = __builtin_fma(m[i * stride + k], m[k * stride + j], part[0]);
-                        return (part[0] + part[1]) + (part[2] + part[3]);
-                    };
-                    if (upper) {
-                        if (i == j) v = T(1) / m[j * stride + j];
-                        else v = -dot(i, j) / m[j * stride + j];
-                    } else {
-                        v = -(m[i * stride + j] + dot(j + 1, i));
-                    }
+                    for (int q = 0; q < 4; ++q) part[q] = __builtin_fma(m_s[at(i, k + q)], a_s[at(k + q, j)], part[q]);
+                for (; k < to; ++k) part[0] = __builtin_fma(m_s[at(i, k)], a_s[at(k, j)], part[0]);
+                return (part[0] + part[1]) + (part[2] + part[3]);
+            };
+            if (upper) {
+                for (int j = i; j < kBlock; ++j) {
+                    const T v = j == i ? T(1) / a_s[at(j, j)] : -dot(j, i, j) / a_s[at(j, j)];
+                    m_s[at(i, j)] = v;
                 }
-                __syncthreads();
-                if (active) m[i * stride + j] = v;
-                __syncthreads();
+            } else {
+                for (int j = i - 1; j >= 0; --j) m_s[at(i, j)] = -(a_s[at(i, j)] + dot(j, j + 1, i));
             }
-            T *block = out + (int64_t)blockIdx.x * kBlock * kBlock;
-            for (int idx = t; idx < kBlock * kBlock; idx += blockDim.x) {
+            __syncthreads();
+            T *block = out + (int64_t)(first + (int)blockIdx.x) * kBlock * kBlock;
+            for (int idx = t; idx < kBlock * kBlock; idx += kBlock) {
                 const int r = idx % kBlock, k = idx / kBlock; // consecutive threads write consecutive r
-                const T value = upper ? (k >= r ? m[r * stride + k] : T(0))
-                                      : (k < r ? m[r * stride + k] : (k == r ? T(1) : T(0)));
+                const T value = upper ? (k >= r ? m_s[at(r, k)] : T(0)) : (k < r ? m_s[at(r, k)] : (k == r ? T(1) : T(0)));
                 block[(int64_t)k * kBlock + r] = value;
             }
+        }
         }
 
         // One triangle of the factors for up to kTriangleColumns right-hand sides, all diagonal blocks in
@@ -2144,11 +2144,12 @@ namespace scihip {
         // in LDS (the inverse read where all lanes share an entry). Each entry is the fma chain over
         // m = 0, 1, ..., kBlock - 1.
         template <typename T, bool upper, int distance>
-        __global__ __launch_bounds__(256) void link_blocks(const T *f, int64_t n, const T *inverses_t, T *links) {
+        __global__ __launch_bounds__(256) void link_blocks(const T *f, int64_t n, const T *inverses_t, T *links,
+                                                           int first) {
             constexpr int M = 32;
             __shared__ T inverse_s[M][kBlock];
             __shared__ T block_s[M][kBlock];
-            const int nb = (int)(n / kBlock), j = (int)blockIdx.x;
+            const int nb = (int)(n / kBlock), j = first + (int)blockIdx.x;
             const int neighbour = upper ? j + distance : j - distance;
             if (neighbour < 0 || neighbour >= nb) return;
             const int t = (int)threadIdx.x, k = t % kBlock, r0 = 64 * (t / kBlock);
@@ -2371,6 +2372,27 @@ namespace scihip {
 
     } // namespace
 
+    namespace {
+        // invert_blocks for diagonal blocks [first, first + count).
+        template <typename T>
+        hipError_t invert_range(const T *f, int64_t n, T *inverses, int first, int count, hipStream_t stream) {
+            if (count <= 0) return hipSuccess;
+            const dim3 grid((unsigned)count);
+            hipLaunchKernelGGL((invert_diagonal<T, false>), grid, dim3(kBlock), 0, stream, f, n, kBlock, inverses, first);
+            hipLaunchKernelGGL((invert_diagonal<T, true>), grid, dim3(kBlock), 0, stream, f, n, kBlock,
+                               inverses + n * kBlock, first);
+            hipLaunchKernelGGL((link_blocks<T, false, 1>), grid, dim3(256), 0, stream, f, n, inverses,
+                               inverses + 2 * n * kBlock, first);
+            hipLaunchKernelGGL((link_blocks<T, true, 1>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
+                               inverses + 3 * n * kBlock, first);
+            hipLaunchKernelGGL((link_blocks<T, false, 2>), grid, dim3(256), 0, stream, f, n, inverses,
+                               inverses + 4 * n * kBlock, first);
+            hipLaunchKernelGGL((link_blocks<T, true, 2>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
+                               inverses + 5 * n * kBlock, first);
+            return hipGetLastError();
+        }
+    } // namespace
+
     // The share of each fp64 update that runs beside the next panel: kPanelMsFp64 milliseconds of work
     // at kAlongsideRateFp64, the rate measured for an fp64 update beside a panel on MI355X. An fp64
     // panel takes about 0.6 ms at n = 16384, but the fp64 solve there measured best with 0.4 ms of
@@ -2386,10 +2408,20 @@ namespace scihip {
     // A float pair's update splits while this many columns or more remain beyond the next panel's.
     constexpr int64_t kFloatSplitColumns = 10240;
 
+    // Panels before the end from which the diagonal blocks done so far are inverted on the third stream
+    // (the last panels leave most compute units idle; the blocks after are inverted after the last one).
+    constexpr int kInvertTail = 24;
+
     template <typename T>
     int factor(T *a, int64_t n, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
-               hipStream_t stream, const char **error) {
+               hipStream_t stream, const char **error, T *inverses) {
         *error = nullptr;
+        if (inverses && (!blocked_substitution(n) || !ws.third || !ws.inverted)) {
+            *error = "LU inverses need an order that is a multiple of 128 (at least 256) and the third stream";
+            return -1;
+        }
+        const int blocks_total = (int)(n / kBlock);
+        const int early_blocks = inverses ? std::max(0, blocks_total - kInvertTail) : 0; // inverted on the third stream
         if (scratch_bytes < factor_scratch(n, mode)) {
             *error = "LU scratch too small";
             return -1;
@@ -2734,6 +2766,20 @@ namespace scihip {
                 if (e != hipSuccess) return (int)e;
                 timer.stage(4);
             }
+            // Blocks [0, early_blocks) are final once this step's work is (later steps touch only rows
+            // and columns beyond them): their inverses and links go on the third stream, which nothing
+            // waits for until the end.
+            if (early_blocks > 0 && j0 / kPanel == early_blocks) {
+                e = hipStreamWaitEvent(ws.third, ws.panel_done, 0);
+                if (e == hipSuccess) e = invert_range(static_cast<const T *>(a), n, inverses, 0, early_blocks, ws.third);
+                if (e == hipSuccess) e = hipEventRecord(ws.inverted, ws.third);
+                if (e != hipSuccess) return (int)e;
+            }
+        }
+        if (inverses) {
+            e = invert_range(static_cast<const T *>(a), n, inverses, early_blocks, blocks_total - early_blocks, stream);
+            if (e == hipSuccess && early_blocks > 0) e = hipStreamWaitEvent(stream, ws.inverted, 0);
+            if (e != hipSuccess) return (int)e;
         }
         return hipGetLastError();
     }
@@ -2750,22 +2796,10 @@ namespace scihip {
         return hipGetLastError();
     }
 
+
     template <typename T> hipError_t invert_blocks(const T *f, int64_t n, T *inverses, hipStream_t stream) {
         if (!blocked_substitution(n)) return hipErrorInvalidValue;
-        const unsigned blocks = (unsigned)(n / kBlock);
-        hipLaunchKernelGGL((invert_diagonal<T, false>), dim3(blocks), dim3(256), 0, stream, f, n, kBlock, inverses);
-        hipLaunchKernelGGL((invert_diagonal<T, true>), dim3(blocks), dim3(256), 0, stream, f, n, kBlock,
-                           inverses + n * kBlock);
-        const dim3 grid(blocks);
-        hipLaunchKernelGGL((link_blocks<T, false, 1>), grid, dim3(256), 0, stream, f, n, inverses,
-                           inverses + 2 * n * kBlock);
-        hipLaunchKernelGGL((link_blocks<T, true, 1>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
-                           inverses + 3 * n * kBlock);
-        hipLaunchKernelGGL((link_blocks<T, false, 2>), grid, dim3(256), 0, stream, f, n, inverses,
-                           inverses + 4 * n * kBlock);
-        hipLaunchKernelGGL((link_blocks<T, true, 2>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
-                           inverses + 5 * n * kBlock);
-        return hipGetLastError();
+        return invert_range(f, n, inverses, 0, (int)(n / kBlock), stream);
     }
 
     template <typename T>
@@ -2822,9 +2856,9 @@ namespace scihip {
     }
 
     template int factor<double>(double *, int64_t, int64_t *, int, Workspace &, void *, size_t, hipStream_t,
-                                const char **);
+                                const char **, double *);
     template int factor<float>(float *, int64_t, int64_t *, int, Workspace &, void *, size_t, hipStream_t,
-                               const char **);
+                               const char **, float *);
     template int substitute<double>(const double *, int64_t, double *, int64_t, Workspace &, unsigned &, hipStream_t,
                                     const char **, const double *, void *);
     template int substitute<float>(const float *, int64_t, float *, int64_t, Workspace &, unsigned &, hipStream_t,

@@ -49,14 +49,19 @@ namespace scihip {
         hipStream_t side = nullptr;         // updates beside the look-ahead panels (default priority)
         hipEvent_t columns_ready = nullptr; // main stream: the next panel's columns are up to date
         hipEvent_t panel_done = nullptr;    // side stream: the step's work there is done
+        hipStream_t third = nullptr;        // diagonal-block inverses of finished blocks, beside the last panels
+        hipEvent_t inverted = nullptr;      // third stream: those inverses are done
     };
 
     // Factors the order x order row-major matrix in place (T = float or double); pivots[k] is the
     // row swapped with row k at step k (absolute). `mode` is the product mode of the trailing update
     // (kFloat64 for double; kFloat32, kSplit or kBfloat16 for float). Workspace buffers are passed in.
+    // With `inverses` (blocked_substitution(order) only), also makes invert_blocks' output there: the
+    // blocks that are final early (from kInvertTail panels before the end) on Workspace::third beside the
+    // last panels, the others after the last one.
     template <typename T>
     int factor(T *matrix, int64_t order, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
-               hipStream_t stream, const char **error);
+               hipStream_t stream, const char **error, T *inverses = nullptr);
 
     // Bytes of scratch factor() needs.
     size_t factor_scratch(int64_t order, int mode);

@@ -480,6 +480,11 @@ namespace scihip {
             if (int rc = status(hipEventCreateWithFlags(&ws.panel_done, hipEventDisableTiming), "hipEventCreate"))
                 return rc;
         }
+        if (!ws.third) {
+            if (int rc = status(hipStreamCreateWithFlags(&ws.third, hipStreamNonBlocking), "hipStreamCreate")) return rc;
+            if (int rc = status(hipEventCreateWithFlags(&ws.inverted, hipEventDisableTiming), "hipEventCreate"))
+                return rc;
+        }
         if (!ws.counter) {
             void *block = nullptr;
             if (int rc = status(hipMalloc(&block, 256), "hipMalloc")) return rc;
@@ -523,6 +528,11 @@ namespace scihip {
             (void)hipStreamSynchronize(ws.side);
             (void)hipStreamDestroy(ws.side);
         }
+        if (ws.third) {
+            (void)hipStreamSynchronize(ws.third);
+            (void)hipStreamDestroy(ws.third);
+        }
+        if (ws.inverted) (void)hipEventDestroy(ws.inverted);
         if (ws.columns_ready) (void)hipEventDestroy(ws.columns_ready);
         if (ws.panel_done) (void)hipEventDestroy(ws.panel_done);
         if (ws.counter) (void)hipFree(ws.counter);
@@ -635,14 +645,15 @@ namespace scihip {
     }
 
     template <typename T>
-    int factor_on_device(scihip_device *device, T *matrix, int64_t order, int mode, std::vector<int64_t> &pivots) {
+    int factor_on_device(scihip_device *device, T *matrix, int64_t order, int mode, std::vector<int64_t> &pivots,
+                         T *inverses = nullptr) {
         int rc = ensure_lu(device, order, sizeof(T));
         Scratch pivot_block(device, size_t(order) * sizeof(int64_t), rc);
         Scratch work(device, factor_scratch(order, mode), rc);
         if (rc) return rc;
         const char *message = nullptr;
         const int e = factor<T>(matrix, order, pivot_block.as<int64_t>(), mode, device->lu, work.pointer,
-                                factor_scratch(order, mode), device->stream, &message);
+                                factor_scratch(order, mode), device->stream, &message, inverses);
         if (e) {
             fail(message ? message : hipGetErrorString(hipError_t(e)));
             return SCIHIP_EDEVICE;
@@ -1024,7 +1035,8 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     Scratch packed_x(device, size_t(w) * size_t(kp) * 8, rc);
     Scratch sums(device, order * 8, rc);
     const int64_t chunks = column_max_chunks(n);
-    Scratch partial(device, size_t(chunks * w) * 8 * 3, rc);
+    // column maxima of the residual, the solution and the right-hand sides, then of |A|'s row sums
+    Scratch partial(device, size_t(chunks * w * 3 + chunks) * 8, rc);
     const bool aligned = n % 16 == 0;
     Scratch packed_a(device, aligned ? 16 : size_t(n) * size_t(kp) * 8, rc);
     const bool blocked = blocked_substitution(n);
@@ -1047,13 +1059,21 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
         if ((rc = status(copy_with_row_sums(matrix, n, n, factors.as<double>(), sums.as<double>(), device->stream),
                          "copy")))
             return rc;
-        rc = factor_on_device<double>(device, factors.as<double>(), n, kFloat64, pivots);
+        if ((rc = status(column_max(sums.as<double>(), n, 1, partial.as<double>() + chunks * w * 3, device->stream),
+                         "norm")))
+            return rc;
+        rc = factor_on_device<double>(device, factors.as<double>(), n, kFloat64, pivots,
+                                      blocked ? inverses.as<double>() : nullptr);
     } else {
         if ((rc = status(copy_with_row_sums(matrix, n, n, factors.as<float>(), sums.as<double>(), device->stream),
                          "convert")))
             return rc;
+        if ((rc = status(column_max(sums.as<double>(), n, 1, partial.as<double>() + chunks * w * 3, device->stream),
+                         "norm")))
+            return rc;
         const int product_mode = mode == SCIHIP_FACTOR_FLOAT32 ? kFloat32 : mode == SCIHIP_FACTOR_SPLIT ? kSplit : kBfloat16;
-        rc = factor_on_device<float>(device, factors.as<float>(), n, product_mode, pivots);
+        rc = factor_on_device<float>(device, factors.as<float>(), n, product_mode, pivots,
+                                     blocked ? inverses.as<float>() : nullptr);
     }
     if (rc) return rc;
     stats->factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
@@ -1062,19 +1082,11 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     if ((rc = status(hipMemcpy(device_rows.pointer, rows.data(), order * sizeof(int64_t), hipMemcpyHostToDevice),
                      "hipMemcpy")))
         return rc;
-    if (blocked) {
-        const hipError_t e = wide ? invert_blocks<double>(factors.as<double>(), n, inverses.as<double>(), device->stream)
-                                  : invert_blocks<float>(factors.as<float>(), n, inverses.as<float>(), device->stream);
-        if ((rc = status(e, "invert blocks"))) return rc;
-    }
+    // (the factorisation also made the diagonal blocks' inverses and links when `blocked`)
 
-    // |A|_inf, once.
-    std::vector<double> host(size_t(std::max<int64_t>(n, chunks * w * 3)));
-    if ((rc = status(hipMemcpyAsync(host.data(), sums.pointer, order * 8, hipMemcpyDeviceToHost, device->stream), "copy")))
-        return rc;
-    if ((rc = status(hipStreamSynchronize(device->stream), "sync"))) return rc;
+    // |A|_inf: its chunk maxima come back with the first residual's (no wait here).
+    std::vector<double> host(size_t(chunks * w * 3 + chunks));
     double norm = 0.0;
-    for (int64_t i = 0; i < n; ++i) norm = std::max(norm, host[size_t(i)]);
 
     // x = solve(b)
     auto solve_into = [&](const double *source, bool add) -> int {
@@ -1130,11 +1142,13 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
         if ((rc = status(column_max(residual.as<double>(), n, w, partials, device->stream), "norm"))) return rc;
         if ((rc = status(column_max(solution, n, w, partials + chunks * w, device->stream), "norm"))) return rc;
         if ((rc = status(column_max(right, n, w, partials + 2 * chunks * w, device->stream), "norm"))) return rc;
-        if ((rc = status(hipMemcpyAsync(host.data(), partials, size_t(chunks * w * 3) * 8, hipMemcpyDeviceToHost,
-                                        device->stream), "copy")))
+        if ((rc = status(hipMemcpyAsync(host.data(), partials, size_t(chunks * w * 3 + (iteration == 0 ? chunks : 0)) * 8,
+                                        hipMemcpyDeviceToHost, device->stream), "copy")))
             return rc;
         if ((rc = check_spins(device))) return rc;
         mark("residual");
+        if (iteration == 0)
+            for (int64_t k = 0; k < chunks; ++k) norm = std::max(norm, host[size_t(chunks * w * 3 + k)]);
         double worst = 0.0;
         for (int64_t c = 0; c < w; ++c) {
             double r = 0.0, x = 0.0, b = 0.0;

@@ -152,10 +152,11 @@ def test_substitute_matches_oracle(device, n, width):
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("n", [1024, 4096])
 @pytest.mark.parametrize("graded", [False, True], ids=["shifted", "graded"])
 @pytest.mark.parametrize("mode", list(FACTOR))
-def test_refined_solve(device, mode, graded):
-    n = 1024
+def test_refined_solve(device, mode, graded, n):
+    # n = 4096: the diagonal blocks of the first 8 of 32 panels are inverted beside the last 24 panels
     a, b = system(n, graded)
     x, stats = gpu_solve(device, a, b, mode)
     print(f"[test] solve n={n} {mode} graded={graded}: {stats.iterations} refinements, backward error "
