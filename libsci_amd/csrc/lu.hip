@@ -1910,19 +1910,40 @@ namespace scihip {
             };
             const int64_t r0 = (int64_t)(first + (int)blockIdx.x) * kBlock;
             const int t = threadIdx.x;
-            // A block of `size` < kBlock rows is completed by the identity, whose inverse it keeps.
-            for (int idx = t; idx < kBlock * kBlock; idx += kBlock) {
-                const int i = idx / kBlock, k = idx % kBlock;
-                if (upper ? k >= i : k < i)
-                    a_s[at(i, k)] = (i < size && k < size) ? f[(r0 + i) * n + r0 + k] : (i == k ? T(1) : T(0));
+            // A block of `size` < kBlock rows is completed by the identity, whose inverse it keeps. Rows
+            // are loaded 16 at a time, all 16 loads in flight before any is stored.
+            for (int i0 = 0; i0 < kBlock; i0 += 16) {
+                T v[16];
+#pragma unroll
+                for (int h = 0; h < 16; ++h) {
+                    const int i = i0 + h, k = t;
+                    const bool kept = upper ? k >= i : k < i;
+                    v[h] = kept && i < size && k < size ? f[(r0 + i) * n + r0 + k] : (i == k ? T(1) : T(0));
+                }
+#pragma unroll
+                for (int h = 0; h < 16; ++h) {
+                    const int i = i0 + h, k = t;
+                    if (upper ? k >= i : k < i) a_s[at(i, k)] = v[h];
+                }
             }
             __syncthreads();
             const int i = t;
             // The dot product of row i of the inverse and column j of the block over k in [from, to), as
-            // four interleaved partial sums, so that four independent fma chains are in flight.
+            // four interleaved partial sums, so that four independent fma chains are in flight; sixteen
+            // entries' loads are issued before their fmas (the same fmas in the same order).
             auto dot = [&](int j, int from, int to) {
                 T part[4] = {T(0), T(0), T(0), T(0)};
                 int k = from;
+                for (; k + 16 <= to; k += 16) {
+                    T x[16], y[16];
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        x[q] = m_s[at(i, k + q)];
+                        y[q] = a_s[at(k + q, j)];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) part[q % 4] = __builtin_fma(x[q], y[q], part[q % 4]);
+                }
                 for (; k + 4 <= to; k += 4)
 #pragma unroll
                     for (int q = 0; q < 4; ++q) part[q] = __builtin_fma(m_s[at(i, k + q)], a_s[at(k + q, j)], part[q]);
@@ -1944,7 +1965,6 @@ namespace scihip {
                 const T value = upper ? (k >= r ? m_s[at(r, k)] : T(0)) : (k < r ? m_s[at(r, k)] : (k == r ? T(1) : T(0)));
                 block[(int64_t)k * kBlock + r] = value;
             }
-        }
         }
 
         // One triangle of the factors for up to kTriangleColumns right-hand sides, all diagonal blocks in
@@ -2159,10 +2179,19 @@ namespace scihip {
 #pragma unroll
             for (int i = 0; i < 64; ++i) acc[i] = T(0);
             for (int m0 = 0; m0 < kBlock; m0 += M) {
-                for (int e = t; e < M * kBlock; e += 256) {
-                    const int mm = e / kBlock, c = e % kBlock;
-                    inverse_s[mm][c] = inverse[(int64_t)(m0 + mm) * kBlock + c];
-                    block_s[mm][c] = block[(int64_t)(m0 + mm) * n + c];
+                // all of a slice's loads in flight before any is stored
+                T iv[M * kBlock / 256], bv[M * kBlock / 256];
+#pragma unroll
+                for (int h = 0; h < M * kBlock / 256; ++h) {
+                    const int e = t + 256 * h, mm = e / kBlock, c = e % kBlock;
+                    iv[h] = inverse[(int64_t)(m0 + mm) * kBlock + c];
+                    bv[h] = block[(int64_t)(m0 + mm) * n + c];
+                }
+#pragma unroll
+                for (int h = 0; h < M * kBlock / 256; ++h) {
+                    const int e = t + 256 * h, mm = e / kBlock, c = e % kBlock;
+                    inverse_s[mm][c] = iv[h];
+                    block_s[mm][c] = bv[h];
                 }
                 __syncthreads();
                 for (int mm = 0; mm < M; ++mm) {
