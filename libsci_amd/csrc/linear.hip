@@ -21,11 +21,12 @@ namespace scihip {
                 target[i] = target[i] + (double)delta[i];
         }
 
-        // One workgroup per row: the row copied to target (converted to D) and the sum of |a_ij| over
-        // it, two elements per thread per 16-byte load where the rows are 16-byte aligned.
+        // One workgroup per row: the row's columns from copy_from on copied to target (converted to D)
+        // and the sum of |a_ij| over the whole row, two elements per thread per 16-byte load where the
+        // rows are 16-byte aligned (copy_from even then).
         template <typename D>
         __global__ __launch_bounds__(256) void copy_row_sums_kernel(const double *a, int64_t columns, D *target,
-                                                                    double *sums) {
+                                                                    double *sums, int64_t copy_from) {
             const int64_t row = blockIdx.x;
             const double *source = a + row * columns;
             D *out = target + row * columns;
@@ -34,14 +35,16 @@ namespace scihip {
                 typedef double Two __attribute__((ext_vector_type(2)));
                 for (int64_t j = 2 * (int64_t)threadIdx.x; j < columns; j += 2 * blockDim.x) {
                     const Two v = *reinterpret_cast<const Two *>(source + j);
-                    out[j] = (D)v[0];
-                    out[j + 1] = (D)v[1];
+                    if (j >= copy_from) {
+                        out[j] = (D)v[0];
+                        out[j + 1] = (D)v[1];
+                    }
                     s += __builtin_fabs(v[0]);
                     s += __builtin_fabs(v[1]);
                 }
             } else {
                 for (int64_t j = threadIdx.x; j < columns; j += blockDim.x) {
-                    out[j] = (D)source[j];
+                    if (j >= copy_from) out[j] = (D)source[j];
                     s += __builtin_fabs(source[j]);
                 }
             }
@@ -51,6 +54,16 @@ namespace scihip {
             if ((threadIdx.x & 63) == 0) waves[threadIdx.x >> 6] = s;
             __syncthreads();
             if (threadIdx.x == 0) sums[row] = (waves[0] + waves[1]) + (waves[2] + waves[3]);
+        }
+
+        // Columns [0, count) of every row of a (row length columns) copied to target (converted to D).
+        template <typename D>
+        __global__ __launch_bounds__(256) void copy_columns_kernel(const double *a, int64_t rows, int64_t columns,
+                                                                   int64_t count, D *target) {
+            const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            if (idx >= rows * count) return;
+            const int64_t r = idx / count, c = idx % count;
+            target[r * columns + c] = (D)a[r * columns + c];
         }
 
         // Partial column maxima of |x| over row chunks: partial[chunk * width + c].
@@ -110,18 +123,34 @@ namespace scihip {
     }
 
     hipError_t copy_with_row_sums(const double *a, int64_t rows, int64_t columns, double *target, double *sums,
-                                  hipStream_t stream) {
+                                  hipStream_t stream, int64_t copy_from) {
         if (rows == 0) return hipSuccess;
         hipLaunchKernelGGL(copy_row_sums_kernel<double>, dim3((unsigned)rows), dim3(256), 0, stream, a, columns, target,
-                           sums);
+                           sums, copy_from);
         return hipGetLastError();
     }
 
     hipError_t copy_with_row_sums(const double *a, int64_t rows, int64_t columns, float *target, double *sums,
-                                  hipStream_t stream) {
+                                  hipStream_t stream, int64_t copy_from) {
         if (rows == 0) return hipSuccess;
         hipLaunchKernelGGL(copy_row_sums_kernel<float>, dim3((unsigned)rows), dim3(256), 0, stream, a, columns, target,
-                           sums);
+                           sums, copy_from);
+        return hipGetLastError();
+    }
+
+    hipError_t copy_columns(const double *a, int64_t rows, int64_t columns, int64_t count, double *target,
+                            hipStream_t stream) {
+        if (rows == 0 || count == 0) return hipSuccess;
+        hipLaunchKernelGGL(copy_columns_kernel<double>, dim3((unsigned)((rows * count + 255) / 256)), dim3(256), 0,
+                           stream, a, rows, columns, count, target);
+        return hipGetLastError();
+    }
+
+    hipError_t copy_columns(const double *a, int64_t rows, int64_t columns, int64_t count, float *target,
+                            hipStream_t stream) {
+        if (rows == 0 || count == 0) return hipSuccess;
+        hipLaunchKernelGGL(copy_columns_kernel<float>, dim3((unsigned)((rows * count + 255) / 256)), dim3(256), 0,
+                           stream, a, rows, columns, count, target);
         return hipGetLastError();
     }
 

@@ -2402,22 +2402,29 @@ namespace scihip {
     } // namespace
 
     namespace {
-        // invert_blocks for diagonal blocks [first, first + count).
+        // invert_blocks for diagonal blocks [first, first + count): L's inverses and links (lower), U's
+        // (upper).
         template <typename T>
-        hipError_t invert_range(const T *f, int64_t n, T *inverses, int first, int count, hipStream_t stream) {
+        hipError_t invert_range(const T *f, int64_t n, T *inverses, int first, int count, hipStream_t stream,
+                                bool lower = true, bool upper = true) {
             if (count <= 0) return hipSuccess;
             const dim3 grid((unsigned)count);
-            hipLaunchKernelGGL((invert_diagonal<T, false>), grid, dim3(kBlock), 0, stream, f, n, kBlock, inverses, first);
-            hipLaunchKernelGGL((invert_diagonal<T, true>), grid, dim3(kBlock), 0, stream, f, n, kBlock,
-                               inverses + n * kBlock, first);
-            hipLaunchKernelGGL((link_blocks<T, false, 1>), grid, dim3(256), 0, stream, f, n, inverses,
-                               inverses + 2 * n * kBlock, first);
-            hipLaunchKernelGGL((link_blocks<T, true, 1>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
-                               inverses + 3 * n * kBlock, first);
-            hipLaunchKernelGGL((link_blocks<T, false, 2>), grid, dim3(256), 0, stream, f, n, inverses,
-                               inverses + 4 * n * kBlock, first);
-            hipLaunchKernelGGL((link_blocks<T, true, 2>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
-                               inverses + 5 * n * kBlock, first);
+            if (lower) {
+                hipLaunchKernelGGL((invert_diagonal<T, false>), grid, dim3(kBlock), 0, stream, f, n, kBlock, inverses,
+                                   first);
+                hipLaunchKernelGGL((link_blocks<T, false, 1>), grid, dim3(256), 0, stream, f, n, inverses,
+                                   inverses + 2 * n * kBlock, first);
+                hipLaunchKernelGGL((link_blocks<T, false, 2>), grid, dim3(256), 0, stream, f, n, inverses,
+                                   inverses + 4 * n * kBlock, first);
+            }
+            if (upper) {
+                hipLaunchKernelGGL((invert_diagonal<T, true>), grid, dim3(kBlock), 0, stream, f, n, kBlock,
+                                   inverses + n * kBlock, first);
+                hipLaunchKernelGGL((link_blocks<T, true, 1>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
+                                   inverses + 3 * n * kBlock, first);
+                hipLaunchKernelGGL((link_blocks<T, true, 2>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
+                                   inverses + 5 * n * kBlock, first);
+            }
             return hipGetLastError();
         }
     } // namespace
@@ -2445,7 +2452,7 @@ namespace scihip {
     int factor(T *a, int64_t n, int64_t *pivots, int mode, Workspace &ws, void *scratch, size_t scratch_bytes,
                hipStream_t stream, const char **error, T *inverses) {
         *error = nullptr;
-        if (inverses && (!blocked_substitution(n) || !ws.third || !ws.inverted)) {
+        if (inverses && (!blocked_substitution(n) || !ws.third || !ws.inverted || !ws.upper_inverted)) {
             *error = "LU inverses need an order that is a multiple of 128 (at least 256) and the third stream";
             return -1;
         }
@@ -2776,9 +2783,10 @@ namespace scihip {
         if (e != hipSuccess) return (int)e;
         for (int64_t j0 = 0; j0 < n; j0 += kPanel) {
             const int w = (int)std::min<int64_t>(kPanel, n - j0);
-            if (j0 > 0) { // the previous step's work on the side stream
+            if (j0 > 0 || ws.head_pending) { // the previous step's work on the side stream (or the caller's)
                 e = hipStreamWaitEvent(stream, ws.panel_done, 0);
                 if (e != hipSuccess) return (int)e;
+                ws.head_pending = false;
             }
             timer.stage(0);
             const int64_t rest = n - j0 - w;
@@ -2806,9 +2814,18 @@ namespace scihip {
             }
         }
         if (inverses) {
-            e = invert_range(static_cast<const T *>(a), n, inverses, early_blocks, blocks_total - early_blocks, stream);
+            // The last blocks: L's part on this stream (the forward substitution comes next), U's on the
+            // third stream, which only the backward substitution waits for (substitute, upper_pending).
+            const int late = blocks_total - early_blocks;
+            e = invert_range(static_cast<const T *>(a), n, inverses, early_blocks, late, stream, true, false);
             if (e == hipSuccess && early_blocks > 0) e = hipStreamWaitEvent(stream, ws.inverted, 0);
+            if (e == hipSuccess) e = hipEventRecord(ws.columns_ready, stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ws.third, ws.columns_ready, 0);
+            if (e == hipSuccess)
+                e = invert_range(static_cast<const T *>(a), n, inverses, early_blocks, late, ws.third, false, true);
+            if (e == hipSuccess) e = hipEventRecord(ws.upper_inverted, ws.third);
             if (e != hipSuccess) return (int)e;
+            ws.upper_pending = true;
         }
         return hipGetLastError();
     }
@@ -2845,6 +2862,10 @@ namespace scihip {
             for (int64_t col0 = 0; col0 < width; col0 += kTriangleColumns)
                 for (int pass = 0; pass < 2; ++pass) {
                     const unsigned tag = ++epoch;
+                    if (pass == 1 && ws.upper_pending) { // U's last inverses, made beside the forward pass
+                        if (hipError_t e = hipStreamWaitEvent(stream, ws.upper_inverted, 0)) return (int)e;
+                        ws.upper_pending = false;
+                    }
                     if (pass == 0)
                         hipLaunchKernelGGL((solve_triangle<T, false>), dim3((unsigned)(4 * nb)), dim3(256), 0, stream,
                                            f, inverse_lower, inverses + 2 * n * kBlock, inverses + 4 * n * kBlock, x,

@@ -51,6 +51,10 @@ namespace scihip {
         hipEvent_t panel_done = nullptr;    // side stream: the step's work there is done
         hipStream_t third = nullptr;        // diagonal-block inverses of finished blocks, beside the last panels
         hipEvent_t inverted = nullptr;      // third stream: those inverses are done
+        hipEvent_t upper_inverted = nullptr; // third stream: U's inverses and links of the last blocks are done
+        bool upper_pending = false;         // the next substitute() waits for upper_inverted before U's pass
+        bool head_pending = false;          // factor() waits for panel_done (recorded by its caller on the side
+                                            // stream) before its first step's work
     };
 
     // Factors the order x order row-major matrix in place (T = float or double); pivots[k] is the

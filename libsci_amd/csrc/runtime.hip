@@ -484,6 +484,8 @@ namespace scihip {
             if (int rc = status(hipStreamCreateWithFlags(&ws.third, hipStreamNonBlocking), "hipStreamCreate")) return rc;
             if (int rc = status(hipEventCreateWithFlags(&ws.inverted, hipEventDisableTiming), "hipEventCreate"))
                 return rc;
+            if (int rc = status(hipEventCreateWithFlags(&ws.upper_inverted, hipEventDisableTiming), "hipEventCreate"))
+                return rc;
         }
         if (!ws.counter) {
             void *block = nullptr;
@@ -533,6 +535,7 @@ namespace scihip {
             (void)hipStreamDestroy(ws.third);
         }
         if (ws.inverted) (void)hipEventDestroy(ws.inverted);
+        if (ws.upper_inverted) (void)hipEventDestroy(ws.upper_inverted);
         if (ws.columns_ready) (void)hipEventDestroy(ws.columns_ready);
         if (ws.panel_done) (void)hipEventDestroy(ws.panel_done);
         if (ws.counter) (void)hipFree(ws.counter);
@@ -1053,24 +1056,35 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     if (rc) return rc;
     if ((rc = ensure_lu(device, n, eb))) return rc;
 
-    // Factor a copy of A in the chosen precision; the copy also gives |A|'s row sums.
+    // Factor a copy of A in the chosen precision; the copy also gives |A|'s row sums. Beyond one panel,
+    // the first panel's columns are copied first, and the rest of the copy and the row sums run on the
+    // LU side stream beside that panel (factor() waits for them before its first step's work).
+    auto copy_in = [&](auto *target) -> int {
+        Workspace &ws = device->lu;
+        const bool split = n > kPanel;
+        hipStream_t s = split ? ws.side : device->stream;
+        if (split) {
+            if (int e = status(copy_columns(matrix, n, n, kPanel, target, device->stream), "copy")) return e;
+            if (int e = status(hipEventRecord(ws.columns_ready, device->stream), "event")) return e;
+            if (int e = status(hipStreamWaitEvent(ws.side, ws.columns_ready, 0), "event")) return e;
+        }
+        if (int e = status(copy_with_row_sums(matrix, n, n, target, sums.as<double>(), s, split ? kPanel : 0), "copy"))
+            return e;
+        if (int e = status(column_max(sums.as<double>(), n, 1, partial.as<double>() + chunks * w * 3, s), "norm"))
+            return e;
+        if (split) {
+            if (int e = status(hipEventRecord(ws.panel_done, ws.side), "event")) return e;
+            ws.head_pending = true;
+        }
+        return SCIHIP_OK;
+    };
     std::vector<int64_t> pivots;
     if (wide) {
-        if ((rc = status(copy_with_row_sums(matrix, n, n, factors.as<double>(), sums.as<double>(), device->stream),
-                         "copy")))
-            return rc;
-        if ((rc = status(column_max(sums.as<double>(), n, 1, partial.as<double>() + chunks * w * 3, device->stream),
-                         "norm")))
-            return rc;
+        if ((rc = copy_in(factors.as<double>()))) return rc;
         rc = factor_on_device<double>(device, factors.as<double>(), n, kFloat64, pivots,
                                       blocked ? inverses.as<double>() : nullptr);
     } else {
-        if ((rc = status(copy_with_row_sums(matrix, n, n, factors.as<float>(), sums.as<double>(), device->stream),
-                         "convert")))
-            return rc;
-        if ((rc = status(column_max(sums.as<double>(), n, 1, partial.as<double>() + chunks * w * 3, device->stream),
-                         "norm")))
-            return rc;
+        if ((rc = copy_in(factors.as<float>()))) return rc;
         const int product_mode = mode == SCIHIP_FACTOR_FLOAT32 ? kFloat32 : mode == SCIHIP_FACTOR_SPLIT ? kSplit : kBfloat16;
         rc = factor_on_device<float>(device, factors.as<float>(), n, product_mode, pivots,
                                      blocked ? inverses.as<float>() : nullptr);
