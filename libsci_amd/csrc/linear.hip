@@ -94,6 +94,37 @@ namespace scihip {
             }
         }
 
+        // One workgroup folds column_max partials over their chunks into pinned host memory: sets of
+        // `width` columns (chunks rows each, set s at partial + s * chunks * width) to out[s * width +
+        // c], then (extra > 0) one more single column of `extra` chunk maxima at partial + sets *
+        // chunks * width to out[sets * width], then *error to the next word. Each fold keeps the
+        // larger of the running value and the next partial, starting from 0 (a NaN partial is passed
+        // over, as a host loop of std::max from 0 would do).
+        __global__ __launch_bounds__(256) void fold_norms_kernel(const double *partial, int64_t chunks, int64_t width,
+                                                                 int sets, int64_t extra, const unsigned *error,
+                                                                 double *out) {
+            for (int64_t idx = threadIdx.x; idx < sets * width; idx += blockDim.x) {
+                const int64_t s = idx / width, c = idx % width;
+                double m = 0.0;
+                for (int64_t k = 0; k < chunks; ++k) {
+                    const double v = partial[(s * chunks + k) * width + c];
+                    m = m < v ? v : m;
+                }
+                out[idx] = m;
+            }
+            if (threadIdx.x == 0) {
+                if (extra > 0) {
+                    double m = 0.0;
+                    for (int64_t k = 0; k < extra; ++k) {
+                        const double v = partial[sets * chunks * width + k];
+                        m = m < v ? v : m;
+                    }
+                    out[sets * width] = m;
+                }
+                out[sets * width + 1] = error ? (double)__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+            }
+        }
+
         unsigned grid_for(int64_t count) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((count + 255) / 256, 8192)); }
 
     } // namespace
@@ -157,6 +188,13 @@ namespace scihip {
     // Chunks of about 64 rows (16 per thread row), at most 512: enough workgroups that the loads of a
     // 16384-row column set are a few latencies deep (1024-row chunks took 96 us for 4 MB).
     int64_t column_max_chunks(int64_t rows) { return std::max<int64_t>(1, std::min<int64_t>(512, (rows + 63) / 64)); }
+
+    hipError_t fold_norms(const double *partial, int64_t chunks, int64_t width, int sets, int64_t extra,
+                          const unsigned *error, double *host_out, hipStream_t stream) {
+        hipLaunchKernelGGL(fold_norms_kernel, dim3(1), dim3(256), 0, stream, partial, chunks, width, sets, extra, error,
+                           host_out);
+        return hipGetLastError();
+    }
 
     hipError_t column_max(const double *x, int64_t rows, int64_t width, double *partial, hipStream_t stream) {
         if (rows == 0 || width == 0) return hipSuccess;

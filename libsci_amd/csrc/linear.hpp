@@ -23,6 +23,11 @@ namespace scihip {
                             hipStream_t stream);
     hipError_t copy_columns(const double *a, int64_t rows, int64_t columns, int64_t count, float *target,
                             hipStream_t stream);
+    // Folds column_max partials into host memory the device can write (pinned): `sets` sets of `width`
+    // columns to host_out[s * width + c]; with extra > 0, one more single column of `extra` chunks to
+    // host_out[sets * width]; then *error (0 if null) to host_out[sets * width + 1].
+    hipError_t fold_norms(const double *partial, int64_t chunks, int64_t width, int sets, int64_t extra,
+                          const unsigned *error, double *host_out, hipStream_t stream);
     // Partial column maxima of |x|: column_max_chunks(rows) rows of `width` values.
     int64_t column_max_chunks(int64_t rows);
     hipError_t column_max(const double *x, int64_t rows, int64_t width, double *partial, hipStream_t stream);

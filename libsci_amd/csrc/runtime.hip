@@ -544,6 +544,35 @@ namespace scihip {
         ws = Workspace{};
     }
 
+    // Grows the pinned host buffers of the LU paths to `doubles` doubles and `indices` 64-bit entries.
+    int ensure_lu_host(scihip_device *device, size_t doubles, size_t indices) {
+        if (doubles > device->lu_host_count) {
+            if (device->lu_host) {
+                (void)hipStreamSynchronize(device->stream);
+                (void)hipHostFree(device->lu_host);
+                device->lu_host = nullptr;
+                device->lu_host_count = 0;
+            }
+            if (int rc = status(hipHostMalloc(reinterpret_cast<void **>(&device->lu_host), doubles * sizeof(double)),
+                                "hipHostMalloc"))
+                return rc;
+            device->lu_host_count = doubles;
+        }
+        if (indices > device->lu_index_count) {
+            if (device->lu_index) {
+                (void)hipStreamSynchronize(device->stream);
+                (void)hipHostFree(device->lu_index);
+                device->lu_index = nullptr;
+                device->lu_index_count = 0;
+            }
+            if (int rc = status(hipHostMalloc(reinterpret_cast<void **>(&device->lu_index), indices * sizeof(int64_t)),
+                                "hipHostMalloc"))
+                return rc;
+            device->lu_index_count = indices;
+        }
+        return SCIHIP_OK;
+    }
+
     // Checks the error flag of the bounded waits after the queued work has finished.
     int check_spins(scihip_device *device) {
         if (int rc = status(hipStreamSynchronize(device->stream), "LU")) return rc;
@@ -658,14 +687,27 @@ namespace scihip {
         const int e = factor<T>(matrix, order, pivot_block.as<int64_t>(), mode, device->lu, work.pointer,
                                 factor_scratch(order, mode), device->stream, &message, inverses);
         if (e) {
-            fail(message ? message : hipGetErrorString(hipError_t(e)));
+            fail(std::string("LU factorisation: ") + (message ? message : hipGetErrorString(hipError_t(e))));
             return SCIHIP_EDEVICE;
         }
-        if ((rc = check_spins(device))) return rc;
-        pivots.resize(size_t(order));
-        return status(hipMemcpy(pivots.data(), pivot_block.pointer, size_t(order) * sizeof(int64_t),
-                                hipMemcpyDeviceToHost),
-                      "hipMemcpy");
+        // The pivots and the bounded waits' error flag into pinned host memory, then one wait.
+        if ((rc = ensure_lu_host(device, 2, size_t(order)))) return rc;
+        unsigned *flag = reinterpret_cast<unsigned *>(device->lu_host);
+        *flag = 0;
+        if ((rc = status(hipMemcpyAsync(device->lu_index, pivot_block.pointer, size_t(order) * sizeof(int64_t),
+                                        hipMemcpyDeviceToHost, device->stream), "hipMemcpy")))
+            return rc;
+        if ((rc = status(hipMemcpyAsync(flag, device->lu.error, sizeof(unsigned), hipMemcpyDeviceToHost, device->stream),
+                         "hipMemcpy")))
+            return rc;
+        if ((rc = status(hipStreamSynchronize(device->stream), "LU"))) return rc;
+        if (*flag) {
+            (void)hipMemset(device->lu.error, 0, sizeof(unsigned));
+            fail(spin_failure());
+            return SCIHIP_EDEVICE;
+        }
+        pivots.assign(device->lu_index, device->lu_index + order);
+        return SCIHIP_OK;
     }
 
     // x = U^-1 L^-1 P source (source in S, solved in T), rows on the device.
@@ -677,7 +719,7 @@ namespace scihip {
         const int e = substitute<T>(factors, order, x, width, device->lu, device->epoch, device->stream, &message,
                                     inverses, packed);
         if (e) {
-            fail(message ? message : hipGetErrorString(hipError_t(e)));
+            fail(std::string("substitution: ") + (message ? message : hipGetErrorString(hipError_t(e))));
             return SCIHIP_EDEVICE;
         }
         return SCIHIP_OK;
@@ -749,6 +791,8 @@ int scihip_close(scihip_device *device) {
         (void)hipFree(device->result);
         (void)hipHostFree(device->staging);
         (void)hipHostFree(device->staging_partial);
+        if (device->lu_host) (void)hipHostFree(device->lu_host);
+        if (device->lu_index) (void)hipHostFree(device->lu_index);
         (void)hipStreamDestroy(device->stream);
     }
     delete device;
@@ -1092,14 +1136,18 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
     if (rc) return rc;
     stats->factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
     mark("factor");
-    const std::vector<int64_t> rows = rows_of(pivots);
-    if ((rc = status(hipMemcpy(device_rows.pointer, rows.data(), order * sizeof(int64_t), hipMemcpyHostToDevice),
-                     "hipMemcpy")))
-        return rc;
+    // The row order goes up from pinned memory without a wait (nothing writes lu_index again before the
+    // waits below).
+    {
+        const std::vector<int64_t> rows = rows_of(pivots);
+        std::copy(rows.begin(), rows.end(), device->lu_index);
+        if ((rc = status(hipMemcpyAsync(device_rows.pointer, device->lu_index, order * sizeof(int64_t),
+                                        hipMemcpyHostToDevice, device->stream), "hipMemcpy")))
+            return rc;
+    }
     // (the factorisation also made the diagonal blocks' inverses and links when `blocked`)
 
-    // |A|_inf: its chunk maxima come back with the first residual's (no wait here).
-    std::vector<double> host(size_t(chunks * w * 3 + chunks));
+    // |A|_inf: folded with the first residual's column maxima (no wait here).
     double norm = 0.0;
 
     // x = solve(b)
@@ -1156,21 +1204,24 @@ int scihip_solve(scihip_device *device, uint32_t mode, const double *matrix, con
         if ((rc = status(column_max(residual.as<double>(), n, w, partials, device->stream), "norm"))) return rc;
         if ((rc = status(column_max(solution, n, w, partials + chunks * w, device->stream), "norm"))) return rc;
         if ((rc = status(column_max(right, n, w, partials + 2 * chunks * w, device->stream), "norm"))) return rc;
-        if ((rc = status(hipMemcpyAsync(host.data(), partials, size_t(chunks * w * 3 + (iteration == 0 ? chunks : 0)) * 8,
-                                        hipMemcpyDeviceToHost, device->stream), "copy")))
+        // The column maxima folded on the device straight into pinned host memory, with |A|_inf on the
+        // first pass and the bounded waits' error flag: one wait, no copy.
+        if ((rc = ensure_lu_host(device, size_t(3 * w + 2), 0))) return rc;
+        double *folded = device->lu_host;
+        if ((rc = status(fold_norms(partials, chunks, w, 3, iteration == 0 ? chunks : 0, device->lu.error, folded,
+                                    device->stream), "norm")))
             return rc;
-        if ((rc = check_spins(device))) return rc;
+        if ((rc = status(hipStreamSynchronize(device->stream), "LU"))) return rc;
+        if (folded[3 * w + 1] != 0.0) {
+            (void)hipMemset(device->lu.error, 0, sizeof(unsigned));
+            fail(spin_failure());
+            return SCIHIP_EDEVICE;
+        }
         mark("residual");
-        if (iteration == 0)
-            for (int64_t k = 0; k < chunks; ++k) norm = std::max(norm, host[size_t(chunks * w * 3 + k)]);
+        if (iteration == 0) norm = folded[3 * w];
         double worst = 0.0;
         for (int64_t c = 0; c < w; ++c) {
-            double r = 0.0, x = 0.0, b = 0.0;
-            for (int64_t k = 0; k < chunks; ++k) {
-                r = std::max(r, host[size_t(k * w + c)]);
-                x = std::max(x, host[size_t((chunks + k) * w + c)]);
-                b = std::max(b, host[size_t((2 * chunks + k) * w + c)]);
-            }
+            const double r = folded[c], x = folded[w + c], b = folded[2 * w + c];
             const double denominator = norm * x + b;
             const double e = denominator > 0.0 ? r / denominator : r;
             worst = std::isnan(e) || std::isnan(worst) ? std::numeric_limits<double>::quiet_NaN() : std::max(worst, e);

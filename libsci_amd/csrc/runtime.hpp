@@ -93,6 +93,10 @@ struct scihip_device {
     double *result = nullptr;   // reduction result on the device
     double *staging = nullptr;  // pinned host word the fold kernel writes a synchronous result into
     scihip::Partial *staging_partial = nullptr; // pinned host partial of a one-workgroup reduction
+    double *lu_host = nullptr;  // pinned host: a solve's folded column maxima and the LU error flag
+    size_t lu_host_count = 0;   // doubles in lu_host
+    int64_t *lu_index = nullptr; // pinned host: a factorisation's pivots, then the solve's row order
+    size_t lu_index_count = 0;  // entries in lu_index
     int policy = 0;             // 0 automatic, 1 interpreter only, 2 specialised only
     scihip::Workspace lu;       // LU barrier counter, error flag, solve flags, panel publications
     unsigned epoch = 0;         // launches of the solve kernels so far (their ready-flag values)
