@@ -1892,78 +1892,143 @@ namespace scihip {
         }
 
         // Inverts each kBlock x kBlock diagonal block of the unit lower L (upper false) or of U (upper
-        // true) in LDS, one workgroup per block, and stores it transposed: out[b][k][r] = inv(block
-        // b)[r][k]. Lower: M L = I gives, for i > j, M[i][j] = -(L[i][j] + sum_{j<k<i} M[i][k] L[k][j]),
-        // columns from the last; upper: N U = I gives N[j][j] = 1 / U[j][j] and, for i < j,
-        // N[i][j] = -(sum_{i<=k<j} N[i][k] U[k][j]) / U[j][j], columns from the first. Thread i makes
-        // row i, which needs only its own earlier entries and the block, so the block and the inverse
-        // sit in separate LDS triangles (packed, so that both fit for doubles) and no thread waits for
-        // another between columns.
+        // true), one workgroup per block, and stores it transposed: out[b][k][r] = inv(block b)[r][k].
+        // Recursive 2 x 2 blocking in place in LDS: the sixteen-row diagonal blocks are inverted first,
+        // a thread per row in registers (lower: M L = I row by row, M[i][j] = -(L[i][j] + sum_{j<k<i}
+        // M[i][k] L[k][j]); upper: N U = I, N[i][i] = 1 / U[i][i], N[i][j] = -(sum_{i<=k<j} N[i][k]
+        // U[k][j]) / U[j][j]); then, for diagonal blocks of 32, 64 and 128 rows, the off-diagonal half
+        // becomes -inv(B) C inv(A) (lower, C below the diagonal halves A and B already inverted) or
+        // -inv(A) B inv(C) (upper, B right of them), as two products whose sums run over the nonzero
+        // triangle only, all threads sharing each level's outputs. (A row's sequential chain of 8,000
+        // multiply-adds took about 290 us per block; this takes a few.) L's diagonal, which holds U's in
+        // the packed factors, is taken as 1.
         template <typename T, bool upper>
-        __global__ __launch_bounds__(kBlock) void invert_diagonal(const T *f, int64_t n, int size, T *out, int first) {
-            // packed triangle: lower strictly below the diagonal, upper with it
-            constexpr int kCount = upper ? kBlock * (kBlock + 1) / 2 : kBlock * (kBlock - 1) / 2;
-            __shared__ T a_s[kCount]; // the block
-            __shared__ T m_s[kCount]; // the inverse
-            auto at = [](int i, int k) -> int {
-                return upper ? i * kBlock - i * (i - 1) / 2 + (k - i) : i * (i - 1) / 2 + k;
-            };
-            const int64_t r0 = (int64_t)(first + (int)blockIdx.x) * kBlock;
+        __global__ __launch_bounds__(256) void invert_diagonal(const T *f, int64_t n, int size, T *out, int first) {
+            constexpr int S = kBlock;
+            constexpr int stride = sizeof(T) == 4 ? S + 1 : S; // doubles: 128 KB, the padding does not fit
+            constexpr int kBase = 16;
+            __shared__ T m[S * stride];
+            const int64_t r0 = (int64_t)(first + (int)blockIdx.x) * S;
             const int t = threadIdx.x;
-            // A block of `size` < kBlock rows is completed by the identity, whose inverse it keeps. Rows
-            // are loaded 16 at a time, all 16 loads in flight before any is stored.
-            for (int i0 = 0; i0 < kBlock; i0 += 16) {
+            // A block of `size` < kBlock rows is completed by the identity, whose inverse it keeps.
+            for (int i0 = 0; i0 < S; i0 += 32) {
                 T v[16];
 #pragma unroll
                 for (int h = 0; h < 16; ++h) {
-                    const int i = i0 + h, k = t;
-                    const bool kept = upper ? k >= i : k < i;
-                    v[h] = kept && i < size && k < size ? f[(r0 + i) * n + r0 + k] : (i == k ? T(1) : T(0));
+                    const int i = i0 + 2 * h + (t >> 7), k = t & (S - 1);
+                    v[h] = (i < size && k < size) ? f[(r0 + i) * n + r0 + k] : (i == k ? T(1) : T(0));
                 }
 #pragma unroll
-                for (int h = 0; h < 16; ++h) {
-                    const int i = i0 + h, k = t;
-                    if (upper ? k >= i : k < i) a_s[at(i, k)] = v[h];
-                }
+                for (int h = 0; h < 16; ++h) m[(i0 + 2 * h + (t >> 7)) * stride + (t & (S - 1))] = v[h];
             }
             __syncthreads();
-            const int i = t;
-            // The dot product of row i of the inverse and column j of the block over k in [from, to), as
-            // four interleaved partial sums, so that four independent fma chains are in flight; sixteen
-            // entries' loads are issued before their fmas (the same fmas in the same order).
-            auto dot = [&](int j, int from, int to) {
-                T part[4] = {T(0), T(0), T(0), T(0)};
-                int k = from;
-                for (; k + 16 <= to; k += 16) {
-                    T x[16], y[16];
+            auto at = [&](int i, int k) -> T & { return m[i * stride + k]; };
+            // The sixteen-row diagonal blocks, a thread per row: the row of the inverse in registers, the
+            // block's other rows read from LDS before anyone writes.
+            {
+                T row[kBase];
+                const int i = t % kBase, base = (t / kBase) * kBase;
+                if (t < S) {
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        x[q] = m_s[at(i, k + q)];
-                        y[q] = a_s[at(k + q, j)];
+                    for (int j = 0; j < kBase; ++j) row[j] = T(0);
+                    if (!upper) {
+                        row[i] = T(1);
+#pragma unroll
+                        for (int j = kBase - 1; j >= 0; --j) {
+                            if (j >= i) continue;
+                            T s = at(base + i, base + j);
+#pragma unroll
+                            for (int k = j + 1; k < kBase; ++k)
+                                if (k < i) s = s + row[k] * at(base + k, base + j);
+                            row[j] = -s;
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < kBase; ++j) {
+                            if (j < i) continue;
+                            if (j == i) {
+                                row[j] = T(1) / at(base + j, base + j);
+                                continue;
+                            }
+                            T s = T(0);
+#pragma unroll
+                            for (int k = 0; k < kBase; ++k)
+                                if (k >= i && k < j) s = s + row[k] * at(base + k, base + j);
+                            row[j] = -s / at(base + j, base + j);
+                        }
                     }
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) part[q % 4] = __builtin_fma(x[q], y[q], part[q % 4]);
                 }
-                for (; k + 4 <= to; k += 4)
+                __syncthreads();
+                if (t < S)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) part[q] = __builtin_fma(m_s[at(i, k + q)], a_s[at(k + q, j)], part[q]);
-                for (; k < to; ++k) part[0] = __builtin_fma(m_s[at(i, k)], a_s[at(k, j)], part[0]);
-                return (part[0] + part[1]) + (part[2] + part[3]);
-            };
-            if (upper) {
-                for (int j = i; j < kBlock; ++j) {
-                    const T v = j == i ? T(1) / a_s[at(j, j)] : -dot(j, i, j) / a_s[at(j, j)];
-                    m_s[at(i, j)] = v;
-                }
-            } else {
-                for (int j = i - 1; j >= 0; --j) m_s[at(i, j)] = -(a_s[at(i, j)] + dot(j, j + 1, i));
+                    for (int j = 0; j < kBase; ++j)
+                        if (upper ? j >= i : j <= i) at(base + i, base + j) = row[j];
+                __syncthreads();
             }
-            __syncthreads();
-            T *block = out + (int64_t)(first + (int)blockIdx.x) * kBlock * kBlock;
-            for (int idx = t; idx < kBlock * kBlock; idx += kBlock) {
-                const int r = idx % kBlock, k = idx / kBlock; // consecutive threads write consecutive r
-                const T value = upper ? (k >= r ? m_s[at(r, k)] : T(0)) : (k < r ? m_s[at(r, k)] : (k == r ? T(1) : T(0)));
-                block[(int64_t)k * kBlock + r] = value;
+            // Levels of 32, 64 and 128 rows: each block's off-diagonal half from its inverted diagonal
+            // halves, h x h outputs per block, spread over the threads (consecutive threads consecutive
+            // columns), in two products with a barrier after each.
+            for (int s = 2 * kBase; s <= S; s *= 2) {
+                const int h = s / 2, outputs = (S / s) * h * h; // at most 4096: 16 per thread
+                T part[16];
+                // first product: lower T = C inv(A) (sum over k >= c, inv(A)[c][c] = 1); upper T = B inv(C)
+                // (sum over k <= c)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int idx = t + 256 * q;
+                    if (idx >= outputs) continue;
+                    const int o = (idx / (h * h)) * s, r = (idx % (h * h)) / h, c = idx % h;
+                    T acc = T(0);
+                    if (!upper) {
+                        acc = at(o + h + r, o + c);
+                        for (int k = c + 1; k < h; ++k) acc = acc + at(o + h + r, o + k) * at(o + k, o + c);
+                    } else {
+                        for (int k = 0; k <= c; ++k) acc = acc + at(o + r, o + h + k) * at(o + h + k, o + h + c);
+                    }
+                    part[q] = acc;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int idx = t + 256 * q;
+                    if (idx >= outputs) continue;
+                    const int o = (idx / (h * h)) * s, r = (idx % (h * h)) / h, c = idx % h;
+                    if (!upper) at(o + h + r, o + c) = part[q];
+                    else at(o + r, o + h + c) = part[q];
+                }
+                __syncthreads();
+                // second product: lower X = -inv(B) T (sum over k <= r, inv(B)[r][r] = 1); upper
+                // X = -inv(A) T (sum over k >= r)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int idx = t + 256 * q;
+                    if (idx >= outputs) continue;
+                    const int o = (idx / (h * h)) * s, r = (idx % (h * h)) / h, c = idx % h;
+                    T acc = T(0);
+                    if (!upper) {
+                        acc = at(o + h + r, o + c);
+                        for (int k = 0; k < r; ++k) acc = acc + at(o + h + r, o + h + k) * at(o + h + k, o + c);
+                    } else {
+                        for (int k = r; k < h; ++k) acc = acc + at(o + r, o + k) * at(o + k, o + h + c);
+                    }
+                    part[q] = -acc;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int idx = t + 256 * q;
+                    if (idx >= outputs) continue;
+                    const int o = (idx / (h * h)) * s, r = (idx % (h * h)) / h, c = idx % h;
+                    if (!upper) at(o + h + r, o + c) = part[q];
+                    else at(o + r, o + h + c) = part[q];
+                }
+                __syncthreads();
+            }
+            T *block = out + (int64_t)(first + (int)blockIdx.x) * S * S;
+            for (int idx = t; idx < S * S; idx += 256) {
+                const int r = idx % S, k = idx / S; // consecutive threads write consecutive r
+                const T value = upper ? (k >= r ? at(r, k) : T(0)) : (k < r ? at(r, k) : (k == r ? T(1) : T(0)));
+                block[(int64_t)k * S + r] = value;
             }
         }
 
@@ -2410,7 +2475,7 @@ namespace scihip {
             if (count <= 0) return hipSuccess;
             const dim3 grid((unsigned)count);
             if (lower) {
-                hipLaunchKernelGGL((invert_diagonal<T, false>), grid, dim3(kBlock), 0, stream, f, n, kBlock, inverses,
+                hipLaunchKernelGGL((invert_diagonal<T, false>), grid, dim3(256), 0, stream, f, n, kBlock, inverses,
                                    first);
                 hipLaunchKernelGGL((link_blocks<T, false, 1>), grid, dim3(256), 0, stream, f, n, inverses,
                                    inverses + 2 * n * kBlock, first);
@@ -2418,7 +2483,7 @@ namespace scihip {
                                    inverses + 4 * n * kBlock, first);
             }
             if (upper) {
-                hipLaunchKernelGGL((invert_diagonal<T, true>), grid, dim3(kBlock), 0, stream, f, n, kBlock,
+                hipLaunchKernelGGL((invert_diagonal<T, true>), grid, dim3(256), 0, stream, f, n, kBlock,
                                    inverses + n * kBlock, first);
                 hipLaunchKernelGGL((link_blocks<T, true, 1>), grid, dim3(256), 0, stream, f, n, inverses + n * kBlock,
                                    inverses + 3 * n * kBlock, first);
